@@ -1,0 +1,159 @@
+#!/usr/bin/env python3
+"""Headline benchmark: fitted pixels/s of the whole imaging-FCS hot path
+(bin + polynomial bleach correction + multi-tau ACF/CCF with blocked variance
++ LM fit) on a 128x128x50,000-frame uint16 stack, on one MI355X.
+
+One step = for each binning b in (1, 2, 4): fused b x b binning, order-4
+polynomial bleach correction, ACF and CCF to the x+b neighbour with blocked
+variances, and the LM fit of N, D, G_inf for every ACF and CCF pixel.  The
+stack is device-resident before timing starts.  ``value`` counts ACF-fitted
+output pixels (the CCF work is inside the timed region but not counted).
+
+Prints one JSON line (see DESIGN.md for the roofline and baseline fields).
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+W = H = 128
+T = 50000
+N_EMITTERS = 2000
+D_QUADRANTS = [0.5, 1.0, 2.0, 5.0]
+BINNINGS = (1, 2, 4)
+PIX = 0.24e-6
+PSF = 0.38e-6
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_stack(seed=2024):
+    import imaging_fcs_1_52_amd as imfcs
+    t0 = time.time()
+    st = imfcs.simulate_2d(W, H, T, N_EMITTERS, D_um2s=D_QUADRANTS, pixel_um=0.24, psf_sigma_um=0.19,
+                           frametime=1e-3, counts_per_frame=20.0, offset=100, tau_bleach_s=15.0, margin=2,
+                           seed=seed, nthreads=min(16, os.cpu_count() or 1))
+    log("simulated %dx%dx%d stack in %.1f s" % (W, H, T, time.time() - t0))
+    return st
+
+
+def configs(stack_min):
+    import imaging_fcs_1_52_amd as imfcs
+    out = []
+    for b in BINNINGS:
+        ax = PIX * b
+        acf = imfcs.make_fit_model(ax, ax, PSF)
+        ccf = imfcs.make_fit_model(ax, ax, PSF, rx=PIX * b)
+        cfg = imfcs.make_pipeline_config(W, H, T, binning=b, cf=(b, 0), background=stack_min, poly_order=4,
+                                         do_acf=True, do_ccf=True, do_fit=True, acf_model=acf, ccf_model=ccf,
+                                         fit_tolerance=1e-16, fit_max_iterations=2000)
+        out.append(cfg)
+    return out
+
+
+def cpu_baseline(stack, stack_min, sample_side, nthreads):
+    """Oracle (C restatement of the Java CPU path) on a bounded sample of the
+    b=1 ACF workload: sample_side^2 pixels, all 50,000 frames."""
+    import oracle
+    oracle.build()
+    m = oracle.make_model(PIX, PIX, PSF)
+    cfg = oracle.make_config(W, H, T, binning=1, cf=(0, 0), background=stack_min, poly_order=4,
+                             roi=(0, 0, sample_side, sample_side), do_fit=True, model=m, nthreads=nthreads)
+    t0 = time.time()
+    r = oracle.pipeline(stack, cfg)
+    dt = time.time() - t0
+    npx = sample_side * sample_side
+    return npx / dt, dt, r
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-sample", type=int, default=24, help="side of the oracle's pixel sample (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (pixels shard without "
+              "communication; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import imaging_fcs_1_52_amd as imfcs
+    stack = make_stack()
+    smin = int(stack.min())
+    eng = imfcs.Engine(0)
+    eng.upload(stack)
+    cfgs = configs(smin)
+    n_fitted_pixels = 0
+    for c in cfgs:
+        ow, oh, _ = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        imfcs.lib().imfcs_plan(ctypes.byref(c), ctypes.byref(ow), ctypes.byref(oh), ctypes.byref(_))
+        n_fitted_pixels += ow.value * oh.value
+
+    def step():
+        stage = []
+        for c in cfgs:
+            eng.run(c)
+            stage.append(eng.stage_ms())
+        return stage
+
+    for i in range(args.warmup):
+        step()
+        log("warmup step %d done" % i)
+    eng.synchronize()
+    stage_sum = np.zeros(5)
+    per_bin = np.zeros((len(BINNINGS), 5))
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        st = step()
+        per_bin += np.asarray(st)
+        log("step %d done" % i)
+    eng.synchronize()
+    dt = time.perf_counter() - t0
+    ms_step = 1e3 * dt / args.steps
+    per_bin /= args.steps
+    stage_sum = per_bin.sum(0)
+    value = n_fitted_pixels / (dt / args.steps)
+
+    # roofline of the dominant kernel (the correlator) at b = 1
+    corr_ms = per_bin[0][3]
+    alg_bytes = 2.0 * W * H * T     # one read of the uint16 stack per correlator launch
+    achieved = alg_bytes / (corr_ms * 1e-3) / 1e9
+    roof = {"kernel": "k_correlate (b=1, ACF+CCF)", "bound": "hbm", "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "avg_ms": round(corr_ms, 3), "alg_bytes": alg_bytes}
+
+    cpu = None
+    if args.cpu_sample > 0:
+        rate, cdt, _ = cpu_baseline(stack, smin, args.cpu_sample, args.cpu_threads)
+        cpu = {"value": round(rate, 3), "unit": "fitted pixels/s", "cores": args.cpu_threads, "kind": "port",
+               "sample": "%dx%d px ROI of the b=1 ACF workload, 50,000 frames, poly4, LM fit (%.1f s)"
+                         % (args.cpu_sample, args.cpu_sample, cdt)}
+
+    line = {"metric": "fitted pixels/s (bin+bleach+ACF+LM), 128^2x50k", "value": round(value, 1),
+            "unit": "fitted pixels/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64", "data": "synthetic (seeded 2D particle simulation)",
+            "config": {"workload": "128x128x50000 uint16; binning 1,2,4; ACF+CCF(x+b); poly4 bleach; LM N,D,G",
+                       "fitted_pixels_per_step": n_fitted_pixels},
+            "stages_ms": {"trace_polyfit": round(stage_sum[0], 3), "means": round(stage_sum[1], 3),
+                          "blocking": round(stage_sum[2], 3), "correlate": round(stage_sum[3], 3),
+                          "fit": round(stage_sum[4], 3)},
+            "roofline": roof, "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

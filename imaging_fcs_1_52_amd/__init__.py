@@ -1,0 +1,332 @@
+"""MI355X-native imaging-FCS engine.
+
+The hot path of Imaging_FCS 1.52 -- binning, polynomial bleach correction,
+multi-tau ACF/CCF with blocked variance and the per-pixel Levenberg-Marquardt
+fit -- runs as hand-written HIP kernels in ``libimfcs_hip.so``.  This module is
+the host-side mirror of the plugin's JNI class ``gpufitImFCS.GpufitImFCS``
+(GpufitImFCS.java): the same method names, argument meaning and error
+behaviour, over the library's C ABI (include/imfcs.h).  There is no CPU
+fallback: every entry point raises if the native library cannot be loaded or
+no GPU is present.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import native
+from .native import (ACFParameters, FitModelGeom, PipelineConfig, MODEL_ACF_1D, MODEL_LINEAR_1D, ESTIMATOR_LSE,
+                     STATE_CONVERGED, STATE_MAX_ITERATION, STATE_SINGULAR_HESSIAN, lib, sim_lib)
+
+__all__ = ["ACFParameters", "FitModel", "FitResult", "Engine", "PipelineConfig", "FitModelGeom", "fit",
+           "get_last_error", "is_cuda_available", "get_cuda_version", "reset_gpu", "calc_data_bleach_correction",
+           "is_binning_memory_sufficient", "calc_binning", "is_acf_memory_sufficient", "calc_acf", "simulate_2d",
+           "correlator_tables", "obsvol", "make_fit_model", "make_pipeline_config"]
+
+
+class ImFCSError(RuntimeError):
+    pass
+
+
+def get_last_error():
+    """GpufitImFCS.getLastError (GpufitImFCS.java:382)."""
+    return lib().imfcs_get_last_error().decode()
+
+
+def is_cuda_available():
+    """GpufitImFCS.isCudaAvailable (GpufitImFCS.java:389-399)."""
+    return bool(lib().imfcs_device_available())
+
+
+def get_cuda_version():
+    """getCudaVersionAsArray (GpufitImFCS.java:403-436): (runtime, driver)
+    version strings "major.minor" of the HIP runtime."""
+    rv, dv = ctypes.c_int(), ctypes.c_int()
+    if lib().imfcs_get_versions(ctypes.byref(rv), ctypes.byref(dv)) != 0:
+        raise ImFCSError(get_last_error())
+    fmt = lambda v: "%d.%d" % (v // 1000, (v % 1000) // 10)
+    return fmt(rv.value), fmt(dv.value)
+
+
+def _check(rc):
+    if rc != 0:
+        raise ImFCSError(get_last_error())
+
+
+def reset_gpu():
+    """GpufitImFCS.resetGPU (GpufitImFCS.java:491)."""
+    _check(lib().imfcs_reset_gpu())
+
+
+class FitModel:
+    """GpufitImFCS.FitModel (GpufitImFCS.java:616-733) with numpy buffers."""
+
+    def __init__(self, number_fits, number_points, with_weights, model_id, tolerance=1e-4,
+                 max_number_iterations=25, num_valid_coefs=0, parameters_to_fit=None,
+                 estimator=ESTIMATOR_LSE, user_info_size=0):
+        npar = {MODEL_ACF_1D: 20, MODEL_LINEAR_1D: 11, 1: 5}[model_id]
+        self.number_fits, self.number_points, self.model_id = number_fits, number_points, model_id
+        self.number_parameters = npar
+        self.data = np.zeros(number_fits * number_points, np.float32)
+        self.weights = np.zeros(number_fits * number_points, np.float32) if with_weights else None
+        self.initial_parameters = np.zeros(number_fits * npar, np.float32)
+        self.tolerance = np.float32(tolerance)
+        self.max_number_iterations = int(max_number_iterations)
+        self.num_valid_coefs = int(num_valid_coefs)
+        self.parameters_to_fit = (np.ones(npar, np.int32) if parameters_to_fit is None
+                                  else np.asarray([1 if p else 0 for p in parameters_to_fit], np.int32))
+        self.estimator = estimator
+        self.user_info = np.zeros(max(0, user_info_size), np.float32)
+
+
+class FitResult:
+    """GpufitImFCS.FitResult (GpufitImFCS.java:745-805)."""
+
+    def __init__(self, number_fits, number_parameters):
+        self.parameters = np.zeros(number_fits * number_parameters, np.float32)
+        self.states = np.zeros(number_fits, np.int32)
+        self.chi_squares = np.zeros(number_fits, np.float32)
+        self.number_iterations = np.zeros(number_fits, np.int32)
+
+
+def _fp(a):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _ip(a):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def fit(fit_model, fit_result=None):
+    """GpufitImFCS.fit (GpufitImFCS.java:340-366): runs the batched LM and
+    raises on a non-zero status with the native error message.  The user
+    info size passed to the native side is the buffer capacity, as in the
+    reference (bytes for the native routine)."""
+    m = fit_model
+    r = fit_result or FitResult(m.number_fits, m.number_parameters)
+    ui = np.ascontiguousarray(m.user_info, np.float32)
+    status = lib().imfcs_fit(m.number_fits, m.number_points, _fp(m.data), _fp(m.weights), m.model_id,
+                             _fp(m.initial_parameters), ctypes.c_float(m.tolerance), m.max_number_iterations,
+                             m.num_valid_coefs, _ip(m.parameters_to_fit), m.estimator, ctypes.c_size_t(ui.size),
+                             ui.ctypes.data_as(ctypes.c_void_p), _fp(r.parameters), _ip(r.states),
+                             _fp(r.chi_squares), _ip(r.number_iterations))
+    if status != 0:
+        raise ImFCSError("status = %d, message = %s" % (status, get_last_error()))
+    return r
+
+
+def calc_data_bleach_correction(pixels, params):
+    """calcDataBleachCorrection (GpufitImFCS.java:501)."""
+    p = params
+    out = np.zeros(p.w_temp * p.h_temp * p.nopit, np.float32)
+    px = np.ascontiguousarray(pixels, np.float32)
+    _check(lib().imfcs_calc_data_bleach_correction(_fp(px), _fp(out), ctypes.byref(p)))
+    return out
+
+
+def is_binning_memory_sufficient(params):
+    rc = lib().imfcs_is_binning_memory_sufficient(ctypes.byref(params))
+    if rc < 0:
+        raise ImFCSError(get_last_error())
+    return rc == 1
+
+
+def calc_binning(indata, params):
+    """calcBinning (GpufitImFCS.java:520)."""
+    p = params
+    T = p.lastframe - p.firstframe + 1
+    out = np.zeros(p.w_temp * p.h_temp * T, np.float32)
+    src = np.ascontiguousarray(indata, np.float32)
+    _check(lib().imfcs_calc_binning(_fp(src), _fp(out), ctypes.byref(p)))
+    return out
+
+
+def is_acf_memory_sufficient(params):
+    rc = lib().imfcs_is_acf_memory_sufficient(ctypes.byref(params))
+    if rc < 0:
+        raise ImFCSError(get_last_error())
+    return rc == 1
+
+
+def calc_acf(pixels, bleachcorr_params, samp, lag, params):
+    """calcACF (GpufitImFCS.java:533-536).  Returns pixels1, blockvararray,
+    blocked1D as plane-major float64 arrays of width*height*chanum."""
+    p = params
+    n = p.width * p.height * p.chanum
+    pixels1, bvar, blocked = np.zeros(n), np.zeros(n), np.zeros(n)
+    nbm, nbc = np.zeros(p.width * p.height), np.zeros(p.width * p.height)
+    px = np.ascontiguousarray(pixels, np.float32)
+    bc = np.ascontiguousarray(bleachcorr_params, np.float64)
+    sp = np.ascontiguousarray(samp, np.float64)
+    lg = np.ascontiguousarray(lag, np.int32)
+    _check(lib().imfcs_calc_acf(_fp(px), _dp(pixels1), _dp(bvar), _dp(nbm), _dp(nbc), _dp(blocked), _dp(bc),
+                                _dp(sp), _ip(lg), ctypes.byref(p)))
+    return pixels1, bvar, blocked
+
+
+def correlator_tables(p, q, num, frametime):
+    """lag, samp, mtab, lagtime of setParameters (IFCS:7391-7424)."""
+    base, hbase = p, p // 2
+    chanum = p + (q - 1) * p // 2 + 1
+    lag = np.zeros(chanum, np.int32)
+    samp = np.zeros(chanum)
+    for x in range(hbase + 1):
+        lag[x] = x
+    for x in range(1, q + 1):
+        for y in range(1, hbase + 1):
+            lag[x * hbase + y] = int(2 ** (x - 1) * y + (base // 4) * 2 ** x)
+    samp[:base + 1] = 1
+    for x in range(2, q + 1):
+        for y in range(1, hbase + 1):
+            samp[x * hbase + y] = 2 ** (x - 1)
+    mtab = np.floor((num - lag) / samp)
+    return lag, samp, mtab, lag * frametime
+
+
+def obsvol(ax, ay, s):
+    """obsvolFCS_ST2D1p (IFCS:13670-13702) in SI units."""
+    from math import erf, exp, sqrt, pi
+    pi_ = 3.14159265359
+    sq = sqrt(pi_)
+    px = 2 * exp(-(ax / s) ** 2) - 2
+    ex = 2 * ax * erf(ax / s)
+    py = 2 * exp(-(ay / s) ** 2) - 2
+    ey = 2 * ay * erf(ay / s)
+    return 4 * (ax * ay) ** 2 / ((s / sq * px + ex) * (s / sq * py + ey))
+
+
+def make_fit_model(ax, ay, s, sz=None, rx=0.0, ry=0.0, q2=1.0, q3=1.0, paramfit=None, paraminit=None):
+    """FitModelGeom in SI units with the fit panel defaults (IFCS:1959-1968):
+    N=1, D=1 um^2/s, G=0; N, D, G free."""
+    m = FitModelGeom()
+    m.ax, m.ay, m.s = ax, ay, s
+    m.sz = 1e6 * 515e-9 / 1.49 if sz is None else sz
+    m.rx, m.ry, m.q2, m.q3 = rx, ry, q2, q3
+    m.fitobsvol = obsvol(ax, ay, s)
+    pf = paramfit if paramfit is not None else [1, 1, 0, 0, 1, 0, 0, 0, 0, 0, 0]
+    pi = paraminit if paraminit is not None else [1.0, 1e-12, 0, 0, 0, 0, 0, 0, 0, 0, 0]
+    for i in range(11):
+        m.paramfit[i] = int(pf[i])
+        m.paraminit[i] = float(pi[i])
+    return m
+
+
+def make_pipeline_config(W, H, T, binning=1, cf=(0, 0), background=0, poly_order=4, frametime=1e-3, p=16, q=8,
+                         roi=None, do_acf=True, do_ccf=None, do_fit=True, acf_model=None, ccf_model=None,
+                         overlap=False, firstframe=1, lastframe=None, fit_max_iterations=2000,
+                         fit_tolerance=1e-16):
+    c = PipelineConfig()
+    c.width, c.height, c.frames = W, H, T
+    c.firstframe = firstframe
+    c.lastframe = T if lastframe is None else lastframe
+    c.binning_x = c.binning_y = binning
+    c.pixbin_x = c.pixbin_y = 1 if overlap else binning
+    c.cf_x, c.cf_y = cf
+    c.background = background
+    c.correlator_p, c.correlator_q = p, q
+    c.frametime = frametime
+    c.poly_order = poly_order
+    rx, ry, rw, rh = roi if roi is not None else (0, 0, W - cf[0], H - cf[1])
+    c.roi_x, c.roi_y, c.roi_w, c.roi_h = rx, ry, rw, rh
+    c.do_acf = int(do_acf)
+    c.do_ccf = int((cf != (0, 0)) if do_ccf is None else do_ccf)
+    c.do_fit = int(do_fit)
+    c.fit_max_iterations = fit_max_iterations
+    c.fit_tolerance = fit_tolerance
+    if acf_model is not None:
+        c.acf_model = acf_model
+    if ccf_model is not None:
+        c.ccf_model = ccf_model
+    elif acf_model is not None:
+        c.ccf_model = acf_model
+    return c
+
+
+class Engine:
+    """A device context for the native pipeline (imfcs_create / imfcs_run)."""
+
+    def __init__(self, device=0):
+        self._lib = lib()
+        self.ctx = self._lib.imfcs_create(device)
+        if not self.ctx:
+            raise ImFCSError(get_last_error())
+        self._stack = None
+        self._stack_bytes = 0
+
+    def close(self):
+        if self.ctx:
+            if self._stack:
+                self._lib.imfcs_device_free(self.ctx, self._stack)
+                self._stack = None
+            self._lib.imfcs_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, stack):
+        """Copies a uint16 [T][H][W] stack to device memory owned by the
+        engine and returns its device pointer."""
+        stack = np.ascontiguousarray(stack, np.uint16)
+        if self._stack is None or self._stack_bytes < stack.nbytes:
+            if self._stack:
+                self._lib.imfcs_device_free(self.ctx, self._stack)
+            self._stack = self._lib.imfcs_device_alloc(self.ctx, ctypes.c_size_t(stack.nbytes))
+            if not self._stack:
+                raise ImFCSError(get_last_error())
+            self._stack_bytes = stack.nbytes
+        _check(self._lib.imfcs_memcpy_h2d(self.ctx, self._stack, stack.ctypes.data_as(ctypes.c_void_p),
+                                          ctypes.c_size_t(stack.nbytes)))
+        return self._stack
+
+    def run(self, cfg, d_stack=None):
+        _check(self._lib.imfcs_run(self.ctx, d_stack or self._stack, ctypes.byref(cfg)))
+
+    def synchronize(self):
+        _check(self._lib.imfcs_synchronize(self.ctx))
+
+    def stage_ms(self):
+        ms = (ctypes.c_float * 5)()
+        _check(self._lib.imfcs_get_stage_ms(self.ctx, ms))
+        return list(ms)
+
+    def results(self, cfg):
+        ow, oh, chanum = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(self._lib.imfcs_plan(ctypes.byref(cfg), ctypes.byref(ow), ctypes.byref(oh), ctypes.byref(chanum)))
+        ow, oh, nc = ow.value, oh.value, chanum.value
+        npairs = ow * oh * (int(cfg.do_acf) + int(cfg.do_ccf))
+        nseries = ow * oh * (2 if cfg.do_ccf else 1)
+        acf, var = np.zeros((npairs, nc)), np.zeros((npairs, nc))
+        ind, bis = np.zeros(npairs, np.int32), np.zeros(npairs, np.int32)
+        fitres, chi2 = np.full((npairs, 11), np.nan), np.full(npairs, np.nan)
+        states, iters = np.full(npairs, -1, np.int32), np.zeros(npairs, np.int32)
+        coef = np.zeros((nseries, max(cfg.poly_order, 0) + 1))
+        _check(self._lib.imfcs_get_results(self.ctx, _dp(acf), _dp(var), _ip(ind), _ip(bis), _dp(fitres),
+                                           _dp(chi2), _ip(states), _ip(iters), _dp(coef)))
+        npair_types = int(cfg.do_acf) + int(cfg.do_ccf)
+        sh = (npair_types, oh, ow)
+        return dict(acf=acf.reshape(sh + (nc,)), var=var.reshape(sh + (nc,)), ind=ind.reshape(sh),
+                    blockIndS=bis.reshape(sh), fitres=fitres.reshape(sh + (11,)), chi2=chi2.reshape(sh),
+                    states=states.reshape(sh), iters=iters.reshape(sh), coef=coef)
+
+
+def simulate_2d(W, H, T, n_emitters, D_um2s=1.0, pixel_um=0.24, psf_sigma_um=0.19, frametime=1e-3,
+                counts_per_frame=20.0, offset=100, tau_bleach_s=15.0, margin=2, seed=1, nthreads=0):
+    """Seeded 2D particle simulation -> uint16 [T][H][W] (see csrc/simulate.cpp)."""
+    out = np.empty((T, H, W), np.uint16)
+    d = np.asarray(D_um2s if np.ndim(D_um2s) else [D_um2s] * 4, np.float64)
+    if d.size != 4:
+        raise ValueError("D_um2s must be a scalar or four quadrant values")
+    rc = sim_lib().imfcs_simulate_2d(out.ctypes.data_as(ctypes.c_void_p), W, H, T, margin, n_emitters, pixel_um,
+                                     psf_sigma_um, frametime, counts_per_frame, offset, tau_bleach_s,
+                                     d.ctypes.data_as(ctypes.c_void_p), ctypes.c_uint64(seed), nthreads)
+    if rc != 0:
+        raise ValueError("imfcs_simulate_2d: bad arguments")
+    return out

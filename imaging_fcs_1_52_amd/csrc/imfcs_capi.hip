@@ -1,0 +1,782 @@
+// Host side of the C ABI declared in include/imfcs.h.
+//
+// The drop-in functions keep the argument meaning and error behaviour of the
+// reference's JNI natives (agpufitjni.cu:4595-5831) but run on persistent
+// device buffers instead of allocating, copying and resetting the device per
+// call.  The native pipeline (imfcs_run) works on a device-resident uint16
+// stack and never leaves the device between stages.
+#include "../../include/imfcs.h"
+#include "imfcs_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+using namespace imfcs;
+
+namespace {
+
+thread_local std::string g_last_error = "";
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(IMFCS_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// A growable device buffer.
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= n) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipSuccess) n = bytes;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    template <class T>
+    T *as() const { return (T *)p; }
+};
+
+// Correlator tables exactly as setParameters builds them (IFCS:7391-7424).
+struct HostTables {
+    int chanum = 0;
+    std::vector<int> lag;
+    std::vector<double> samp, mtab, lagtime;
+};
+
+int make_host_tables(int p, int q, int num, double frametime, HostTables &t) {
+    if (p < 4 || p % 4 != 0 || q < 1) return fail(IMFCS_ERR_ARG, "correlator p must be a multiple of 4, q >= 1");
+    int base = p, hbase = p / 2, lagnum = q;
+    t.chanum = p + (q - 1) * p / 2 + 1;
+    if (t.chanum > kMaxChan) return fail(IMFCS_ERR_ARG, "too many correlator channels");
+    t.lag.assign(t.chanum, 0);
+    t.samp.assign(t.chanum, 0);
+    t.mtab.assign(t.chanum, 0);
+    t.lagtime.assign(t.chanum, 0);
+    for (int x = 0; x <= hbase; x++) t.lag[x] = x;
+    for (int x = 1; x <= lagnum; x++)
+        for (int y = 1; y <= hbase; y++)
+            t.lag[x * hbase + y] = (int)(std::pow(2.0, x - 1) * y + (base / 4) * std::pow(2.0, x));
+    for (int x = 0; x <= base; x++) t.samp[x] = 1;
+    for (int x = 2; x <= lagnum; x++)
+        for (int y = 1; y <= hbase; y++) t.samp[x * hbase + y] = std::pow(2.0, x - 1);
+    for (int x = 0; x < t.chanum; x++) {
+        t.mtab[x] = (int)std::floor((num - t.lag[x]) / t.samp[x]);
+        t.lagtime[x] = t.lag[x] * frametime;
+    }
+    return 0;
+}
+
+// Derives the level layout, prefix positions and blocking constants.
+int make_corr_tables(const HostTables &h, int num, int q, CorrTables &ct) {
+    std::memset(&ct, 0, sizeof(ct));
+    ct.chanum = h.chanum;
+    ct.T = num;
+    ct.q = q;
+    int nlev = 0;
+    for (int x = 0; x < h.chanum; x++) {
+        int s = (int)h.samp[x];
+        int c = 0;
+        while ((1 << c) < s) c++;
+        if ((1 << c) != s) return fail(IMFCS_ERR_ARG, "sample times must be powers of two");
+        ct.level[x] = c;
+        ct.del[x] = h.lag[x] / s;
+        nlev = std::max(nlev, c + 1);
+    }
+    if (nlev > kMaxLevel) return fail(IMFCS_ERR_ARG, "too many correlator levels");
+    ct.nlevels = nlev;
+    for (int c = 0; c < nlev; c++) {
+        ct.first_chan[c] = -1;
+        ct.nchan[c] = 0;
+        ct.numbin[c] = num >> c;
+    }
+    for (int x = 0; x < h.chanum; x++) {
+        int c = ct.level[x];
+        if (ct.first_chan[c] < 0) ct.first_chan[c] = x;
+        ct.nchan[c]++;
+    }
+    // the kernels are specialised for p = 16: level 0 holds delays 0..16 and
+    // every other level delays 9..16
+    for (int c = 0; c < nlev; c++) {
+        int nexp = (c == 0) ? 17 : 8, dmin = (c == 0) ? 0 : 9;
+        if (ct.nchan[c] != nexp) return fail(IMFCS_ERR_ARG, "only correlator p = 16 is supported");
+        for (int i = 0; i < nexp; i++)
+            if (ct.del[ct.first_chan[c] + i] != dmin + i) return fail(IMFCS_ERR_ARG, "unexpected lag table");
+        if (ct.numbin[c] - 16 < 2) return fail(IMFCS_ERR_ARG, "not enough frames for the correlator");
+    }
+    ct.mtab_last = h.mtab[h.chanum - 1];
+    ct.log2_samp_last = std::log(h.samp[h.chanum - 1]) / std::log(2.0);
+    ct.blocknum = (int)std::floor(std::log(h.mtab[1]) / std::log(2.0)) - 2;
+    if (ct.blocknum < 4 || ct.blocknum > kMaxBlock) return fail(IMFCS_ERR_ARG, "frame count outside the blocking range");
+    // prefix positions
+    std::vector<int> pos;
+    for (int x = 0; x < h.chanum; x++) {
+        int c = ct.level[x];
+        pos.push_back((ct.numbin[c] - ct.del[x]) << c);
+        pos.push_back(ct.del[x] << c);
+        pos.push_back(ct.numbin[c] << c);
+    }
+    pos.push_back(num - 2);
+    pos.push_back(num - 1);
+    pos.push_back(num);
+    std::sort(pos.begin(), pos.end());
+    pos.erase(std::unique(pos.begin(), pos.end()), pos.end());
+    if ((int)pos.size() > kMaxPos) return fail(IMFCS_ERR_ARG, "too many prefix positions");
+    ct.npos = (int)pos.size();
+    for (int i = 0; i < ct.npos; i++) ct.pos[i] = pos[i];
+    auto idx = [&](int v) { return (int16_t)(std::lower_bound(pos.begin(), pos.end(), v) - pos.begin()); };
+    for (int x = 0; x < h.chanum; x++) {
+        int c = ct.level[x];
+        ct.pos_a_end[x] = idx((ct.numbin[c] - ct.del[x]) << c);
+        ct.pos_b_beg[x] = idx(ct.del[x] << c);
+        ct.pos_b_end[x] = idx(ct.numbin[c] << c);
+    }
+    ct.pos_blk_a = idx(num - 2);
+    ct.pos_blk_b = idx(num - 1);
+    ct.pos_total = idx(num);
+    return 0;
+}
+
+bool device_present() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return false;
+    return n > 0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------
+// Context
+// ------------------------------------------------------------------
+struct imfcs_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[6] = {};
+    DevBuf part, coef, ctot, rec, F, bpart, ind, bis, corav, bvar, fdata, fw, fx, finit, fpfit, fout, fstate, fchi2,
+        fiter;
+    // last run
+    int npix = 0, npairs = 0, nseries = 0, chanum = 0, order = 0, ow = 0, oh = 0, did_fit = 0, nfree = 0;
+    double chi2_norm = 1.0;
+    std::vector<double> coef_scale;   // u-basis -> seconds conversion per power
+};
+
+extern "C" {
+
+const char *imfcs_get_last_error(void) { return g_last_error.c_str(); }
+
+int imfcs_device_available(void) { return device_present() ? 1 : 0; }
+
+int imfcs_get_versions(int *runtime_version, int *driver_version) {
+    int rv = 0, dv = 0;
+    HIP_TRY(hipRuntimeGetVersion(&rv));
+    HIP_TRY(hipDriverGetVersion(&dv));
+    // HIP encodes major*10000000 + minor*100000 + patch
+    auto enc = [](int v) { return (v / 10000000) * 1000 + ((v / 100000) % 100) * 10; };
+    if (runtime_version) *runtime_version = enc(rv);
+    if (driver_version) *driver_version = enc(dv);
+    return 0;
+}
+
+int imfcs_plan(const imfcs_config *c, int32_t *out_w, int32_t *out_h, int32_t *chanum) {
+    if (!c) return fail(IMFCS_ERR_ARG, "null config");
+    if (c->pixbin_x <= 0 || c->pixbin_y <= 0 || c->binning_x <= 0 || c->binning_y <= 0)
+        return fail(IMFCS_ERR_ARG, "binning and pixbin must be positive");
+    // correlateROI ROI mapping (IFCS:8458-8466)
+    int sx = (int)std::ceil((double)c->roi_x / c->pixbin_x);
+    int sy = (int)std::ceil((double)c->roi_y / c->pixbin_y);
+    int ex = (int)std::floor((double)(c->roi_x + c->roi_w - c->binning_x) / c->pixbin_x);
+    int ey = (int)std::floor((double)(c->roi_y + c->roi_h - c->binning_y) / c->pixbin_y);
+    if (out_w) *out_w = ex - sx + 1;
+    if (out_h) *out_h = ey - sy + 1;
+    if (chanum) *chanum = c->correlator_p + (c->correlator_q - 1) * c->correlator_p / 2 + 1;
+    return 0;
+}
+
+imfcs_ctx *imfcs_create(int device) {
+    if (!device_present()) {
+        fail(IMFCS_ERR_DEVICE, "no HIP device available");
+        return nullptr;
+    }
+    if (hipSetDevice(device) != hipSuccess) {
+        fail(IMFCS_ERR_DEVICE, "hipSetDevice failed");
+        return nullptr;
+    }
+    imfcs_ctx *ctx = new imfcs_ctx();
+    ctx->device = device;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        fail(IMFCS_ERR_DEVICE, "hipStreamCreate failed");
+        return nullptr;
+    }
+    for (auto &e : ctx->ev) (void)hipEventCreate(&e);
+    return ctx;
+}
+
+static void release_all(imfcs_ctx *ctx) {
+    DevBuf *bufs[] = {&ctx->part, &ctx->coef, &ctx->ctot, &ctx->rec, &ctx->F, &ctx->bpart, &ctx->ind,
+                      &ctx->bis, &ctx->corav, &ctx->bvar, &ctx->fdata, &ctx->fw, &ctx->fx, &ctx->finit,
+                      &ctx->fpfit, &ctx->fout, &ctx->fstate, &ctx->fchi2, &ctx->fiter};
+    for (DevBuf *b : bufs) b->release();
+}
+
+void imfcs_destroy(imfcs_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    release_all(ctx);
+    for (auto &e : ctx->ev) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+void *imfcs_device_alloc(imfcs_ctx *ctx, size_t bytes) {
+    void *p = nullptr;
+    if (!ctx) return nullptr;
+    (void)hipSetDevice(ctx->device);
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        fail(IMFCS_ERR_MEMORY, "hipMalloc failed");
+        return nullptr;
+    }
+    return p;
+}
+
+int imfcs_device_free(imfcs_ctx *ctx, void *ptr) {
+    if (!ctx) return fail(IMFCS_ERR_ARG, "null context");
+    HIP_TRY(hipFree(ptr));
+    return 0;
+}
+
+int imfcs_memcpy_h2d(imfcs_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx) return fail(IMFCS_ERR_ARG, "null context");
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int imfcs_memcpy_d2h(imfcs_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx) return fail(IMFCS_ERR_ARG, "null context");
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int imfcs_synchronize(imfcs_ctx *ctx) {
+    if (!ctx) return fail(IMFCS_ERR_ARG, "null context");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------
+// Shared correlation engine (native pipeline and drop-in calcACF)
+// ------------------------------------------------------------------
+namespace {
+
+struct EngineArgs {
+    SeriesGeom g;
+    int order;             // polynomial order (0 = none)
+    int fit_trace;         // 1: trace + fit on device; 0: coefficients given in host_coef
+    const double *host_coef;   // [nseries][order+1] in seconds (drop-in)
+    int nopit, ave;
+    double frametime;
+    int p, q;
+    PairDesc pd;
+};
+
+int run_engine(imfcs_ctx *ctx, const EngineArgs &ea, HostTables &ht, CorrTables &ct) {
+    hipStream_t st = ctx->stream;
+    const SeriesGeom &g = ea.g;
+    int rc = make_host_tables(ea.p, ea.q, g.T, ea.frametime, ht);
+    if (rc) return rc;
+    rc = make_corr_tables(ht, g.T, ea.q, ct);
+    if (rc) return rc;
+    const int ns = g.nseries, np = ea.pd.npairs, n = ea.order + 1;
+    ctx->npairs = np;
+    ctx->nseries = ns;
+    ctx->chanum = ct.chanum;
+    ctx->order = ea.order;
+    BleachDesc bd{};
+    bd.order = ea.order;
+    bd.frametime = ea.frametime;
+    HIP_TRY(hipEventRecord(ctx->ev[0], st));
+    if (ea.order > 0) {
+        if (ea.order > kMaxOrder) return fail(IMFCS_ERR_ARG, "polynomial order above 8");
+        HIP_TRY(ctx->coef.ensure(sizeof(double) * (size_t)ns * n));
+        if (ea.fit_trace) {
+            // abscissa normalized by the last trace time
+            double tscale = ea.frametime * (ea.nopit - 0.5) * ea.ave;
+            int nchunk = std::min(40, ea.nopit);
+            HIP_TRY(ctx->part.ensure(sizeof(double) * (size_t)nchunk * ns * n));
+            launch_trace_moments(g, ea.nopit, ea.ave, tscale, ea.frametime, ea.order, 1, ctx->part.as<double>(),
+                                 nchunk, st);
+            launch_polyfit(ctx->part.as<double>(), nchunk, ns, ea.nopit, ea.ave, tscale, ea.frametime, ea.order,
+                           ctx->coef.as<double>(), st);
+            bd.tscale = tscale;
+            ctx->coef_scale.assign(n, 1.0);
+            for (int k = 1; k < n; k++) ctx->coef_scale[k] = ctx->coef_scale[k - 1] / tscale;
+        } else {
+            HIP_TRY(hipMemcpyAsync(ctx->coef.p, ea.host_coef, sizeof(double) * (size_t)ns * n,
+                                   hipMemcpyHostToDevice, st));
+            bd.tscale = 1.0;
+            ctx->coef_scale.assign(n, 1.0);
+        }
+        bd.coef = ctx->coef.as<double>();
+        bd.t_off = 1.5;   // t = frametime * (x + 0.5), x 1-based (IFCS:9747)
+    } else {
+        bd.tscale = 1.0;
+        bd.coef = nullptr;
+    }
+    HIP_TRY(hipEventRecord(ctx->ev[1], st));
+    // window means
+    const int plen = 4096, npc = (g.T + plen - 1) / plen;
+    HIP_TRY(ctx->ctot.ensure(sizeof(double) * (size_t)npc * ns));
+    HIP_TRY(ctx->rec.ensure(sizeof(double) * (size_t)ns * ct.npos));
+    HIP_TRY(ctx->F.ensure(sizeof(double) * (size_t)ns * ct.npos));
+    launch_prefix(g, bd, ct, ctx->ctot.as<double>(), ctx->rec.as<double>(), npc, plen, st);
+    launch_prefix_combine(ct, ns, ctx->ctot.as<double>(), ctx->rec.as<double>(), npc, plen, ctx->F.as<double>(), st);
+    HIP_TRY(hipEventRecord(ctx->ev[2], st));
+    // blocking
+    const int blen = 1 << (ct.blocknum - 1), nbc = (g.T - 1 + blen - 1) / blen;
+    HIP_TRY(ctx->bpart.ensure(sizeof(double) * (size_t)nbc * np * ct.blocknum * 2));
+    HIP_TRY(ctx->ind.ensure(sizeof(int32_t) * (size_t)np));
+    HIP_TRY(ctx->bis.ensure(sizeof(int32_t) * (size_t)np));
+    launch_blocking(g, bd, ct, ea.pd, ctx->F.as<double>(), ctx->bpart.as<double>(), nbc, blen, st);
+    launch_blocking_finalize(ct, ea.pd, ctx->F.as<double>(), ctx->bpart.as<double>(), nbc, ctx->ind.as<int32_t>(),
+                             ctx->bis.as<int32_t>(), st);
+    HIP_TRY(hipEventRecord(ctx->ev[3], st));
+    // correlation
+    HIP_TRY(ctx->corav.ensure(sizeof(double) * (size_t)np * ct.chanum));
+    HIP_TRY(ctx->bvar.ensure(sizeof(double) * (size_t)np * ct.chanum));
+    launch_correlate(g, bd, ct, ea.pd, ctx->F.as<double>(), ctx->ind.as<int32_t>(), ctx->corav.as<double>(),
+                     ctx->bvar.as<double>(), st);
+    HIP_TRY(hipEventRecord(ctx->ev[4], st));
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int check_geom(const imfcs_config *c, int ow, int oh) {
+    if (c->width <= 0 || c->height <= 0 || c->frames <= 0) return fail(IMFCS_ERR_ARG, "empty stack");
+    if (c->firstframe < 1 || c->lastframe > c->frames || c->lastframe - c->firstframe + 1 < 2)
+        return fail(IMFCS_ERR_ARG, "frame range outside the stack");
+    if (ow <= 0 || oh <= 0) return fail(IMFCS_ERR_ARG, "ROI holds no output pixel");
+    if (c->cf_x < 0 || c->cf_y < 0)
+        return fail(IMFCS_ERR_ARG, "negative CF distances are not supported (as in IFCS:15296-15299)");
+    int sx = (int)std::ceil((double)c->roi_x / c->pixbin_x), sy = (int)std::ceil((double)c->roi_y / c->pixbin_y);
+    if (sx < 0 || sy < 0) return fail(IMFCS_ERR_ARG, "ROI starts outside the stack");
+    int cfx = c->do_ccf ? c->cf_x : 0, cfy = c->do_ccf ? c->cf_y : 0;
+    if ((sx + ow - 1) * c->pixbin_x + c->binning_x + cfx > c->width ||
+        (sy + oh - 1) * c->pixbin_y + c->binning_y + cfy > c->height)
+        return fail(IMFCS_ERR_ARG, "ROI (with CCF partner) exceeds the stack");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int imfcs_run(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_config *c) {
+    if (!ctx || !d_stack || !c) return fail(IMFCS_ERR_ARG, "null argument");
+    if (!c->do_acf && !c->do_ccf) return fail(IMFCS_ERR_ARG, "neither ACF nor CCF requested");
+    HIP_TRY(hipSetDevice(ctx->device));
+    int ow, oh, chanum;
+    int rc = imfcs_plan(c, &ow, &oh, &chanum);
+    if (rc) return rc;
+    rc = check_geom(c, ow, oh);
+    if (rc) return rc;
+    EngineArgs ea{};
+    SeriesGeom &g = ea.g;
+    g.data = d_stack;
+    g.src = kSrcU16;
+    g.plane = (int64_t)c->width * c->height;
+    g.stride = c->width;
+    g.f0 = c->firstframe - 1;
+    g.T = c->lastframe - c->firstframe + 1;
+    g.bx = c->binning_x;
+    g.by = c->binning_y;
+    g.bckg_total = c->background * c->binning_x * c->binning_y;
+    g.ow = ow;
+    g.oh = oh;
+    g.npix = ow * oh;
+    g.ox0 = (int)std::ceil((double)c->roi_x / c->pixbin_x) * c->pixbin_x;
+    g.oy0 = (int)std::ceil((double)c->roi_y / c->pixbin_y) * c->pixbin_y;
+    g.pbx = c->pixbin_x;
+    g.pby = c->pixbin_y;
+    g.cfx = c->cf_x;
+    g.cfy = c->cf_y;
+    g.nseries = g.npix * (c->do_ccf ? 2 : 1);
+    ea.order = c->poly_order;
+    ea.fit_trace = 1;
+    ea.nopit = std::min(g.T, 1000);   // IFCS:7323-7327
+    ea.ave = g.T / ea.nopit;
+    ea.frametime = c->frametime;
+    ea.p = c->correlator_p;
+    ea.q = c->correlator_q;
+    ea.pd.npix = g.npix;
+    ea.pd.acf_first = c->do_acf ? 1 : 0;
+    ea.pd.has_ccf = c->do_ccf ? 1 : 0;
+    ea.pd.npairs = g.npix * ((c->do_acf ? 1 : 0) + (c->do_ccf ? 1 : 0));
+    ctx->npix = g.npix;
+    ctx->ow = ow;
+    ctx->oh = oh;
+    HostTables ht;
+    CorrTables ct;
+    rc = run_engine(ctx, ea, ht, ct);
+    if (rc) return rc;
+    hipStream_t st = ctx->stream;
+    ctx->did_fit = c->do_fit;
+    if (c->do_fit) {
+        const int np = ea.pd.npairs, npts = ct.chanum - 1;
+        HIP_TRY(ctx->fdata.ensure(sizeof(float) * (size_t)np * npts));
+        HIP_TRY(ctx->fw.ensure(sizeof(float) * (size_t)np * npts));
+        HIP_TRY(ctx->fx.ensure(sizeof(float) * npts));
+        HIP_TRY(ctx->finit.ensure(sizeof(float) * (size_t)np * 20));
+        HIP_TRY(ctx->fpfit.ensure(sizeof(int) * 20));
+        HIP_TRY(ctx->fout.ensure(sizeof(float) * (size_t)np * 20));
+        HIP_TRY(ctx->fstate.ensure(sizeof(int) * np));
+        HIP_TRY(ctx->fchi2.ensure(sizeof(float) * np));
+        HIP_TRY(ctx->fiter.ensure(sizeof(int) * np));
+        launch_pack_fit_inputs(ct, np, ctx->corav.as<double>(), ctx->bvar.as<double>(), ctx->fdata.as<float>(),
+                               ctx->fw.as<float>(), st);
+        // lag times of channels 1..chanum-1 (IFCS:15122-15125) and initial
+        // parameters with the fixed geometry in slots 11..19 (IFCS:15090-15110)
+        std::vector<float> xs(npts), init((size_t)np * 20);
+        for (int i = 0; i < npts; i++) xs[i] = (float)ht.lagtime[i + 1];
+        std::vector<int> pfit(20, 0);
+        const imfcs_fit_model *models[2] = {&c->acf_model, &c->ccf_model};
+        for (int k = 0; k < 11; k++) pfit[k] = c->acf_model.paramfit[k] ? 1 : 0;
+        int nfree = 0;
+        for (int k = 0; k < 11; k++) nfree += pfit[k];
+        for (int p = 0; p < np; p++) {
+            const imfcs_fit_model *m = models[(c->do_acf && p < g.npix) ? 0 : 1];
+            float *q = &init[(size_t)p * 20];
+            for (int k = 0; k < 11; k++) q[k] = (float)m->paraminit[k];
+            q[11] = (float)m->ax;
+            q[12] = (float)m->ay;
+            q[13] = (float)m->s;
+            q[14] = (float)m->sz;
+            q[15] = (float)m->rx;
+            q[16] = (float)m->ry;
+            q[17] = (float)m->fitobsvol;
+            q[18] = (float)m->q2;
+            q[19] = (float)m->q3;
+        }
+        HIP_TRY(hipMemcpyAsync(ctx->fx.p, xs.data(), sizeof(float) * npts, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(ctx->finit.p, init.data(), sizeof(float) * init.size(), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(ctx->fpfit.p, pfit.data(), sizeof(int) * 20, hipMemcpyHostToDevice, st));
+        launch_lm_acf(np, npts, ctx->fdata.as<float>(), ctx->fw.as<float>(), ctx->fx.as<float>(),
+                      ctx->finit.as<float>(), ctx->fpfit.as<int>(), LMModelGeom{}, c->fit_tolerance,
+                      c->fit_max_iterations, ctx->fout.as<float>(), ctx->fstate.as<int>(), ctx->fchi2.as<float>(),
+                      ctx->fiter.as<int>(), st);
+        ctx->nfree = nfree;
+        // chi2 normalization of GPU_ACF_Fit (IFCS:15234)
+        ctx->chi2_norm = (double)((npts - 1) - nfree - 1);
+        // the host vectors above must outlive the async copies
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    HIP_TRY(hipEventRecord(ctx->ev[5], st));
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int imfcs_get_stage_ms(imfcs_ctx *ctx, float *ms) {
+    if (!ctx || !ms) return fail(IMFCS_ERR_ARG, "null argument");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < 5; i++) {
+        float t = 0;
+        HIP_TRY(hipEventElapsedTime(&t, ctx->ev[i], ctx->ev[i + 1]));
+        ms[i] = t;
+    }
+    return 0;
+}
+
+extern "C" int imfcs_get_results(imfcs_ctx *ctx, double *acf, double *var, int32_t *ind, int32_t *blocked,
+                                 double *fitres, double *chi2, int32_t *states, int32_t *iters, double *coeffs) {
+    if (!ctx) return fail(IMFCS_ERR_ARG, "null context");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const size_t np = ctx->npairs, nc = ctx->chanum;
+    if (acf) HIP_TRY(hipMemcpy(acf, ctx->corav.p, sizeof(double) * np * nc, hipMemcpyDeviceToHost));
+    if (var) HIP_TRY(hipMemcpy(var, ctx->bvar.p, sizeof(double) * np * nc, hipMemcpyDeviceToHost));
+    if (ind) HIP_TRY(hipMemcpy(ind, ctx->ind.p, sizeof(int32_t) * np, hipMemcpyDeviceToHost));
+    if (blocked) HIP_TRY(hipMemcpy(blocked, ctx->bis.p, sizeof(int32_t) * np, hipMemcpyDeviceToHost));
+    if (coeffs && ctx->order > 0) {
+        size_t n = ctx->order + 1;
+        std::vector<double> cu(ctx->nseries * n);
+        HIP_TRY(hipMemcpy(cu.data(), ctx->coef.p, sizeof(double) * cu.size(), hipMemcpyDeviceToHost));
+        for (size_t s = 0; s < (size_t)ctx->nseries; s++)
+            for (size_t k = 0; k < n; k++) coeffs[s * n + k] = cu[s * n + k] * ctx->coef_scale[k];
+    }
+    if (ctx->did_fit) {
+        std::vector<float> out(np * 20), c2(np);
+        HIP_TRY(hipMemcpy(out.data(), ctx->fout.p, sizeof(float) * out.size(), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(c2.data(), ctx->fchi2.p, sizeof(float) * np, hipMemcpyDeviceToHost));
+        if (fitres)
+            for (size_t p = 0; p < np; p++)
+                for (int k = 0; k < 11; k++) fitres[p * 11 + k] = out[p * 20 + k];
+        if (chi2)
+            for (size_t p = 0; p < np; p++) chi2[p] = c2[p] / ctx->chi2_norm;
+        if (states) HIP_TRY(hipMemcpy(states, ctx->fstate.p, sizeof(int32_t) * np, hipMemcpyDeviceToHost));
+        if (iters) HIP_TRY(hipMemcpy(iters, ctx->fiter.p, sizeof(int32_t) * np, hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------------
+// Drop-in layer
+// ------------------------------------------------------------------
+namespace {
+
+std::mutex g_dropin_mu;
+imfcs_ctx *g_dropin_ctx = nullptr;
+
+imfcs_ctx *dropin_ctx() {
+    if (!g_dropin_ctx) g_dropin_ctx = imfcs_create(0);
+    return g_dropin_ctx;
+}
+
+// free device memory fraction test used by the is*MemorySufficient calls
+int mem_sufficient(double bytes) {
+    size_t fr = 0, tot = 0;
+    if (!device_present()) return fail(IMFCS_ERR_DEVICE, "no HIP device available");
+    HIP_TRY(hipMemGetInfo(&fr, &tot));
+    return (bytes > (double)fr * 0.9) ? 0 : 1;
+}
+
+}  // namespace
+
+extern "C" int imfcs_reset_gpu(void) {
+    std::lock_guard<std::mutex> lk(g_dropin_mu);
+    if (g_dropin_ctx) {
+        imfcs_destroy(g_dropin_ctx);
+        g_dropin_ctx = nullptr;
+    }
+    return 0;
+}
+
+extern "C" int imfcs_reset_device(void) { return 0; }
+
+extern "C" int imfcs_is_binning_memory_sufficient(const imfcs_acf_params *p) {
+    if (!p) return fail(IMFCS_ERR_ARG, "null parameters");
+    int T = p->lastframe - p->firstframe + 1;
+    double bytes = ((double)p->win_star * p->hin_star + (double)p->w_temp * p->h_temp) * T * 4.0;
+    return mem_sufficient(bytes);
+}
+
+extern "C" int imfcs_is_acf_memory_sufficient(const imfcs_acf_params *p) {
+    if (!p) return fail(IMFCS_ERR_ARG, "null parameters");
+    // float input, per-pair outputs and the per-chunk partial sums of the
+    // streaming kernels (no T x pixels product buffer as in .cu:5031-5084)
+    double T = p->lastframe - p->firstframe + 1;
+    double pix = (double)p->width * p->height;
+    double bytes = (double)p->w_temp * p->h_temp * T * 4.0 + pix * p->chanum * 8.0 * 6 + pix * 2 * 400 * 8.0 +
+                   pix * 2 * 64.0 * (T / 4096 + 20);
+    return mem_sufficient(bytes);
+}
+
+extern "C" int imfcs_calc_binning(const float *indata, float *outdata, const imfcs_acf_params *p) {
+    if (!indata || !outdata || !p) return fail(IMFCS_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(g_dropin_mu);
+    imfcs_ctx *ctx = dropin_ctx();
+    if (!ctx) return IMFCS_ERR_DEVICE;
+    int T = p->lastframe - p->firstframe + 1;
+    size_t nin = (size_t)p->win_star * p->hin_star * T, nout = (size_t)p->w_temp * p->h_temp * T;
+    HIP_TRY(ctx->part.ensure(nin * 4));
+    HIP_TRY(ctx->fdata.ensure(nout * 4));
+    HIP_TRY(hipMemcpyAsync(ctx->part.p, indata, nin * 4, hipMemcpyHostToDevice, ctx->stream));
+    launch_binning_f32(ctx->part.as<float>(), ctx->fdata.as<float>(), p->win_star, p->hin_star, p->w_temp,
+                       p->h_temp, T, p->binningX, p->binningY, ctx->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(outdata, ctx->fdata.p, nout * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+extern "C" int imfcs_calc_data_bleach_correction(const float *pixels, float *outdata, const imfcs_acf_params *p) {
+    if (!pixels || !outdata || !p) return fail(IMFCS_ERR_ARG, "null argument");
+    if (p->nopit <= 0 || p->ave <= 0) return fail(IMFCS_ERR_ARG, "nopit and ave must be positive");
+    std::lock_guard<std::mutex> lk(g_dropin_mu);
+    imfcs_ctx *ctx = dropin_ctx();
+    if (!ctx) return IMFCS_ERR_DEVICE;
+    int T = p->lastframe - p->firstframe + 1;
+    size_t nin = (size_t)p->w_temp * p->h_temp * T, nout = (size_t)p->w_temp * p->h_temp * p->nopit;
+    HIP_TRY(ctx->part.ensure(nin * 4));
+    HIP_TRY(ctx->fdata.ensure(nout * 4));
+    HIP_TRY(hipMemcpyAsync(ctx->part.p, pixels, nin * 4, hipMemcpyHostToDevice, ctx->stream));
+    launch_bleach_average_f32(ctx->part.as<float>(), ctx->fdata.as<float>(), p->w_temp, p->h_temp, p->nopit, p->ave,
+                              ctx->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(outdata, ctx->fdata.p, nout * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+extern "C" int imfcs_calc_acf(const float *pixels, double *pixels1, double *blockvararray, double *NBmean,
+                              double *NBcov, double *blocked1D, const double *bleachcorr_params, const double *samp,
+                              const int *lag, const imfcs_acf_params *p) {
+    (void)samp;
+    (void)lag;
+    if (!pixels || !pixels1 || !blockvararray || !blocked1D || !p) return fail(IMFCS_ERR_ARG, "null argument");
+    if (p->isNBcalculation) return fail(IMFCS_ERR_ARG, "N&B calculation is not implemented");
+    std::lock_guard<std::mutex> lk(g_dropin_mu);
+    imfcs_ctx *ctx = dropin_ctx();
+    if (!ctx) return IMFCS_ERR_DEVICE;
+    int T = p->lastframe - p->firstframe + 1;
+    size_t nin = (size_t)p->w_temp * p->h_temp * T;
+    // input lives in the generic scratch buffer `rec` is needed later, so use fdata
+    DevBuf input;
+    HIP_TRY(input.ensure(nin * 4));
+    HIP_TRY(hipMemcpyAsync(input.p, pixels, nin * 4, hipMemcpyHostToDevice, ctx->stream));
+    EngineArgs ea{};
+    SeriesGeom &g = ea.g;
+    g.data = input.p;
+    g.src = kSrcF32;
+    g.plane = (int64_t)p->w_temp * p->h_temp;
+    g.stride = p->w_temp;
+    g.f0 = 0;
+    g.T = T;
+    g.bx = g.by = 1;
+    g.bckg_total = 0;
+    g.ow = p->width;
+    g.oh = p->height;
+    g.npix = p->width * p->height;
+    g.ox0 = g.oy0 = 0;
+    g.pbx = p->pixbinX;
+    g.pby = p->pixbinY;
+    g.cfx = p->cfXDistance;
+    g.cfy = p->cfYDistance;
+    bool ccf = (p->cfXDistance != 0 || p->cfYDistance != 0);
+    g.nseries = g.npix * (ccf ? 2 : 1);
+    if ((g.ow - 1) * g.pbx + g.cfx >= p->w_temp || (g.oh - 1) * g.pby + g.cfy >= p->h_temp) {
+        input.release();
+        return fail(IMFCS_ERR_ARG, "output grid exceeds the binned input");
+    }
+    int order = p->bleachcorr_gpu ? p->bleachcorr_order - 1 : 0;
+    std::vector<double> hc;
+    if (order > 0) {
+        int n = order + 1;
+        hc.resize((size_t)g.nseries * n);
+        for (int s = 0; s < g.nseries; s++) {
+            int k = s < g.npix ? s : s - g.npix;
+            int ox = (k % g.ow) * g.pbx + (s >= g.npix ? g.cfx : 0);
+            int oy = (k / g.ow) * g.pby + (s >= g.npix ? g.cfy : 0);
+            const double *src = bleachcorr_params + ((size_t)oy * p->w_temp + ox) * p->bleachcorr_order;
+            for (int i = 0; i < n; i++) hc[(size_t)s * n + i] = src[i];
+        }
+    }
+    ea.order = order;
+    ea.fit_trace = 0;
+    ea.host_coef = hc.data();
+    ea.frametime = p->frametime;
+    ea.p = (int)p->correlatorp;
+    ea.q = (int)p->correlatorq;
+    ea.pd.npix = g.npix;
+    ea.pd.acf_first = ccf ? 0 : 1;
+    ea.pd.has_ccf = ccf ? 1 : 0;
+    ea.pd.npairs = g.npix;
+    ctx->npix = g.npix;
+    HostTables ht;
+    CorrTables ct;
+    int rc = run_engine(ctx, ea, ht, ct);
+    if (rc == 0) {
+        size_t np = g.npix, nc = ct.chanum;
+        std::vector<double> cav(np * nc), bv(np * nc);
+        std::vector<int32_t> iv(np), bi(np);
+        hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess) e = hipMemcpy(cav.data(), ctx->corav.p, sizeof(double) * np * nc, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(bv.data(), ctx->bvar.p, sizeof(double) * np * nc, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(iv.data(), ctx->ind.p, sizeof(int32_t) * np, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(bi.data(), ctx->bis.p, sizeof(int32_t) * np, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            rc = fail(IMFCS_ERR_DEVICE, std::string("copy back: ") + hipGetErrorString(e));
+        } else {
+            // plane-major [z][y][x] outputs (agpufitjni.cu:5748-5774)
+            for (size_t k = 0; k < np; k++)
+                for (size_t z = 0; z < nc; z++) {
+                    pixels1[z * np + k] = cav[k * nc + z];
+                    blockvararray[z * np + k] = bv[k * nc + z];
+                }
+            for (size_t i = 0; i < np * nc; i++) blocked1D[i] = 0.0;
+            for (size_t k = 0; k < np; k++) {
+                blocked1D[k] = iv[k];
+                blocked1D[np + k] = bi[k];
+            }
+        }
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    input.release();
+    (void)NBmean;
+    (void)NBcov;
+    return rc;
+}
+
+extern "C" int imfcs_fit(int n_fits, int n_points, const float *data, const float *weights, int model_id,
+                         const float *initial_parameters, float tolerance, int max_n_iterations, int num_valid_coefs,
+                         const int *parameters_to_fit, int estimator_id, size_t user_info_size, const void *user_info,
+                         float *output_parameters, int *output_states, float *output_chi_squares,
+                         int *output_n_iterations) {
+    if (n_fits <= 0 || n_points <= 0) return fail(IMFCS_ERROR, "number of fits and points must be positive");
+    if (!data || !initial_parameters || !parameters_to_fit || !output_parameters || !output_states ||
+        !output_chi_squares || !output_n_iterations)
+        return fail(IMFCS_ERROR, "null buffer");
+    if (estimator_id != IMFCS_ESTIMATOR_LSE) return fail(IMFCS_ERROR, "only the LSE estimator is implemented");
+    int npar;
+    if (model_id == IMFCS_MODEL_ACF_1D) npar = 20;
+    else if (model_id == IMFCS_MODEL_LINEAR_1D) npar = 11;
+    else return fail(IMFCS_ERROR, "unsupported model id");
+    if (!user_info || user_info_size < sizeof(float) * (size_t)n_points)
+        return fail(IMFCS_ERROR, "user_info must hold one float abscissa per point");
+    if (model_id == IMFCS_MODEL_LINEAR_1D && (num_valid_coefs < 1 || num_valid_coefs > 11))
+        return fail(IMFCS_ERROR, "num_valid_coefs out of range");
+    std::lock_guard<std::mutex> lk(g_dropin_mu);
+    imfcs_ctx *ctx = dropin_ctx();
+    if (!ctx) return IMFCS_ERROR;
+    hipStream_t st = ctx->stream;
+    size_t nd = (size_t)n_fits * n_points;
+    HIP_TRY(ctx->fdata.ensure(nd * 4));
+    HIP_TRY(ctx->fw.ensure(nd * 4));
+    HIP_TRY(ctx->fx.ensure((size_t)n_points * 4));
+    HIP_TRY(ctx->finit.ensure((size_t)n_fits * npar * 4));
+    HIP_TRY(ctx->fpfit.ensure(npar * sizeof(int)));
+    HIP_TRY(ctx->fout.ensure((size_t)n_fits * npar * 4));
+    HIP_TRY(ctx->fstate.ensure((size_t)n_fits * 4));
+    HIP_TRY(ctx->fchi2.ensure((size_t)n_fits * 4));
+    HIP_TRY(ctx->fiter.ensure((size_t)n_fits * 4));
+    HIP_TRY(hipMemcpyAsync(ctx->fdata.p, data, nd * 4, hipMemcpyHostToDevice, st));
+    if (weights) HIP_TRY(hipMemcpyAsync(ctx->fw.p, weights, nd * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(ctx->fx.p, user_info, (size_t)n_points * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(ctx->finit.p, initial_parameters, (size_t)n_fits * npar * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(ctx->fpfit.p, parameters_to_fit, npar * sizeof(int), hipMemcpyHostToDevice, st));
+    const float *w = weights ? ctx->fw.as<float>() : nullptr;
+    if (model_id == IMFCS_MODEL_ACF_1D)
+        launch_lm_acf(n_fits, n_points, ctx->fdata.as<float>(), w, ctx->fx.as<float>(), ctx->finit.as<float>(),
+                      ctx->fpfit.as<int>(), LMModelGeom{}, tolerance, max_n_iterations, ctx->fout.as<float>(),
+                      ctx->fstate.as<int>(), ctx->fchi2.as<float>(), ctx->fiter.as<int>(), st);
+    else
+        launch_lm_poly(n_fits, n_points, ctx->fdata.as<float>(), w, ctx->fx.as<float>(), ctx->finit.as<float>(),
+                       ctx->fpfit.as<int>(), num_valid_coefs, tolerance, max_n_iterations, ctx->fout.as<float>(),
+                       ctx->fstate.as<int>(), ctx->fchi2.as<float>(), ctx->fiter.as<int>(), st);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(output_parameters, ctx->fout.p, (size_t)n_fits * npar * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(output_states, ctx->fstate.p, (size_t)n_fits * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(output_chi_squares, ctx->fchi2.p, (size_t)n_fits * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(output_n_iterations, ctx->fiter.p, (size_t)n_fits * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}

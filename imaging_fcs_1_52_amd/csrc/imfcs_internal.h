@@ -1,0 +1,114 @@
+// Internal types shared by the HIP kernels and the host-side C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace imfcs {
+
+constexpr int kMaxChan = 128;   // correlator channels (73 for p=16, q=8)
+constexpr int kMaxLevel = 12;   // multi-tau levels (q)
+constexpr int kMaxOrder = 8;    // polynomial bleach order (BCmaxorder, IFCS:201)
+constexpr int kMaxBlock = 20;   // blocking-curve levels
+constexpr int kMaxPos = 2 * kMaxChan + kMaxLevel + 8;  // prefix positions
+
+enum SourceType : int32_t { kSrcU16 = 0, kSrcF32 = 1 };
+
+// Where the intensity series live and how one series value is formed.
+// Series s < npix is output pixel s (row-major over ow x oh); series
+// s >= npix is the CCF partner of pixel s - npix.
+struct SeriesGeom {
+    const void *data;
+    int32_t src;          // SourceType
+    int64_t plane;        // elements per frame
+    int32_t stride;       // elements per row
+    int32_t f0;           // first frame (0-based index into data)
+    int32_t T;            // frames in each series (num)
+    int32_t bx, by;       // window summed per series value (1 for F32)
+    int32_t bckg_total;   // background * bx * by, subtracted per value
+    int32_t ow, oh, npix;
+    int32_t ox0, oy0;     // raw origin of map pixel (0,0)
+    int32_t pbx, pby;     // stride between map pixels
+    int32_t cfx, cfy;     // partner offset
+    int32_t nseries;
+};
+
+// Polynomial bleach correction (per series) in a normalized abscissa
+// u = (frame + t_off) * frametime / tscale.  coef[s][k] are the
+// coefficients of f(u); order 0 means "no correction".
+struct BleachDesc {
+    const double *coef;   // [nseries][order+1]
+    int32_t order;
+    double frametime, tscale, t_off;
+};
+
+// Multi-tau correlator layout derived from the lag/samp tables.
+struct CorrTables {
+    int32_t chanum, nlevels, T;
+    int32_t level[kMaxChan];      // log2(samp)
+    int32_t del[kMaxChan];        // lag / samp
+    int32_t first_chan[kMaxLevel], nchan[kMaxLevel];
+    int32_t numbin[kMaxLevel];    // floor(T / 2^level)
+    double mtab_last, log2_samp_last;
+    int32_t q;
+    int32_t blocknum;             // blocking levels of the lag-1 curve
+    // prefix-sum positions used to form every channel mean
+    int32_t npos;
+    int32_t pos[kMaxPos];
+    int16_t pos_a_end[kMaxChan];  // index into pos of 2^c (numbin_c - del)
+    int16_t pos_b_beg[kMaxChan];  // index of lag
+    int16_t pos_b_end[kMaxChan];  // index of 2^c numbin_c
+    int16_t pos_blk_a, pos_blk_b, pos_total;
+};
+
+// Pair p (< npairs) correlates series sa(p) with series sb(p).
+struct PairDesc {
+    int32_t npix, npairs;
+    int32_t acf_first;   // 1: pairs [0,npix) are ACF
+    int32_t has_ccf;     // 1: CCF pairs follow
+};
+
+__host__ __device__ inline void pair_series(const PairDesc &pd, int p, int &sa, int &sb) {
+    if (pd.acf_first && p < pd.npix) {
+        sa = p;
+        sb = p;
+    } else {
+        int k = pd.acf_first ? p - pd.npix : p;
+        sa = k;
+        sb = pd.npix + k;
+    }
+}
+
+// Fit model with fixed geometry (ACF_1D parameter slots 11..19).
+struct LMModelGeom {
+    float ax, ay, s, sz, rx, ry, obsvol, q2, q3;   // micrometres
+};
+
+// Launchers (imfcs_kernels.hip)
+void launch_trace_moments(const SeriesGeom &g, int nopit, int ave, double tscale, double frametime,
+                          int order, int integer_trace, double *part, int nchunk, hipStream_t st);
+void launch_polyfit(const double *part, int nchunk, int nseries, int nopit, int ave, double tscale,
+                    double frametime, int order, double *coef, hipStream_t st);
+void launch_prefix(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, double *ctot,
+                   double *rec, int nchunk, int chunk_len, hipStream_t st);
+void launch_prefix_combine(const CorrTables &ct, int nseries, const double *ctot, const double *rec,
+                           int nchunk, int chunk_len, double *F, hipStream_t st);
+void launch_blocking(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const PairDesc &pd,
+                     const double *F, double *bpart, int nchunk, int chunk_len, hipStream_t st);
+void launch_blocking_finalize(const CorrTables &ct, const PairDesc &pd, const double *F, const double *bpart,
+                              int nchunk, int32_t *ind, int32_t *bis, hipStream_t st);
+void launch_correlate(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const PairDesc &pd,
+                      const double *F, const int32_t *ind, double *corav, double *bvar, hipStream_t st);
+void launch_lm_acf(int nfits, int npts, const float *data, const float *weights, const float *x,
+                   const float *init, const int *pfit, const LMModelGeom &geom, float tol, int maxit,
+                   float *out_params, int *states, float *chi2, int *iters, hipStream_t st);
+void launch_lm_poly(int nfits, int npts, const float *data, const float *weights, const float *x,
+                    const float *init, const int *pfit, int ncoef, float tol, int maxit,
+                    float *out_params, int *states, float *chi2, int *iters, hipStream_t st);
+void launch_binning_f32(const float *in, float *out, int win, int hin, int wt, int ht, int T, int bx, int by,
+                        hipStream_t st);
+void launch_bleach_average_f32(const float *in, float *out, int wt, int ht, int nopit, int ave,
+                               hipStream_t st);
+void launch_pack_fit_inputs(const CorrTables &ct, int npairs, const double *corav, const double *bvar,
+                            float *data, float *weights, hipStream_t st);
+
+}  // namespace imfcs

@@ -1,0 +1,938 @@
+// HIP kernels of the imaging-FCS engine for CDNA4 (gfx950).
+//
+// Stage map (reference semantics followed, IFCS = Imaging_FCS_1_52.java):
+//   k_trace_moments / k_polyfit : calcIntensityTrace (IFCS:9822-9855) and the
+//        least-squares polynomial of PolynomFit (IFCS:10953-11004), solved
+//        as FP64 normal equations in a normalized abscissa.
+//   bleach correction (inline)  : getIntensity polynomial branch
+//        (IFCS:9734-9750), applied on the fly to every loaded value.
+//   k_prefix / k_prefix_combine : the window means ("direct/delayed
+//        monitors") of every channel (IFCS:9487-9494, 9262-9270).
+//   k_blocking / _finalize      : blockTransform (IFCS:9217-9422).
+//   k_correlate                 : calculateCF (IFCS:9425-9521) -- G and the
+//        blocked variance of every channel.
+//   k_lm                        : batched Levenberg-Marquardt with the
+//        step control of Gpufit (agpufitjni.cu:2044-2458), model FCS_3p
+//        (IFCS:13131-13408) or a polynomial (agpufitjni.cu:362-420).
+//
+// All series arithmetic is FP64 (full rate on CDNA4 VALU); the LM model is
+// evaluated in FP32 and its normal equations are accumulated and solved in
+// FP64.
+#include "imfcs_internal.h"
+
+#include <math.h>
+
+namespace imfcs {
+
+// ------------------------------------------------------------------
+// Series access
+// ------------------------------------------------------------------
+__device__ __forceinline__ void series_origin(const SeriesGeom &g, int s, int &ox, int &oy) {
+    int k = s < g.npix ? s : s - g.npix;
+    int x = k % g.ow, y = k / g.ow;
+    ox = g.ox0 + x * g.pbx;
+    oy = g.oy0 + y * g.pby;
+    if (s >= g.npix) {
+        ox += g.cfx;
+        oy += g.cfy;
+    }
+}
+
+// Exact integer window sum minus background (IFCS:9668-9678).
+__device__ __forceinline__ int raw_u16(const SeriesGeom &g, int ox, int oy, int f) {
+    const uint16_t *base = (const uint16_t *)g.data + (int64_t)(g.f0 + f) * g.plane +
+                           (int64_t)oy * g.stride + ox;
+    if (g.bx == 1 && g.by == 1) return (int)base[0] - g.bckg_total;
+    int acc = 0;
+    for (int k = 0; k < g.by; k++)
+        for (int i = 0; i < g.bx; i++) acc += base[(int64_t)k * g.stride + i];
+    return acc - g.bckg_total;
+}
+
+__device__ __forceinline__ double raw_value(const SeriesGeom &g, int ox, int oy, int f) {
+    if (g.src == kSrcU16) return (double)raw_u16(g, ox, oy, f);
+    const float *base = (const float *)g.data + (int64_t)(g.f0 + f) * g.plane + (int64_t)oy * g.stride + ox;
+    return (double)base[0];
+}
+
+// Per-series bleach polynomial held in registers.
+struct Bleach {
+    double c[kMaxOrder + 1];
+    int order;
+    double c0, scale;   // u = (f + off) * scale
+    double off;
+    __device__ void load(const BleachDesc &bd, int s) {
+        order = bd.order;
+#pragma unroll
+        for (int k = 0; k <= kMaxOrder; k++) c[k] = (k <= order && order > 0) ? bd.coef[(int64_t)s * (order + 1) + k] : 0.0;
+        c0 = c[0];
+        scale = bd.frametime / bd.tscale;
+        off = bd.t_off;
+    }
+    // I' = I / sqrt(f/c0) + c0 (1 - sqrt(f/c0))   (IFCS:9744-9750)
+    __device__ __forceinline__ double apply(double v, int f) const {
+        if (order == 0) return v;
+        double u = ((double)f + off) * scale;
+        double fu = 0.0;
+#pragma unroll
+        for (int k = kMaxOrder; k >= 0; k--)
+            if (k <= order) fu = fma(fu, u, c[k]);
+        double sq = sqrt(fu / c0);
+        return v / sq + c0 * (1.0 - sq);
+    }
+};
+
+// ------------------------------------------------------------------
+// Stage 1: intensity trace moments and polynomial fit
+// ------------------------------------------------------------------
+__global__ void k_trace_moments(SeriesGeom g, int nopit, int ave, double tscale, double frametime, int order,
+                                int integer_trace, int pts_per_chunk, double *part) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    int chunk = blockIdx.y;
+    if (s >= g.nseries) return;
+    int ox, oy;
+    series_origin(g, s, ox, oy);
+    int x0 = chunk * pts_per_chunk, x1 = min(nopit, x0 + pts_per_chunk);
+    double acc[kMaxOrder + 1];
+#pragma unroll
+    for (int k = 0; k <= kMaxOrder; k++) acc[k] = 0.0;
+    for (int x = x0; x < x1; x++) {
+        double y;
+        if (integer_trace && g.src == kSrcU16) {
+            // Java int sum and int division by ave (IFCS:9841-9852)
+            int isum = 0;
+            for (int f = x * ave; f < (x + 1) * ave; f++) isum += raw_u16(g, ox, oy, f);
+            y = (double)(isum / ave);
+        } else {
+            double sum = 0.0;
+            for (int f = x * ave; f < (x + 1) * ave; f++) sum += raw_value(g, ox, oy, f);
+            y = sum / ave;
+        }
+        double u = frametime * (x + 0.5) * ave / tscale, up = 1.0;
+#pragma unroll
+        for (int k = 0; k <= kMaxOrder; k++) {
+            if (k <= order) acc[k] += y * up;
+            up *= u;
+        }
+    }
+    int64_t base = ((int64_t)chunk * g.nseries + s) * (order + 1);
+#pragma unroll
+    for (int k = 0; k <= kMaxOrder; k++)
+        if (k <= order) part[base + k] = acc[k];
+}
+
+struct PolyL {
+    double L[(kMaxOrder + 1) * (kMaxOrder + 1)];
+};
+
+// Solves M c = r with the shared Cholesky factor of the moment matrix.
+__global__ void k_polyfit(const double *part, int nchunk, int nseries, int order, PolyL Lf, double *coef) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseries) return;
+    int n = order + 1;
+    double r[kMaxOrder + 1], z[kMaxOrder + 1];
+#pragma unroll
+    for (int k = 0; k <= kMaxOrder; k++) r[k] = 0.0;
+    for (int c = 0; c < nchunk; c++) {
+        int64_t base = ((int64_t)c * nseries + s) * n;
+#pragma unroll
+        for (int k = 0; k <= kMaxOrder; k++)
+            if (k < n) r[k] += part[base + k];
+    }
+#pragma unroll
+    for (int i = 0; i <= kMaxOrder; i++) {
+        if (i < n) {
+            double v = r[i];
+#pragma unroll
+            for (int k = 0; k < i; k++) v -= Lf.L[i * n + k] * z[k];
+            z[i] = v / Lf.L[i * n + i];
+        }
+    }
+#pragma unroll
+    for (int i = kMaxOrder; i >= 0; i--) {
+        if (i < n) {
+            double v = z[i];
+#pragma unroll
+            for (int k = kMaxOrder; k > i; k--)
+                if (k < n) v -= Lf.L[k * n + i] * r[k];
+            r[i] = v / Lf.L[i * n + i];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k <= kMaxOrder; k++)
+        if (k < n) coef[(int64_t)s * n + k] = r[k];
+}
+
+// ------------------------------------------------------------------
+// Stage 2: prefix sums of the corrected series at the positions that
+// define every channel's window means.
+// ------------------------------------------------------------------
+__global__ void k_prefix(SeriesGeom g, BleachDesc bd, CorrTables ct, int chunk_len, double *ctot, double *rec) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    int chunk = blockIdx.y;
+    int nchunk = gridDim.y;
+    if (s >= g.nseries) return;
+    int ox, oy;
+    series_origin(g, s, ox, oy);
+    Bleach bl;
+    bl.load(bd, s);
+    int f0 = chunk * chunk_len, f1 = min(g.T, f0 + chunk_len);
+    int k = 0;
+    while (k < ct.npos && ct.pos[k] < f0) k++;
+    double run = 0.0;
+    for (int f = f0; f < f1; f++) {
+        while (k < ct.npos && ct.pos[k] == f) {
+            rec[(int64_t)s * ct.npos + k] = run;
+            k++;
+        }
+        run += bl.apply(raw_value(g, ox, oy, f), f);
+    }
+    if (chunk == nchunk - 1)
+        while (k < ct.npos && ct.pos[k] >= f1) {
+            rec[(int64_t)s * ct.npos + k] = run;
+            k++;
+        }
+    ctot[(int64_t)chunk * g.nseries + s] = run;
+}
+
+__global__ void k_prefix_combine(CorrTables ct, int nseries, const double *ctot, const double *rec, int nchunk,
+                                 int chunk_len, double *F) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseries) return;
+    double acc = 0.0;
+    int c = 0;
+    for (int k = 0; k < ct.npos; k++) {
+        int ck = min(ct.pos[k] / chunk_len, nchunk - 1);
+        while (c < ck) {
+            acc += ctot[(int64_t)c * nseries + s];
+            c++;
+        }
+        F[(int64_t)s * ct.npos + k] = acc + rec[(int64_t)s * ct.npos + k];
+    }
+}
+
+// ------------------------------------------------------------------
+// Stage 3: lag-1 blocking curve (blockTransform).  The Java copies the
+// 1-based trace into a 0-based array, so the series is a'_y = a[y-1]
+// (a'_0 = 0) against b'_{y+1} = b[y], y = 0..T-2 (IFCS:9259-9262).
+// ------------------------------------------------------------------
+__global__ void k_blocking(SeriesGeom g, BleachDesc bd, CorrTables ct, PairDesc pd, const double *F, double *bpart,
+                           int chunk_len) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int chunk = blockIdx.y;
+    if (p >= pd.npairs) return;
+    int sa, sb;
+    pair_series(pd, p, sa, sb);
+    int oxa, oya, oxb, oyb;
+    series_origin(g, sa, oxa, oya);
+    series_origin(g, sb, oxb, oyb);
+    Bleach ba, bb;
+    ba.load(bd, sa);
+    bb.load(bd, sb);
+    const int n0 = g.T - 1;
+    const double am = F[(int64_t)sa * ct.npos + ct.pos_blk_a] / n0;
+    const double bm = F[(int64_t)sb * ct.npos + ct.pos_blk_b] / n0;
+    const int K = ct.blocknum;
+    double pend[kMaxBlock], s1[kMaxBlock], s2[kMaxBlock];
+#pragma unroll
+    for (int k = 0; k < kMaxBlock; k++) pend[k] = s1[k] = s2[k] = 0.0;
+    int y0 = chunk * chunk_len, y1 = min(n0, y0 + chunk_len);
+    double aprev = (y0 == 0) ? 0.0 : ba.apply(raw_value(g, oxa, oya, y0 - 1), y0 - 1);
+    for (int y = y0; y < y1; y++) {
+        double av = ba.apply(raw_value(g, oxa, oya, y), y);
+        double bv = (sa == sb) ? av : bb.apply(raw_value(g, oxb, oyb, y), y);
+        double carry = (aprev - am) * (bv - bm);
+        aprev = av;
+#pragma unroll
+        for (int k = 0; k < kMaxBlock; k++) {
+            if (k < K) {
+                s1[k] += carry;
+                s2[k] += carry * carry;
+                if (k == K - 1 || ((y >> k) & 1) == 0) {
+                    if (k < K - 1) pend[k] = carry;
+                    break;
+                }
+                carry += pend[k];
+            }
+        }
+    }
+    int64_t base = ((int64_t)chunk * pd.npairs + p) * K * 2;
+    for (int k = 0; k < K; k++) {
+        bpart[base + 2 * k] = s1[k];
+        bpart[base + 2 * k + 1] = s2[k];
+    }
+}
+
+__global__ void k_blocking_finalize(CorrTables ct, PairDesc pd, int Tn, const double *F, const double *bpart,
+                                    int nchunk, int32_t *ind_out, int32_t *bis_out) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= pd.npairs) return;
+    int sa, sb;
+    pair_series(pd, p, sa, sb);
+    const int n0 = Tn - 1;
+    const double am = F[(int64_t)sa * ct.npos + ct.pos_blk_a] / n0;
+    const double bm = F[(int64_t)sb * ct.npos + ct.pos_blk_b] / n0;
+    const int K = ct.blocknum;
+    double sd[kMaxBlock], upper[kMaxBlock], lower[kMaxBlock];
+    int crt[kMaxBlock], cr12[kMaxBlock], cr3[kMaxBlock], diffpos[kMaxBlock];
+    double dd = am * bm;
+    double nb = (double)n0;
+    for (int k = 0; k < K; k++) {
+        double s1 = 0.0, s2 = 0.0;
+        for (int c = 0; c < nchunk; c++) {
+            int64_t base = ((int64_t)c * pd.npairs + p) * K * 2;
+            s1 += bpart[base + 2 * k];
+            s2 += bpart[base + 2 * k + 1];
+        }
+        if (k > 0) nb = floor(nb / 2);
+        double scale = ldexp(1.0, -k);
+        double sp = s1 * scale, sp2 = s2 * scale * scale;
+        double v = (sp2 / nb - (sp / nb) * (sp / nb)) / (nb * (dd * dd));
+        sd[k] = sqrt(v);
+        double err = sd[k] / sqrt(2 * (nb - 1));
+        upper[k] = sd[k] + err;
+        lower[k] = sd[k] - err;
+        crt[k] = cr12[k] = cr3[k] = diffpos[k] = 0;
+    }
+    for (int x = 0; x < K - 1; x++)
+        if (upper[x] > lower[x + 1] && upper[x + 1] > lower[x]) crt[x] = 1;
+    for (int x = 0; x < K - 2; x++)
+        if (crt[x] * crt[x + 1] == 1) cr12[x] = 1;
+    for (int x = 0; x < K - 1; x++)
+        if (sd[x + 1] - sd[x] > 0) diffpos[x] = 1;
+    for (int x = 0; x < K - 2; x++)
+        if (diffpos[x] * diffpos[x + 1] == 1) cr3[x] = 1;
+    int last0 = 0, ind = 0, bis;
+    for (int x = 0; x < K - 2; x++)
+        if (cr3[x] == 1 && cr12[x] == 0) last0 = x;
+    for (int x = 0; x <= last0; x++) cr12[x] = 0;
+    cr12[K - 3] = 0;
+    cr12[K - 4] = 0;
+    for (int x = K - 5; x > 0; x--)
+        if (cr12[x] == 1) ind = x + 1;
+    if (ind == 0) {
+        bis = 0;
+        ind = (K - 3 > 0) ? K - 3 : K - 1;
+    } else {
+        bis = 1;
+    }
+    if (ind < ct.q - 1) ind = ct.q - 1;
+    ind_out[p] = ind;
+    bis_out[p] = bis;
+}
+
+// ------------------------------------------------------------------
+// Stage 4: multi-tau correlation with blocked variance (calculateCF).
+//
+// One thread per (pair, level).  Level-c values are sums of 2^c frames
+// (the pairwise-sum rebinning of IFCS:9477-9478).  Products are grouped by
+// their a-index in groups of G values; block boundaries of the blocked
+// variance (2^(ind-c) products) are multiples of G.  Values are shifted by
+// the series mean so FP64 block sums carry no large offsets; every channel's
+// exact window means (alpha, beta relative to the shift) are folded in
+// analytically:
+//   sum_y (a_y - abar)(b_{y+d} - bbar) = sum_y a~ b~ - n alpha beta
+//   block sum S = Sab - beta Sa - alpha Sb + B alpha beta
+// ------------------------------------------------------------------
+struct LevelCtx {
+    const SeriesGeom *g;
+    int oxa, oya, oxb, oyb, c, numbin, same;
+    double shift_a, shift_b;
+    const Bleach *ba, *bb;
+    __device__ __forceinline__ double val_a(int y) const {
+        if (y >= numbin) return 0.0;
+        int f0 = y << c;
+        double acc = 0.0;
+        for (int i = 0; i < (1 << c); i++) acc += ba->apply(raw_value(*g, oxa, oya, f0 + i), f0 + i);
+        return acc - shift_a;
+    }
+    __device__ __forceinline__ double val_b(int y) const {
+        if (same) return val_a(y);
+        if (y >= numbin) return 0.0;
+        int f0 = y << c;
+        double acc = 0.0;
+        for (int i = 0; i < (1 << c); i++) acc += bb->apply(raw_value(*g, oxb, oyb, f0 + i), f0 + i);
+        return acc - shift_b;
+    }
+};
+
+template <int G, int NL, int DMIN>
+__device__ void correlate_level(const LevelCtx &L, const CorrTables &ct, int chan0, const double *alpha,
+                                const double *beta, int B, int pnum, double *cross, double *a1, double *a2) {
+    constexpr int WN = G + 16;
+    double w[WN];   // b~ values [y0, y0 + WN)
+    double ga[G];   // a~ values [y0, y0 + G)
+    double sab[NL], sb[NL], sa = 0.0;
+#pragma unroll
+    for (int i = 0; i < NL; i++) {
+        sab[i] = sb[i] = 0.0;
+        cross[i] = a1[i] = a2[i] = 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < WN; k++) w[k] = L.val_b(k);
+    const int numbin = L.numbin;
+    for (int y0 = 0; y0 < numbin; y0 += G) {
+#pragma unroll
+        for (int k = 0; k < G; k++) ga[k] = L.same ? w[k] : L.val_a(y0 + k);
+        if (y0 + WN <= numbin) {
+            double sg = 0.0;
+#pragma unroll
+            for (int k = 0; k < G; k++) sg += ga[k];
+            sa += sg;
+#pragma unroll
+            for (int i = 0; i < NL; i++) {
+                double s = 0.0, t = 0.0;
+#pragma unroll
+                for (int k = 0; k < G; k++) {
+                    s = fma(ga[k], w[k + DMIN + i], s);
+                    t += w[k + DMIN + i];
+                }
+                sab[i] += s;
+                sb[i] += t;
+            }
+        } else {
+            // series tail: only products with y + d < numbin exist
+#pragma unroll
+            for (int i = 0; i < NL; i++) {
+                double s = 0.0;
+#pragma unroll
+                for (int k = 0; k < G; k++)
+                    if (y0 + k + DMIN + i < numbin) s = fma(ga[k], w[k + DMIN + i], s);
+                sab[i] += s;
+            }
+        }
+        int yend = y0 + G;
+        if ((yend & (B - 1)) == 0) {
+            int j = yend / B - 1;
+            bool inblk = j < pnum;
+#pragma unroll
+            for (int i = 0; i < NL; i++) {
+                if (inblk) {
+                    double S = sab[i] - beta[i] * sa - alpha[i] * sb[i] + (double)B * alpha[i] * beta[i];
+                    a1[i] += S;
+                    a2[i] += S * S;
+                }
+                cross[i] += sab[i];
+                sab[i] = 0.0;
+                sb[i] = 0.0;
+            }
+            sa = 0.0;
+        }
+        // slide the window by G
+#pragma unroll
+        for (int k = 0; k < 16; k++) w[k] = w[k + G];
+#pragma unroll
+        for (int k = 0; k < G; k++) w[16 + k] = L.val_b(y0 + WN + k);
+    }
+#pragma unroll
+    for (int i = 0; i < NL; i++) cross[i] += sab[i];
+}
+
+__global__ void __launch_bounds__(64) k_correlate(SeriesGeom g, BleachDesc bd, CorrTables ct, PairDesc pd,
+                                                  const double *F, const int32_t *ind, double *corav, double *bvar) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = blockIdx.y;
+    if (p >= pd.npairs || c >= ct.nlevels) return;
+    int sa, sb;
+    pair_series(pd, p, sa, sb);
+    LevelCtx L;
+    L.g = &g;
+    series_origin(g, sa, L.oxa, L.oya);
+    series_origin(g, sb, L.oxb, L.oyb);
+    L.c = c;
+    L.numbin = ct.numbin[c];
+    L.same = (sa == sb);
+    Bleach ba, bb;
+    ba.load(bd, sa);
+    bb.load(bd, sb);
+    L.ba = &ba;
+    L.bb = &bb;
+    const double *Fa = F + (int64_t)sa * ct.npos, *Fb = F + (int64_t)sb * ct.npos;
+    double mu_a = Fa[ct.pos_total] / g.T, mu_b = Fb[ct.pos_total] / g.T;
+    L.shift_a = mu_a * (double)(1 << c);
+    L.shift_b = mu_b * (double)(1 << c);
+    const int chan0 = ct.first_chan[c], nch = ct.nchan[c];
+    double am[17], bmn[17], alpha[17], beta[17];
+    for (int i = 0; i < nch; i++) {
+        int x = chan0 + i;
+        double n = (double)(ct.numbin[c] - ct.del[x]);
+        am[i] = Fa[ct.pos_a_end[x]] / n;
+        bmn[i] = (Fb[ct.pos_b_end[x]] - Fb[ct.pos_b_beg[x]]) / n;
+        alpha[i] = am[i] - L.shift_a;
+        beta[i] = bmn[i] - L.shift_b;
+    }
+    int indp = ind[p];
+    int B = 1 << (indp - c);
+    int pnum = (int)floor(ct.mtab_last / pow(2.0, fmax(indp - ct.log2_samp_last, 0.0)));
+    double cross[17], a1[17], a2[17];
+    if (c == 0) {
+        correlate_level<16, 17, 0>(L, ct, chan0, alpha, beta, B, pnum, cross, a1, a2);
+    } else {
+        int G = 16;
+        int bmin = 1 << max(ct.q - 1 - c, 0);
+        if (bmin < G) G = bmin;
+        switch (G) {
+            case 16: correlate_level<16, 8, 9>(L, ct, chan0, alpha, beta, B, pnum, cross, a1, a2); break;
+            case 8: correlate_level<8, 8, 9>(L, ct, chan0, alpha, beta, B, pnum, cross, a1, a2); break;
+            case 4: correlate_level<4, 8, 9>(L, ct, chan0, alpha, beta, B, pnum, cross, a1, a2); break;
+            case 2: correlate_level<2, 8, 9>(L, ct, chan0, alpha, beta, B, pnum, cross, a1, a2); break;
+            default: correlate_level<1, 8, 9>(L, ct, chan0, alpha, beta, B, pnum, cross, a1, a2); break;
+        }
+    }
+    for (int i = 0; i < nch; i++) {
+        int x = chan0 + i;
+        double n = (double)(ct.numbin[c] - ct.del[x]);
+        double tot = cross[i] - n * alpha[i] * beta[i];
+        double ab = am[i] * bmn[i];
+        corav[(int64_t)p * ct.chanum + x] = tot / (n * am[i] * bmn[i]);
+        double sp = a1[i] / B, sp2 = a2[i] / ((double)B * B);
+        double pn = (double)pnum;
+        bvar[(int64_t)p * ct.chanum + x] = (sp2 / pn - (sp / pn) * (sp / pn)) / ((pn - 1) * (ab * ab));
+    }
+}
+
+// ------------------------------------------------------------------
+// Stage 5: batched Levenberg-Marquardt, one wavefront per fit.
+// ------------------------------------------------------------------
+constexpr int kLMMaxFree = 11;
+// fit states (GpufitImFCS.java:817-866)
+constexpr int kStateConverged = 0, kStateMaxIter = 1, kStateSingular = 2;
+
+// FCS_3p value and gradient (IFCS:13147-13408) in micrometre/second units.
+struct Comp {
+    float plat, dDplat, pspim, dDpspim, dvx, dvy, acf;
+};
+
+__device__ __forceinline__ void fcs_component(const LMModelGeom &m, float x, float D, float vx, float vy, bool grad,
+                                              Comp &o) {
+    const float sqrpi = 1.7724538509055159f;
+    float p0t = sqrtf(4.f * D * x + m.s * m.s);
+    float ip0 = 1.f / p0t, ip02 = ip0 * ip0;
+    float p1xt = m.ax + m.rx - vx * x, p2xt = m.ax - m.rx + vx * x, p3xt = m.rx - vx * x;
+    float p1yt = m.ay + m.ry - vy * x, p2yt = m.ay - m.ry + vy * x, p3yt = m.ry - vy * x;
+    float e1x = erff(p1xt * ip0), e2x = erff(p2xt * ip0), e3x = erff(p3xt * ip0);
+    float e1y = erff(p1yt * ip0), e2y = erff(p2yt * ip0), e3y = erff(p3yt * ip0);
+    float pexpx = expf(-(p1xt * ip0) * (p1xt * ip0)) + expf(-(p2xt * ip0) * (p2xt * ip0)) -
+                  2.f * expf(-(p3xt * ip0) * (p3xt * ip0));
+    float pexpy = expf(-(p1yt * ip0) * (p1yt * ip0)) + expf(-(p2yt * ip0) * (p2yt * ip0)) -
+                  2.f * expf(-(p3yt * ip0) * (p3yt * ip0));
+    float perfx = p1xt * e1x + p2xt * e2x - 2.f * p3xt * e3x;
+    float perfy = p1yt * e1y + p2yt * e2y - 2.f * p3yt * e3y;
+    float lx = p0t / sqrpi * pexpx + perfx, ly = p0t / sqrpi * pexpy + perfy;
+    float norm = 4.f * (m.ax * m.ay) * (m.ax * m.ay) / m.obsvol;
+    o.plat = lx * ly / norm;
+    float szi2 = 1.f / (m.sz * m.sz);
+    float q = 1.f + 4.f * D * x * szi2;
+    o.pspim = rsqrtf(q);
+    o.acf = o.plat * o.pspim;
+    if (grad) {
+        // dDpexp = 2 exp(-p4/p0^2)(exp(p5/p0^2) + exp(p6/p0^2) - 2 exp(p7/p0^2)), exponents combined
+        float p4x = 2.f * m.ax * m.ax + 3.f * m.rx * m.rx - 6.f * x * m.rx * vx + 3.f * (x * vx) * (x * vx);
+        float p5x = p3xt * p3xt + p1xt * p1xt, p6x = p3xt * p3xt + p2xt * p2xt;
+        float p7x = 2.f * (m.ax * m.ax + m.rx * m.rx - 2.f * x * m.rx * vx + (x * vx) * (x * vx));
+        float p4y = 2.f * m.ay * m.ay + 3.f * m.ry * m.ry - 6.f * x * m.ry * vy + 3.f * (x * vy) * (x * vy);
+        float p5y = p3yt * p3yt + p1yt * p1yt, p6y = p3yt * p3yt + p2yt * p2yt;
+        float p7y = 2.f * (m.ay * m.ay + m.ry * m.ry - 2.f * x * m.ry * vy + (x * vy) * (x * vy));
+        float dDpexpx = 2.f * (expf((p5x - p4x) * ip02) + expf((p6x - p4x) * ip02) - 2.f * expf((p7x - p4x) * ip02));
+        float dDpexpy = 2.f * (expf((p5y - p4y) * ip02) + expf((p6y - p4y) * ip02) - 2.f * expf((p7y - p4y) * ip02));
+        o.dDplat = (ip0 / sqrpi) * (dDpexpy * x * lx + dDpexpx * x * ly) / norm;
+        o.dDpspim = -4.f * x / (2.f * m.sz * m.sz * q * sqrtf(q));
+        o.dvx = ly * ((e2x + 2.f * e3x - e1x) * x);
+        o.dvy = lx * ((e2y + 2.f * e3y - e1y) * x);
+    }
+}
+
+// pq: N, D, vx, vy, G, F2, D2, F3, D3, Ftrip, Ttrip (um, s units)
+__device__ float fcs3p(const LMModelGeom &m, float x, const float *pq, bool grad, bool c2on, bool c3on,
+                       float *gr) {
+    Comp c1, c2 = {0, 0, 0, 0, 0, 0, 0}, c3 = {0, 0, 0, 0, 0, 0, 0};
+    fcs_component(m, x, pq[1], pq[2], pq[3], grad, c1);
+    if (c2on) fcs_component(m, x, pq[6], pq[2], pq[3], grad, c2);
+    if (c3on) fcs_component(m, x, pq[8], pq[2], pq[3], grad, c3);
+    float q2 = m.q2, q3 = m.q3, F2 = pq[5], F3 = pq[7];
+    float et = (pq[10] != 0.f) ? expf(-x / pq[10]) : 0.f;
+    float triplet = 1.f + pq[9] / (1.f - pq[9]) * et;
+    float den = 1.f - F2 - F3 + q2 * F2 + q3 * F3;
+    float invN = 1.f / pq[0];
+    float mix = (1.f - F2 - F3) * c1.acf + q2 * q2 * F2 * c2.acf + q3 * q3 * F3 * c3.acf;
+    float pacf = invN * mix / (den * den) * triplet + pq[4];
+    if (grad) {
+        float pf1 = (1.f - F2 - F3) / den, pf2 = q2 * q2 * F2 / den, pf3 = q3 * q3 * F3 / den;
+        float dfnom = den * den * den;
+        float df21 = 1.f - F2 - F3 + q2 * F2 - q3 * F3 + 2.f * q2 * F3 - 2.f * q2;
+        float df22 = q2 * q2 * (1.f + F2 - F3 - q2 * F2 + q3 * F3);
+        float df23 = 2.f * F3 * q3 * q3 * (1.f - q2);
+        float df31 = 1.f - F2 - F3 - q2 * F2 + 2.f * q3 * F2 - 2.f * q3 + q3 * F3;
+        float df32 = 2.f * F2 * q2 * q2 * (1.f - q3);
+        float df33 = q3 * q3 * (1.f - F2 + F3 + q2 * F2 - q3 * F3);
+        float norm = 4.f * (m.ax * m.ay) * (m.ax * m.ay) / m.obsvol;
+        float dtF = et * (1.f / (1.f - pq[9]) + pq[9] / ((1.f - pq[9]) * (1.f - pq[9])));
+        float dtT = (pq[10] != 0.f) ? et * (pq[9] * x) / ((1.f - pq[9]) * pq[10] * pq[10]) : 0.f;
+        gr[0] = -invN * invN * (pf1 * c1.acf + pf2 * c2.acf + pf3 * c3.acf) * triplet;
+        gr[1] = invN * pf1 * (c1.plat * c1.dDpspim + c1.pspim * c1.dDplat);   // no triplet, IFCS:13310
+        gr[2] = invN * (pf1 * c1.dvx * c1.pspim + pf2 * c2.dvx * c2.pspim + pf3 * c3.dvx * c3.pspim) / norm * triplet;
+        gr[3] = invN * (pf1 * c1.dvy * c1.pspim + pf2 * c2.dvy * c2.pspim + pf3 * c3.dvy * c3.pspim) / norm * triplet;
+        gr[4] = 1.f;
+        gr[5] = invN / dfnom * (df21 * c1.acf + df22 * c2.acf + df23 * c3.acf) * triplet;
+        gr[6] = invN * pf2 * (c2.plat * c2.dDpspim + c2.pspim * c2.dDplat) * triplet;
+        gr[7] = invN / dfnom * (df31 * c1.acf + df32 * c2.acf + df33 * c3.acf) * triplet;
+        gr[8] = invN * pf3 * (c3.plat * c3.dDpspim + c3.pspim * c3.dDplat) * triplet;
+        gr[9] = dtF * pacf;
+        gr[10] = dtT * pacf;
+    }
+    return pacf;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// In-place Cholesky solve of the n x n SPD system A x = b (FP64). Returns
+// false if A is not positive definite.
+__device__ bool chol_solve(double *A, double *b, int n) {
+    for (int j = 0; j < n; j++) {
+        double s = A[j * kLMMaxFree + j];
+        for (int k = 0; k < j; k++) s -= A[j * kLMMaxFree + k] * A[j * kLMMaxFree + k];
+        if (!(s > 0.0)) return false;
+        double d = sqrt(s);
+        A[j * kLMMaxFree + j] = d;
+        for (int i = j + 1; i < n; i++) {
+            double t = A[i * kLMMaxFree + j];
+            for (int k = 0; k < j; k++) t -= A[i * kLMMaxFree + k] * A[j * kLMMaxFree + k];
+            A[i * kLMMaxFree + j] = t / d;
+        }
+    }
+    for (int i = 0; i < n; i++) {
+        double t = b[i];
+        for (int k = 0; k < i; k++) t -= A[i * kLMMaxFree + k] * b[k];
+        b[i] = t / A[i * kLMMaxFree + i];
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        double t = b[i];
+        for (int k = i + 1; k < n; k++) t -= A[k * kLMMaxFree + i] * b[k];
+        b[i] = t / A[i * kLMMaxFree + i];
+    }
+    return true;
+}
+
+enum ModelKind { kModelFCS = 0, kModelPoly = 1 };
+
+struct LMArgs {
+    int nfits, npts, nparams, kind, ncoef;
+    const float *data, *weights, *x, *init;
+    const int *pfit;
+    LMModelGeom geom;
+    float tol;
+    int maxit;
+    float *out_params;
+    int *states;
+    float *chi2;
+    int *iters;
+};
+
+// Evaluates chi^2 and (optionally) gradient / hessian sums for the fit, with
+// every lane handling points lane, lane+64, ...  Results are wave totals.
+__device__ void lm_eval(const LMArgs &a, int fit, const float *par, const int *fidx, int nfree, bool need_jac,
+                        double &chi2, double *grad, double *hess) {
+    const int lane = threadIdx.x & 63;
+    const float *y = a.data + (int64_t)fit * a.npts;
+    const float *w = a.weights ? a.weights + (int64_t)fit * a.npts : nullptr;
+    double c2 = 0.0, gl[kLMMaxFree], hl[kLMMaxFree * (kLMMaxFree + 1) / 2];
+    int nh = nfree * (nfree + 1) / 2;
+    for (int i = 0; i < nfree; i++) gl[i] = 0.0;
+    for (int i = 0; i < nh; i++) hl[i] = 0.0;
+    bool c2on = (par[5] != 0.f) || a.pfit[5] || a.pfit[6];
+    bool c3on = (par[7] != 0.f) || a.pfit[7] || a.pfit[8];
+    for (int i = lane; i < a.npts; i += 64) {
+        float xi = a.x[i];
+        float jf[kLMMaxFree];
+        float f;
+        if (a.kind == kModelFCS) {
+            float gfull[11];
+            f = fcs3p(a.geom, xi, par, need_jac, c2on, c3on, gfull);
+            if (need_jac)
+                for (int k = 0; k < nfree; k++) jf[k] = gfull[fidx[k]];
+        } else {
+            // LINEAR_1D: value = sum p_i x^i with FP64 accumulation (agpufitjni.cu:402-418)
+            double tx = 1.0, sv = (double)par[0];
+            float gfull[kLMMaxFree];
+            gfull[0] = 1.f;
+            for (int k = 1; k < a.ncoef; k++) {
+                tx *= xi;
+                gfull[k] = (float)tx;
+                sv += (double)par[k] * tx;
+            }
+            for (int k = a.ncoef; k < kLMMaxFree; k++) gfull[k] = 0.f;
+            f = (float)sv;
+            if (need_jac)
+                for (int k = 0; k < nfree; k++) jf[k] = gfull[fidx[k]];
+        }
+        double wi = w ? (double)w[i] : 1.0;
+        double r = (double)y[i] - (double)f;
+        c2 += wi * r * r;
+        if (need_jac) {
+            int h = 0;
+            for (int k = 0; k < nfree; k++) {
+                double jk = (double)jf[k];
+                gl[k] += wi * r * jk;
+                for (int l = 0; l <= k; l++) hl[h++] += wi * jk * (double)jf[l];
+            }
+        }
+    }
+    chi2 = wave_sum(c2);
+    if (need_jac) {
+        for (int k = 0; k < nfree; k++) grad[k] = wave_sum(gl[k]);
+        for (int k = 0; k < nh; k++) hess[k] = wave_sum(hl[k]);
+    }
+}
+
+__global__ void __launch_bounds__(64) k_lm(LMArgs a) {
+    const int fit = blockIdx.x;
+    if (fit >= a.nfits) return;
+    const int lane = threadIdx.x & 63;
+    // parameters in model units
+    float par[20], prev[20];
+    int fidx[kLMMaxFree], nfree = 0;
+    for (int k = 0; k < a.nparams; k++) {
+        par[k] = a.init[(int64_t)fit * a.nparams + k];
+        if (a.pfit[k] && nfree < kLMMaxFree && (a.kind == kModelPoly ? k < a.ncoef : k < 11)) fidx[nfree++] = k;
+    }
+    for (int k = a.nparams; k < 20; k++) par[k] = 0.f;
+    LMArgs ac = a;
+    if (a.kind == kModelFCS) {
+        // SI -> micrometre/second units; geometry from slots 11..19
+        // (the ACF_1D parameter layout of GPU_ACF_Fit, IFCS:15090-15110)
+        par[1] *= 1e12f;
+        par[6] *= 1e12f;
+        par[8] *= 1e12f;
+        par[2] *= 1e6f;
+        par[3] *= 1e6f;
+        ac.geom.ax = par[11] * 1e6f;
+        ac.geom.ay = par[12] * 1e6f;
+        ac.geom.s = par[13] * 1e6f;
+        ac.geom.sz = par[14] * 1e6f;
+        ac.geom.rx = par[15] * 1e6f;
+        ac.geom.ry = par[16] * 1e6f;
+        ac.geom.obsvol = par[17] * 1e12f;
+        ac.geom.q2 = par[18];
+        ac.geom.q3 = par[19];
+    }
+    double chi2, grad[kLMMaxFree], hess[kLMMaxFree * (kLMMaxFree + 1) / 2];
+    double H[kLMMaxFree * kLMMaxFree], scale[kLMMaxFree], delta[kLMMaxFree];
+    lm_eval(ac, fit, par, fidx, nfree, true, chi2, grad, hess);
+    int state = kStateConverged;
+    int iters = 0;
+    if (nfree > 0) {
+        double lambda = 1e-3, prev_chi2 = chi2;
+        bool failed = false;
+        for (int k = 0; k < nfree; k++) scale[k] = 0.0;
+        // hessian as a full matrix; the diagonal keeps its lambda shift
+        // between failed iterations, as Gpufit does
+        for (int k = 0, h = 0; k < nfree; k++)
+            for (int l = 0; l <= k; l++, h++) H[k * kLMMaxFree + l] = H[l * kLMMaxFree + k] = hess[h];
+        bool finished = false;
+        for (int it = 0; it < a.maxit && !finished; it++) {
+            // cuda_modify_step_widths (agpufitjni.cu:2044-2076)
+            for (int k = 0; k < nfree; k++) {
+                double &hd = H[k * kLMMaxFree + k];
+                if (failed) hd -= scale[k] * lambda / 10.0;
+                scale[k] = fmax(scale[k], hd);
+                hd += scale[k] * lambda;
+            }
+            double A[kLMMaxFree * kLMMaxFree];
+            for (int k = 0; k < nfree * kLMMaxFree; k++) A[k] = H[k];
+            for (int k = 0; k < nfree; k++) delta[k] = grad[k];
+            bool ok = chol_solve(A, delta, nfree);
+            iters = it + 1;
+            if (!ok) {
+                state = kStateSingular;
+                break;
+            }
+            for (int k = 0; k < 20; k++) prev[k] = par[k];
+            for (int k = 0; k < nfree; k++) par[fidx[k]] = (float)((double)par[fidx[k]] + delta[k]);
+            double nchi2, ngrad[kLMMaxFree], nhess[kLMMaxFree * (kLMMaxFree + 1) / 2];
+            lm_eval(ac, fit, par, fidx, nfree, false, nchi2, ngrad, nhess);
+            bool decreased = nchi2 < prev_chi2;
+            if (decreased) lm_eval(ac, fit, par, fidx, nfree, true, nchi2, ngrad, nhess);
+            // cuda_check_for_convergence (agpufitjni.cu:2289-2318)
+            if (fabs(nchi2 - prev_chi2) < (double)a.tol * fmax(1.0, nchi2)) finished = true;
+            else if (it == a.maxit - 1) state = kStateMaxIter;
+            // cuda_prepare_next_iteration (agpufitjni.cu:2458-2490)
+            if (decreased) {
+                lambda *= 0.1;
+                prev_chi2 = nchi2;
+                chi2 = nchi2;
+                for (int k = 0; k < nfree; k++) grad[k] = ngrad[k];
+                for (int k = 0, h = 0; k < nfree; k++)
+                    for (int l = 0; l <= k; l++, h++) H[k * kLMMaxFree + l] = H[l * kLMMaxFree + k] = nhess[h];
+                failed = false;
+            } else {
+                lambda *= 10.0;
+                for (int k = 0; k < 20; k++) par[k] = prev[k];
+                failed = true;
+            }
+        }
+        chi2 = prev_chi2;
+    }
+    if (a.kind == kModelFCS) {
+        par[1] *= 1e-12f;
+        par[6] *= 1e-12f;
+        par[8] *= 1e-12f;
+        par[2] *= 1e-6f;
+        par[3] *= 1e-6f;
+    }
+    if (lane == 0) {
+        for (int k = 0; k < a.nparams; k++) a.out_params[(int64_t)fit * a.nparams + k] = par[k];
+        a.states[fit] = state;
+        a.chi2[fit] = (float)chi2;
+        a.iters[fit] = iters;
+    }
+}
+
+// ------------------------------------------------------------------
+// Drop-in helpers: float binning and bleach-trace averaging
+// ------------------------------------------------------------------
+__global__ void k_binning_f32(const float *in, float *out, int win, int hin, int wt, int ht, int T, int bx, int by) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int yz = blockIdx.y;
+    int y = yz % ht, t = yz / ht;
+    if (x >= wt || t >= T) return;
+    const float *b = in + ((int64_t)t * hin + y) * win + x;
+    float s = 0.f;
+    for (int i = 0; i < bx; i++)
+        for (int j = 0; j < by; j++) s += b[(int64_t)j * win + i];
+    out[((int64_t)t * ht + y) * wt + x] = s;
+}
+
+__global__ void k_bleach_average_f32(const float *in, float *out, int wt, int ht, int nopit, int ave) {
+    int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    int z = blockIdx.y;
+    if (pix >= wt * ht || z >= nopit) return;
+    double s = 0.0;
+    for (int f = z * ave; f < (z + 1) * ave; f++) s += (float)in[(int64_t)f * wt * ht + pix];
+    out[(int64_t)pix * nopit + z] = (float)(s / ave);
+}
+
+__global__ void k_pack_fit_inputs(int chanum, int npairs, const double *corav, const double *bvar, float *data,
+                                  float *weights) {
+    int p = blockIdx.x;
+    int z = threadIdx.x;
+    if (p >= npairs || z >= chanum - 1) return;
+    int64_t src = (int64_t)p * chanum + z + 1, dst = (int64_t)p * (chanum - 1) + z;
+    data[dst] = (float)corav[src];
+    weights[dst] = 1.0f / (float)bvar[src];
+}
+
+// ------------------------------------------------------------------
+// Launchers
+// ------------------------------------------------------------------
+static inline int cdiv(int64_t a, int b) { return (int)((a + b - 1) / b); }
+
+void launch_trace_moments(const SeriesGeom &g, int nopit, int ave, double tscale, double frametime, int order,
+                          int integer_trace, double *part, int nchunk, hipStream_t st) {
+    int ppc = cdiv(nopit, nchunk);
+    dim3 grid(cdiv(g.nseries, 64), nchunk);
+    hipLaunchKernelGGL(k_trace_moments, grid, dim3(64), 0, st, g, nopit, ave, tscale, frametime, order,
+                       integer_trace, ppc, part);
+}
+
+void launch_polyfit(const double *part, int nchunk, int nseries, int nopit, int ave, double tscale,
+                    double frametime, int order, double *coef, hipStream_t st) {
+    // moment matrix of the normalized abscissa and its Cholesky factor
+    int n = order + 1;
+    double M[(kMaxOrder + 1) * (kMaxOrder + 1)] = {0};
+    for (int x = 0; x < nopit; x++) {
+        double u = frametime * (x + 0.5) * ave / tscale, up[2 * kMaxOrder + 1];
+        up[0] = 1.0;
+        for (int k = 1; k < 2 * n - 1; k++) up[k] = up[k - 1] * u;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) M[i * n + j] += up[i + j];
+    }
+    PolyL Lf;
+    for (int i = 0; i < (kMaxOrder + 1) * (kMaxOrder + 1); i++) Lf.L[i] = 0.0;
+    for (int j = 0; j < n; j++) {
+        double s = M[j * n + j];
+        for (int k = 0; k < j; k++) s -= Lf.L[j * n + k] * Lf.L[j * n + k];
+        Lf.L[j * n + j] = sqrt(s);
+        for (int i = j + 1; i < n; i++) {
+            double t = M[i * n + j];
+            for (int k = 0; k < j; k++) t -= Lf.L[i * n + k] * Lf.L[j * n + k];
+            Lf.L[i * n + j] = t / Lf.L[j * n + j];
+        }
+    }
+    hipLaunchKernelGGL(k_polyfit, dim3(cdiv(nseries, 64)), dim3(64), 0, st, part, nchunk, nseries, order, Lf, coef);
+}
+
+void launch_prefix(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, double *ctot, double *rec,
+                   int nchunk, int chunk_len, hipStream_t st) {
+    dim3 grid(cdiv(g.nseries, 64), nchunk);
+    hipLaunchKernelGGL(k_prefix, grid, dim3(64), 0, st, g, bd, ct, chunk_len, ctot, rec);
+}
+
+void launch_prefix_combine(const CorrTables &ct, int nseries, const double *ctot, const double *rec, int nchunk,
+                           int chunk_len, double *F, hipStream_t st) {
+    hipLaunchKernelGGL(k_prefix_combine, dim3(cdiv(nseries, 64)), dim3(64), 0, st, ct, nseries, ctot, rec, nchunk,
+                       chunk_len, F);
+}
+
+void launch_blocking(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const PairDesc &pd,
+                     const double *F, double *bpart, int nchunk, int chunk_len, hipStream_t st) {
+    dim3 grid(cdiv(pd.npairs, 64), nchunk);
+    hipLaunchKernelGGL(k_blocking, grid, dim3(64), 0, st, g, bd, ct, pd, F, bpart, chunk_len);
+}
+
+void launch_blocking_finalize(const CorrTables &ct, const PairDesc &pd, const double *F, const double *bpart,
+                              int nchunk, int32_t *ind, int32_t *bis, hipStream_t st) {
+    hipLaunchKernelGGL(k_blocking_finalize, dim3(cdiv(pd.npairs, 64)), dim3(64), 0, st, ct, pd, ct.T, F, bpart,
+                       nchunk, ind, bis);
+}
+
+void launch_correlate(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const PairDesc &pd,
+                      const double *F, const int32_t *ind, double *corav, double *bvar, hipStream_t st) {
+    dim3 grid(cdiv(pd.npairs, 64), ct.nlevels);
+    hipLaunchKernelGGL(k_correlate, grid, dim3(64), 0, st, g, bd, ct, pd, F, ind, corav, bvar);
+}
+
+static void launch_lm(const LMArgs &a, hipStream_t st) {
+    if (a.nfits <= 0) return;
+    hipLaunchKernelGGL(k_lm, dim3(a.nfits), dim3(64), 0, st, a);
+}
+
+void launch_lm_acf(int nfits, int npts, const float *data, const float *weights, const float *x, const float *init,
+                   const int *pfit, const LMModelGeom &geom, float tol, int maxit, float *out_params, int *states,
+                   float *chi2, int *iters, hipStream_t st) {
+    LMArgs a{nfits, npts, 20, kModelFCS, 0, data, weights, x, init, pfit, geom, tol, maxit, out_params, states, chi2,
+             iters};
+    launch_lm(a, st);
+}
+
+void launch_lm_poly(int nfits, int npts, const float *data, const float *weights, const float *x, const float *init,
+                    const int *pfit, int ncoef, float tol, int maxit, float *out_params, int *states, float *chi2,
+                    int *iters, hipStream_t st) {
+    LMModelGeom g{};
+    LMArgs a{nfits, npts, 11, kModelPoly, ncoef, data, weights, x, init, pfit, g, tol, maxit, out_params, states,
+             chi2, iters};
+    launch_lm(a, st);
+}
+
+void launch_binning_f32(const float *in, float *out, int win, int hin, int wt, int ht, int T, int bx, int by,
+                        hipStream_t st) {
+    dim3 grid(cdiv(wt, 64), (unsigned)((int64_t)ht * T));
+    hipLaunchKernelGGL(k_binning_f32, grid, dim3(64), 0, st, in, out, win, hin, wt, ht, T, bx, by);
+}
+
+void launch_bleach_average_f32(const float *in, float *out, int wt, int ht, int nopit, int ave, hipStream_t st) {
+    dim3 grid(cdiv((int64_t)wt * ht, 64), nopit);
+    hipLaunchKernelGGL(k_bleach_average_f32, grid, dim3(64), 0, st, in, out, wt, ht, nopit, ave);
+}
+
+void launch_pack_fit_inputs(const CorrTables &ct, int npairs, const double *corav, const double *bvar, float *data,
+                            float *weights, hipStream_t st) {
+    if (npairs <= 0) return;
+    hipLaunchKernelGGL(k_pack_fit_inputs, dim3(npairs), dim3(128), 0, st, ct.chanum, npairs, corav, bvar, data,
+                       weights);
+}
+
+}  // namespace imfcs

@@ -1,0 +1,214 @@
+/*
+ * imfcs.h -- C ABI of the MI355X imaging-FCS engine (libimfcs_hip.so).
+ *
+ * Two layers share one set of HIP kernels:
+ *
+ *  1. The drop-in layer: one function per native method of the plugin's
+ *     JNI class gpufitImFCS.GpufitImFCS (GpufitImFCS.java:323-538).  Plain
+ *     host pointers and sizes, the same argument meaning and the same error
+ *     behaviour (status codes / last-error string).  The JNI exports in
+ *     jni_shim.cpp are thin wrappers over these.
+ *
+ *  2. The native pipeline: a 64-bit, device-resident path for uint16 stacks
+ *     (bin -> polynomial bleach correction -> multi-tau ACF/CCF with blocked
+ *     variance -> LM fit) that has no Java array limits.
+ *
+ * Every function returns 0 (IMFCS_OK) on success and a negative code on
+ * failure; imfcs_get_last_error() then describes the failure.  The last
+ * error is thread-local.
+ */
+#ifndef IMFCS_H
+#define IMFCS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IMFCS_OK 0
+#define IMFCS_ERROR (-1)
+#define IMFCS_ERR_ARG (-2)
+#define IMFCS_ERR_DEVICE (-3)
+#define IMFCS_ERR_MEMORY (-4)
+
+/* Fit model ids: GpufitImFCS.java:879-891, agpufitjni.cu:177-182. */
+#define IMFCS_MODEL_GAUSS_2D 1
+#define IMFCS_MODEL_ACF_1D 2
+#define IMFCS_MODEL_LINEAR_1D 3
+/* Estimator ids: GpufitImFCS.java:570-587 (only LSE is implemented). */
+#define IMFCS_ESTIMATOR_LSE 0
+#define IMFCS_ESTIMATOR_MLE 1
+/* Fit states: GpufitImFCS.java:817-866, agpufitjni.cu:187. */
+#define IMFCS_STATE_CONVERGED 0
+#define IMFCS_STATE_MAX_ITERATION 1
+#define IMFCS_STATE_SINGULAR_HESSIAN 2
+#define IMFCS_STATE_NEG_CURVATURE_MLE 3
+#define IMFCS_STATE_GPU_NOT_READY 4
+
+/* ------------------------------------------------------------------ */
+/* 1. Drop-in layer                                                    */
+/* ------------------------------------------------------------------ */
+
+/* Field-for-field mirror of GpufitImFCS.ACFParameters
+ * (GpufitImFCS.java:453-486); booleans are 0/1. */
+typedef struct imfcs_acf_params {
+    int32_t width, height;          /* output grid */
+    int32_t win_star, hin_star;     /* input region */
+    int32_t w_temp, h_temp;         /* stride-1 binned grid */
+    int32_t pixbinX, pixbinY, binningX, binningY;
+    int32_t firstframe, lastframe, framediff;
+    int32_t cfXDistance, cfYDistance;
+    double correlatorp, correlatorq, frametime;
+    int32_t background;
+    double mtab1, mtabchanumminus1, sampchanumminus1;
+    int32_t chanum;
+    int32_t isNBcalculation, bleachcorr_gpu;
+    int32_t bleachcorr_order, nopit, ave;
+} imfcs_acf_params;
+
+/* Batched Levenberg-Marquardt fit.  Replaces gpufit() behind the JNI
+ * method `fit` (GpufitImFCS.java:323, agpufitjni.cu:4614-4636, 4332).
+ * data/weights: n_fits*n_points (weights may be NULL); initial_parameters
+ * and output_parameters: n_fits*n_parameters(model); parameters_to_fit:
+ * n_parameters(model) flags; user_info: user_info_size BYTES (lag times or
+ * bleach abscissae as float).  Models ACF_1D and LINEAR_1D; estimator LSE.
+ * Returns 0 or -1 (message in imfcs_get_last_error), like gpufit(). */
+int imfcs_fit(int n_fits, int n_points, const float *data, const float *weights,
+              int model_id, const float *initial_parameters, float tolerance,
+              int max_n_iterations, int num_valid_coefs,
+              const int *parameters_to_fit, int estimator_id,
+              size_t user_info_size, const void *user_info,
+              float *output_parameters, int *output_states,
+              float *output_chi_squares, int *output_n_iterations);
+
+/* getLastError (GpufitImFCS.java:382, agpufitjni.cu:4643-4647). */
+const char *imfcs_get_last_error(void);
+
+/* isCudaAvailableInt (GpufitImFCS.java:389, agpufitjni.cu:4656-4659):
+ * 1 if a usable GPU is present, else 0. */
+int imfcs_device_available(void);
+
+/* getCudaVersionAsArray (GpufitImFCS.java:403, agpufitjni.cu:4669-4683):
+ * HIP runtime and driver versions as major*1000 + minor*10. */
+int imfcs_get_versions(int *runtime_version, int *driver_version);
+
+/* resetGPU (GpufitImFCS.java:491, agpufitjni.cu:4688-4700): drops every
+ * cached device allocation of this library (a full device reset would
+ * destroy other users' contexts in the same process). */
+int imfcs_reset_gpu(void);
+
+/* resetDevice (GpufitImFCS.java:538): declared by the reference but never
+ * implemented; here a no-op that returns 0. */
+int imfcs_reset_device(void);
+
+/* calcDataBleachCorrection (GpufitImFCS.java:501, agpufitjni.cu:4708-4809):
+ * pixels float [framediff][h_temp][w_temp]; out float [h_temp][w_temp][nopit]
+ * = means of `ave` consecutive frames. */
+int imfcs_calc_data_bleach_correction(const float *pixels, float *outdata,
+                                      const imfcs_acf_params *p);
+
+/* isBinningMemorySufficient (GpufitImFCS.java:510, agpufitjni.cu:4816-4853):
+ * 1 if (win*hin + w_temp*h_temp)*T*4 bytes fit in 90% of free memory. */
+int imfcs_is_binning_memory_sufficient(const imfcs_acf_params *p);
+
+/* calcBinning (GpufitImFCS.java:520, agpufitjni.cu:4860-4967): stride-1
+ * binningX x binningY window sums of a float [T][hin_star][win_star] stack
+ * into float [T][h_temp][w_temp]. */
+int imfcs_calc_binning(const float *indata, float *outdata, const imfcs_acf_params *p);
+
+/* isACFmemorySufficient (GpufitImFCS.java:531, agpufitjni.cu:4974-5092). */
+int imfcs_is_acf_memory_sufficient(const imfcs_acf_params *p);
+
+/* calcACF (GpufitImFCS.java:533-536, agpufitjni.cu:5364-5831).
+ * pixels: float [framediff][h_temp][w_temp], background already removed.
+ * Outputs (double, plane-major [z][height][width]): pixels1 = G(z),
+ * blockvararray = blocked variance, blocked1D plane 0 = blocking index,
+ * plane 1 = blocking success flag.  NBmean/NBcov ([height][width]) are
+ * written when isNBcalculation.  bleachcorr_params [(y*w_temp+x)*order+i]
+ * is read when bleachcorr_gpu (polynomial in seconds, as fitted by the
+ * LINEAR_1D model).  samp/lag: chanum-entry correlator tables. */
+int imfcs_calc_acf(const float *pixels, double *pixels1, double *blockvararray,
+                   double *NBmean, double *NBcov, double *blocked1D,
+                   const double *bleachcorr_params, const double *samp,
+                   const int *lag, const imfcs_acf_params *p);
+
+/* ------------------------------------------------------------------ */
+/* 2. Native pipeline (device-resident uint16 stacks)                  */
+/* ------------------------------------------------------------------ */
+
+typedef struct imfcs_ctx imfcs_ctx;
+
+/* Fit geometry of the FCS model in SI units (metres, seconds), as
+ * prepareFit / GPU_ACF_Fit set it up (Imaging_FCS_1_52.java:9986-10039,
+ * 15090-15110).  paramfit/paraminit follow the 11 fit parameters
+ * N, D, vx, vy, G, F2, D2, F3, D3, Ftrip, Ttrip. */
+typedef struct imfcs_fit_model {
+    double ax, ay, s, sz, rx, ry, fitobsvol, q2, q3;
+    int32_t paramfit[11];
+    double paraminit[11];
+} imfcs_fit_model;
+
+typedef struct imfcs_config {
+    int32_t width, height, frames;   /* stack dims W, H, T */
+    int32_t firstframe, lastframe;   /* 1-based inclusive frame range */
+    int32_t binning_x, binning_y;    /* b x b window sums */
+    int32_t pixbin_x, pixbin_y;      /* output stride (== binning unless overlap) */
+    int32_t cf_x, cf_y;              /* CCF partner offset in raw pixels */
+    int32_t background;              /* subtracted from every raw pixel */
+    int32_t correlator_p, correlator_q;
+    double frametime;                /* seconds */
+    int32_t poly_order;              /* polynomial bleach order, 0 = none */
+    int32_t roi_x, roi_y, roi_w, roi_h;
+    int32_t do_acf, do_ccf;          /* which correlation pairs */
+    int32_t do_fit;                  /* LM fit of every computed pair */
+    int32_t fit_max_iterations;      /* 2000 in the plugin */
+    float fit_tolerance;             /* relative chi^2 change for convergence */
+    imfcs_fit_model acf_model, ccf_model;
+} imfcs_config;
+
+/* Output geometry for a config: out_w x out_h pixels, chanum channels. */
+int imfcs_plan(const imfcs_config *cfg, int32_t *out_w, int32_t *out_h, int32_t *chanum);
+
+imfcs_ctx *imfcs_create(int device);
+void imfcs_destroy(imfcs_ctx *ctx);
+
+/* Device buffer helpers (so hosts need no other GPU runtime). */
+void *imfcs_device_alloc(imfcs_ctx *ctx, size_t bytes);
+int imfcs_device_free(imfcs_ctx *ctx, void *ptr);
+int imfcs_memcpy_h2d(imfcs_ctx *ctx, void *dst, const void *src, size_t bytes);
+int imfcs_memcpy_d2h(imfcs_ctx *ctx, void *dst, const void *src, size_t bytes);
+int imfcs_synchronize(imfcs_ctx *ctx);
+
+/* Runs the pipeline on a device-resident uint16 stack [T][H][W].  Results
+ * stay on the device inside ctx until imfcs_get_results.  Asynchronous
+ * on the context's stream. */
+int imfcs_run(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_config *cfg);
+
+/* Copies the results of the last imfcs_run to host arrays (any may be NULL).
+ * pair 0 = ACF, pair 1 = CCF (when both computed, ACF first).
+ * acf/var: [pair][out_h][out_w][chanum]; ind/blocked: [pair][out_h][out_w];
+ * fitres: [pair][out_h][out_w][11] (SI units); chi2: [pair][pixel];
+ * states/iters: [pair][pixel]; coeffs: [series][poly_order+1] bleach
+ * polynomial in seconds (series = pixels, then CCF partners). */
+int imfcs_get_results(imfcs_ctx *ctx, double *acf, double *var, int32_t *ind,
+                      int32_t *blocked, double *fitres, double *chi2,
+                      int32_t *states, int32_t *iters, double *coeffs);
+
+/* Per-stage device time of the last imfcs_run in milliseconds:
+ * [0] trace+polyfit, [1] means, [2] blocking, [3] correlation, [4] fit. */
+int imfcs_get_stage_ms(imfcs_ctx *ctx, float *stage_ms5);
+
+/* Seeded 2D particle simulator into a host uint16 [T][H][W] buffer
+ * (physics of the plugin's simulateACF2D, Imaging_FCS_1_52.java:13963-14393).
+ * Lives in libimfcs_sim.so. D_um2s: 4 quadrant diffusion coefficients. */
+int imfcs_simulate_2d(uint16_t *out, int W, int H, int T, int margin, int n_emitters,
+                      double pixel_um, double psf_sigma_um, double frametime_s,
+                      double counts_per_frame, int offset, double tau_bleach_s,
+                      const double *D_um2s, uint64_t seed, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
