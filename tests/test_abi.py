@@ -1,0 +1,109 @@
+"""The drop-in boundary without compute: the C-ABI library loads, exports
+every symbol include/imfcs.h declares and the 11 JNI natives of
+GpufitImFCS.java, and reports errors the way the reference does.  Runs on
+the CPU (no kernel is launched)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from imaging_fcs_1_52_amd import native
+from fake_jni import ACF_FIELDS, FakeEnv
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "imfcs.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(imfcs_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_every_declared_symbol_is_exported():
+    hip = native.lib()
+    sim = native.sim_lib()
+    names = _declared()
+    assert "imfcs_calc_acf" in names and "imfcs_fit" in names
+    for n in names:
+        target = sim if n == "imfcs_simulate_2d" else hip
+        assert hasattr(target, n), n
+    assert set(names) - {"imfcs_simulate_2d"} == set(native.SIGNATURES)
+
+
+def test_jni_natives_exported():
+    hip = native.lib()
+    for m in native.JNI_EXPORTS:
+        assert hasattr(hip, "Java_gpufitImFCS_GpufitImFCS_" + m), m
+
+
+def test_fit_rejects_bad_arguments_with_status_and_message():
+    L = native.lib()
+    rc = L.imfcs_fit(0, 10, None, None, 2, None, ctypes.c_float(1e-4), 10, 0, None, 0, 0, None, None, None, None,
+                     None)
+    assert rc == -1
+    assert b"positive" in L.imfcs_get_last_error()
+
+
+def test_plan_geometry_matches_correlate_roi():
+    import imaging_fcs_1_52_amd as imfcs
+    c = imfcs.make_pipeline_config(128, 128, 50000, binning=4, cf=(4, 0))
+    ow, oh, ch = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    assert native.lib().imfcs_plan(ctypes.byref(c), ctypes.byref(ow), ctypes.byref(oh), ctypes.byref(ch)) == 0
+    # startXmap = 0, endXmap = floor((124 - 4) / 4) = 30 (IFCS:8458-8466)
+    assert (ow.value, oh.value, ch.value) == (31, 32, 73)
+
+
+def test_jni_fit_status_and_last_error():
+    env = FakeEnv()
+    st = env.call("fit", 0, 72, None, None, 2, None, 1e-4, 10, 0, None, 0, 0, None, None, None, None, None,
+                  restype=ctypes.c_int32)
+    assert st == -1
+    h = env.call("getLastError", restype=ctypes.c_void_p)
+    assert "positive" in env.get(h)
+    assert not env.trapped
+
+
+def test_jni_is_cuda_available_matches_c_abi():
+    env = FakeEnv()
+    r = env.call("isCudaAvailableInt", restype=ctypes.c_uint8)
+    assert r == native.lib().imfcs_device_available()
+
+
+def _params(env):
+    return env.acf_parameters(width=4, height=4, win_star=5, hin_star=4, w_temp=5, h_temp=4, pixbinX=1, pixbinY=1,
+                              binningX=1, binningY=1, firstframe=1, lastframe=4096, framediff=4096, cfXDistance=1,
+                              cfYDistance=0, correlatorp=16.0, correlatorq=8.0, frametime=1e-3, background=0,
+                              mtab1=4095.0, mtabchanumminus1=16.0, sampchanumminus1=128.0, chanum=73,
+                              isNBcalculation=False, bleachcorr_gpu=False, bleachcorr_order=0, nopit=1000, ave=4)
+
+
+def test_jni_reads_acf_parameters_by_name_and_type():
+    env = FakeEnv()
+    ok = env.call("isBinningMemorySufficient", _params(env), restype=ctypes.c_uint8)
+    for name, sig in env.field_requests:
+        assert ACF_FIELDS[name] == sig, (name, sig)
+    assert {n for n, _ in env.field_requests} >= {"win_star", "hin_star", "w_temp", "h_temp", "firstframe",
+                                                  "lastframe"}
+    if native.lib().imfcs_device_available():
+        assert ok == 1 and not env.exceptions
+    else:
+        assert ok == 0 and env.exceptions and env.exceptions[0][0] == "java/lang/Exception"
+    assert not env.trapped
+
+
+@pytest.mark.skipif(native.lib().imfcs_device_available() == 1, reason="exercises the no-device error path")
+def test_jni_calc_acf_throws_java_exception_without_device():
+    env = FakeEnv()
+    n = 4 * 4 * 73
+    pixels = env.handle(np.zeros(5 * 4 * 4096, np.float32))
+    outs = [env.handle(np.zeros(n)) for _ in range(3)]
+    nb = [env.handle(np.zeros(16)) for _ in range(2)]
+    bc = env.handle(np.zeros(1))
+    samp = env.handle(np.ones(73))
+    lag = env.handle(np.arange(73, dtype=np.int32))
+    env.call("calcACF", pixels, outs[0], outs[1], nb[0], nb[1], outs[2], bc, samp, lag, _params(env))
+    assert env.exceptions and env.exceptions[0][0] == "java/lang/Exception"
+    assert "device" in env.exceptions[0][1]
+    assert not env.trapped
