@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -172,9 +173,11 @@ struct imfcs_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev[6] = {};
     DevBuf part, coef, ctot, rec, F, bpart, ind, bis, corav, bvar, fdata, fw, fx, finit, fpfit, fout, fstate, fchi2,
-        fiter;
+        fiter, cpart, hpart, l3, ctab;
+    CorrTables host_ct;   // source of the async copy into ctab
     // last run
     int npix = 0, npairs = 0, nseries = 0, chanum = 0, order = 0, ow = 0, oh = 0, did_fit = 0, nfree = 0;
+    int force_v1 = 0;   // IMFCS_CORRELATOR=generic selects the one-thread-per-level kernel
     double chi2_norm = 1.0;
     std::vector<double> coef_scale;   // u-basis -> seconds conversion per power
 };
@@ -228,13 +231,16 @@ imfcs_ctx *imfcs_create(int device) {
         return nullptr;
     }
     for (auto &e : ctx->ev) (void)hipEventCreate(&e);
+    const char *sel = std::getenv("IMFCS_CORRELATOR");
+    ctx->force_v1 = (sel && std::strcmp(sel, "generic") == 0) ? 1 : 0;
     return ctx;
 }
 
 static void release_all(imfcs_ctx *ctx) {
     DevBuf *bufs[] = {&ctx->part, &ctx->coef, &ctx->ctot, &ctx->rec, &ctx->F, &ctx->bpart, &ctx->ind,
                       &ctx->bis, &ctx->corav, &ctx->bvar, &ctx->fdata, &ctx->fw, &ctx->fx, &ctx->finit,
-                      &ctx->fpfit, &ctx->fout, &ctx->fstate, &ctx->fchi2, &ctx->fiter};
+                      &ctx->fpfit, &ctx->fout, &ctx->fstate, &ctx->fchi2, &ctx->fiter, &ctx->cpart,
+                      &ctx->hpart, &ctx->l3, &ctx->ctab};
     for (DevBuf *b : bufs) b->release();
 }
 
@@ -310,6 +316,11 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea, HostTables &ht, CorrTables 
     if (rc) return rc;
     rc = make_corr_tables(ht, g.T, ea.q, ct);
     if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(st));   // host_ct may still feed a previous copy
+    ctx->host_ct = ct;
+    HIP_TRY(ctx->ctab.ensure(sizeof(CorrTables)));
+    HIP_TRY(hipMemcpyAsync(ctx->ctab.p, &ctx->host_ct, sizeof(CorrTables), hipMemcpyHostToDevice, st));
+    const CorrTables *dct = ctx->ctab.as<CorrTables>();
     const int ns = g.nseries, np = ea.pd.npairs, n = ea.order + 1;
     ctx->npairs = np;
     ctx->nseries = ns;
@@ -352,23 +363,34 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea, HostTables &ht, CorrTables 
     HIP_TRY(ctx->ctot.ensure(sizeof(double) * (size_t)npc * ns));
     HIP_TRY(ctx->rec.ensure(sizeof(double) * (size_t)ns * ct.npos));
     HIP_TRY(ctx->F.ensure(sizeof(double) * (size_t)ns * ct.npos));
-    launch_prefix(g, bd, ct, ctx->ctot.as<double>(), ctx->rec.as<double>(), npc, plen, st);
-    launch_prefix_combine(ct, ns, ctx->ctot.as<double>(), ctx->rec.as<double>(), npc, plen, ctx->F.as<double>(), st);
+    launch_prefix(g, bd, ct, dct, ctx->ctot.as<double>(), ctx->rec.as<double>(), npc, plen, st);
+    launch_prefix_combine(dct, ns, ctx->ctot.as<double>(), ctx->rec.as<double>(), npc, plen, ctx->F.as<double>(), st);
     HIP_TRY(hipEventRecord(ctx->ev[2], st));
     // blocking
     const int blen = 1 << (ct.blocknum - 1), nbc = (g.T - 1 + blen - 1) / blen;
     HIP_TRY(ctx->bpart.ensure(sizeof(double) * (size_t)nbc * np * ct.blocknum * 2));
     HIP_TRY(ctx->ind.ensure(sizeof(int32_t) * (size_t)np));
     HIP_TRY(ctx->bis.ensure(sizeof(int32_t) * (size_t)np));
-    launch_blocking(g, bd, ct, ea.pd, ctx->F.as<double>(), ctx->bpart.as<double>(), nbc, blen, st);
-    launch_blocking_finalize(ct, ea.pd, ctx->F.as<double>(), ctx->bpart.as<double>(), nbc, ctx->ind.as<int32_t>(),
-                             ctx->bis.as<int32_t>(), st);
+    launch_blocking(g, bd, dct, ea.pd, ctx->F.as<double>(), ctx->bpart.as<double>(), nbc, blen, st);
+    launch_blocking_finalize(ct, dct, ea.pd, ctx->F.as<double>(), ctx->bpart.as<double>(), nbc,
+                             ctx->ind.as<int32_t>(), ctx->bis.as<int32_t>(), st);
     HIP_TRY(hipEventRecord(ctx->ev[3], st));
     // correlation
     HIP_TRY(ctx->corav.ensure(sizeof(double) * (size_t)np * ct.chanum));
     HIP_TRY(ctx->bvar.ensure(sizeof(double) * (size_t)np * ct.chanum));
-    launch_correlate(g, bd, ct, ea.pd, ctx->F.as<double>(), ctx->ind.as<int32_t>(), ctx->corav.as<double>(),
-                     ctx->bvar.as<double>(), st);
+    if (correlate_fast_supported(ct) && !ctx->force_v1) {
+        const int clen = corr_fast_chunk_len(ct, np);
+        const int nch = (g.T + clen - 1) / clen;
+        HIP_TRY(ctx->cpart.ensure(sizeof(float) * (size_t)nch * np * 33 * 3));
+        HIP_TRY(ctx->hpart.ensure(sizeof(double) * (size_t)np * (ct.chanum - 33) * 3));
+        HIP_TRY(ctx->l3.ensure(sizeof(float) * (size_t)ct.numbin[3] * ns));
+        launch_correlate_fast(g, bd, ct, dct, ea.pd, ctx->F.as<double>(), ctx->ind.as<int32_t>(), ctx->cpart.as<float>(),
+                              ctx->hpart.as<double>(), ctx->l3.as<float>(), clen, ctx->corav.as<double>(),
+                              ctx->bvar.as<double>(), st);
+    } else {
+        launch_correlate(g, bd, ct, dct, ea.pd, ctx->F.as<double>(), ctx->ind.as<int32_t>(),
+                         ctx->corav.as<double>(), ctx->bvar.as<double>(), st);
+    }
     HIP_TRY(hipEventRecord(ctx->ev[4], st));
     HIP_TRY(hipGetLastError());
     return 0;
@@ -481,8 +503,10 @@ extern "C" int imfcs_run(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_co
         HIP_TRY(hipMemcpyAsync(ctx->fx.p, xs.data(), sizeof(float) * npts, hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemcpyAsync(ctx->finit.p, init.data(), sizeof(float) * init.size(), hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemcpyAsync(ctx->fpfit.p, pfit.data(), sizeof(int) * 20, hipMemcpyHostToDevice, st));
+        unsigned mask = 0;
+        for (int k = 0; k < 11; k++) mask |= (pfit[k] ? 1u : 0u) << k;
         launch_lm_acf(np, npts, ctx->fdata.as<float>(), ctx->fw.as<float>(), ctx->fx.as<float>(),
-                      ctx->finit.as<float>(), ctx->fpfit.as<int>(), LMModelGeom{}, c->fit_tolerance,
+                      ctx->finit.as<float>(), ctx->fpfit.as<int>(), mask, LMModelGeom{}, c->fit_tolerance,
                       c->fit_max_iterations, ctx->fout.as<float>(), ctx->fstate.as<int>(), ctx->fchi2.as<float>(),
                       ctx->fiter.as<int>(), st);
         ctx->nfree = nfree;
@@ -764,14 +788,17 @@ extern "C" int imfcs_fit(int n_fits, int n_points, const float *data, const floa
     HIP_TRY(hipMemcpyAsync(ctx->finit.p, initial_parameters, (size_t)n_fits * npar * 4, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(ctx->fpfit.p, parameters_to_fit, npar * sizeof(int), hipMemcpyHostToDevice, st));
     const float *w = weights ? ctx->fw.as<float>() : nullptr;
+    unsigned mask = 0;
+    for (int k = 0; k < npar && k < 32; k++) mask |= (parameters_to_fit[k] ? 1u : 0u) << k;
     if (model_id == IMFCS_MODEL_ACF_1D)
         launch_lm_acf(n_fits, n_points, ctx->fdata.as<float>(), w, ctx->fx.as<float>(), ctx->finit.as<float>(),
-                      ctx->fpfit.as<int>(), LMModelGeom{}, tolerance, max_n_iterations, ctx->fout.as<float>(),
-                      ctx->fstate.as<int>(), ctx->fchi2.as<float>(), ctx->fiter.as<int>(), st);
+                      ctx->fpfit.as<int>(), mask & 0x7ffu, LMModelGeom{}, tolerance, max_n_iterations,
+                      ctx->fout.as<float>(), ctx->fstate.as<int>(), ctx->fchi2.as<float>(), ctx->fiter.as<int>(), st);
     else
         launch_lm_poly(n_fits, n_points, ctx->fdata.as<float>(), w, ctx->fx.as<float>(), ctx->finit.as<float>(),
-                       ctx->fpfit.as<int>(), num_valid_coefs, tolerance, max_n_iterations, ctx->fout.as<float>(),
-                       ctx->fstate.as<int>(), ctx->fchi2.as<float>(), ctx->fiter.as<int>(), st);
+                       ctx->fpfit.as<int>(), mask & ((1u << num_valid_coefs) - 1u), num_valid_coefs, tolerance,
+                       max_n_iterations, ctx->fout.as<float>(), ctx->fstate.as<int>(), ctx->fchi2.as<float>(),
+                       ctx->fiter.as<int>(), st);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(output_parameters, ctx->fout.p, (size_t)n_fits * npar * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(output_states, ctx->fstate.p, (size_t)n_fits * 4, hipMemcpyDeviceToHost, st));

@@ -1,0 +1,480 @@
+// Multi-tau correlator with blocked variance (calculateCF, IFCS:9425-9521),
+// LDS-staged and wave-specialised for CDNA4.
+//
+// k_corr_main: one workgroup per (64 output pixels, time chunk).  Loader
+//   waves read the uint16 frames once (fused b x b binning), apply the
+//   polynomial bleach correction in FP64, subtract the series mean and stage
+//   FP32 values for 64 frames in LDS, together with the pair-summed level-1
+//   and level-2 values; the level-3 (8-frame) sums go to global memory for
+//   k_corr_high.  Five role waves per correlation type then consume the tile:
+//     role 0..2: level 0, delays 0-5 / 6-11 / 12-16
+//     role 3   : level 1, delays 9-16
+//     role 4   : level 2, delays 9-16
+//   Each lane keeps a 32-value register window of its level and processes
+//   products in groups of 16 a-indices; block sums of the blocked variance
+//   (2^(ind-level) products, multiples of 16) are closed at group ends.
+// k_corr_high: levels 3..q-1 from the level-3 series, one thread per pair.
+// k_corr_finalize: combines the per-chunk sums into G and the variance.
+//
+// With the series shifted by its mean mu, a~ = a - mu, every channel's exact
+// window means enter through alpha = abar - mu_c, beta = bbar - mu_c:
+//   sum_y (a_y - abar)(b_{y+d} - bbar) = sum a~ b~ - n alpha beta
+//   block sum S_j = Sab - beta Sa - alpha Sb + B alpha beta
+#include "imfcs_device.h"
+#include "imfcs_internal.h"
+
+namespace imfcs {
+
+constexpr int kTileF = 64;     // frames per LDS tile
+constexpr int kMainChan = 33;  // channels of levels 0..2 (p = 16)
+constexpr int kNS = 128;       // staged series per tile (a and partner b)
+
+struct MainArgs {
+    SeriesGeom g;
+    BleachDesc bd;
+    const CorrTables *ctp;
+    PairDesc pd;
+    const double *F;
+    const int32_t *ind;
+    int chunk_len, nchunk;
+    int has_acf, has_ccf;
+    float *part;    // [chunk][pair][kMainChan][3] : cross, A1, A2
+    float *l3;      // [numbin3][nseries]
+};
+
+struct Tile {
+    float l0[kTileF][kNS];
+    float l1[kTileF / 2][kNS];
+    float l2[kTileF / 4][kNS];
+    double coef[kNS][kMaxOrder + 2];   // normalized bleach polynomial, c0, mean
+};
+
+// Per-lane loader: one series, 16 consecutive frames of every tile.  The
+// series' bleach polynomial and mean live in LDS so they are not held in
+// registers across the role loops.
+struct Loader {
+    bool active;
+    int sl, s, fb, ox, oy, order;
+    double scale, off;
+    __device__ void init(const MainArgs &A, Tile &tl, int wave, int lane, int p0, int nloader_waves) {
+        active = wave < nloader_waves;
+        int half = (nloader_waves == 8) ? (wave & 1) : 0;
+        fb = (nloader_waves == 8) ? (wave >> 1) : wave;
+        sl = half * 64 + lane;
+        int p = p0 + lane;
+        if (p >= A.g.npix) p = A.g.npix - 1;   // padding lanes duplicate the last pixel
+        s = half ? A.g.npix + p : p;
+        ox = oy = 0;
+        order = A.bd.order;
+        scale = A.bd.frametime / A.bd.tscale;
+        off = A.bd.t_off;
+        if (active) {
+            series_origin(A.g, s, ox, oy);
+            if (fb == 0) {
+                double c0 = (order > 0) ? A.bd.coef[(int64_t)s * (order + 1)] : 1.0;
+                for (int k = 0; k <= kMaxOrder; k++)
+                    tl.coef[sl][k] = (k <= order && order > 0) ? A.bd.coef[(int64_t)s * (order + 1) + k] / c0 : 0.0;
+                tl.coef[sl][kMaxOrder + 1] = A.F[(int64_t)s * A.ctp->npos + A.ctp->pos_total] / A.g.T;
+            }
+        }
+    }
+    __device__ __forceinline__ double corrected(const MainArgs &A, const double *cf, double c0, double mu, int f) const {
+        double v = raw_value(A.g, ox, oy, f);
+        if (order > 0) {
+            double u = ((double)f + off) * scale, g = 0.0;
+#pragma unroll
+            for (int k = kMaxOrder; k >= 0; k--)
+                if (k <= order) g = fma(g, u, cf[k]);
+            double r = (double)rsqrtf((float)g);
+            r = r * fma(-0.5 * g * r, r, 1.5);
+            v = fma(v, r, c0 * (1.0 - g * r));
+        }
+        return v - mu;
+    }
+    __device__ void load(const MainArgs &A, Tile &tl, int f0, int chunk_end, bool own) {
+        if (!active) return;
+        const int T = A.g.T;
+        const int fbase = f0 + 16 * fb;
+        double cf[kMaxOrder + 1];
+#pragma unroll
+        for (int k = 0; k <= kMaxOrder; k++) cf[k] = (k <= order) ? tl.coef[sl][k] : 0.0;
+        const double c0 = (order > 0) ? A.bd.coef[(int64_t)s * (order + 1)] : 1.0;
+        const double mu = tl.coef[sl][kMaxOrder + 1];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            double v[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                int f = fbase + 8 * h + k;
+                v[k] = (f < T) ? corrected(A, cf, c0, mu, f) : 0.0;
+                tl.l0[16 * fb + 8 * h + k][sl] = (float)v[k];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                tl.l1[8 * fb + 4 * h + k][sl] = (fbase + 8 * h + 2 * k + 1 < T) ? (float)(v[2 * k] + v[2 * k + 1]) : 0.f;
+#pragma unroll
+            for (int k = 0; k < 2; k++)
+                tl.l2[4 * fb + 2 * h + k][sl] =
+                    (fbase + 8 * h + 4 * k + 3 < T) ? (float)((v[4 * k] + v[4 * k + 1]) + (v[4 * k + 2] + v[4 * k + 3]))
+                                                    : 0.f;
+            if (own && fbase < chunk_end) {
+                int y3 = fbase / 8 + h;
+                if (y3 < A.ctp->numbin[3]) {
+                    double s8 = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+                    A.l3[(int64_t)y3 * A.g.nseries + s] = (float)s8;
+                }
+            }
+        }
+    }
+};
+
+// Per-lag sums of one chunk.  A chunk spans at most 2^(blocknum-1) frames,
+// i.e. at most 32 blocks of the blocked variance, so FP32 sums stay within
+// ~1e-6 relative; chunks are combined in FP64 by k_corr_finalize.
+template <int NL>
+struct LagAcc {
+    float sab[NL], sb[NL], alpha[NL], beta[NL];
+    float a1[NL], a2[NL], cross[NL];
+};
+
+// One role: NL delays starting at DMIN on level LEVEL (group size 16).
+template <int NL, int DMIN, int LEVEL, bool CCF>
+__device__ void run_role(const MainArgs &A, Tile &tl, Loader &ld, int lane, int p0, int chunk) {
+    constexpr int type = CCF ? 1 : 0;
+    constexpr int GPT = kTileF / (16 << LEVEL);   // groups per tile
+    const CorrTables &ct = *A.ctp;
+    const int p = p0 + lane;
+    const bool live = (p < A.g.npix) && (type == 0 ? A.has_acf : A.has_ccf);
+    constexpr bool ccf = CCF;
+    const int pair = type == 0 ? p : (A.has_acf ? A.g.npix + p : p);
+    const int sa = p < A.g.npix ? p : A.g.npix - 1;
+    const int sb = ccf ? A.g.npix + sa : sa;
+    const int asl = lane, bsl = ccf ? 64 + lane : lane;
+    LagAcc<NL> acc;
+    float sa_blk = 0.f;
+    int B = 1, pnum = 0;
+    {
+        const double *Fa = A.F + (int64_t)sa * ct.npos, *Fb = A.F + (int64_t)sb * ct.npos;
+        double mua = Fa[ct.pos_total] / A.g.T * (double)(1 << LEVEL);
+        double mub = Fb[ct.pos_total] / A.g.T * (double)(1 << LEVEL);
+        const int x0 = ct.first_chan[LEVEL] + (LEVEL == 0 ? DMIN : DMIN - 9);
+#pragma unroll
+        for (int i = 0; i < NL; i++) {
+            int x = x0 + i;
+            double n = (double)(ct.numbin[LEVEL] - ct.del[x]);
+            acc.alpha[i] = (float)(Fa[ct.pos_a_end[x]] / n - mua);
+            acc.beta[i] = (float)((Fb[ct.pos_b_end[x]] - Fb[ct.pos_b_beg[x]]) / n - mub);
+            acc.sab[i] = acc.sb[i] = 0.f;
+            acc.a1[i] = acc.a2[i] = acc.cross[i] = 0.f;
+        }
+        int indp = live ? A.ind[pair] : ct.q - 1;
+        B = 1 << (indp - LEVEL);
+        pnum = (int)floor(ct.mtab_last / pow(2.0, fmax(indp - ct.log2_samp_last, 0.0)));
+    }
+    const int cstart = chunk * A.chunk_len;
+    const int cend = min(A.g.T, cstart + A.chunk_len);
+    const bool last_chunk = (chunk == A.nchunk - 1);
+    const int ystart = cstart >> LEVEL;
+    const int yend = last_chunk ? ct.numbin[LEVEL] : (cend >> LEVEL);
+    const int ntiles = (cend - cstart + kTileF - 1) / kTileF + 2;
+    float w[32], ga_cur[CCF ? 16 : 1], ga_nxt[CCF ? 16 : 1];
+#pragma unroll
+    for (int k = 0; k < 32; k++) w[k] = 0.f;
+    if (CCF) {
+#pragma unroll
+        for (int k = 0; k < 16; k++) ga_cur[k] = ga_nxt[k] = 0.f;
+    }
+    int ynext = ystart;   // level index of the next value to load
+    for (int t = 0; t < ntiles; t++) {
+        const int f0 = cstart + t * kTileF;
+        ld.load(A, tl, f0, cend, true);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < GPT; q++) {
+            // shift in 16 new level values
+#pragma unroll
+            for (int k = 0; k < 16; k++) {
+                w[k] = w[k + 16];
+                if (CCF) ga_cur[k] = ga_nxt[k];
+            }
+#pragma unroll
+            for (int k = 0; k < 16; k++) {
+                const float *row = (LEVEL == 0) ? tl.l0[16 * q + k] : (LEVEL == 1) ? tl.l1[16 * q + k] : tl.l2[16 * q + k];
+                w[16 + k] = row[bsl];
+                if (CCF) ga_nxt[k] = row[asl];
+            }
+            ynext += 16;
+            const int y0 = ynext - 32;   // group whose window is complete
+            if (live && y0 >= ystart && y0 < yend) {
+                float gs = 0.f;
+                const float *ga = CCF ? ga_cur : w;
+#pragma unroll
+                for (int k = 0; k < 16; k++) gs += ga[k];
+                sa_blk += gs;
+                float tsum = 0.f;
+#pragma unroll
+                for (int k = 0; k < 16; k++) tsum += w[k + DMIN];
+#pragma unroll
+                for (int i = 0; i < NL; i++) {
+                    const int d = DMIN + i;
+                    if (i > 0) tsum = tsum - w[d - 1] + w[15 + d];
+                    float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+                    for (int k = 0; k < 16; k += 2) {
+                        s0 = fmaf(ga[k], w[k + d], s0);
+                        s1 = fmaf(ga[k + 1], w[k + 1 + d], s1);
+                    }
+                    acc.sab[i] += s0 + s1;
+                    acc.sb[i] += tsum;
+                }
+                const int ye = y0 + 16;
+                if ((ye & (B - 1)) == 0) {
+                    const bool inblk = (ye / B - 1) < pnum;
+#pragma unroll
+                    for (int i = 0; i < NL; i++) {
+                        float S = acc.sab[i] - acc.beta[i] * sa_blk - acc.alpha[i] * acc.sb[i] +
+                                  (float)B * acc.alpha[i] * acc.beta[i];
+                        if (inblk) {
+                            acc.a1[i] += S;
+                            acc.a2[i] = fmaf(S, S, acc.a2[i]);
+                        }
+                        acc.cross[i] += acc.sab[i];
+                        acc.sab[i] = 0.f;
+                        acc.sb[i] = 0.f;
+                    }
+                    sa_blk = 0.f;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (live) {
+        const int x0 = ct.first_chan[LEVEL] + (LEVEL == 0 ? DMIN : DMIN - 9);
+        float *out = A.part + ((int64_t)chunk * A.pd.npairs + pair) * kMainChan * 3;
+#pragma unroll
+        for (int i = 0; i < NL; i++) {
+            out[(x0 + i) * 3 + 0] = acc.cross[i] + acc.sab[i];
+            out[(x0 + i) * 3 + 1] = acc.a1[i];
+            out[(x0 + i) * 3 + 2] = acc.a2[i];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(640) k_corr_main(MainArgs A) {
+    __shared__ Tile tl;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int p0 = blockIdx.x * 64;
+    const int chunk = blockIdx.y;
+    const int nwaves = blockDim.x >> 6;
+    const int nloader = (nwaves == 10) ? 8 : 4;
+    Loader ld;
+    ld.init(A, tl, wave, lane, p0, nloader);
+    __syncthreads();
+    switch (wave) {
+        case 0: run_role<6, 0, 0, false>(A, tl, ld, lane, p0, chunk); break;
+        case 1: run_role<6, 6, 0, false>(A, tl, ld, lane, p0, chunk); break;
+        case 2: run_role<5, 12, 0, false>(A, tl, ld, lane, p0, chunk); break;
+        case 3: run_role<8, 9, 1, false>(A, tl, ld, lane, p0, chunk); break;
+        case 4: run_role<8, 9, 2, false>(A, tl, ld, lane, p0, chunk); break;
+        case 5: run_role<6, 0, 0, true>(A, tl, ld, lane, p0, chunk); break;
+        case 6: run_role<6, 6, 0, true>(A, tl, ld, lane, p0, chunk); break;
+        case 7: run_role<5, 12, 0, true>(A, tl, ld, lane, p0, chunk); break;
+        case 8: run_role<8, 9, 1, true>(A, tl, ld, lane, p0, chunk); break;
+        default: run_role<8, 9, 2, true>(A, tl, ld, lane, p0, chunk); break;
+    }
+}
+
+// ------------------------------------------------------------------
+// Levels 3..q-1 from the level-3 series.
+// ------------------------------------------------------------------
+struct L3Ctx {
+    const float *l3;
+    int nseries, sa, sb, c, numbin;
+    __device__ __forceinline__ double val(int s, int y) const {
+        if (y >= numbin) return 0.0;
+        const int r = 1 << (c - 3);
+        double acc = 0.0;
+        for (int i = 0; i < r; i++) acc += (double)l3[(int64_t)(y * r + i) * nseries + s];
+        return acc;
+    }
+};
+
+// Levels >= 3 hold few, large (rebinned) values: FP64 throughout.
+template <int G>
+__device__ void high_level(const L3Ctx &L, bool same, const double *alpha, const double *beta, int B, int pnum,
+                           double *cross, double *a1, double *a2) {
+    constexpr int WN = G + 16;
+    double w[WN], ga[G], sab[8], sb[8];
+    double sa = 0.0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        sab[i] = sb[i] = 0.0;
+        cross[i] = a1[i] = a2[i] = 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < WN; k++) w[k] = L.val(L.sb, k);
+    for (int y0 = 0; y0 < L.numbin; y0 += G) {
+#pragma unroll
+        for (int k = 0; k < G; k++) ga[k] = same ? w[k] : L.val(L.sa, y0 + k);
+        double gs = 0.0;
+#pragma unroll
+        for (int k = 0; k < G; k++) gs += ga[k];
+        sa += gs;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            double sv = 0.0, t = 0.0;
+#pragma unroll
+            for (int k = 0; k < G; k++) {
+                sv = fma(ga[k], w[k + 9 + i], sv);
+                t += w[k + 9 + i];
+            }
+            sab[i] += sv;
+            sb[i] += t;
+        }
+        const int ye = y0 + G;
+        if ((ye & (B - 1)) == 0) {
+            const bool inblk = (ye / B - 1) < pnum;
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                double S = sab[i] - beta[i] * sa - alpha[i] * sb[i] + (double)B * alpha[i] * beta[i];
+                if (inblk) {
+                    a1[i] += S;
+                    a2[i] += S * S;
+                }
+                cross[i] += sab[i];
+                sab[i] = sb[i] = 0.0;
+            }
+            sa = 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 16; k++) w[k] = w[k + G];
+#pragma unroll
+        for (int k = 0; k < G; k++) w[16 + k] = L.val(L.sb, y0 + WN + k);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) cross[i] += sab[i];
+}
+
+__global__ void __launch_bounds__(64) k_corr_high(MainArgs A, double *hpart /* [pair][nhigh][3] */, int nhigh) {
+    const int pidx = blockIdx.x * 64 + threadIdx.x;
+    const int c = 3 + blockIdx.y;
+    const CorrTables &ct = *A.ctp;
+    if (pidx >= A.pd.npairs || c >= ct.nlevels) return;
+    int sa, sb;
+    pair_series(A.pd, pidx, sa, sb);
+    L3Ctx L{A.l3, A.g.nseries, sa, sb, c, ct.numbin[c]};
+    const double *Fa = A.F + (int64_t)sa * ct.npos, *Fb = A.F + (int64_t)sb * ct.npos;
+    const double mua = Fa[ct.pos_total] / A.g.T * (double)(1 << c);
+    const double mub = Fb[ct.pos_total] / A.g.T * (double)(1 << c);
+    const int x0 = ct.first_chan[c];
+    double alpha[8], beta[8];
+    for (int i = 0; i < 8; i++) {
+        int x = x0 + i;
+        double n = (double)(ct.numbin[c] - ct.del[x]);
+        alpha[i] = Fa[ct.pos_a_end[x]] / n - mua;
+        beta[i] = (Fb[ct.pos_b_end[x]] - Fb[ct.pos_b_beg[x]]) / n - mub;
+    }
+    const int indp = A.ind[pidx];
+    const int B = 1 << (indp - c);
+    const int pnum = (int)floor(ct.mtab_last / pow(2.0, fmax(indp - ct.log2_samp_last, 0.0)));
+    double cross[8], a1[8], a2[8];
+    int G = min(16, 1 << max(ct.q - 1 - c, 0));
+    switch (G) {
+        case 16: high_level<16>(L, sa == sb, alpha, beta, B, pnum, cross, a1, a2); break;
+        case 8: high_level<8>(L, sa == sb, alpha, beta, B, pnum, cross, a1, a2); break;
+        case 4: high_level<4>(L, sa == sb, alpha, beta, B, pnum, cross, a1, a2); break;
+        case 2: high_level<2>(L, sa == sb, alpha, beta, B, pnum, cross, a1, a2); break;
+        default: high_level<1>(L, sa == sb, alpha, beta, B, pnum, cross, a1, a2); break;
+    }
+    double *out = hpart + ((int64_t)pidx * nhigh + (x0 - kMainChan)) * 3;
+    for (int i = 0; i < 8; i++) {
+        out[i * 3 + 0] = cross[i];
+        out[i * 3 + 1] = a1[i];
+        out[i * 3 + 2] = a2[i];
+    }
+}
+
+// G and blocked variance of every channel from the accumulated sums.
+__global__ void k_corr_finalize(MainArgs A, const double *hpart, int nhigh, double *corav, double *bvar) {
+    const int pidx = blockIdx.x;
+    const int x = threadIdx.x;
+    const CorrTables &ct = *A.ctp;
+    if (pidx >= A.pd.npairs || x >= ct.chanum) return;
+    int sa, sb;
+    pair_series(A.pd, pidx, sa, sb);
+    const int c = ct.level[x];
+    const double *Fa = A.F + (int64_t)sa * ct.npos, *Fb = A.F + (int64_t)sb * ct.npos;
+    const double n = (double)(ct.numbin[c] - ct.del[x]);
+    const double am = Fa[ct.pos_a_end[x]] / n;
+    const double bm = (Fb[ct.pos_b_end[x]] - Fb[ct.pos_b_beg[x]]) / n;
+    const double mua = Fa[ct.pos_total] / A.g.T * (double)(1 << c);
+    const double mub = Fb[ct.pos_total] / A.g.T * (double)(1 << c);
+    const double alpha = (double)(float)(am - mua), beta = (double)(float)(bm - mub);
+    double cross = 0.0, a1 = 0.0, a2 = 0.0;
+    if (x < kMainChan) {
+        for (int k = 0; k < A.nchunk; k++) {
+            const float *o = A.part + (((int64_t)k * A.pd.npairs + pidx) * kMainChan + x) * 3;
+            cross += (double)o[0];
+            a1 += (double)o[1];
+            a2 += (double)o[2];
+        }
+    } else {
+        const double *o = hpart + ((int64_t)pidx * nhigh + (x - kMainChan)) * 3;
+        cross = o[0];
+        a1 = o[1];
+        a2 = o[2];
+    }
+    const int indp = A.ind[pidx];
+    const double B = (double)(1 << (indp - c));
+    const double pn = floor(ct.mtab_last / pow(2.0, fmax(indp - ct.log2_samp_last, 0.0)));
+    // sum_y (a_y - abar)(b_{y+d} - bbar) = sum a~ b~ - n alpha beta, with the
+    // alpha, beta values (FP32-rounded) used inside the block sums
+    const double tot = cross - n * (am - mua) * (bm - mub);
+    corav[(int64_t)pidx * ct.chanum + x] = tot / (n * am * bm);
+    (void)alpha;
+    (void)beta;
+    const double sp = a1 / B, sp2 = a2 / (B * B), ab = am * bm;
+    bvar[(int64_t)pidx * ct.chanum + x] = (sp2 / pn - (sp / pn) * (sp / pn)) / ((pn - 1) * (ab * ab));
+}
+
+bool correlate_fast_supported(const CorrTables &ct) {
+    // level layout of p = 16 with at least levels 0..3 and blocks >= 16
+    // values on level 2 (ind >= q - 1 >= 6)
+    return ct.nlevels >= 7 && ct.nlevels <= 12 && ct.q == ct.nlevels && ct.numbin[3] >= 32;
+}
+
+int corr_fast_chunk_len(const CorrTables &ct, int npairs) {
+    // one chunk per largest blocking block: boundaries are multiples of
+    // every block size and a chunk holds at most 32 blocks of 2^ind frames
+    (void)npairs;
+    return 1 << (ct.blocknum - 1);
+}
+
+void launch_correlate_fast(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const CorrTables *dct,
+                           const PairDesc &pd,
+                           const double *F, const int32_t *ind, float *part, double *hpart, float *l3,
+                           int chunk_len, double *corav, double *bvar, hipStream_t st) {
+    MainArgs A;
+    A.g = g;
+    A.bd = bd;
+    A.ctp = dct;
+    A.pd = pd;
+    A.F = F;
+    A.ind = ind;
+    A.chunk_len = chunk_len;
+    A.nchunk = (g.T + chunk_len - 1) / chunk_len;
+    A.has_acf = pd.acf_first;
+    A.has_ccf = pd.has_ccf;
+    A.part = part;
+    A.l3 = l3;
+    const int tiles = (g.npix + 63) / 64;
+    const int threads = pd.has_ccf ? 640 : 320;
+    hipLaunchKernelGGL(k_corr_main, dim3(tiles, A.nchunk), dim3(threads), 0, st, A);
+    const int nhigh = ct.chanum - kMainChan;
+    hipLaunchKernelGGL(k_corr_high, dim3((pd.npairs + 63) / 64, ct.nlevels - 3), dim3(64), 0, st, A, hpart, nhigh);
+    hipLaunchKernelGGL(k_corr_finalize, dim3(pd.npairs), dim3(128), 0, st, A, (const double *)hpart, nhigh, corav,
+                       bvar);
+}
+
+}  // namespace imfcs

@@ -88,21 +88,28 @@ void launch_trace_moments(const SeriesGeom &g, int nopit, int ave, double tscale
                           int order, int integer_trace, double *part, int nchunk, hipStream_t st);
 void launch_polyfit(const double *part, int nchunk, int nseries, int nopit, int ave, double tscale,
                     double frametime, int order, double *coef, hipStream_t st);
-void launch_prefix(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, double *ctot,
-                   double *rec, int nchunk, int chunk_len, hipStream_t st);
-void launch_prefix_combine(const CorrTables &ct, int nseries, const double *ctot, const double *rec,
+void launch_prefix(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const CorrTables *dct,
+                   double *ctot, double *rec, int nchunk, int chunk_len, hipStream_t st);
+void launch_prefix_combine(const CorrTables *dct, int nseries, const double *ctot, const double *rec,
                            int nchunk, int chunk_len, double *F, hipStream_t st);
-void launch_blocking(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const PairDesc &pd,
+void launch_blocking(const SeriesGeom &g, const BleachDesc &bd, const CorrTables *dct, const PairDesc &pd,
                      const double *F, double *bpart, int nchunk, int chunk_len, hipStream_t st);
-void launch_blocking_finalize(const CorrTables &ct, const PairDesc &pd, const double *F, const double *bpart,
-                              int nchunk, int32_t *ind, int32_t *bis, hipStream_t st);
-void launch_correlate(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const PairDesc &pd,
-                      const double *F, const int32_t *ind, double *corav, double *bvar, hipStream_t st);
+void launch_blocking_finalize(const CorrTables &ct, const CorrTables *dct, const PairDesc &pd, const double *F,
+                              const double *bpart, int nchunk, int32_t *ind, int32_t *bis, hipStream_t st);
+void launch_correlate(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const CorrTables *dct,
+                      const PairDesc &pd, const double *F, const int32_t *ind, double *corav, double *bvar,
+                      hipStream_t st);
+bool correlate_fast_supported(const CorrTables &ct);
+int corr_fast_chunk_len(const CorrTables &ct, int npairs);
+void launch_correlate_fast(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const CorrTables *dct,
+                           const PairDesc &pd,
+                           const double *F, const int32_t *ind, float *part, double *hpart, float *l3,
+                           int chunk_len, double *corav, double *bvar, hipStream_t st);
 void launch_lm_acf(int nfits, int npts, const float *data, const float *weights, const float *x,
-                   const float *init, const int *pfit, const LMModelGeom &geom, float tol, int maxit,
+                   const float *init, const int *pfit, unsigned free_mask, const LMModelGeom &geom, float tol, int maxit,
                    float *out_params, int *states, float *chi2, int *iters, hipStream_t st);
 void launch_lm_poly(int nfits, int npts, const float *data, const float *weights, const float *x,
-                    const float *init, const int *pfit, int ncoef, float tol, int maxit,
+                    const float *init, const int *pfit, unsigned free_mask, int ncoef, float tol, int maxit,
                     float *out_params, int *states, float *chi2, int *iters, hipStream_t st);
 void launch_binning_f32(const float *in, float *out, int win, int hin, int wt, int ht, int T, int bx, int by,
                         hipStream_t st);

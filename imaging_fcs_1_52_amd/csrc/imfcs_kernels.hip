@@ -19,68 +19,11 @@
 // evaluated in FP32 and its normal equations are accumulated and solved in
 // FP64.
 #include "imfcs_internal.h"
+#include "imfcs_device.h"
 
 #include <math.h>
 
 namespace imfcs {
-
-// ------------------------------------------------------------------
-// Series access
-// ------------------------------------------------------------------
-__device__ __forceinline__ void series_origin(const SeriesGeom &g, int s, int &ox, int &oy) {
-    int k = s < g.npix ? s : s - g.npix;
-    int x = k % g.ow, y = k / g.ow;
-    ox = g.ox0 + x * g.pbx;
-    oy = g.oy0 + y * g.pby;
-    if (s >= g.npix) {
-        ox += g.cfx;
-        oy += g.cfy;
-    }
-}
-
-// Exact integer window sum minus background (IFCS:9668-9678).
-__device__ __forceinline__ int raw_u16(const SeriesGeom &g, int ox, int oy, int f) {
-    const uint16_t *base = (const uint16_t *)g.data + (int64_t)(g.f0 + f) * g.plane +
-                           (int64_t)oy * g.stride + ox;
-    if (g.bx == 1 && g.by == 1) return (int)base[0] - g.bckg_total;
-    int acc = 0;
-    for (int k = 0; k < g.by; k++)
-        for (int i = 0; i < g.bx; i++) acc += base[(int64_t)k * g.stride + i];
-    return acc - g.bckg_total;
-}
-
-__device__ __forceinline__ double raw_value(const SeriesGeom &g, int ox, int oy, int f) {
-    if (g.src == kSrcU16) return (double)raw_u16(g, ox, oy, f);
-    const float *base = (const float *)g.data + (int64_t)(g.f0 + f) * g.plane + (int64_t)oy * g.stride + ox;
-    return (double)base[0];
-}
-
-// Per-series bleach polynomial held in registers.
-struct Bleach {
-    double c[kMaxOrder + 1];
-    int order;
-    double c0, scale;   // u = (f + off) * scale
-    double off;
-    __device__ void load(const BleachDesc &bd, int s) {
-        order = bd.order;
-#pragma unroll
-        for (int k = 0; k <= kMaxOrder; k++) c[k] = (k <= order && order > 0) ? bd.coef[(int64_t)s * (order + 1) + k] : 0.0;
-        c0 = c[0];
-        scale = bd.frametime / bd.tscale;
-        off = bd.t_off;
-    }
-    // I' = I / sqrt(f/c0) + c0 (1 - sqrt(f/c0))   (IFCS:9744-9750)
-    __device__ __forceinline__ double apply(double v, int f) const {
-        if (order == 0) return v;
-        double u = ((double)f + off) * scale;
-        double fu = 0.0;
-#pragma unroll
-        for (int k = kMaxOrder; k >= 0; k--)
-            if (k <= order) fu = fma(fu, u, c[k]);
-        double sq = sqrt(fu / c0);
-        return v / sq + c0 * (1.0 - sq);
-    }
-};
 
 // ------------------------------------------------------------------
 // Stage 1: intensity trace moments and polynomial fit
@@ -167,7 +110,9 @@ __global__ void k_polyfit(const double *part, int nchunk, int nseries, int order
 // Stage 2: prefix sums of the corrected series at the positions that
 // define every channel's window means.
 // ------------------------------------------------------------------
-__global__ void k_prefix(SeriesGeom g, BleachDesc bd, CorrTables ct, int chunk_len, double *ctot, double *rec) {
+__global__ void k_prefix(SeriesGeom g, BleachDesc bd, const CorrTables *__restrict__ ctp, int chunk_len, double *ctot,
+                         double *rec) {
+    const CorrTables &ct = *ctp;
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     int chunk = blockIdx.y;
     int nchunk = gridDim.y;
@@ -177,13 +122,16 @@ __global__ void k_prefix(SeriesGeom g, BleachDesc bd, CorrTables ct, int chunk_l
     Bleach bl;
     bl.load(bd, s);
     int f0 = chunk * chunk_len, f1 = min(g.T, f0 + chunk_len);
+    const int npos = ct.npos;
     int k = 0;
-    while (k < ct.npos && ct.pos[k] < f0) k++;
+    while (k < npos && ct.pos[k] < f0) k++;
+    int next = (k < npos) ? ct.pos[k] : 0x7fffffff;
     double run = 0.0;
     for (int f = f0; f < f1; f++) {
-        while (k < ct.npos && ct.pos[k] == f) {
-            rec[(int64_t)s * ct.npos + k] = run;
+        while (next == f) {
+            rec[(int64_t)s * npos + k] = run;
             k++;
+            next = (k < npos) ? ct.pos[k] : 0x7fffffff;
         }
         run += bl.apply(raw_value(g, ox, oy, f), f);
     }
@@ -195,8 +143,9 @@ __global__ void k_prefix(SeriesGeom g, BleachDesc bd, CorrTables ct, int chunk_l
     ctot[(int64_t)chunk * g.nseries + s] = run;
 }
 
-__global__ void k_prefix_combine(CorrTables ct, int nseries, const double *ctot, const double *rec, int nchunk,
+__global__ void k_prefix_combine(const CorrTables *__restrict__ ctp, int nseries, const double *ctot, const double *rec, int nchunk,
                                  int chunk_len, double *F) {
+    const CorrTables &ct = *ctp;
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nseries) return;
     double acc = 0.0;
@@ -216,8 +165,9 @@ __global__ void k_prefix_combine(CorrTables ct, int nseries, const double *ctot,
 // 1-based trace into a 0-based array, so the series is a'_y = a[y-1]
 // (a'_0 = 0) against b'_{y+1} = b[y], y = 0..T-2 (IFCS:9259-9262).
 // ------------------------------------------------------------------
-__global__ void k_blocking(SeriesGeom g, BleachDesc bd, CorrTables ct, PairDesc pd, const double *F, double *bpart,
-                           int chunk_len) {
+__global__ void k_blocking(SeriesGeom g, BleachDesc bd, const CorrTables *__restrict__ ctp, PairDesc pd,
+                           const double *F, double *bpart, int chunk_len) {
+    const CorrTables &ct = *ctp;
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     int chunk = blockIdx.y;
     if (p >= pd.npairs) return;
@@ -243,28 +193,36 @@ __global__ void k_blocking(SeriesGeom g, BleachDesc bd, CorrTables ct, PairDesc 
         double bv = (sa == sb) ? av : bb.apply(raw_value(g, oxb, oyb, y), y);
         double carry = (aprev - am) * (bv - bm);
         aprev = av;
+        // binary cascade: the level-k block containing product y closes
+        // when the low k bits of y are all ones (y is wave-uniform)
+        bool open = true;
 #pragma unroll
         for (int k = 0; k < kMaxBlock; k++) {
-            if (k < K) {
+            if (open && k < K) {
                 s1[k] += carry;
-                s2[k] += carry * carry;
+                s2[k] = fma(carry, carry, s2[k]);
                 if (k == K - 1 || ((y >> k) & 1) == 0) {
-                    if (k < K - 1) pend[k] = carry;
-                    break;
+                    pend[k] = carry;
+                    open = false;
+                } else {
+                    carry += pend[k];
                 }
-                carry += pend[k];
             }
         }
     }
     int64_t base = ((int64_t)chunk * pd.npairs + p) * K * 2;
-    for (int k = 0; k < K; k++) {
-        bpart[base + 2 * k] = s1[k];
-        bpart[base + 2 * k + 1] = s2[k];
+#pragma unroll
+    for (int k = 0; k < kMaxBlock; k++) {
+        if (k < K) {
+            bpart[base + 2 * k] = s1[k];
+            bpart[base + 2 * k + 1] = s2[k];
+        }
     }
 }
 
-__global__ void k_blocking_finalize(CorrTables ct, PairDesc pd, int Tn, const double *F, const double *bpart,
-                                    int nchunk, int32_t *ind_out, int32_t *bis_out) {
+__global__ void k_blocking_finalize(const CorrTables *__restrict__ ctp, PairDesc pd, int Tn, const double *F,
+                                    const double *bpart, int nchunk, int32_t *ind_out, int32_t *bis_out) {
+    const CorrTables &ct = *ctp;
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= pd.npairs) return;
     int sa, sb;
@@ -277,39 +235,50 @@ __global__ void k_blocking_finalize(CorrTables ct, PairDesc pd, int Tn, const do
     int crt[kMaxBlock], cr12[kMaxBlock], cr3[kMaxBlock], diffpos[kMaxBlock];
     double dd = am * bm;
     double nb = (double)n0;
-    for (int k = 0; k < K; k++) {
-        double s1 = 0.0, s2 = 0.0;
-        for (int c = 0; c < nchunk; c++) {
-            int64_t base = ((int64_t)c * pd.npairs + p) * K * 2;
-            s1 += bpart[base + 2 * k];
-            s2 += bpart[base + 2 * k + 1];
-        }
-        if (k > 0) nb = floor(nb / 2);
-        double scale = ldexp(1.0, -k);
-        double sp = s1 * scale, sp2 = s2 * scale * scale;
-        double v = (sp2 / nb - (sp / nb) * (sp / nb)) / (nb * (dd * dd));
-        sd[k] = sqrt(v);
-        double err = sd[k] / sqrt(2 * (nb - 1));
-        upper[k] = sd[k] + err;
-        lower[k] = sd[k] - err;
+#pragma unroll
+    for (int k = 0; k < kMaxBlock; k++) {
+        sd[k] = upper[k] = lower[k] = 0.0;
         crt[k] = cr12[k] = cr3[k] = diffpos[k] = 0;
+        if (k < K) {
+            double s1 = 0.0, s2 = 0.0;
+            for (int c = 0; c < nchunk; c++) {
+                int64_t base = ((int64_t)c * pd.npairs + p) * K * 2;
+                s1 += bpart[base + 2 * k];
+                s2 += bpart[base + 2 * k + 1];
+            }
+            if (k > 0) nb = floor(nb / 2);
+            double scale = ldexp(1.0, -k);
+            double sp = s1 * scale, sp2 = s2 * scale * scale;
+            double v = (sp2 / nb - (sp / nb) * (sp / nb)) / (nb * (dd * dd));
+            sd[k] = sqrt(v);
+            double err = sd[k] / sqrt(2 * (nb - 1));
+            upper[k] = sd[k] + err;
+            lower[k] = sd[k] - err;
+        }
     }
-    for (int x = 0; x < K - 1; x++)
-        if (upper[x] > lower[x + 1] && upper[x + 1] > lower[x]) crt[x] = 1;
-    for (int x = 0; x < K - 2; x++)
-        if (crt[x] * crt[x + 1] == 1) cr12[x] = 1;
-    for (int x = 0; x < K - 1; x++)
-        if (sd[x + 1] - sd[x] > 0) diffpos[x] = 1;
-    for (int x = 0; x < K - 2; x++)
-        if (diffpos[x] * diffpos[x + 1] == 1) cr3[x] = 1;
+    // criteria of blockTransform (IFCS:9364-9406)
+#pragma unroll
+    for (int x = 0; x < kMaxBlock - 1; x++)
+        if (x < K - 1 && upper[x] > lower[x + 1] && upper[x + 1] > lower[x]) crt[x] = 1;
+#pragma unroll
+    for (int x = 0; x < kMaxBlock - 2; x++)
+        if (x < K - 2 && crt[x] * crt[x + 1] == 1) cr12[x] = 1;
+#pragma unroll
+    for (int x = 0; x < kMaxBlock - 1; x++)
+        if (x < K - 1 && sd[x + 1] - sd[x] > 0) diffpos[x] = 1;
+#pragma unroll
+    for (int x = 0; x < kMaxBlock - 2; x++)
+        if (x < K - 2 && diffpos[x] * diffpos[x + 1] == 1) cr3[x] = 1;
     int last0 = 0, ind = 0, bis;
-    for (int x = 0; x < K - 2; x++)
-        if (cr3[x] == 1 && cr12[x] == 0) last0 = x;
-    for (int x = 0; x <= last0; x++) cr12[x] = 0;
-    cr12[K - 3] = 0;
-    cr12[K - 4] = 0;
-    for (int x = K - 5; x > 0; x--)
-        if (cr12[x] == 1) ind = x + 1;
+#pragma unroll
+    for (int x = 0; x < kMaxBlock - 2; x++)
+        if (x < K - 2 && cr3[x] == 1 && cr12[x] == 0) last0 = x;
+#pragma unroll
+    for (int x = 0; x < kMaxBlock; x++)
+        if (x <= last0 || x == K - 3 || x == K - 4) cr12[x] = 0;
+#pragma unroll
+    for (int x = kMaxBlock - 1; x > 0; x--)
+        if (x <= K - 5 && cr12[x] == 1) ind = x + 1;
     if (ind == 0) {
         bis = 0;
         ind = (K - 3 > 0) ? K - 3 : K - 1;
@@ -428,8 +397,10 @@ __device__ void correlate_level(const LevelCtx &L, const CorrTables &ct, int cha
     for (int i = 0; i < NL; i++) cross[i] += sab[i];
 }
 
-__global__ void __launch_bounds__(64) k_correlate(SeriesGeom g, BleachDesc bd, CorrTables ct, PairDesc pd,
-                                                  const double *F, const int32_t *ind, double *corav, double *bvar) {
+__global__ void __launch_bounds__(64) k_correlate(SeriesGeom g, BleachDesc bd, const CorrTables *__restrict__ ctp,
+                                                  PairDesc pd, const double *F, const int32_t *ind, double *corav,
+                                                  double *bvar) {
+    const CorrTables &ct = *ctp;
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     int c = blockIdx.y;
     if (p >= pd.npairs || c >= ct.nlevels) return;
@@ -792,6 +763,228 @@ __global__ void __launch_bounds__(64) k_lm(LMArgs a) {
 }
 
 // ------------------------------------------------------------------
+// Specialised LM for a compile-time set of free parameters: the plugin's
+// default fit of N, D, G_inf (FCS, slots 0, 1, 4) and the contiguous
+// polynomial coefficients of the bleach fit.  All arrays are statically
+// indexed, so the state stays in registers.
+// ------------------------------------------------------------------
+template <int KIND, int NF>
+struct FixedSlots {
+    static __device__ __forceinline__ constexpr int slot(int k) {
+        return KIND == kModelFCS ? (k == 0 ? 0 : (k == 1 ? 1 : 4)) : k;
+    }
+};
+
+template <int KIND, int NF>
+__device__ void lm_eval_fixed(const LMArgs &a, int fit, const float *par, bool need_jac, double &chi2, double *grad,
+                              double *hess) {
+    constexpr int NH = NF * (NF + 1) / 2;
+    const int lane = threadIdx.x & 63;
+    const float *y = a.data + (int64_t)fit * a.npts;
+    const float *w = a.weights ? a.weights + (int64_t)fit * a.npts : nullptr;
+    double c2 = 0.0, gl[NF], hl[NH];
+#pragma unroll
+    for (int i = 0; i < NF; i++) gl[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < NH; i++) hl[i] = 0.0;
+    bool c2on = false, c3on = false;
+    if (KIND == kModelFCS) {
+        c2on = (par[5] != 0.f) || a.pfit[6];
+        c3on = (par[7] != 0.f) || a.pfit[8];
+    }
+    for (int i = lane; i < a.npts; i += 64) {
+        const float xi = a.x[i];
+        float jf[NF];
+        float f;
+        if (KIND == kModelFCS) {
+            float gfull[11];
+            f = fcs3p(a.geom, xi, par, need_jac, c2on, c3on, gfull);
+#pragma unroll
+            for (int k = 0; k < NF; k++) jf[k] = gfull[FixedSlots<KIND, NF>::slot(k)];
+        } else {
+            double tx = 1.0, sv = (double)par[0];
+            jf[0] = 1.f;
+#pragma unroll
+            for (int k = 1; k < NF; k++) {
+                tx *= xi;
+                jf[k] = (float)tx;
+                sv += (double)par[k] * tx;
+            }
+            f = (float)sv;
+        }
+        const double wi = w ? (double)w[i] : 1.0;
+        const double r = (double)y[i] - (double)f;
+        c2 += wi * r * r;
+        if (need_jac) {
+            int h = 0;
+#pragma unroll
+            for (int k = 0; k < NF; k++) {
+                const double jk = (double)jf[k];
+                gl[k] += wi * r * jk;
+#pragma unroll
+                for (int l = 0; l <= k; l++) hl[h++] += wi * jk * (double)jf[l];
+            }
+        }
+    }
+    chi2 = wave_sum(c2);
+    if (need_jac) {
+#pragma unroll
+        for (int k = 0; k < NF; k++) grad[k] = wave_sum(gl[k]);
+#pragma unroll
+        for (int k = 0; k < NH; k++) hess[k] = wave_sum(hl[k]);
+    }
+}
+
+template <int NF>
+__device__ __forceinline__ bool chol_solve_fixed(double (&A)[NF][NF], double (&b)[NF]) {
+#pragma unroll
+    for (int j = 0; j < NF; j++) {
+        double s = A[j][j];
+#pragma unroll
+        for (int k = 0; k < j; k++) s -= A[j][k] * A[j][k];
+        if (!(s > 0.0)) return false;
+        const double d = sqrt(s);
+        A[j][j] = d;
+#pragma unroll
+        for (int i = j + 1; i < NF; i++) {
+            double t = A[i][j];
+#pragma unroll
+            for (int k = 0; k < j; k++) t -= A[i][k] * A[j][k];
+            A[i][j] = t / d;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NF; i++) {
+        double t = b[i];
+#pragma unroll
+        for (int k = 0; k < i; k++) t -= A[i][k] * b[k];
+        b[i] = t / A[i][i];
+    }
+#pragma unroll
+    for (int i = NF - 1; i >= 0; i--) {
+        double t = b[i];
+#pragma unroll
+        for (int k = i + 1; k < NF; k++) t -= A[k][i] * b[k];
+        b[i] = t / A[i][i];
+    }
+    return true;
+}
+
+template <int KIND, int NF>
+__global__ void __launch_bounds__(64) k_lm_fixed(LMArgs a) {
+    constexpr int NP = (KIND == kModelFCS) ? 20 : 11;
+    constexpr int NH = NF * (NF + 1) / 2;
+    const int fit = blockIdx.x;
+    if (fit >= a.nfits) return;
+    const int lane = threadIdx.x & 63;
+    float par[NP];
+#pragma unroll
+    for (int k = 0; k < NP; k++) par[k] = a.init[(int64_t)fit * NP + k];
+    LMArgs ac = a;
+    if (KIND == kModelFCS) {
+        par[1] *= 1e12f;
+        par[6] *= 1e12f;
+        par[8] *= 1e12f;
+        par[2] *= 1e6f;
+        par[3] *= 1e6f;
+        ac.geom.ax = par[11] * 1e6f;
+        ac.geom.ay = par[12] * 1e6f;
+        ac.geom.s = par[13] * 1e6f;
+        ac.geom.sz = par[14] * 1e6f;
+        ac.geom.rx = par[15] * 1e6f;
+        ac.geom.ry = par[16] * 1e6f;
+        ac.geom.obsvol = par[17] * 1e12f;
+        ac.geom.q2 = par[18];
+        ac.geom.q3 = par[19];
+    }
+    double chi2, grad[NF], hess[NH], H[NF][NF], scale[NF];
+    lm_eval_fixed<KIND, NF>(ac, fit, par, true, chi2, grad, hess);
+    int state = kStateConverged, iters = 0;
+    double lambda = 1e-3, prev_chi2 = chi2;
+    bool failed = false;
+#pragma unroll
+    for (int k = 0; k < NF; k++) scale[k] = 0.0;
+    {
+        int h = 0;
+#pragma unroll
+        for (int k = 0; k < NF; k++)
+#pragma unroll
+            for (int l = 0; l <= k; l++, h++) H[k][l] = H[l][k] = hess[h];
+    }
+    bool finished = false;
+    for (int it = 0; it < a.maxit && !finished; it++) {
+        // cuda_modify_step_widths (agpufitjni.cu:2044-2076)
+#pragma unroll
+        for (int k = 0; k < NF; k++) {
+            if (failed) H[k][k] -= scale[k] * lambda / 10.0;
+            scale[k] = fmax(scale[k], H[k][k]);
+            H[k][k] += scale[k] * lambda;
+        }
+        double A[NF][NF], delta[NF];
+#pragma unroll
+        for (int k = 0; k < NF; k++) {
+            delta[k] = grad[k];
+#pragma unroll
+            for (int l = 0; l < NF; l++) A[k][l] = H[k][l];
+        }
+        iters = it + 1;
+        if (!chol_solve_fixed<NF>(A, delta)) {
+            state = kStateSingular;
+            break;
+        }
+        float prev[NF];
+#pragma unroll
+        for (int k = 0; k < NF; k++) {
+            constexpr int dummy = 0;
+            (void)dummy;
+            const int sl = FixedSlots<KIND, NF>::slot(k);
+            prev[k] = par[sl];
+            par[sl] = (float)((double)par[sl] + delta[k]);
+        }
+        double nchi2, ngrad[NF], nhess[NH];
+        lm_eval_fixed<KIND, NF>(ac, fit, par, false, nchi2, ngrad, nhess);
+        const bool decreased = nchi2 < prev_chi2;
+        if (decreased) lm_eval_fixed<KIND, NF>(ac, fit, par, true, nchi2, ngrad, nhess);
+        // cuda_check_for_convergence (agpufitjni.cu:2289-2318)
+        if (fabs(nchi2 - prev_chi2) < (double)a.tol * fmax(1.0, nchi2)) finished = true;
+        else if (it == a.maxit - 1) state = kStateMaxIter;
+        // cuda_prepare_next_iteration (agpufitjni.cu:2458-2490)
+        if (decreased) {
+            lambda *= 0.1;
+            prev_chi2 = nchi2;
+#pragma unroll
+            for (int k = 0; k < NF; k++) grad[k] = ngrad[k];
+            int h = 0;
+#pragma unroll
+            for (int k = 0; k < NF; k++)
+#pragma unroll
+                for (int l = 0; l <= k; l++, h++) H[k][l] = H[l][k] = nhess[h];
+            failed = false;
+        } else {
+            lambda *= 10.0;
+#pragma unroll
+            for (int k = 0; k < NF; k++) par[FixedSlots<KIND, NF>::slot(k)] = prev[k];
+            failed = true;
+        }
+    }
+    chi2 = prev_chi2;
+    if (KIND == kModelFCS) {
+        par[1] *= 1e-12f;
+        par[6] *= 1e-12f;
+        par[8] *= 1e-12f;
+        par[2] *= 1e-6f;
+        par[3] *= 1e-6f;
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NP; k++) a.out_params[(int64_t)fit * NP + k] = par[k];
+        a.states[fit] = state;
+        a.chi2[fit] = (float)chi2;
+        a.iters[fit] = iters;
+    }
+}
+
+// ------------------------------------------------------------------
 // Drop-in helpers: float binning and bleach-trace averaging
 // ------------------------------------------------------------------
 __global__ void k_binning_f32(const float *in, float *out, int win, int hin, int wt, int ht, int T, int bx, int by) {
@@ -865,56 +1058,77 @@ void launch_polyfit(const double *part, int nchunk, int nseries, int nopit, int 
     hipLaunchKernelGGL(k_polyfit, dim3(cdiv(nseries, 64)), dim3(64), 0, st, part, nchunk, nseries, order, Lf, coef);
 }
 
-void launch_prefix(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, double *ctot, double *rec,
-                   int nchunk, int chunk_len, hipStream_t st) {
+void launch_prefix(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const CorrTables *dct,
+                   double *ctot, double *rec, int nchunk, int chunk_len, hipStream_t st) {
+    (void)ct;
     dim3 grid(cdiv(g.nseries, 64), nchunk);
-    hipLaunchKernelGGL(k_prefix, grid, dim3(64), 0, st, g, bd, ct, chunk_len, ctot, rec);
+    hipLaunchKernelGGL(k_prefix, grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec);
 }
 
-void launch_prefix_combine(const CorrTables &ct, int nseries, const double *ctot, const double *rec, int nchunk,
+void launch_prefix_combine(const CorrTables *dct, int nseries, const double *ctot, const double *rec, int nchunk,
                            int chunk_len, double *F, hipStream_t st) {
-    hipLaunchKernelGGL(k_prefix_combine, dim3(cdiv(nseries, 64)), dim3(64), 0, st, ct, nseries, ctot, rec, nchunk,
+    hipLaunchKernelGGL(k_prefix_combine, dim3(cdiv(nseries, 64)), dim3(64), 0, st, dct, nseries, ctot, rec, nchunk,
                        chunk_len, F);
 }
 
-void launch_blocking(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const PairDesc &pd,
+void launch_blocking(const SeriesGeom &g, const BleachDesc &bd, const CorrTables *dct, const PairDesc &pd,
                      const double *F, double *bpart, int nchunk, int chunk_len, hipStream_t st) {
     dim3 grid(cdiv(pd.npairs, 64), nchunk);
-    hipLaunchKernelGGL(k_blocking, grid, dim3(64), 0, st, g, bd, ct, pd, F, bpart, chunk_len);
+    hipLaunchKernelGGL(k_blocking, grid, dim3(64), 0, st, g, bd, dct, pd, F, bpart, chunk_len);
 }
 
-void launch_blocking_finalize(const CorrTables &ct, const PairDesc &pd, const double *F, const double *bpart,
-                              int nchunk, int32_t *ind, int32_t *bis, hipStream_t st) {
-    hipLaunchKernelGGL(k_blocking_finalize, dim3(cdiv(pd.npairs, 64)), dim3(64), 0, st, ct, pd, ct.T, F, bpart,
+void launch_blocking_finalize(const CorrTables &ct, const CorrTables *dct, const PairDesc &pd, const double *F,
+                              const double *bpart, int nchunk, int32_t *ind, int32_t *bis, hipStream_t st) {
+    hipLaunchKernelGGL(k_blocking_finalize, dim3(cdiv(pd.npairs, 64)), dim3(64), 0, st, dct, pd, ct.T, F, bpart,
                        nchunk, ind, bis);
 }
 
-void launch_correlate(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const PairDesc &pd,
-                      const double *F, const int32_t *ind, double *corav, double *bvar, hipStream_t st) {
+void launch_correlate(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const CorrTables *dct,
+                      const PairDesc &pd, const double *F, const int32_t *ind, double *corav, double *bvar,
+                      hipStream_t st) {
     dim3 grid(cdiv(pd.npairs, 64), ct.nlevels);
-    hipLaunchKernelGGL(k_correlate, grid, dim3(64), 0, st, g, bd, ct, pd, F, ind, corav, bvar);
+    hipLaunchKernelGGL(k_correlate, grid, dim3(64), 0, st, g, bd, dct, pd, F, ind, corav, bvar);
 }
 
-static void launch_lm(const LMArgs &a, hipStream_t st) {
+static void launch_lm(const LMArgs &a, unsigned free_mask, hipStream_t st) {
     if (a.nfits <= 0) return;
+    if (a.kind == kModelFCS && free_mask == 0x13u) {   // N, D, G_inf
+        hipLaunchKernelGGL((k_lm_fixed<kModelFCS, 3>), dim3(a.nfits), dim3(64), 0, st, a);
+        return;
+    }
+    if (a.kind == kModelPoly) {
+        unsigned contiguous = (1u << a.ncoef) - 1u;
+        if (free_mask == contiguous) {
+            switch (a.ncoef) {
+                case 1: hipLaunchKernelGGL((k_lm_fixed<kModelPoly, 1>), dim3(a.nfits), dim3(64), 0, st, a); return;
+                case 2: hipLaunchKernelGGL((k_lm_fixed<kModelPoly, 2>), dim3(a.nfits), dim3(64), 0, st, a); return;
+                case 3: hipLaunchKernelGGL((k_lm_fixed<kModelPoly, 3>), dim3(a.nfits), dim3(64), 0, st, a); return;
+                case 4: hipLaunchKernelGGL((k_lm_fixed<kModelPoly, 4>), dim3(a.nfits), dim3(64), 0, st, a); return;
+                case 5: hipLaunchKernelGGL((k_lm_fixed<kModelPoly, 5>), dim3(a.nfits), dim3(64), 0, st, a); return;
+                case 6: hipLaunchKernelGGL((k_lm_fixed<kModelPoly, 6>), dim3(a.nfits), dim3(64), 0, st, a); return;
+                case 7: hipLaunchKernelGGL((k_lm_fixed<kModelPoly, 7>), dim3(a.nfits), dim3(64), 0, st, a); return;
+                default: break;
+            }
+        }
+    }
     hipLaunchKernelGGL(k_lm, dim3(a.nfits), dim3(64), 0, st, a);
 }
 
 void launch_lm_acf(int nfits, int npts, const float *data, const float *weights, const float *x, const float *init,
-                   const int *pfit, const LMModelGeom &geom, float tol, int maxit, float *out_params, int *states,
-                   float *chi2, int *iters, hipStream_t st) {
+                   const int *pfit, unsigned free_mask, const LMModelGeom &geom, float tol, int maxit,
+                   float *out_params, int *states, float *chi2, int *iters, hipStream_t st) {
     LMArgs a{nfits, npts, 20, kModelFCS, 0, data, weights, x, init, pfit, geom, tol, maxit, out_params, states, chi2,
              iters};
-    launch_lm(a, st);
+    launch_lm(a, free_mask, st);
 }
 
 void launch_lm_poly(int nfits, int npts, const float *data, const float *weights, const float *x, const float *init,
-                    const int *pfit, int ncoef, float tol, int maxit, float *out_params, int *states, float *chi2,
-                    int *iters, hipStream_t st) {
+                    const int *pfit, unsigned free_mask, int ncoef, float tol, int maxit, float *out_params,
+                    int *states, float *chi2, int *iters, hipStream_t st) {
     LMModelGeom g{};
     LMArgs a{nfits, npts, 11, kModelPoly, ncoef, data, weights, x, init, pfit, g, tol, maxit, out_params, states,
              chi2, iters};
-    launch_lm(a, st);
+    launch_lm(a, free_mask, st);
 }
 
 void launch_binning_f32(const float *in, float *out, int win, int hin, int wt, int ht, int T, int bx, int by,
