@@ -81,7 +81,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-sample", type=int, default=24, help="side of the oracle's pixel sample (0 = skip)")
+    ap.add_argument("--cpu-sample", type=int, default=48, help="side of the oracle's pixel sample (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=1)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -134,6 +134,16 @@ def main():
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "avg_ms": round(corr_ms, 3), "alg_bytes": alg_bytes}
 
+    # LM iteration statistics of every binning (untimed re-run)
+    iters = {}
+    for b, c in zip(BINNINGS, cfgs):
+        eng.run(c)
+        r = eng.results(c)
+        it = np.asarray(r["iters"]).ravel()
+        stt = np.asarray(r["states"]).ravel()
+        iters["b%d" % b] = {"median": float(np.median(it)), "p99": float(np.percentile(it, 99)),
+                            "max": int(it.max()), "converged_frac": round(float((stt == 0).mean()), 4)}
+
     cpu = None
     if args.cpu_sample > 0:
         rate, cdt, _ = cpu_baseline(stack, smin, args.cpu_sample, args.cpu_threads)
@@ -150,7 +160,7 @@ def main():
             "stages_ms": {"trace_polyfit": round(stage_sum[0], 3), "means": round(stage_sum[1], 3),
                           "blocking": round(stage_sum[2], 3), "correlate": round(stage_sum[3], 3),
                           "fit": round(stage_sum[4], 3)},
-            "roofline": roof, "cpu_baseline": cpu}
+            "fit_iterations": iters, "roofline": roof, "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)
     eng.close()
 

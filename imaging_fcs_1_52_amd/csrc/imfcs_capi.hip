@@ -173,10 +173,12 @@ struct imfcs_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev[6] = {};
     DevBuf part, coef, ctot, rec, F, bpart, ind, bis, corav, bvar, fdata, fw, fx, finit, fpfit, fout, fstate, fchi2,
-        fiter, cpart, hpart, l3, ctab;
-    CorrTables host_ct;   // source of the async copy into ctab
+        fiter, cpart, hpart, l3, ctab, csum, slots, X;
+    CorrTables host_ct;
+    std::vector<int32_t> host_slots;   // slot_of then slot origins (source of the async copy)
+    std::vector<double> host_coef;     // drop-in bleach coefficients per slot (source of the async copy)   // source of the async copy into ctab
     // last run
-    int npix = 0, npairs = 0, nseries = 0, chanum = 0, order = 0, ow = 0, oh = 0, did_fit = 0, nfree = 0;
+    int npix = 0, npairs = 0, nseries = 0, nslots = 0, chanum = 0, order = 0, ow = 0, oh = 0, did_fit = 0, nfree = 0;
     int force_v1 = 0;   // IMFCS_CORRELATOR=generic selects the one-thread-per-level kernel
     double chi2_norm = 1.0;
     std::vector<double> coef_scale;   // u-basis -> seconds conversion per power
@@ -240,7 +242,7 @@ static void release_all(imfcs_ctx *ctx) {
     DevBuf *bufs[] = {&ctx->part, &ctx->coef, &ctx->ctot, &ctx->rec, &ctx->F, &ctx->bpart, &ctx->ind,
                       &ctx->bis, &ctx->corav, &ctx->bvar, &ctx->fdata, &ctx->fw, &ctx->fx, &ctx->finit,
                       &ctx->fpfit, &ctx->fout, &ctx->fstate, &ctx->fchi2, &ctx->fiter, &ctx->cpart,
-                      &ctx->hpart, &ctx->l3, &ctx->ctab};
+                      &ctx->hpart, &ctx->l3, &ctx->ctab, &ctx->csum, &ctx->slots, &ctx->X};
     for (DevBuf *b : bufs) b->release();
 }
 
@@ -298,6 +300,43 @@ int imfcs_synchronize(imfcs_ctx *ctx) {
 // ------------------------------------------------------------------
 namespace {
 
+// Distinct series ("slots").  Slots form a row-major grid over the output
+// pixels extended by the CCF partner offset when that offset is a multiple
+// of the pixel pitch (then every partner is a grid point and shares the
+// slot of the pixel at that point); otherwise the partners get a second
+// grid of their own.  Grid rows are padded to SWP slots (a multiple of 64,
+// or a power of two below 64) so that the 64 lanes of a wave read whole,
+// aligned row segments of the stack: unaligned uint16 rows cost ~10x in
+// load throughput.  Padding slots repeat the last window of their row.
+// buf = slot_of[nseries] followed by the window origins [nslots][2].
+int build_slots(const SeriesGeom &g, std::vector<int32_t> &buf) {
+    const bool ccf = g.nseries > g.npix;
+    const bool on_grid = ccf && (g.cfx % g.pbx == 0) && (g.cfy % g.pby == 0);
+    const int dx = on_grid ? g.cfx / g.pbx : 0, dy = on_grid ? g.cfy / g.pby : 0;
+    const int sw = g.ow + dx, sh = g.oh + dy;
+    int swp = 1;
+    while (swp < sw && swp < 64) swp <<= 1;
+    if (sw > 64) swp = (sw + 63) / 64 * 64;
+    const int ngrid = (ccf && !on_grid) ? 2 : 1;
+    const int nslots = ngrid * sh * swp;
+    buf.assign((size_t)g.nseries + 2 * (size_t)nslots, 0);
+    int32_t *org = buf.data() + g.nseries;
+    for (int gi = 0; gi < ngrid; gi++)
+        for (int y = 0; y < sh; y++)
+            for (int xs = 0; xs < swp; xs++) {
+                const int x = std::min(xs, sw - 1);
+                const size_t k = ((size_t)gi * sh + y) * swp + xs;
+                org[2 * k] = g.ox0 + x * g.pbx + (gi ? g.cfx : 0);
+                org[2 * k + 1] = g.oy0 + y * g.pby + (gi ? g.cfy : 0);
+            }
+    for (int k = 0; k < g.npix; k++) {
+        const int x = k % g.ow, y = k / g.ow;
+        buf[k] = y * swp + x;
+        if (ccf) buf[g.npix + k] = on_grid ? (y + dy) * swp + (x + dx) : (sh + y) * swp + x;
+    }
+    return nslots;
+}
+
 struct EngineArgs {
     SeriesGeom g;
     int order;             // polynomial order (0 = none)
@@ -309,21 +348,29 @@ struct EngineArgs {
     PairDesc pd;
 };
 
-int run_engine(imfcs_ctx *ctx, const EngineArgs &ea, HostTables &ht, CorrTables &ct) {
+int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTables &ct) {
     hipStream_t st = ctx->stream;
-    const SeriesGeom &g = ea.g;
+    EngineArgs ea = ea_in;
+    SeriesGeom &g = ea.g;
     int rc = make_host_tables(ea.p, ea.q, g.T, ea.frametime, ht);
     if (rc) return rc;
     rc = make_corr_tables(ht, g.T, ea.q, ct);
     if (rc) return rc;
-    HIP_TRY(hipStreamSynchronize(st));   // host_ct may still feed a previous copy
+    HIP_TRY(hipStreamSynchronize(st));   // host_ct / host_slots may still feed a previous copy
     ctx->host_ct = ct;
     HIP_TRY(ctx->ctab.ensure(sizeof(CorrTables)));
     HIP_TRY(hipMemcpyAsync(ctx->ctab.p, &ctx->host_ct, sizeof(CorrTables), hipMemcpyHostToDevice, st));
     const CorrTables *dct = ctx->ctab.as<CorrTables>();
-    const int ns = g.nseries, np = ea.pd.npairs, n = ea.order + 1;
+    g.nslots = build_slots(g, ctx->host_slots);
+    HIP_TRY(ctx->slots.ensure(sizeof(int32_t) * ctx->host_slots.size()));
+    HIP_TRY(hipMemcpyAsync(ctx->slots.p, ctx->host_slots.data(), sizeof(int32_t) * ctx->host_slots.size(),
+                           hipMemcpyHostToDevice, st));
+    g.slot_of = ctx->slots.as<int32_t>();
+    g.slot_org = ctx->slots.as<int32_t>() + g.nseries;
+    ctx->nslots = g.nslots;
+    const int ns = g.nslots, np = ea.pd.npairs, n = ea.order + 1;
     ctx->npairs = np;
-    ctx->nseries = ns;
+    ctx->nseries = g.nseries;
     ctx->chanum = ct.chanum;
     ctx->order = ea.order;
     BleachDesc bd{};
@@ -336,7 +383,7 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea, HostTables &ht, CorrTables 
         if (ea.fit_trace) {
             // abscissa normalized by the last trace time
             double tscale = ea.frametime * (ea.nopit - 0.5) * ea.ave;
-            int nchunk = std::min(40, ea.nopit);
+            int nchunk = std::min(std::min(ea.nopit, 128), std::max(40, (1 << 19) / std::max(1, ns)));
             HIP_TRY(ctx->part.ensure(sizeof(double) * (size_t)nchunk * ns * n));
             launch_trace_moments(g, ea.nopit, ea.ave, tscale, ea.frametime, ea.order, 1, ctx->part.as<double>(),
                                  nchunk, st);
@@ -346,7 +393,12 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea, HostTables &ht, CorrTables 
             ctx->coef_scale.assign(n, 1.0);
             for (int k = 1; k < n; k++) ctx->coef_scale[k] = ctx->coef_scale[k - 1] / tscale;
         } else {
-            HIP_TRY(hipMemcpyAsync(ctx->coef.p, ea.host_coef, sizeof(double) * (size_t)ns * n,
+            // coefficients per series -> per slot (a slot's series share the window)
+            ctx->host_coef.assign((size_t)ns * n, 0.0);
+            for (int s2 = g.nseries - 1; s2 >= 0; s2--)
+                for (int k = 0; k < n; k++)
+                    ctx->host_coef[(size_t)ctx->host_slots[s2] * n + k] = ea.host_coef[(size_t)s2 * n + k];
+            HIP_TRY(hipMemcpyAsync(ctx->coef.p, ctx->host_coef.data(), sizeof(double) * (size_t)ns * n,
                                    hipMemcpyHostToDevice, st));
             bd.tscale = 1.0;
             ctx->coef_scale.assign(n, 1.0);
@@ -359,21 +411,27 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea, HostTables &ht, CorrTables 
     }
     HIP_TRY(hipEventRecord(ctx->ev[1], st));
     // window means
-    const int plen = 4096, npc = (g.T + plen - 1) / plen;
+    // chunk length: about 2^19 threads in flight, 256..4096 frames each
+    int plen = 4096;
+    while (plen > 256 && (int64_t)ns * ((g.T + plen - 1) / plen) < (1 << 19)) plen >>= 1;
+    const int npc = (g.T + plen - 1) / plen;
     HIP_TRY(ctx->ctot.ensure(sizeof(double) * (size_t)npc * ns));
     HIP_TRY(ctx->rec.ensure(sizeof(double) * (size_t)ns * ct.npos));
     HIP_TRY(ctx->F.ensure(sizeof(double) * (size_t)ns * ct.npos));
-    launch_prefix(g, bd, ct, dct, ctx->ctot.as<double>(), ctx->rec.as<double>(), npc, plen, st);
+    HIP_TRY(ctx->X.ensure(sizeof(float) * (size_t)ns * g.T));
+    launch_apply(g, bd, dct, ctx->ctot.as<double>(), ctx->rec.as<double>(), ctx->X.as<float>(), npc, plen, st);
     launch_prefix_combine(dct, ns, ctx->ctot.as<double>(), ctx->rec.as<double>(), npc, plen, ctx->F.as<double>(), st);
     HIP_TRY(hipEventRecord(ctx->ev[2], st));
     // blocking
-    const int blen = 1 << (ct.blocknum - 1), nbc = (g.T - 1 + blen - 1) / blen;
-    HIP_TRY(ctx->bpart.ensure(sizeof(double) * (size_t)nbc * np * ct.blocknum * 2));
+    const BlockingPlan bp = plan_blocking(ct, g.npix);
+    HIP_TRY(ctx->bpart.ensure(sizeof(double) * (size_t)bp.nsplit * np * 8 * 2));
+    HIP_TRY(ctx->csum.ensure(sizeof(double) * (size_t)std::max(1, bp.nchunk_prod) * np));
     HIP_TRY(ctx->ind.ensure(sizeof(int32_t) * (size_t)np));
     HIP_TRY(ctx->bis.ensure(sizeof(int32_t) * (size_t)np));
-    launch_blocking(g, bd, dct, ea.pd, ctx->F.as<double>(), ctx->bpart.as<double>(), nbc, blen, st);
-    launch_blocking_finalize(ct, dct, ea.pd, ctx->F.as<double>(), ctx->bpart.as<double>(), nbc,
-                             ctx->ind.as<int32_t>(), ctx->bis.as<int32_t>(), st);
+    launch_blocking(g, bd, dct, ea.pd, ctx->F.as<double>(), ctx->X.as<float>(), bp, ctx->bpart.as<double>(),
+                    ctx->csum.as<double>(), st);
+    launch_blocking_finalize(ct, dct, g, ea.pd, ctx->F.as<double>(), bp, ctx->bpart.as<double>(),
+                             ctx->csum.as<double>(), ctx->ind.as<int32_t>(), ctx->bis.as<int32_t>(), st);
     HIP_TRY(hipEventRecord(ctx->ev[3], st));
     // correlation
     HIP_TRY(ctx->corav.ensure(sizeof(double) * (size_t)np * ct.chanum));
@@ -382,13 +440,13 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea, HostTables &ht, CorrTables 
         const int clen = corr_fast_chunk_len(ct, np);
         const int nch = (g.T + clen - 1) / clen;
         HIP_TRY(ctx->cpart.ensure(sizeof(float) * (size_t)nch * np * 33 * 3));
-        HIP_TRY(ctx->hpart.ensure(sizeof(double) * (size_t)np * (ct.chanum - 33) * 3));
-        HIP_TRY(ctx->l3.ensure(sizeof(float) * (size_t)ct.numbin[3] * ns));
-        launch_correlate_fast(g, bd, ct, dct, ea.pd, ctx->F.as<double>(), ctx->ind.as<int32_t>(), ctx->cpart.as<float>(),
-                              ctx->hpart.as<double>(), ctx->l3.as<float>(), clen, ctx->corav.as<double>(),
-                              ctx->bvar.as<double>(), st);
+        HIP_TRY(ctx->hpart.ensure(sizeof(double) * (size_t)nch * np * (ct.chanum - 33) * 3));
+        HIP_TRY(ctx->l3.ensure(sizeof(float) * (size_t)ct.numbin[3] * g.nseries));
+        launch_correlate_fast(g, bd, ct, dct, ea.pd, ctx->F.as<double>(), ctx->X.as<float>(), ctx->ind.as<int32_t>(),
+                              ctx->cpart.as<float>(), ctx->hpart.as<double>(), ctx->l3.as<float>(), clen,
+                              ctx->corav.as<double>(), ctx->bvar.as<double>(), st);
     } else {
-        launch_correlate(g, bd, ct, dct, ea.pd, ctx->F.as<double>(), ctx->ind.as<int32_t>(),
+        launch_correlate(g, bd, ct, dct, ea.pd, ctx->F.as<double>(), ctx->X.as<float>(), ctx->ind.as<int32_t>(),
                          ctx->corav.as<double>(), ctx->bvar.as<double>(), st);
     }
     HIP_TRY(hipEventRecord(ctx->ev[4], st));
@@ -542,10 +600,12 @@ extern "C" int imfcs_get_results(imfcs_ctx *ctx, double *acf, double *var, int32
     if (blocked) HIP_TRY(hipMemcpy(blocked, ctx->bis.p, sizeof(int32_t) * np, hipMemcpyDeviceToHost));
     if (coeffs && ctx->order > 0) {
         size_t n = ctx->order + 1;
-        std::vector<double> cu(ctx->nseries * n);
+        std::vector<double> cu(ctx->nslots * n);
         HIP_TRY(hipMemcpy(cu.data(), ctx->coef.p, sizeof(double) * cu.size(), hipMemcpyDeviceToHost));
-        for (size_t s = 0; s < (size_t)ctx->nseries; s++)
-            for (size_t k = 0; k < n; k++) coeffs[s * n + k] = cu[s * n + k] * ctx->coef_scale[k];
+        for (size_t s = 0; s < (size_t)ctx->nseries; s++) {
+            const size_t k0 = (size_t)ctx->host_slots[s] * n;
+            for (size_t k = 0; k < n; k++) coeffs[s * n + k] = cu[k0 + k] * ctx->coef_scale[k];
+        }
     }
     if (ctx->did_fit) {
         std::vector<float> out(np * 20), c2(np);

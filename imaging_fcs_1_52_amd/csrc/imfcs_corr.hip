@@ -2,10 +2,9 @@
 // LDS-staged and wave-specialised for CDNA4.
 //
 // k_corr_main: one workgroup per (64 output pixels, time chunk).  Loader
-//   waves read the uint16 frames once (fused b x b binning), apply the
-//   polynomial bleach correction in FP64, subtract the series mean and stage
-//   FP32 values for 64 frames in LDS, together with the pair-summed level-1
-//   and level-2 values; the level-3 (8-frame) sums go to global memory for
+//   waves read the bleach-corrected FP32 deviations X of the a and b series
+//   and stage 64 frames in LDS, together with the pair-summed level-1 and
+//   level-2 values; the level-3 (8-frame) sums go to global memory for
 //   k_corr_high.  Five role waves per correlation type then consume the tile:
 //     role 0..2: level 0, delays 0-5 / 6-11 / 12-16
 //     role 3   : level 1, delays 9-16
@@ -16,7 +15,7 @@
 // k_corr_high: levels 3..q-1 from the level-3 series, one thread per pair.
 // k_corr_finalize: combines the per-chunk sums into G and the variance.
 //
-// With the series shifted by its mean mu, a~ = a - mu, every channel's exact
+// With the series shifted by its slot's shift mu, a~ = a - mu, every channel's exact
 // window means enter through alpha = abar - mu_c, beta = bbar - mu_c:
 //   sum_y (a_y - abar)(b_{y+d} - bbar) = sum a~ b~ - n alpha beta
 //   block sum S_j = Sab - beta Sa - alpha Sb + B alpha beta
@@ -35,6 +34,7 @@ struct MainArgs {
     const CorrTables *ctp;
     PairDesc pd;
     const double *F;
+    const float *X;
     const int32_t *ind;
     int chunk_len, nchunk;
     int has_acf, has_ccf;
@@ -46,17 +46,19 @@ struct Tile {
     float l0[kTileF][kNS];
     float l1[kTileF / 2][kNS];
     float l2[kTileF / 4][kNS];
-    double coef[kNS][kMaxOrder + 2];   // normalized bleach polynomial, c0, mean
 };
 
-// Per-lane loader: one series, 16 consecutive frames of every tile.  The
-// series' bleach polynomial and mean live in LDS so they are not held in
-// registers across the role loops.
+// Shift of series s at level c (2^c frames per value).
+__device__ __forceinline__ double level_shift(const MainArgs &A, int s, int c) {
+    return slot_mu(A.bd.coef, A.bd.order, A.g.slot_of[s]) * (double)(1 << c);
+}
+
+// Per-lane loader: one series, 16 consecutive frames of every tile.
 struct Loader {
     bool active;
-    int sl, s, fb, ox, oy, order;
-    double scale, off;
+    int sl, s, fb, slot;
     __device__ void init(const MainArgs &A, Tile &tl, int wave, int lane, int p0, int nloader_waves) {
+        (void)tl;
         active = wave < nloader_waves;
         int half = (nloader_waves == 8) ? (wave & 1) : 0;
         fb = (nloader_waves == 8) ? (wave >> 1) : wave;
@@ -64,50 +66,28 @@ struct Loader {
         int p = p0 + lane;
         if (p >= A.g.npix) p = A.g.npix - 1;   // padding lanes duplicate the last pixel
         s = half ? A.g.npix + p : p;
-        ox = oy = 0;
-        order = A.bd.order;
-        scale = A.bd.frametime / A.bd.tscale;
-        off = A.bd.t_off;
-        if (active) {
-            series_origin(A.g, s, ox, oy);
-            if (fb == 0) {
-                double c0 = (order > 0) ? A.bd.coef[(int64_t)s * (order + 1)] : 1.0;
-                for (int k = 0; k <= kMaxOrder; k++)
-                    tl.coef[sl][k] = (k <= order && order > 0) ? A.bd.coef[(int64_t)s * (order + 1) + k] / c0 : 0.0;
-                tl.coef[sl][kMaxOrder + 1] = A.F[(int64_t)s * A.ctp->npos + A.ctp->pos_total] / A.g.T;
-            }
-        }
-    }
-    __device__ __forceinline__ double corrected(const MainArgs &A, const double *cf, double c0, double mu, int f) const {
-        double v = raw_value(A.g, ox, oy, f);
-        if (order > 0) {
-            double u = ((double)f + off) * scale, g = 0.0;
-#pragma unroll
-            for (int k = kMaxOrder; k >= 0; k--)
-                if (k <= order) g = fma(g, u, cf[k]);
-            double r = (double)rsqrtf((float)g);
-            r = r * fma(-0.5 * g * r, r, 1.5);
-            v = fma(v, r, c0 * (1.0 - g * r));
-        }
-        return v - mu;
+        slot = A.g.slot_of[s];
     }
     __device__ void load(const MainArgs &A, Tile &tl, int f0, int chunk_end, bool own) {
         if (!active) return;
         const int T = A.g.T;
         const int fbase = f0 + 16 * fb;
-        double cf[kMaxOrder + 1];
+        const int64_t ld = A.g.nslots;
+        float xr[16];
+        const float *xq = A.X + (int64_t)min(fbase, T - 1) * ld;   // wave-uniform
 #pragma unroll
-        for (int k = 0; k <= kMaxOrder; k++) cf[k] = (k <= order) ? tl.coef[sl][k] : 0.0;
-        const double c0 = (order > 0) ? A.bd.coef[(int64_t)s * (order + 1)] : 1.0;
-        const double mu = tl.coef[sl][kMaxOrder + 1];
+        for (int k = 0; k < 16; k++) {
+            const int f = fbase + k;
+            const float x = xq[(uint32_t)min(k, max(T - 1 - fbase, 0)) * (uint32_t)ld + (uint32_t)slot];
+            xr[k] = (f < T) ? x : 0.f;
+        }
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             double v[8];
 #pragma unroll
             for (int k = 0; k < 8; k++) {
-                int f = fbase + 8 * h + k;
-                v[k] = (f < T) ? corrected(A, cf, c0, mu, f) : 0.0;
-                tl.l0[16 * fb + 8 * h + k][sl] = (float)v[k];
+                v[k] = (double)xr[8 * h + k];
+                tl.l0[16 * fb + 8 * h + k][sl] = xr[8 * h + k];
             }
 #pragma unroll
             for (int k = 0; k < 4; k++)
@@ -154,9 +134,9 @@ __device__ void run_role(const MainArgs &A, Tile &tl, Loader &ld, int lane, int 
     float sa_blk = 0.f;
     int B = 1, pnum = 0;
     {
-        const double *Fa = A.F + (int64_t)sa * ct.npos, *Fb = A.F + (int64_t)sb * ct.npos;
-        double mua = Fa[ct.pos_total] / A.g.T * (double)(1 << LEVEL);
-        double mub = Fb[ct.pos_total] / A.g.T * (double)(1 << LEVEL);
+        const double *Fa = A.F + (int64_t)A.g.slot_of[sa] * ct.npos, *Fb = A.F + (int64_t)A.g.slot_of[sb] * ct.npos;
+        double mua = level_shift(A, sa, LEVEL);
+        double mub = level_shift(A, sb, LEVEL);
         const int x0 = ct.first_chan[LEVEL] + (LEVEL == 0 ? DMIN : DMIN - 9);
 #pragma unroll
         for (int i = 0; i < NL; i++) {
@@ -301,9 +281,12 @@ struct L3Ctx {
 };
 
 // Levels >= 3 hold few, large (rebinned) values: FP64 throughout.
+// Values [ybeg, yend) of one level; yend - ybeg and ybeg are multiples of G
+// and of the block length B, so every block of the blocked variance lies in
+// one call.
 template <int G>
 __device__ void high_level(const L3Ctx &L, bool same, const double *alpha, const double *beta, int B, int pnum,
-                           double *cross, double *a1, double *a2) {
+                           int ybeg, int yend, double *cross, double *a1, double *a2) {
     constexpr int WN = G + 16;
     double w[WN], ga[G], sab[8], sb[8];
     double sa = 0.0;
@@ -313,8 +296,8 @@ __device__ void high_level(const L3Ctx &L, bool same, const double *alpha, const
         cross[i] = a1[i] = a2[i] = 0.0;
     }
 #pragma unroll
-    for (int k = 0; k < WN; k++) w[k] = L.val(L.sb, k);
-    for (int y0 = 0; y0 < L.numbin; y0 += G) {
+    for (int k = 0; k < WN; k++) w[k] = L.val(L.sb, ybeg + k);
+    for (int y0 = ybeg; y0 < yend; y0 += G) {
 #pragma unroll
         for (int k = 0; k < G; k++) ga[k] = same ? w[k] : L.val(L.sa, y0 + k);
         double gs = 0.0;
@@ -356,17 +339,20 @@ __device__ void high_level(const L3Ctx &L, bool same, const double *alpha, const
     for (int i = 0; i < 8; i++) cross[i] += sab[i];
 }
 
-__global__ void __launch_bounds__(64) k_corr_high(MainArgs A, double *hpart /* [pair][nhigh][3] */, int nhigh) {
+// One thread per (pair, level >= 3, time chunk of the main kernel).
+__global__ void __launch_bounds__(64) k_corr_high(MainArgs A, double *hpart /* [chunk][pair][nhigh][3] */,
+                                                  int nhigh) {
     const int pidx = blockIdx.x * 64 + threadIdx.x;
     const int c = 3 + blockIdx.y;
+    const int chunk = blockIdx.z;
     const CorrTables &ct = *A.ctp;
     if (pidx >= A.pd.npairs || c >= ct.nlevels) return;
     int sa, sb;
     pair_series(A.pd, pidx, sa, sb);
     L3Ctx L{A.l3, A.g.nseries, sa, sb, c, ct.numbin[c]};
-    const double *Fa = A.F + (int64_t)sa * ct.npos, *Fb = A.F + (int64_t)sb * ct.npos;
-    const double mua = Fa[ct.pos_total] / A.g.T * (double)(1 << c);
-    const double mub = Fb[ct.pos_total] / A.g.T * (double)(1 << c);
+    const double *Fa = A.F + (int64_t)A.g.slot_of[sa] * ct.npos, *Fb = A.F + (int64_t)A.g.slot_of[sb] * ct.npos;
+    const double mua = level_shift(A, sa, c);
+    const double mub = level_shift(A, sb, c);
     const int x0 = ct.first_chan[c];
     double alpha[8], beta[8];
     for (int i = 0; i < 8; i++) {
@@ -379,15 +365,18 @@ __global__ void __launch_bounds__(64) k_corr_high(MainArgs A, double *hpart /* [
     const int B = 1 << (indp - c);
     const int pnum = (int)floor(ct.mtab_last / pow(2.0, fmax(indp - ct.log2_samp_last, 0.0)));
     double cross[8], a1[8], a2[8];
+    const int len = A.chunk_len >> c;
+    const int ybeg = chunk * len;
+    const int yend = (chunk == A.nchunk - 1) ? ct.numbin[c] : ybeg + len;
     int G = min(16, 1 << max(ct.q - 1 - c, 0));
     switch (G) {
-        case 16: high_level<16>(L, sa == sb, alpha, beta, B, pnum, cross, a1, a2); break;
-        case 8: high_level<8>(L, sa == sb, alpha, beta, B, pnum, cross, a1, a2); break;
-        case 4: high_level<4>(L, sa == sb, alpha, beta, B, pnum, cross, a1, a2); break;
-        case 2: high_level<2>(L, sa == sb, alpha, beta, B, pnum, cross, a1, a2); break;
-        default: high_level<1>(L, sa == sb, alpha, beta, B, pnum, cross, a1, a2); break;
+        case 16: high_level<16>(L, sa == sb, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2); break;
+        case 8: high_level<8>(L, sa == sb, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2); break;
+        case 4: high_level<4>(L, sa == sb, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2); break;
+        case 2: high_level<2>(L, sa == sb, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2); break;
+        default: high_level<1>(L, sa == sb, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2); break;
     }
-    double *out = hpart + ((int64_t)pidx * nhigh + (x0 - kMainChan)) * 3;
+    double *out = hpart + (((int64_t)chunk * A.pd.npairs + pidx) * nhigh + (x0 - kMainChan)) * 3;
     for (int i = 0; i < 8; i++) {
         out[i * 3 + 0] = cross[i];
         out[i * 3 + 1] = a1[i];
@@ -404,12 +393,12 @@ __global__ void k_corr_finalize(MainArgs A, const double *hpart, int nhigh, doub
     int sa, sb;
     pair_series(A.pd, pidx, sa, sb);
     const int c = ct.level[x];
-    const double *Fa = A.F + (int64_t)sa * ct.npos, *Fb = A.F + (int64_t)sb * ct.npos;
+    const double *Fa = A.F + (int64_t)A.g.slot_of[sa] * ct.npos, *Fb = A.F + (int64_t)A.g.slot_of[sb] * ct.npos;
     const double n = (double)(ct.numbin[c] - ct.del[x]);
     const double am = Fa[ct.pos_a_end[x]] / n;
     const double bm = (Fb[ct.pos_b_end[x]] - Fb[ct.pos_b_beg[x]]) / n;
-    const double mua = Fa[ct.pos_total] / A.g.T * (double)(1 << c);
-    const double mub = Fb[ct.pos_total] / A.g.T * (double)(1 << c);
+    const double mua = level_shift(A, sa, c);
+    const double mub = level_shift(A, sb, c);
     const double alpha = (double)(float)(am - mua), beta = (double)(float)(bm - mub);
     double cross = 0.0, a1 = 0.0, a2 = 0.0;
     if (x < kMainChan) {
@@ -420,10 +409,12 @@ __global__ void k_corr_finalize(MainArgs A, const double *hpart, int nhigh, doub
             a2 += (double)o[2];
         }
     } else {
-        const double *o = hpart + ((int64_t)pidx * nhigh + (x - kMainChan)) * 3;
-        cross = o[0];
-        a1 = o[1];
-        a2 = o[2];
+        for (int k = 0; k < A.nchunk; k++) {
+            const double *o = hpart + (((int64_t)k * A.pd.npairs + pidx) * nhigh + (x - kMainChan)) * 3;
+            cross += o[0];
+            a1 += o[1];
+            a2 += o[2];
+        }
     }
     const int indp = A.ind[pidx];
     const double B = (double)(1 << (indp - c));
@@ -440,8 +431,9 @@ __global__ void k_corr_finalize(MainArgs A, const double *hpart, int nhigh, doub
 
 bool correlate_fast_supported(const CorrTables &ct) {
     // level layout of p = 16 with at least levels 0..3 and blocks >= 16
-    // values on level 2 (ind >= q - 1 >= 6)
-    return ct.nlevels >= 7 && ct.nlevels <= 12 && ct.q == ct.nlevels && ct.numbin[3] >= 32;
+    // values on level 2 (ind >= q - 1 >= 6); blocks never longer than the
+    // 2^(blocknum-1)-frame chunks (ind <= blocknum - 1 needs blocknum >= q)
+    return ct.nlevels >= 7 && ct.nlevels <= 12 && ct.q == ct.nlevels && ct.numbin[3] >= 32 && ct.blocknum >= ct.q;
 }
 
 int corr_fast_chunk_len(const CorrTables &ct, int npairs) {
@@ -452,15 +444,15 @@ int corr_fast_chunk_len(const CorrTables &ct, int npairs) {
 }
 
 void launch_correlate_fast(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const CorrTables *dct,
-                           const PairDesc &pd,
-                           const double *F, const int32_t *ind, float *part, double *hpart, float *l3,
-                           int chunk_len, double *corav, double *bvar, hipStream_t st) {
+                           const PairDesc &pd, const double *F, const float *X, const int32_t *ind, float *part,
+                           double *hpart, float *l3, int chunk_len, double *corav, double *bvar, hipStream_t st) {
     MainArgs A;
     A.g = g;
     A.bd = bd;
     A.ctp = dct;
     A.pd = pd;
     A.F = F;
+    A.X = X;
     A.ind = ind;
     A.chunk_len = chunk_len;
     A.nchunk = (g.T + chunk_len - 1) / chunk_len;
@@ -472,7 +464,8 @@ void launch_correlate_fast(const SeriesGeom &g, const BleachDesc &bd, const Corr
     const int threads = pd.has_ccf ? 640 : 320;
     hipLaunchKernelGGL(k_corr_main, dim3(tiles, A.nchunk), dim3(threads), 0, st, A);
     const int nhigh = ct.chanum - kMainChan;
-    hipLaunchKernelGGL(k_corr_high, dim3((pd.npairs + 63) / 64, ct.nlevels - 3), dim3(64), 0, st, A, hpart, nhigh);
+    hipLaunchKernelGGL(k_corr_high, dim3((pd.npairs + 63) / 64, ct.nlevels - 3, A.nchunk), dim3(64), 0, st, A, hpart,
+                       nhigh);
     hipLaunchKernelGGL(k_corr_finalize, dim3(pd.npairs), dim3(128), 0, st, A, (const double *)hpart, nhigh, corav,
                        bvar);
 }

@@ -19,6 +19,11 @@ __device__ __forceinline__ void series_origin(const SeriesGeom &g, int s, int &o
     }
 }
 
+__device__ __forceinline__ void slot_origin(const SeriesGeom &g, int slot, int &ox, int &oy) {
+    ox = g.slot_org[2 * slot];
+    oy = g.slot_org[2 * slot + 1];
+}
+
 // Exact integer window sum minus background (IFCS:9668-9678).
 __device__ __forceinline__ int raw_u16(const SeriesGeom &g, int ox, int oy, int f) {
     const uint16_t *base = (const uint16_t *)g.data + (int64_t)(g.f0 + f) * g.plane +
@@ -30,25 +35,67 @@ __device__ __forceinline__ int raw_u16(const SeriesGeom &g, int ox, int oy, int 
     return acc - g.bckg_total;
 }
 
+// Window sums (uint16 source, minus background) of 16 consecutive frames
+// f0 .. f0+15; frames past fmax repeat frame fmax.  The frame base address
+// is wave-uniform (f0 is), so each load needs one 32-bit lane offset; the
+// window loops are outermost so the 16 loads of a window element are in
+// flight together.
+__device__ __forceinline__ void raw_ints16(const SeriesGeom &g, int ox, int oy, int f0, int fmax, int (&acc)[16]) {
+    const int nv = min(16, fmax - f0 + 1);
+    const uint32_t pl = (uint32_t)g.plane;
+    const uint32_t lo = (uint32_t)(oy * g.stride + ox);
+    const uint16_t *q = (const uint16_t *)g.data + (int64_t)(g.f0 + f0) * g.plane;
+#pragma unroll
+    for (int j = 0; j < 16; j++) acc[j] = -g.bckg_total;
+    for (int k = 0; k < g.by; k++)
+        for (int i = 0; i < g.bx; i++) {
+            const uint32_t d = lo + (uint32_t)(k * g.stride + i);
+            uint16_t r[16];
+#pragma unroll
+            for (int j = 0; j < 16; j++) r[j] = q[d + (uint32_t)min(j, nv - 1) * pl];
+#pragma unroll
+            for (int j = 0; j < 16; j++) acc[j] += r[j];
+        }
+}
+
+__device__ __forceinline__ void raw_values16(const SeriesGeom &g, int ox, int oy, int f0, int fmax, double (&v)[16]) {
+    if (g.src == kSrcU16) {
+        int acc[16];
+        raw_ints16(g, ox, oy, f0, fmax, acc);
+#pragma unroll
+        for (int j = 0; j < 16; j++) v[j] = (double)acc[j];
+    } else {
+        const int nv = min(16, fmax - f0 + 1);
+        const uint32_t pl = (uint32_t)g.plane;
+        const uint32_t lo = (uint32_t)(oy * g.stride + ox);
+        const float *q = (const float *)g.data + (int64_t)(g.f0 + f0) * g.plane;
+#pragma unroll
+        for (int j = 0; j < 16; j++) v[j] = (double)q[lo + (uint32_t)min(j, nv - 1) * pl];
+    }
+}
+
 __device__ __forceinline__ double raw_value(const SeriesGeom &g, int ox, int oy, int f) {
     if (g.src == kSrcU16) return (double)raw_u16(g, ox, oy, f);
     const float *base = (const float *)g.data + (int64_t)(g.f0 + f) * g.plane + (int64_t)oy * g.stride + ox;
     return (double)base[0];
 }
 
-// Per-series bleach polynomial held in registers, normalized by its value
-// at t = 0 so that g(u) = f(u) / c0.
-struct Bleach {
-    double c[kMaxOrder + 1];
-    int order;
-    double c0, scale;   // u = (f + off) * scale
-    double off;
-    __device__ void load(const BleachDesc &bd, int s) {
-        order = bd.order;
-        c0 = (order > 0) ? bd.coef[(int64_t)s * (order + 1)] : 1.0;
+// Per-slot bleach polynomial held in registers, normalized by its value at
+// t = 0 so that g(u) = f(u) / c0.  MAXO is a compile-time degree >= the run's
+// order; the coefficients above the order are zero, which leaves Horner's
+// result unchanged (0 * u + 0 = 0 until the leading coefficient).
+template <int MAXO>
+struct BleachP {
+    double c[MAXO + 1];
+    double c0, scale, off;
+    bool on;
+    __device__ void load(const BleachDesc &bd, int slot) {
+        const int order = bd.order;
+        on = order > 0;
+        c0 = on ? bd.coef[(int64_t)slot * (order + 1)] : 1.0;
 #pragma unroll
-        for (int k = 0; k <= kMaxOrder; k++)
-            c[k] = (k <= order && order > 0) ? bd.coef[(int64_t)s * (order + 1) + k] / c0 : 0.0;
+        for (int k = 0; k <= MAXO; k++)
+            c[k] = (on && k <= order) ? bd.coef[(int64_t)slot * (order + 1) + k] / c0 : 0.0;
         scale = bd.frametime / bd.tscale;
         off = bd.t_off;
     }
@@ -56,17 +103,19 @@ struct Bleach {
     // 1/sqrt(g) from the FP32 reciprocal square root refined by one FP64
     // Newton step (relative error ~1e-14).  g <= 0 yields NaN as in Java.
     __device__ __forceinline__ double apply(double v, int f) const {
-        if (order == 0) return v;
-        double u = ((double)f + off) * scale;
+        if (MAXO == 0 || !on) return v;
+        const double u = ((double)f + off) * scale;
         double g = 0.0;
 #pragma unroll
-        for (int k = kMaxOrder; k >= 0; k--)
-            if (k <= order) g = fma(g, u, c[k]);
+        for (int k = MAXO; k >= 0; k--) g = fma(g, u, c[k]);
         double r = (double)rsqrtf((float)g);
         r = r * fma(-0.5 * g * r, r, 1.5);
-        double sq = g * r;
+        const double sq = g * r;
         return fma(v, r, c0 * (1.0 - sq));
     }
 };
+
+// Compile-time degree used for a run's polynomial order.
+inline int bleach_maxo(int order) { return order <= 0 ? 0 : (order <= 4 ? 4 : kMaxOrder); }
 
 }  // namespace imfcs

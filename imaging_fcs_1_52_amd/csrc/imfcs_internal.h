@@ -15,7 +15,10 @@ enum SourceType : int32_t { kSrcU16 = 0, kSrcF32 = 1 };
 
 // Where the intensity series live and how one series value is formed.
 // Series s < npix is output pixel s (row-major over ow x oh); series
-// s >= npix is the CCF partner of pixel s - npix.
+// s >= npix is the CCF partner of pixel s - npix.  Distinct series are
+// held once, in "slots": a CCF partner that coincides with another output
+// pixel (partner offset a multiple of the pixel pitch) shares its slot.
+// Slot k reads the window at (slot_org[2k], slot_org[2k+1]).
 struct SeriesGeom {
     const void *data;
     int32_t src;          // SourceType
@@ -30,7 +33,19 @@ struct SeriesGeom {
     int32_t pbx, pby;     // stride between map pixels
     int32_t cfx, cfy;     // partner offset
     int32_t nseries;
+    int32_t nslots;
+    const int32_t *slot_of;    // [nseries] device
+    const int32_t *slot_org;   // [nslots][2] device
 };
+
+// The bleach-corrected series of every slot, stored once per pipeline run
+// as FP32 deviations from a per-slot shift mu (X[f * nslots + slot] =
+// I'(f) - mu): mu is the constant term c0 of the bleach polynomial, around
+// which the corrected series fluctuates (I' -> c0 when I follows the fit),
+// or 0 without correction, where I' is an integer and X exact.
+__host__ __device__ inline double slot_mu(const double *coef, int order, int slot) {
+    return order > 0 ? coef[(int64_t)slot * (order + 1)] : 0.0;
+}
 
 // Polynomial bleach correction (per series) in a normalized abscissa
 // u = (frame + t_off) * frametime / tscale.  coef[s][k] are the
@@ -88,23 +103,32 @@ void launch_trace_moments(const SeriesGeom &g, int nopit, int ave, double tscale
                           int order, int integer_trace, double *part, int nchunk, hipStream_t st);
 void launch_polyfit(const double *part, int nchunk, int nseries, int nopit, int ave, double tscale,
                     double frametime, int order, double *coef, hipStream_t st);
-void launch_prefix(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const CorrTables *dct,
-                   double *ctot, double *rec, int nchunk, int chunk_len, hipStream_t st);
-void launch_prefix_combine(const CorrTables *dct, int nseries, const double *ctot, const double *rec,
+void launch_apply(const SeriesGeom &g, const BleachDesc &bd, const CorrTables *dct, double *ctot, double *rec,
+                  float *X, int nchunk, int chunk_len, hipStream_t st);
+void launch_prefix_combine(const CorrTables *dct, int nslots, const double *ctot, const double *rec,
                            int nchunk, int chunk_len, double *F, hipStream_t st);
+// Work split of the blocking pass: products in chunks of 256 (levels 0..7
+// inside a chunk, levels >= 8 cascaded from the chunk totals).
+struct BlockingPlan {
+    int nsplit;            // grid.y
+    int chunks_per_split;  // 256-product chunks per thread
+    int nchunk_prod;       // chunks of the whole series (csum rows)
+};
+BlockingPlan plan_blocking(const CorrTables &ct, int npix);
 void launch_blocking(const SeriesGeom &g, const BleachDesc &bd, const CorrTables *dct, const PairDesc &pd,
-                     const double *F, double *bpart, int nchunk, int chunk_len, hipStream_t st);
-void launch_blocking_finalize(const CorrTables &ct, const CorrTables *dct, const PairDesc &pd, const double *F,
-                              const double *bpart, int nchunk, int32_t *ind, int32_t *bis, hipStream_t st);
+                     const double *F, const float *X, const BlockingPlan &bp, double *bpart, double *csum,
+                     hipStream_t st);
+void launch_blocking_finalize(const CorrTables &ct, const CorrTables *dct, const SeriesGeom &g, const PairDesc &pd,
+                              const double *F, const BlockingPlan &bp, const double *bpart, const double *csum,
+                              int32_t *ind, int32_t *bis, hipStream_t st);
 void launch_correlate(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const CorrTables *dct,
-                      const PairDesc &pd, const double *F, const int32_t *ind, double *corav, double *bvar,
-                      hipStream_t st);
+                      const PairDesc &pd, const double *F, const float *X, const int32_t *ind, double *corav,
+                      double *bvar, hipStream_t st);
 bool correlate_fast_supported(const CorrTables &ct);
 int corr_fast_chunk_len(const CorrTables &ct, int npairs);
 void launch_correlate_fast(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const CorrTables *dct,
-                           const PairDesc &pd,
-                           const double *F, const int32_t *ind, float *part, double *hpart, float *l3,
-                           int chunk_len, double *corav, double *bvar, hipStream_t st);
+                           const PairDesc &pd, const double *F, const float *X, const int32_t *ind, float *part,
+                           double *hpart, float *l3, int chunk_len, double *corav, double *bvar, hipStream_t st);
 void launch_lm_acf(int nfits, int npts, const float *data, const float *weights, const float *x,
                    const float *init, const int *pfit, unsigned free_mask, const LMModelGeom &geom, float tol, int maxit,
                    float *out_params, int *states, float *chi2, int *iters, hipStream_t st);

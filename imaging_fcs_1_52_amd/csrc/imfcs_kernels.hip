@@ -30,11 +30,11 @@ namespace imfcs {
 // ------------------------------------------------------------------
 __global__ void k_trace_moments(SeriesGeom g, int nopit, int ave, double tscale, double frametime, int order,
                                 int integer_trace, int pts_per_chunk, double *part) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    int s = blockIdx.x * blockDim.x + threadIdx.x;   // slot
     int chunk = blockIdx.y;
-    if (s >= g.nseries) return;
+    if (s >= g.nslots) return;
     int ox, oy;
-    series_origin(g, s, ox, oy);
+    slot_origin(g, s, ox, oy);
     int x0 = chunk * pts_per_chunk, x1 = min(nopit, x0 + pts_per_chunk);
     double acc[kMaxOrder + 1];
 #pragma unroll
@@ -44,7 +44,13 @@ __global__ void k_trace_moments(SeriesGeom g, int nopit, int ave, double tscale,
         if (integer_trace && g.src == kSrcU16) {
             // Java int sum and int division by ave (IFCS:9841-9852)
             int isum = 0;
-            for (int f = x * ave; f < (x + 1) * ave; f++) isum += raw_u16(g, ox, oy, f);
+            const int fe = (x + 1) * ave;
+            for (int f = x * ave; f < fe; f += 16) {
+                int v[16];
+                raw_ints16(g, ox, oy, f, fe - 1, v);
+#pragma unroll
+                for (int i = 0; i < 16; i++) isum += (f + i < fe) ? v[i] : 0;
+            }
             y = (double)(isum / ave);
         } else {
             double sum = 0.0;
@@ -58,7 +64,7 @@ __global__ void k_trace_moments(SeriesGeom g, int nopit, int ave, double tscale,
             up *= u;
         }
     }
-    int64_t base = ((int64_t)chunk * g.nseries + s) * (order + 1);
+    int64_t base = ((int64_t)chunk * g.nslots + s) * (order + 1);
 #pragma unroll
     for (int k = 0; k <= kMaxOrder; k++)
         if (k <= order) part[base + k] = acc[k];
@@ -107,53 +113,69 @@ __global__ void k_polyfit(const double *part, int nchunk, int nseries, int order
 }
 
 // ------------------------------------------------------------------
-// Stage 2: prefix sums of the corrected series at the positions that
-// define every channel's window means.
+// Stage 2: bleach correction of every slot, written once as FP32
+// deviations X = I' - mu, and the FP64 prefix sums of I' at the positions
+// that form every channel's window means ("direct/delayed monitors",
+// IFCS:9487-9494, 9262-9270).  One thread per (slot, frame chunk); frames
+// are read in groups of 16 so the loads of a group are in flight together.
 // ------------------------------------------------------------------
-__global__ void k_prefix(SeriesGeom g, BleachDesc bd, const CorrTables *__restrict__ ctp, int chunk_len, double *ctot,
-                         double *rec) {
+template <int MAXO>
+__global__ void __launch_bounds__(64) k_apply(SeriesGeom g, BleachDesc bd, const CorrTables *__restrict__ ctp,
+                                              int chunk_len, double *ctot, double *rec, float *X) {
     const CorrTables &ct = *ctp;
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    int chunk = blockIdx.y;
-    int nchunk = gridDim.y;
-    if (s >= g.nseries) return;
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    const int chunk = blockIdx.y, nchunk = gridDim.y;
+    if (k >= g.nslots) return;
     int ox, oy;
-    series_origin(g, s, ox, oy);
-    Bleach bl;
-    bl.load(bd, s);
-    int f0 = chunk * chunk_len, f1 = min(g.T, f0 + chunk_len);
+    slot_origin(g, k, ox, oy);
+    BleachP<MAXO> bl;
+    bl.load(bd, k);
+    const double mu = slot_mu(bd.coef, bd.order, k);
+    const int f0 = chunk * chunk_len, f1 = min(g.T, f0 + chunk_len);
     const int npos = ct.npos;
-    int k = 0;
-    while (k < npos && ct.pos[k] < f0) k++;
-    int next = (k < npos) ? ct.pos[k] : 0x7fffffff;
+    int q = 0;
+    while (q < npos && ct.pos[q] < f0) q++;
+    int next = (q < npos) ? ct.pos[q] : 0x7fffffff;
     double run = 0.0;
-    for (int f = f0; f < f1; f++) {
-        while (next == f) {
-            rec[(int64_t)s * npos + k] = run;
-            k++;
-            next = (k < npos) ? ct.pos[k] : 0x7fffffff;
+    const uint32_t ld = (uint32_t)g.nslots;
+    for (int fb = f0; fb < f1; fb += 16) {
+        double v[16];
+        raw_values16(g, ox, oy, fb, f1 - 1, v);
+        float *xw = X + (int64_t)fb * g.nslots;   // wave-uniform
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int f = fb + i;
+            if (f < f1) {
+                while (next == f) {
+                    rec[(int64_t)k * npos + q] = run;
+                    q++;
+                    next = (q < npos) ? ct.pos[q] : 0x7fffffff;
+                }
+                const double a = bl.apply(v[i], f);
+                run += a;
+                xw[(uint32_t)i * ld + (uint32_t)k] = (float)(a - mu);
+            }
         }
-        run += bl.apply(raw_value(g, ox, oy, f), f);
     }
     if (chunk == nchunk - 1)
-        while (k < ct.npos && ct.pos[k] >= f1) {
-            rec[(int64_t)s * ct.npos + k] = run;
-            k++;
+        while (q < npos && ct.pos[q] >= f1) {
+            rec[(int64_t)k * npos + q] = run;
+            q++;
         }
-    ctot[(int64_t)chunk * g.nseries + s] = run;
+    ctot[(int64_t)chunk * g.nslots + k] = run;
 }
 
-__global__ void k_prefix_combine(const CorrTables *__restrict__ ctp, int nseries, const double *ctot, const double *rec, int nchunk,
-                                 int chunk_len, double *F) {
+__global__ void k_prefix_combine(const CorrTables *__restrict__ ctp, int nslots, const double *ctot,
+                                 const double *rec, int nchunk, int chunk_len, double *F) {
     const CorrTables &ct = *ctp;
     int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nseries) return;
+    if (s >= nslots) return;
     double acc = 0.0;
     int c = 0;
     for (int k = 0; k < ct.npos; k++) {
         int ck = min(ct.pos[k] / chunk_len, nchunk - 1);
         while (c < ck) {
-            acc += ctot[(int64_t)c * nseries + s];
+            acc += ctot[(int64_t)c * nslots + s];
             c++;
         }
         F[(int64_t)s * ct.npos + k] = acc + rec[(int64_t)s * ct.npos + k];
@@ -164,88 +186,224 @@ __global__ void k_prefix_combine(const CorrTables *__restrict__ ctp, int nseries
 // Stage 3: lag-1 blocking curve (blockTransform).  The Java copies the
 // 1-based trace into a 0-based array, so the series is a'_y = a[y-1]
 // (a'_0 = 0) against b'_{y+1} = b[y], y = 0..T-2 (IFCS:9259-9262).
+//
+// One thread per output pixel computes the curve of its ACF and of its CCF
+// (both share the a series).  The products are cut into chunks of
+// 2^kBlkJ = 256, each processed as 16 groups of 16 products held in
+// registers.  Levels 0..3 are reduced inside a group, levels 4..7 by a
+// cascade over the (wave-uniform) group index, and the level-8 block -- the
+// chunk total -- goes to csum[chunk][pair] for the finalize kernel to
+// cascade into the levels >= 8.  Every level keeps the pairwise tree of
+// blockTransform, so each block value equals the Java block average times
+// 2^level up to the FP32 rounding of the stored series.
 // ------------------------------------------------------------------
-__global__ void k_blocking(SeriesGeom g, BleachDesc bd, const CorrTables *__restrict__ ctp, PairDesc pd,
-                           const double *F, double *bpart, int chunk_len) {
+constexpr int kBlkJ = 8;
+
+template <bool FULL>
+__device__ __forceinline__ void blk_group_stats(const double (&c)[16], int lim, double (&s1)[kBlkJ],
+                                                double (&s2)[kBlkJ], double &h) {
+    double d[8], e[4], f2[2];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        double v = (FULL || i < lim) ? c[i] : 0.0;
+        s1[0] += v;
+        s2[0] = fma(v, v, s2[0]);
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        d[j] = c[2 * j] + c[2 * j + 1];
+        double v = (FULL || 2 * j + 2 <= lim) ? d[j] : 0.0;
+        s1[1] += v;
+        s2[1] = fma(v, v, s2[1]);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        e[j] = d[2 * j] + d[2 * j + 1];
+        double v = (FULL || 4 * j + 4 <= lim) ? e[j] : 0.0;
+        s1[2] += v;
+        s2[2] = fma(v, v, s2[2]);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        f2[j] = e[2 * j] + e[2 * j + 1];
+        double v = (FULL || 8 * j + 8 <= lim) ? f2[j] : 0.0;
+        s1[3] += v;
+        s2[3] = fma(v, v, s2[3]);
+    }
+    h = f2[0] + f2[1];
+}
+
+// Levels 4..8 of one curve over the group index gi of a full group.
+struct BlkCascade {
+    double pend4, pend5, pend6, pend7;
+    __device__ __forceinline__ void reset() { pend4 = pend5 = pend6 = pend7 = 0.0; }
+    __device__ __forceinline__ void push(double carry, int gi, double (&s1)[kBlkJ], double (&s2)[kBlkJ],
+                                         double *csum_out) {
+        s1[4] += carry;
+        s2[4] = fma(carry, carry, s2[4]);
+        if (!(gi & 1)) { pend4 = carry; return; }
+        carry += pend4;
+        s1[5] += carry;
+        s2[5] = fma(carry, carry, s2[5]);
+        if (!(gi & 2)) { pend5 = carry; return; }
+        carry += pend5;
+        s1[6] += carry;
+        s2[6] = fma(carry, carry, s2[6]);
+        if (!(gi & 4)) { pend6 = carry; return; }
+        carry += pend6;
+        s1[7] += carry;
+        s2[7] = fma(carry, carry, s2[7]);
+        if (!(gi & 8)) { pend7 = carry; return; }
+        *csum_out = carry + pend7;
+    }
+};
+
+template <bool ACF, bool CCF>
+__global__ void __launch_bounds__(64) k_blocking(SeriesGeom g, const CorrTables *__restrict__ ctp, PairDesc pd,
+                                                 const double *F, const float *__restrict__ X, const double *coef,
+                                                 int order, double *bpart, double *csum, int chunks_per_split) {
     const CorrTables &ct = *ctp;
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    int chunk = blockIdx.y;
-    if (p >= pd.npairs) return;
-    int sa, sb;
-    pair_series(pd, p, sa, sb);
-    int oxa, oya, oxb, oyb;
-    series_origin(g, sa, oxa, oya);
-    series_origin(g, sb, oxb, oyb);
-    Bleach ba, bb;
-    ba.load(bd, sa);
-    bb.load(bd, sb);
+    const int p = blockIdx.x * 64 + threadIdx.x;
+    if (p >= pd.npix) return;
+    const int pa = p, pc = ACF ? pd.npix + p : p;   // pair indices of the ACF and the CCF
+    const int ka = g.slot_of[p], kb = CCF ? g.slot_of[pd.npix + p] : ka;
+    const int64_t ld = g.nslots;
     const int n0 = g.T - 1;
-    const double am = F[(int64_t)sa * ct.npos + ct.pos_blk_a] / n0;
-    const double bm = F[(int64_t)sb * ct.npos + ct.pos_blk_b] / n0;
-    const int K = ct.blocknum;
-    double pend[kMaxBlock], s1[kMaxBlock], s2[kMaxBlock];
+    const double mua = slot_mu(coef, order, ka), mub = slot_mu(coef, order, kb);
+    // means of a'_0..a'_{n0-1} and b'_1..b'_{n0}, relative to the shifts
+    const double alpha = F[(int64_t)ka * ct.npos + ct.pos_blk_a] / n0 - mua;
+    const double beta_a = F[(int64_t)ka * ct.npos + ct.pos_blk_b] / n0 - mua;
+    const double beta_b = F[(int64_t)kb * ct.npos + ct.pos_blk_b] / n0 - mub;
+    const int nchunk = (n0 + (1 << kBlkJ) - 1) >> kBlkJ;
+    const int t0 = blockIdx.y * chunks_per_split, t1 = min(nchunk, t0 + chunks_per_split);
+    double s1a[kBlkJ], s2a[kBlkJ], s1c[kBlkJ], s2c[kBlkJ];
 #pragma unroll
-    for (int k = 0; k < kMaxBlock; k++) pend[k] = s1[k] = s2[k] = 0.0;
-    int y0 = chunk * chunk_len, y1 = min(n0, y0 + chunk_len);
-    double aprev = (y0 == 0) ? 0.0 : ba.apply(raw_value(g, oxa, oya, y0 - 1), y0 - 1);
-    for (int y = y0; y < y1; y++) {
-        double av = ba.apply(raw_value(g, oxa, oya, y), y);
-        double bv = (sa == sb) ? av : bb.apply(raw_value(g, oxb, oyb, y), y);
-        double carry = (aprev - am) * (bv - bm);
-        aprev = av;
-        // binary cascade: the level-k block containing product y closes
-        // when the low k bits of y are all ones (y is wave-uniform)
-        bool open = true;
+    for (int k = 0; k < kBlkJ; k++) s1a[k] = s2a[k] = s1c[k] = s2c[k] = 0.0;
+    BlkCascade cas_a, cas_c;
+    for (int t = t0; t < t1; t++) {
+        const int y0 = t << kBlkJ;
+        // a'_{y-1} - abar for the first product of the chunk (a'_{-1} := 0)
+        double aprev = (y0 == 0) ? -mua - alpha : (double)X[(int64_t)(y0 - 1) * ld + ka] - alpha;
+        cas_a.reset();
+        cas_c.reset();
+        const int ngroups = min(16, (n0 - y0 + 15) >> 4);
+        for (int gi = 0; gi < ngroups; gi++) {
+            const int yb = y0 + 16 * gi;
+            const int lim = min(16, n0 - yb);   // valid products, wave-uniform
+            float xa[16], xb[16];
+            const float *xq = X + (int64_t)yb * ld;   // wave-uniform
+            const int nv = min(16, n0 - yb);
 #pragma unroll
-        for (int k = 0; k < kMaxBlock; k++) {
-            if (open && k < K) {
-                s1[k] += carry;
-                s2[k] = fma(carry, carry, s2[k]);
-                if (k == K - 1 || ((y >> k) & 1) == 0) {
-                    pend[k] = carry;
-                    open = false;
+            for (int i = 0; i < 16; i++) {
+                const uint32_t o = (uint32_t)min(i, nv - 1) * (uint32_t)ld;
+                xa[i] = xq[o + (uint32_t)ka];
+                if (CCF) xb[i] = xq[o + (uint32_t)kb];
+            }
+            double am[16];
+#pragma unroll
+            for (int i = 0; i < 16; i++) am[i] = i ? (double)xa[i - 1] - alpha : aprev;
+            aprev = (double)xa[15] - alpha;
+            double h;
+            if (ACF) {
+                double c[16];
+#pragma unroll
+                for (int i = 0; i < 16; i++) c[i] = am[i] * ((double)xa[i] - beta_a);
+                if (lim == 16) {
+                    blk_group_stats<true>(c, 16, s1a, s2a, h);
+                    cas_a.push(h, gi, s1a, s2a, &csum[(int64_t)t * pd.npairs + pa]);
                 } else {
-                    carry += pend[k];
+                    blk_group_stats<false>(c, lim, s1a, s2a, h);
+                }
+            }
+            if (CCF) {
+                double c[16];
+#pragma unroll
+                for (int i = 0; i < 16; i++) c[i] = am[i] * ((double)xb[i] - beta_b);
+                if (lim == 16) {
+                    blk_group_stats<true>(c, 16, s1c, s2c, h);
+                    cas_c.push(h, gi, s1c, s2c, &csum[(int64_t)t * pd.npairs + pc]);
+                } else {
+                    blk_group_stats<false>(c, lim, s1c, s2c, h);
                 }
             }
         }
     }
-    int64_t base = ((int64_t)chunk * pd.npairs + p) * K * 2;
+    if (ACF) {
+        const int64_t base = ((int64_t)blockIdx.y * pd.npairs + pa) * kBlkJ * 2;
 #pragma unroll
-    for (int k = 0; k < kMaxBlock; k++) {
-        if (k < K) {
-            bpart[base + 2 * k] = s1[k];
-            bpart[base + 2 * k + 1] = s2[k];
+        for (int k = 0; k < kBlkJ; k++) {
+            bpart[base + 2 * k] = s1a[k];
+            bpart[base + 2 * k + 1] = s2a[k];
+        }
+    }
+    if (CCF) {
+        const int64_t base = ((int64_t)blockIdx.y * pd.npairs + pc) * kBlkJ * 2;
+#pragma unroll
+        for (int k = 0; k < kBlkJ; k++) {
+            bpart[base + 2 * k] = s1c[k];
+            bpart[base + 2 * k + 1] = s2c[k];
         }
     }
 }
 
-__global__ void k_blocking_finalize(const CorrTables *__restrict__ ctp, PairDesc pd, int Tn, const double *F,
-                                    const double *bpart, int nchunk, int32_t *ind_out, int32_t *bis_out) {
+// bpart holds [nsplit][pair][kBlkJ][2] partial sums of the levels < kBlkJ;
+// the levels >= kBlkJ are cascaded from the 256-product chunk totals
+// csum[chunk][pair].
+__global__ void k_blocking_finalize(const CorrTables *__restrict__ ctp, SeriesGeom g, PairDesc pd,
+                                    const double *F, const double *bpart, int nsplit, const double *csum,
+                                    int32_t *ind_out, int32_t *bis_out) {
     const CorrTables &ct = *ctp;
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= pd.npairs) return;
     int sa, sb;
     pair_series(pd, p, sa, sb);
-    const int n0 = Tn - 1;
-    const double am = F[(int64_t)sa * ct.npos + ct.pos_blk_a] / n0;
-    const double bm = F[(int64_t)sb * ct.npos + ct.pos_blk_b] / n0;
+    const int ka = g.slot_of[sa], kb = g.slot_of[sb];
+    const int n0 = g.T - 1;
+    const double am = F[(int64_t)ka * ct.npos + ct.pos_blk_a] / n0;
+    const double bm = F[(int64_t)kb * ct.npos + ct.pos_blk_b] / n0;
     const int K = ct.blocknum;
     double sd[kMaxBlock], upper[kMaxBlock], lower[kMaxBlock];
     int crt[kMaxBlock], cr12[kMaxBlock], cr3[kMaxBlock], diffpos[kMaxBlock];
     double dd = am * bm;
     double nb = (double)n0;
+    double S1[kMaxBlock], S2[kMaxBlock], pend[kMaxBlock];
+#pragma unroll
+    for (int k = 0; k < kMaxBlock; k++) {
+        S1[k] = S2[k] = pend[k] = 0.0;
+        if (k < K && k < kBlkJ) {
+            for (int c = 0; c < nsplit; c++) {
+                int64_t base = ((int64_t)c * pd.npairs + p) * kBlkJ * 2;
+                S1[k] += bpart[base + 2 * k];
+                S2[k] += bpart[base + 2 * k + 1];
+            }
+        }
+    }
+    if (K > kBlkJ) {
+        const int nfull = n0 >> kBlkJ;
+        for (int t = 0; t < nfull; t++) {
+            double carry = csum[(int64_t)t * pd.npairs + p];
+            bool open = true;
+#pragma unroll
+            for (int k = kBlkJ; k < kMaxBlock; k++) {
+                if (open && k < K) {
+                    S1[k] += carry;
+                    S2[k] = fma(carry, carry, S2[k]);
+                    if (((t >> (k - kBlkJ)) & 1) == 0) {
+                        pend[k] = carry;
+                        open = false;
+                    } else {
+                        carry += pend[k];
+                    }
+                }
+            }
+        }
+    }
 #pragma unroll
     for (int k = 0; k < kMaxBlock; k++) {
         sd[k] = upper[k] = lower[k] = 0.0;
         crt[k] = cr12[k] = cr3[k] = diffpos[k] = 0;
         if (k < K) {
-            double s1 = 0.0, s2 = 0.0;
-            for (int c = 0; c < nchunk; c++) {
-                int64_t base = ((int64_t)c * pd.npairs + p) * K * 2;
-                s1 += bpart[base + 2 * k];
-                s2 += bpart[base + 2 * k + 1];
-            }
+            double s1 = S1[k], s2 = S2[k];
             if (k > 0) nb = floor(nb / 2);
             double scale = ldexp(1.0, -k);
             double sp = s1 * scale, sp2 = s2 * scale * scale;
@@ -304,25 +462,19 @@ __global__ void k_blocking_finalize(const CorrTables *__restrict__ ctp, PairDesc
 //   block sum S = Sab - beta Sa - alpha Sb + B alpha beta
 // ------------------------------------------------------------------
 struct LevelCtx {
-    const SeriesGeom *g;
-    int oxa, oya, oxb, oyb, c, numbin, same;
-    double shift_a, shift_b;
-    const Bleach *ba, *bb;
-    __device__ __forceinline__ double val_a(int y) const {
+    const float *X;
+    int64_t ld;
+    int ka, kb, c, numbin, same;
+    // level-c value: sum of 2^c stored deviations, i.e. I'-sum minus 2^c mu
+    __device__ __forceinline__ double val(int k, int y) const {
         if (y >= numbin) return 0.0;
-        int f0 = y << c;
+        const int64_t f0 = (int64_t)y << c;
         double acc = 0.0;
-        for (int i = 0; i < (1 << c); i++) acc += ba->apply(raw_value(*g, oxa, oya, f0 + i), f0 + i);
-        return acc - shift_a;
+        for (int i = 0; i < (1 << c); i++) acc += (double)X[(f0 + i) * ld + k];
+        return acc;
     }
-    __device__ __forceinline__ double val_b(int y) const {
-        if (same) return val_a(y);
-        if (y >= numbin) return 0.0;
-        int f0 = y << c;
-        double acc = 0.0;
-        for (int i = 0; i < (1 << c); i++) acc += bb->apply(raw_value(*g, oxb, oyb, f0 + i), f0 + i);
-        return acc - shift_b;
-    }
+    __device__ __forceinline__ double val_a(int y) const { return val(ka, y); }
+    __device__ __forceinline__ double val_b(int y) const { return val(kb, y); }
 };
 
 template <int G, int NL, int DMIN>
@@ -398,8 +550,8 @@ __device__ void correlate_level(const LevelCtx &L, const CorrTables &ct, int cha
 }
 
 __global__ void __launch_bounds__(64) k_correlate(SeriesGeom g, BleachDesc bd, const CorrTables *__restrict__ ctp,
-                                                  PairDesc pd, const double *F, const int32_t *ind, double *corav,
-                                                  double *bvar) {
+                                                  PairDesc pd, const double *F, const float *X, const int32_t *ind,
+                                                  double *corav, double *bvar) {
     const CorrTables &ct = *ctp;
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     int c = blockIdx.y;
@@ -407,21 +559,16 @@ __global__ void __launch_bounds__(64) k_correlate(SeriesGeom g, BleachDesc bd, c
     int sa, sb;
     pair_series(pd, p, sa, sb);
     LevelCtx L;
-    L.g = &g;
-    series_origin(g, sa, L.oxa, L.oya);
-    series_origin(g, sb, L.oxb, L.oyb);
+    L.X = X;
+    L.ld = g.nslots;
+    L.ka = g.slot_of[sa];
+    L.kb = g.slot_of[sb];
     L.c = c;
     L.numbin = ct.numbin[c];
-    L.same = (sa == sb);
-    Bleach ba, bb;
-    ba.load(bd, sa);
-    bb.load(bd, sb);
-    L.ba = &ba;
-    L.bb = &bb;
-    const double *Fa = F + (int64_t)sa * ct.npos, *Fb = F + (int64_t)sb * ct.npos;
-    double mu_a = Fa[ct.pos_total] / g.T, mu_b = Fb[ct.pos_total] / g.T;
-    L.shift_a = mu_a * (double)(1 << c);
-    L.shift_b = mu_b * (double)(1 << c);
+    L.same = (L.ka == L.kb);
+    const double *Fa = F + (int64_t)L.ka * ct.npos, *Fb = F + (int64_t)L.kb * ct.npos;
+    const double shift_a = slot_mu(bd.coef, bd.order, L.ka) * (double)(1 << c);
+    const double shift_b = slot_mu(bd.coef, bd.order, L.kb) * (double)(1 << c);
     const int chan0 = ct.first_chan[c], nch = ct.nchan[c];
     double am[17], bmn[17], alpha[17], beta[17];
     for (int i = 0; i < nch; i++) {
@@ -429,8 +576,8 @@ __global__ void __launch_bounds__(64) k_correlate(SeriesGeom g, BleachDesc bd, c
         double n = (double)(ct.numbin[c] - ct.del[x]);
         am[i] = Fa[ct.pos_a_end[x]] / n;
         bmn[i] = (Fb[ct.pos_b_end[x]] - Fb[ct.pos_b_beg[x]]) / n;
-        alpha[i] = am[i] - L.shift_a;
-        beta[i] = bmn[i] - L.shift_b;
+        alpha[i] = am[i] - shift_a;
+        beta[i] = bmn[i] - shift_b;
     }
     int indp = ind[p];
     int B = 1 << (indp - c);
@@ -560,6 +707,25 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// FP32 chi-square with the reduction order of Gpufit: points are grouped in
+// pieces of 512, each piece is summed as a binary tree in FP32 (strides
+// 256, 128, ..., 1, agpufitjni.cu:1326-1345, 1546-1603, with 4 x the next
+// power of two threads per fit, 2722-2731) and the pieces are added in FP64
+// (1385-1405).  Lane l holds the piece's elements l + 64 k in e[k].
+__device__ __forceinline__ float piece_tree(float (&e)[8]) {
+    e[0] += e[4];
+    e[1] += e[5];
+    e[2] += e[6];
+    e[3] += e[7];
+    e[0] += e[2];
+    e[1] += e[3];
+    e[0] += e[1];
+    float v = e[0];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return __shfl(v, 0, 64);
+}
+
 // In-place Cholesky solve of the n x n SPD system A x = b (FP64). Returns
 // false if A is not positive definite.
 __device__ bool chol_solve(double *A, double *b, int n) {
@@ -606,7 +772,7 @@ struct LMArgs {
 // Evaluates chi^2 and (optionally) gradient / hessian sums for the fit, with
 // every lane handling points lane, lane+64, ...  Results are wave totals.
 __device__ void lm_eval(const LMArgs &a, int fit, const float *par, const int *fidx, int nfree, bool need_jac,
-                        double &chi2, double *grad, double *hess) {
+                        float &chi2, double *grad, double *hess) {
     const int lane = threadIdx.x & 63;
     const float *y = a.data + (int64_t)fit * a.npts;
     const float *w = a.weights ? a.weights + (int64_t)fit * a.npts : nullptr;
@@ -616,7 +782,14 @@ __device__ void lm_eval(const LMArgs &a, int fit, const float *par, const int *f
     for (int i = 0; i < nh; i++) hl[i] = 0.0;
     bool c2on = (par[5] != 0.f) || a.pfit[5] || a.pfit[6];
     bool c3on = (par[7] != 0.f) || a.pfit[7] || a.pfit[8];
-    for (int i = lane; i < a.npts; i += 64) {
+    for (int j0 = 0; j0 < a.npts; j0 += 512) {
+      float e[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) e[k] = 0.f;
+#pragma unroll 1
+      for (int k = 0; k < 8; k++) {
+        const int i = j0 + 64 * k + lane;
+        if (i >= a.npts) continue;
         float xi = a.x[i];
         float jf[kLMMaxFree];
         float f;
@@ -640,19 +813,24 @@ __device__ void lm_eval(const LMArgs &a, int fit, const float *par, const int *f
             if (need_jac)
                 for (int k = 0; k < nfree; k++) jf[k] = gfull[fidx[k]];
         }
+        const float dev = f - y[i];
+        const float ek = w ? dev * dev * w[i] : dev * dev;
+#pragma unroll
+        for (int q = 0; q < 8; q++) e[q] = (q == k) ? ek : e[q];
         double wi = w ? (double)w[i] : 1.0;
         double r = (double)y[i] - (double)f;
-        c2 += wi * r * r;
         if (need_jac) {
             int h = 0;
-            for (int k = 0; k < nfree; k++) {
-                double jk = (double)jf[k];
-                gl[k] += wi * r * jk;
-                for (int l = 0; l <= k; l++) hl[h++] += wi * jk * (double)jf[l];
+            for (int q = 0; q < nfree; q++) {
+                double jq = (double)jf[q];
+                gl[q] += wi * r * jq;
+                for (int l = 0; l <= q; l++) hl[h++] += wi * jq * (double)jf[l];
             }
         }
+      }
+      c2 += (double)piece_tree(e);
     }
-    chi2 = wave_sum(c2);
+    chi2 = (float)c2;
     if (need_jac) {
         for (int k = 0; k < nfree; k++) grad[k] = wave_sum(gl[k]);
         for (int k = 0; k < nh; k++) hess[k] = wave_sum(hl[k]);
@@ -690,13 +868,15 @@ __global__ void __launch_bounds__(64) k_lm(LMArgs a) {
         ac.geom.q2 = par[18];
         ac.geom.q3 = par[19];
     }
-    double chi2, grad[kLMMaxFree], hess[kLMMaxFree * (kLMMaxFree + 1) / 2];
+    float chi2;
+    double grad[kLMMaxFree], hess[kLMMaxFree * (kLMMaxFree + 1) / 2];
     double H[kLMMaxFree * kLMMaxFree], scale[kLMMaxFree], delta[kLMMaxFree];
     lm_eval(ac, fit, par, fidx, nfree, true, chi2, grad, hess);
     int state = kStateConverged;
     int iters = 0;
     if (nfree > 0) {
-        double lambda = 1e-3, prev_chi2 = chi2;
+        double lambda = 1e-3;
+        float prev_chi2 = chi2;
         bool failed = false;
         for (int k = 0; k < nfree; k++) scale[k] = 0.0;
         // hessian as a full matrix; the diagonal keeps its lambda shift
@@ -723,26 +903,30 @@ __global__ void __launch_bounds__(64) k_lm(LMArgs a) {
             }
             for (int k = 0; k < 20; k++) prev[k] = par[k];
             for (int k = 0; k < nfree; k++) par[fidx[k]] = (float)((double)par[fidx[k]] + delta[k]);
-            double nchi2, ngrad[kLMMaxFree], nhess[kLMMaxFree * (kLMMaxFree + 1) / 2];
+            float nchi2;
+            double ngrad[kLMMaxFree], nhess[kLMMaxFree * (kLMMaxFree + 1) / 2];
             lm_eval(ac, fit, par, fidx, nfree, false, nchi2, ngrad, nhess);
-            bool decreased = nchi2 < prev_chi2;
-            if (decreased) lm_eval(ac, fit, par, fidx, nfree, true, nchi2, ngrad, nhess);
+            const bool decreased = nchi2 < prev_chi2;
+            // cuda_check_fit_improvement (agpufitjni.cu:1453-1476): a step
+            // from chi^2 = 0 never counts as failed
+            const bool step_failed = (prev_chi2 != 0.f) && !decreased;
+            if (!step_failed) lm_eval(ac, fit, par, fidx, nfree, true, nchi2, ngrad, nhess);
             // cuda_check_for_convergence (agpufitjni.cu:2289-2318)
-            if (fabs(nchi2 - prev_chi2) < (double)a.tol * fmax(1.0, nchi2)) finished = true;
+            if ((double)fabsf(nchi2 - prev_chi2) < (double)a.tol * fmax(1.0, (double)nchi2)) finished = true;
             else if (it == a.maxit - 1) state = kStateMaxIter;
+            if (!step_failed) {
+                for (int k = 0; k < nfree; k++) grad[k] = ngrad[k];
+                for (int k = 0, h = 0; k < nfree; k++)
+                    for (int l = 0; l <= k; l++, h++) H[k * kLMMaxFree + l] = H[l * kLMMaxFree + k] = nhess[h];
+            }
+            failed = step_failed;
             // cuda_prepare_next_iteration (agpufitjni.cu:2458-2490)
             if (decreased) {
                 lambda *= 0.1;
                 prev_chi2 = nchi2;
-                chi2 = nchi2;
-                for (int k = 0; k < nfree; k++) grad[k] = ngrad[k];
-                for (int k = 0, h = 0; k < nfree; k++)
-                    for (int l = 0; l <= k; l++, h++) H[k * kLMMaxFree + l] = H[l * kLMMaxFree + k] = nhess[h];
-                failed = false;
             } else {
                 lambda *= 10.0;
                 for (int k = 0; k < 20; k++) par[k] = prev[k];
-                failed = true;
             }
         }
         chi2 = prev_chi2;
@@ -776,7 +960,7 @@ struct FixedSlots {
 };
 
 template <int KIND, int NF>
-__device__ void lm_eval_fixed(const LMArgs &a, int fit, const float *par, bool need_jac, double &chi2, double *grad,
+__device__ void lm_eval_fixed(const LMArgs &a, int fit, const float *par, bool need_jac, float &chi2, double *grad,
                               double *hess) {
     constexpr int NH = NF * (NF + 1) / 2;
     const int lane = threadIdx.x & 63;
@@ -792,7 +976,15 @@ __device__ void lm_eval_fixed(const LMArgs &a, int fit, const float *par, bool n
         c2on = (par[5] != 0.f) || a.pfit[6];
         c3on = (par[7] != 0.f) || a.pfit[8];
     }
-    for (int i = lane; i < a.npts; i += 64) {
+    for (int j0 = 0; j0 < a.npts; j0 += 512) {
+      float e[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) e[k] = 0.f;
+      const int kend = min(8, (a.npts - j0 + 63) >> 6);   // wave-uniform
+#pragma unroll 1
+      for (int k = 0; k < kend; k++) {
+        const int i = j0 + 64 * k + lane;
+        if (i >= a.npts) continue;
         const float xi = a.x[i];
         float jf[NF];
         float f;
@@ -812,21 +1004,26 @@ __device__ void lm_eval_fixed(const LMArgs &a, int fit, const float *par, bool n
             }
             f = (float)sv;
         }
+        const float dev = f - y[i];
+        const float ek = w ? dev * dev * w[i] : dev * dev;
+#pragma unroll
+        for (int q = 0; q < 8; q++) e[q] = (q == k) ? ek : e[q];
         const double wi = w ? (double)w[i] : 1.0;
         const double r = (double)y[i] - (double)f;
-        c2 += wi * r * r;
         if (need_jac) {
             int h = 0;
 #pragma unroll
-            for (int k = 0; k < NF; k++) {
-                const double jk = (double)jf[k];
-                gl[k] += wi * r * jk;
+            for (int q = 0; q < NF; q++) {
+                const double jq = (double)jf[q];
+                gl[q] += wi * r * jq;
 #pragma unroll
-                for (int l = 0; l <= k; l++) hl[h++] += wi * jk * (double)jf[l];
+                for (int l = 0; l <= q; l++) hl[h++] += wi * jq * (double)jf[l];
             }
         }
+      }
+      c2 += (double)piece_tree(e);
     }
-    chi2 = wave_sum(c2);
+    chi2 = (float)c2;
     if (need_jac) {
 #pragma unroll
         for (int k = 0; k < NF; k++) grad[k] = wave_sum(gl[k]);
@@ -897,10 +1094,12 @@ __global__ void __launch_bounds__(64) k_lm_fixed(LMArgs a) {
         ac.geom.q2 = par[18];
         ac.geom.q3 = par[19];
     }
-    double chi2, grad[NF], hess[NH], H[NF][NF], scale[NF];
+    float chi2;
+    double grad[NF], hess[NH], H[NF][NF], scale[NF];
     lm_eval_fixed<KIND, NF>(ac, fit, par, true, chi2, grad, hess);
     int state = kStateConverged, iters = 0;
-    double lambda = 1e-3, prev_chi2 = chi2;
+    double lambda = 1e-3;
+    float prev_chi2 = chi2;
     bool failed = false;
 #pragma unroll
     for (int k = 0; k < NF; k++) scale[k] = 0.0;
@@ -941,17 +1140,17 @@ __global__ void __launch_bounds__(64) k_lm_fixed(LMArgs a) {
             prev[k] = par[sl];
             par[sl] = (float)((double)par[sl] + delta[k]);
         }
-        double nchi2, ngrad[NF], nhess[NH];
+        float nchi2;
+        double ngrad[NF], nhess[NH];
         lm_eval_fixed<KIND, NF>(ac, fit, par, false, nchi2, ngrad, nhess);
         const bool decreased = nchi2 < prev_chi2;
-        if (decreased) lm_eval_fixed<KIND, NF>(ac, fit, par, true, nchi2, ngrad, nhess);
+        // cuda_check_fit_improvement (agpufitjni.cu:1453-1476)
+        const bool step_failed = (prev_chi2 != 0.f) && !decreased;
+        if (!step_failed) lm_eval_fixed<KIND, NF>(ac, fit, par, true, nchi2, ngrad, nhess);
         // cuda_check_for_convergence (agpufitjni.cu:2289-2318)
-        if (fabs(nchi2 - prev_chi2) < (double)a.tol * fmax(1.0, nchi2)) finished = true;
+        if ((double)fabsf(nchi2 - prev_chi2) < (double)a.tol * fmax(1.0, (double)nchi2)) finished = true;
         else if (it == a.maxit - 1) state = kStateMaxIter;
-        // cuda_prepare_next_iteration (agpufitjni.cu:2458-2490)
-        if (decreased) {
-            lambda *= 0.1;
-            prev_chi2 = nchi2;
+        if (!step_failed) {
 #pragma unroll
             for (int k = 0; k < NF; k++) grad[k] = ngrad[k];
             int h = 0;
@@ -959,12 +1158,16 @@ __global__ void __launch_bounds__(64) k_lm_fixed(LMArgs a) {
             for (int k = 0; k < NF; k++)
 #pragma unroll
                 for (int l = 0; l <= k; l++, h++) H[k][l] = H[l][k] = nhess[h];
-            failed = false;
+        }
+        failed = step_failed;
+        // cuda_prepare_next_iteration (agpufitjni.cu:2458-2490)
+        if (decreased) {
+            lambda *= 0.1;
+            prev_chi2 = nchi2;
         } else {
             lambda *= 10.0;
 #pragma unroll
             for (int k = 0; k < NF; k++) par[FixedSlots<KIND, NF>::slot(k)] = prev[k];
-            failed = true;
         }
     }
     chi2 = prev_chi2;
@@ -1026,7 +1229,7 @@ static inline int cdiv(int64_t a, int b) { return (int)((a + b - 1) / b); }
 void launch_trace_moments(const SeriesGeom &g, int nopit, int ave, double tscale, double frametime, int order,
                           int integer_trace, double *part, int nchunk, hipStream_t st) {
     int ppc = cdiv(nopit, nchunk);
-    dim3 grid(cdiv(g.nseries, 64), nchunk);
+    dim3 grid(cdiv(g.nslots, 64), nchunk);
     hipLaunchKernelGGL(k_trace_moments, grid, dim3(64), 0, st, g, nopit, ave, tscale, frametime, order,
                        integer_trace, ppc, part);
 }
@@ -1058,36 +1261,64 @@ void launch_polyfit(const double *part, int nchunk, int nseries, int nopit, int 
     hipLaunchKernelGGL(k_polyfit, dim3(cdiv(nseries, 64)), dim3(64), 0, st, part, nchunk, nseries, order, Lf, coef);
 }
 
-void launch_prefix(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const CorrTables *dct,
-                   double *ctot, double *rec, int nchunk, int chunk_len, hipStream_t st) {
-    (void)ct;
-    dim3 grid(cdiv(g.nseries, 64), nchunk);
-    hipLaunchKernelGGL(k_prefix, grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec);
+void launch_apply(const SeriesGeom &g, const BleachDesc &bd, const CorrTables *dct, double *ctot, double *rec,
+                  float *X, int nchunk, int chunk_len, hipStream_t st) {
+    dim3 grid(cdiv(g.nslots, 64), nchunk);
+    switch (bleach_maxo(bd.order)) {
+        case 0: hipLaunchKernelGGL(k_apply<0>, grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec, X); break;
+        case 4: hipLaunchKernelGGL(k_apply<4>, grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec, X); break;
+        default:
+            hipLaunchKernelGGL(k_apply<kMaxOrder>, grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec, X);
+            break;
+    }
 }
 
-void launch_prefix_combine(const CorrTables *dct, int nseries, const double *ctot, const double *rec, int nchunk,
+void launch_prefix_combine(const CorrTables *dct, int nslots, const double *ctot, const double *rec, int nchunk,
                            int chunk_len, double *F, hipStream_t st) {
-    hipLaunchKernelGGL(k_prefix_combine, dim3(cdiv(nseries, 64)), dim3(64), 0, st, dct, nseries, ctot, rec, nchunk,
+    hipLaunchKernelGGL(k_prefix_combine, dim3(cdiv(nslots, 64)), dim3(64), 0, st, dct, nslots, ctot, rec, nchunk,
                        chunk_len, F);
 }
 
-void launch_blocking(const SeriesGeom &g, const BleachDesc &bd, const CorrTables *dct, const PairDesc &pd,
-                     const double *F, double *bpart, int nchunk, int chunk_len, hipStream_t st) {
-    dim3 grid(cdiv(pd.npairs, 64), nchunk);
-    hipLaunchKernelGGL(k_blocking, grid, dim3(64), 0, st, g, bd, dct, pd, F, bpart, chunk_len);
+BlockingPlan plan_blocking(const CorrTables &ct, int npix) {
+    BlockingPlan bp;
+    const int n0 = ct.T - 1;
+    bp.nchunk_prod = (n0 + (1 << kBlkJ) - 1) >> kBlkJ;
+    // about 2^19 threads in flight, whole chunks per split
+    int want = std::max(1, (1 << 19) / std::max(1, npix));
+    bp.nsplit = std::min(bp.nchunk_prod, want);
+    bp.chunks_per_split = (bp.nchunk_prod + bp.nsplit - 1) / bp.nsplit;
+    bp.nsplit = (bp.nchunk_prod + bp.chunks_per_split - 1) / bp.chunks_per_split;
+    return bp;
 }
 
-void launch_blocking_finalize(const CorrTables &ct, const CorrTables *dct, const PairDesc &pd, const double *F,
-                              const double *bpart, int nchunk, int32_t *ind, int32_t *bis, hipStream_t st) {
-    hipLaunchKernelGGL(k_blocking_finalize, dim3(cdiv(pd.npairs, 64)), dim3(64), 0, st, dct, pd, ct.T, F, bpart,
-                       nchunk, ind, bis);
+void launch_blocking(const SeriesGeom &g, const BleachDesc &bd, const CorrTables *dct, const PairDesc &pd,
+                     const double *F, const float *X, const BlockingPlan &bp, double *bpart, double *csum,
+                     hipStream_t st) {
+    dim3 grid(cdiv(pd.npix, 64), bp.nsplit);
+    if (pd.acf_first && pd.has_ccf)
+        hipLaunchKernelGGL((k_blocking<true, true>), grid, dim3(64), 0, st, g, dct, pd, F, X, bd.coef, bd.order,
+                           bpart, csum, bp.chunks_per_split);
+    else if (pd.acf_first)
+        hipLaunchKernelGGL((k_blocking<true, false>), grid, dim3(64), 0, st, g, dct, pd, F, X, bd.coef, bd.order,
+                           bpart, csum, bp.chunks_per_split);
+    else
+        hipLaunchKernelGGL((k_blocking<false, true>), grid, dim3(64), 0, st, g, dct, pd, F, X, bd.coef, bd.order,
+                           bpart, csum, bp.chunks_per_split);
+}
+
+void launch_blocking_finalize(const CorrTables &ct, const CorrTables *dct, const SeriesGeom &g, const PairDesc &pd,
+                              const double *F, const BlockingPlan &bp, const double *bpart, const double *csum,
+                              int32_t *ind, int32_t *bis, hipStream_t st) {
+    (void)ct;
+    hipLaunchKernelGGL(k_blocking_finalize, dim3(cdiv(pd.npairs, 64)), dim3(64), 0, st, dct, g, pd, F, bpart,
+                       bp.nsplit, csum, ind, bis);
 }
 
 void launch_correlate(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const CorrTables *dct,
-                      const PairDesc &pd, const double *F, const int32_t *ind, double *corav, double *bvar,
-                      hipStream_t st) {
+                      const PairDesc &pd, const double *F, const float *X, const int32_t *ind, double *corav,
+                      double *bvar, hipStream_t st) {
     dim3 grid(cdiv(pd.npairs, 64), ct.nlevels);
-    hipLaunchKernelGGL(k_correlate, grid, dim3(64), 0, st, g, bd, dct, pd, F, ind, corav, bvar);
+    hipLaunchKernelGGL(k_correlate, grid, dim3(64), 0, st, g, bd, dct, pd, F, X, ind, corav, bvar);
 }
 
 static void launch_lm(const LMArgs &a, unsigned free_mask, hipStream_t st) {
