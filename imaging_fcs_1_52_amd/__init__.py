@@ -218,7 +218,10 @@ def make_fit_model(ax, ay, s, sz=None, rx=0.0, ry=0.0, q2=1.0, q3=1.0, paramfit=
 def make_pipeline_config(W, H, T, binning=1, cf=(0, 0), background=0, poly_order=4, frametime=1e-3, p=16, q=8,
                          roi=None, do_acf=True, do_ccf=None, do_fit=True, acf_model=None, ccf_model=None,
                          overlap=False, firstframe=1, lastframe=None, fit_max_iterations=2000,
-                         fit_tolerance=1e-16):
+                         fit_tolerance=1e-16, fit_chi2_fp64=True):
+    """imfcs_config of one pipeline run.  fit_chi2_fp64 selects FP64 step
+    acceptance and convergence on chi^2 (the optimum the Java CPU path
+    reaches); False keeps Gpufit's FP32 chi^2 of the reference GPU fit."""
     c = PipelineConfig()
     c.width, c.height, c.frames = W, H, T
     c.firstframe = firstframe
@@ -237,6 +240,7 @@ def make_pipeline_config(W, H, T, binning=1, cf=(0, 0), background=0, poly_order
     c.do_fit = int(do_fit)
     c.fit_max_iterations = fit_max_iterations
     c.fit_tolerance = fit_tolerance
+    c.fit_chi2_fp64 = int(bool(fit_chi2_fp64))
     if acf_model is not None:
         c.acf_model = acf_model
     if ccf_model is not None:
@@ -293,9 +297,17 @@ class Engine:
         _check(self._lib.imfcs_synchronize(self.ctx))
 
     def stage_ms(self):
+        """Device times (ms) of the last run's stages: trace + polynomial fit,
+        bleach correction + window means, blocking, correlation, LM fit."""
         ms = (ctypes.c_float * 5)()
         _check(self._lib.imfcs_get_stage_ms(self.ctx, ms))
         return list(ms)
+
+    def correlator_kernel_ms(self):
+        """Device time (ms) of the last run's main correlator kernel."""
+        ms = ctypes.c_float()
+        _check(self._lib.imfcs_get_correlator_kernel_ms(self.ctx, ctypes.byref(ms)))
+        return ms.value
 
     def results(self, cfg):
         ow, oh, chanum = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

@@ -172,8 +172,14 @@ struct imfcs_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[6] = {};
+    hipEvent_t ev_h2d = nullptr;   // after the last host-to-device copy of a run
+    hipEvent_t evk[2] = {};        // around the main correlator kernel
+    int evk_valid = 0;
+    // host sources of a run's asynchronous copies; reused only after ev_h2d
+    std::vector<float> h_xs, h_init;
+    std::vector<int> h_pfit;
     DevBuf part, coef, ctot, rec, F, bpart, ind, bis, corav, bvar, fdata, fw, fx, finit, fpfit, fout, fstate, fchi2,
-        fiter, cpart, hpart, l3, ctab, csum, slots, X;
+        fiter, cpart, hpart, l3, lvd, ctab, csum, slots, X;
     CorrTables host_ct;
     std::vector<int32_t> host_slots;   // slot_of then slot origins (source of the async copy)
     std::vector<double> host_coef;     // drop-in bleach coefficients per slot (source of the async copy)   // source of the async copy into ctab
@@ -233,6 +239,8 @@ imfcs_ctx *imfcs_create(int device) {
         return nullptr;
     }
     for (auto &e : ctx->ev) (void)hipEventCreate(&e);
+    (void)hipEventCreateWithFlags(&ctx->ev_h2d, hipEventDisableTiming);
+    for (auto &e : ctx->evk) (void)hipEventCreate(&e);
     const char *sel = std::getenv("IMFCS_CORRELATOR");
     ctx->force_v1 = (sel && std::strcmp(sel, "generic") == 0) ? 1 : 0;
     return ctx;
@@ -242,7 +250,8 @@ static void release_all(imfcs_ctx *ctx) {
     DevBuf *bufs[] = {&ctx->part, &ctx->coef, &ctx->ctot, &ctx->rec, &ctx->F, &ctx->bpart, &ctx->ind,
                       &ctx->bis, &ctx->corav, &ctx->bvar, &ctx->fdata, &ctx->fw, &ctx->fx, &ctx->finit,
                       &ctx->fpfit, &ctx->fout, &ctx->fstate, &ctx->fchi2, &ctx->fiter, &ctx->cpart,
-                      &ctx->hpart, &ctx->l3, &ctx->ctab, &ctx->csum, &ctx->slots, &ctx->X};
+                      &ctx->hpart, &ctx->l3, &ctx->lvd, &ctx->ctab, &ctx->csum, &ctx->slots,
+                      &ctx->X};
     for (DevBuf *b : bufs) b->release();
 }
 
@@ -252,6 +261,8 @@ void imfcs_destroy(imfcs_ctx *ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     release_all(ctx);
     for (auto &e : ctx->ev) (void)hipEventDestroy(e);
+    if (ctx->ev_h2d) (void)hipEventDestroy(ctx->ev_h2d);
+    for (auto &e : ctx->evk) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -356,7 +367,8 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTabl
     if (rc) return rc;
     rc = make_corr_tables(ht, g.T, ea.q, ct);
     if (rc) return rc;
-    HIP_TRY(hipStreamSynchronize(st));   // host_ct / host_slots may still feed a previous copy
+    // the host sources of the previous run's copies are reused below
+    HIP_TRY(hipEventSynchronize(ctx->ev_h2d));
     ctx->host_ct = ct;
     HIP_TRY(ctx->ctab.ensure(sizeof(CorrTables)));
     HIP_TRY(hipMemcpyAsync(ctx->ctab.p, &ctx->host_ct, sizeof(CorrTables), hipMemcpyHostToDevice, st));
@@ -409,6 +421,7 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTabl
         bd.tscale = 1.0;
         bd.coef = nullptr;
     }
+    HIP_TRY(hipEventRecord(ctx->ev_h2d, st));
     HIP_TRY(hipEventRecord(ctx->ev[1], st));
     // window means
     // chunk length: about 2^19 threads in flight, 256..4096 frames each
@@ -441,13 +454,21 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTabl
         const int nch = (g.T + clen - 1) / clen;
         HIP_TRY(ctx->cpart.ensure(sizeof(float) * (size_t)nch * np * 33 * 3));
         HIP_TRY(ctx->hpart.ensure(sizeof(double) * (size_t)nch * np * (ct.chanum - 33) * 3));
+        size_t nlv = 0;
+        for (int c = 4; c < ct.nlevels; c++) nlv += (size_t)ct.numbin[c];
         HIP_TRY(ctx->l3.ensure(sizeof(float) * (size_t)ct.numbin[3] * g.nseries));
+        HIP_TRY(ctx->lvd.ensure(sizeof(double) * std::max<size_t>(nlv, 1) * g.nseries));
         launch_correlate_fast(g, bd, ct, dct, ea.pd, ctx->F.as<double>(), ctx->X.as<float>(), ctx->ind.as<int32_t>(),
-                              ctx->cpart.as<float>(), ctx->hpart.as<double>(), ctx->l3.as<float>(), clen,
-                              ctx->corav.as<double>(), ctx->bvar.as<double>(), st);
+                              ctx->cpart.as<float>(), ctx->hpart.as<double>(), ctx->l3.as<float>(),
+                              ctx->lvd.as<double>(), clen,
+                              ctx->corav.as<double>(), ctx->bvar.as<double>(), st, ctx->evk[0], ctx->evk[1]);
+        ctx->evk_valid = 1;
     } else {
+        HIP_TRY(hipEventRecord(ctx->evk[0], st));
         launch_correlate(g, bd, ct, dct, ea.pd, ctx->F.as<double>(), ctx->X.as<float>(), ctx->ind.as<int32_t>(),
                          ctx->corav.as<double>(), ctx->bvar.as<double>(), st);
+        HIP_TRY(hipEventRecord(ctx->evk[1], st));
+        ctx->evk_valid = 1;
     }
     HIP_TRY(hipEventRecord(ctx->ev[4], st));
     HIP_TRY(hipGetLastError());
@@ -537,9 +558,12 @@ extern "C" int imfcs_run(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_co
                                ctx->fw.as<float>(), st);
         // lag times of channels 1..chanum-1 (IFCS:15122-15125) and initial
         // parameters with the fixed geometry in slots 11..19 (IFCS:15090-15110)
-        std::vector<float> xs(npts), init((size_t)np * 20);
+        std::vector<float> &xs = ctx->h_xs, &init = ctx->h_init;
+        std::vector<int> &pfit = ctx->h_pfit;
+        xs.assign(npts, 0.f);
+        init.assign((size_t)np * 20, 0.f);
+        pfit.assign(20, 0);
         for (int i = 0; i < npts; i++) xs[i] = (float)ht.lagtime[i + 1];
-        std::vector<int> pfit(20, 0);
         const imfcs_fit_model *models[2] = {&c->acf_model, &c->ccf_model};
         for (int k = 0; k < 11; k++) pfit[k] = c->acf_model.paramfit[k] ? 1 : 0;
         int nfree = 0;
@@ -561,17 +585,16 @@ extern "C" int imfcs_run(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_co
         HIP_TRY(hipMemcpyAsync(ctx->fx.p, xs.data(), sizeof(float) * npts, hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemcpyAsync(ctx->finit.p, init.data(), sizeof(float) * init.size(), hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemcpyAsync(ctx->fpfit.p, pfit.data(), sizeof(int) * 20, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipEventRecord(ctx->ev_h2d, st));
         unsigned mask = 0;
         for (int k = 0; k < 11; k++) mask |= (pfit[k] ? 1u : 0u) << k;
         launch_lm_acf(np, npts, ctx->fdata.as<float>(), ctx->fw.as<float>(), ctx->fx.as<float>(),
                       ctx->finit.as<float>(), ctx->fpfit.as<int>(), mask, LMModelGeom{}, c->fit_tolerance,
-                      c->fit_max_iterations, ctx->fout.as<float>(), ctx->fstate.as<int>(), ctx->fchi2.as<float>(),
-                      ctx->fiter.as<int>(), st);
+                      c->fit_max_iterations, c->fit_chi2_fp64 ? 1 : 0, ctx->fout.as<float>(),
+                      ctx->fstate.as<int>(), ctx->fchi2.as<float>(), ctx->fiter.as<int>(), st);
         ctx->nfree = nfree;
         // chi2 normalization of GPU_ACF_Fit (IFCS:15234)
         ctx->chi2_norm = (double)((npts - 1) - nfree - 1);
-        // the host vectors above must outlive the async copies
-        HIP_TRY(hipStreamSynchronize(st));
     }
     HIP_TRY(hipEventRecord(ctx->ev[5], st));
     HIP_TRY(hipGetLastError());
@@ -586,6 +609,14 @@ extern "C" int imfcs_get_stage_ms(imfcs_ctx *ctx, float *ms) {
         HIP_TRY(hipEventElapsedTime(&t, ctx->ev[i], ctx->ev[i + 1]));
         ms[i] = t;
     }
+    return 0;
+}
+
+extern "C" int imfcs_get_correlator_kernel_ms(imfcs_ctx *ctx, float *ms) {
+    if (!ctx || !ms) return fail(IMFCS_ERR_ARG, "null argument");
+    if (!ctx->evk_valid) return fail(IMFCS_ERR_ARG, "no correlation run yet");
+    HIP_TRY(hipEventSynchronize(ctx->evk[1]));
+    HIP_TRY(hipEventElapsedTime(ms, ctx->evk[0], ctx->evk[1]));
     return 0;
 }
 
@@ -852,12 +883,12 @@ extern "C" int imfcs_fit(int n_fits, int n_points, const float *data, const floa
     for (int k = 0; k < npar && k < 32; k++) mask |= (parameters_to_fit[k] ? 1u : 0u) << k;
     if (model_id == IMFCS_MODEL_ACF_1D)
         launch_lm_acf(n_fits, n_points, ctx->fdata.as<float>(), w, ctx->fx.as<float>(), ctx->finit.as<float>(),
-                      ctx->fpfit.as<int>(), mask & 0x7ffu, LMModelGeom{}, tolerance, max_n_iterations,
+                      ctx->fpfit.as<int>(), mask & 0x7ffu, LMModelGeom{}, tolerance, max_n_iterations, 0,
                       ctx->fout.as<float>(), ctx->fstate.as<int>(), ctx->fchi2.as<float>(), ctx->fiter.as<int>(), st);
     else
         launch_lm_poly(n_fits, n_points, ctx->fdata.as<float>(), w, ctx->fx.as<float>(), ctx->finit.as<float>(),
                        ctx->fpfit.as<int>(), mask & ((1u << num_valid_coefs) - 1u), num_valid_coefs, tolerance,
-                       max_n_iterations, ctx->fout.as<float>(), ctx->fstate.as<int>(), ctx->fchi2.as<float>(),
+                       max_n_iterations, 0, ctx->fout.as<float>(), ctx->fstate.as<int>(), ctx->fchi2.as<float>(),
                        ctx->fiter.as<int>(), st);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(output_parameters, ctx->fout.p, (size_t)n_fits * npar * 4, hipMemcpyDeviceToHost, st));

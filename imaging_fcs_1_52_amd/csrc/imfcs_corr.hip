@@ -39,7 +39,9 @@ struct MainArgs {
     int chunk_len, nchunk;
     int has_acf, has_ccf;
     float *part;    // [chunk][pair][kMainChan][3] : cross, A1, A2
-    float *l3;      // [numbin3][nseries]
+    float *l3;      // level-3 values [numbin_3][nseries]
+    double *lvd;    // levels >= 4, level c at lvd + loff[c]: [numbin_c][nseries]
+    int64_t loff[kMaxLevel];
 };
 
 struct Tile {
@@ -269,16 +271,26 @@ __global__ void __launch_bounds__(640) k_corr_main(MainArgs A) {
 // Levels 3..q-1 from the level-3 series.
 // ------------------------------------------------------------------
 struct L3Ctx {
-    const float *l3;
+    const float *l3;    // level 3 (FP32 sums of 8 frames)
+    const double *lv;   // this level's values when c >= 4 (FP64)
     int nseries, sa, sb, c, numbin;
     __device__ __forceinline__ double val(int s, int y) const {
         if (y >= numbin) return 0.0;
-        const int r = 1 << (c - 3);
-        double acc = 0.0;
-        for (int i = 0; i < r; i++) acc += (double)l3[(int64_t)(y * r + i) * nseries + s];
-        return acc;
+        const int64_t i = (int64_t)y * nseries + s;
+        return c == 3 ? (double)l3[i] : lv[i];
     }
 };
+
+// Level c >= 4 from level c - 1 by the pairwise-sum rebinning
+// (IFCS:9477-9478): v_c[y] = v_{c-1}[2y] + v_{c-1}[2y+1], in FP64.
+__global__ void k_corr_rebin(const float *l3, double *lvd, int64_t off_src, int64_t off_dst, int from_l3,
+                             int nseries, int numbin) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)numbin * nseries) return;
+    const int64_t y = i / nseries, s = i - y * nseries;
+    const int64_t i0 = (2 * y) * nseries + s, i1 = (2 * y + 1) * nseries + s;
+    lvd[off_dst + i] = from_l3 ? (double)l3[i0] + (double)l3[i1] : lvd[off_src + i0] + lvd[off_src + i1];
+}
 
 // Levels >= 3 hold few, large (rebinned) values: FP64 throughout.
 // Values [ybeg, yend) of one level; yend - ybeg and ybeg are multiples of G
@@ -349,7 +361,7 @@ __global__ void __launch_bounds__(64) k_corr_high(MainArgs A, double *hpart /* [
     if (pidx >= A.pd.npairs || c >= ct.nlevels) return;
     int sa, sb;
     pair_series(A.pd, pidx, sa, sb);
-    L3Ctx L{A.l3, A.g.nseries, sa, sb, c, ct.numbin[c]};
+    L3Ctx L{A.l3, A.lvd + A.loff[c], A.g.nseries, sa, sb, c, ct.numbin[c]};
     const double *Fa = A.F + (int64_t)A.g.slot_of[sa] * ct.npos, *Fb = A.F + (int64_t)A.g.slot_of[sb] * ct.npos;
     const double mua = level_shift(A, sa, c);
     const double mub = level_shift(A, sb, c);
@@ -445,7 +457,8 @@ int corr_fast_chunk_len(const CorrTables &ct, int npairs) {
 
 void launch_correlate_fast(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const CorrTables *dct,
                            const PairDesc &pd, const double *F, const float *X, const int32_t *ind, float *part,
-                           double *hpart, float *l3, int chunk_len, double *corav, double *bvar, hipStream_t st) {
+                           double *hpart, float *l3, double *lvd, int chunk_len, double *corav, double *bvar,
+                           hipStream_t st, hipEvent_t ev_main_begin, hipEvent_t ev_main_end) {
     MainArgs A;
     A.g = g;
     A.bd = bd;
@@ -460,9 +473,22 @@ void launch_correlate_fast(const SeriesGeom &g, const BleachDesc &bd, const Corr
     A.has_ccf = pd.has_ccf;
     A.part = part;
     A.l3 = l3;
+    A.lvd = lvd;
+    int64_t off = 0;
+    for (int c = 0; c < kMaxLevel; c++) {
+        A.loff[c] = off;
+        if (c >= 4 && c < ct.nlevels) off += (int64_t)ct.numbin[c] * g.nseries;
+    }
     const int tiles = (g.npix + 63) / 64;
     const int threads = pd.has_ccf ? 640 : 320;
+    if (ev_main_begin) (void)hipEventRecord(ev_main_begin, st);
     hipLaunchKernelGGL(k_corr_main, dim3(tiles, A.nchunk), dim3(threads), 0, st, A);
+    if (ev_main_end) (void)hipEventRecord(ev_main_end, st);
+    for (int c = 4; c < ct.nlevels; c++) {
+        const int64_t n = (int64_t)ct.numbin[c] * g.nseries;
+        hipLaunchKernelGGL(k_corr_rebin, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (const float *)l3, lvd,
+                           A.loff[c - 1], A.loff[c], c == 4 ? 1 : 0, g.nseries, ct.numbin[c]);
+    }
     const int nhigh = ct.chanum - kMainChan;
     hipLaunchKernelGGL(k_corr_high, dim3((pd.npairs + 63) / 64, ct.nlevels - 3, A.nchunk), dim3(64), 0, st, A, hpart,
                        nhigh);

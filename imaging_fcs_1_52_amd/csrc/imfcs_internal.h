@@ -128,13 +128,14 @@ bool correlate_fast_supported(const CorrTables &ct);
 int corr_fast_chunk_len(const CorrTables &ct, int npairs);
 void launch_correlate_fast(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const CorrTables *dct,
                            const PairDesc &pd, const double *F, const float *X, const int32_t *ind, float *part,
-                           double *hpart, float *l3, int chunk_len, double *corav, double *bvar, hipStream_t st);
+                           double *hpart, float *l3, double *lvd, int chunk_len, double *corav, double *bvar,
+                           hipStream_t st, hipEvent_t ev_main_begin = nullptr, hipEvent_t ev_main_end = nullptr);
 void launch_lm_acf(int nfits, int npts, const float *data, const float *weights, const float *x,
                    const float *init, const int *pfit, unsigned free_mask, const LMModelGeom &geom, float tol, int maxit,
-                   float *out_params, int *states, float *chi2, int *iters, hipStream_t st);
+                   int chi2_fp64, float *out_params, int *states, float *chi2, int *iters, hipStream_t st);
 void launch_lm_poly(int nfits, int npts, const float *data, const float *weights, const float *x,
                     const float *init, const int *pfit, unsigned free_mask, int ncoef, float tol, int maxit,
-                    float *out_params, int *states, float *chi2, int *iters, hipStream_t st);
+                    int chi2_fp64, float *out_params, int *states, float *chi2, int *iters, hipStream_t st);
 void launch_binning_f32(const float *in, float *out, int win, int hin, int wt, int ht, int T, int bx, int by,
                         hipStream_t st);
 void launch_bleach_average_f32(const float *in, float *out, int wt, int ht, int nopit, int ave,

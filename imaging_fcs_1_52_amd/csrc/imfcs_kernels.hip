@@ -763,6 +763,7 @@ struct LMArgs {
     LMModelGeom geom;
     float tol;
     int maxit;
+    int chi2_fp64;   // 0: Gpufit's FP32 chi-square tree; 1: FP64 sum of FP64 residual squares
     float *out_params;
     int *states;
     float *chi2;
@@ -772,11 +773,11 @@ struct LMArgs {
 // Evaluates chi^2 and (optionally) gradient / hessian sums for the fit, with
 // every lane handling points lane, lane+64, ...  Results are wave totals.
 __device__ void lm_eval(const LMArgs &a, int fit, const float *par, const int *fidx, int nfree, bool need_jac,
-                        float &chi2, double *grad, double *hess) {
+                        double &chi2, double *grad, double *hess) {
     const int lane = threadIdx.x & 63;
     const float *y = a.data + (int64_t)fit * a.npts;
     const float *w = a.weights ? a.weights + (int64_t)fit * a.npts : nullptr;
-    double c2 = 0.0, gl[kLMMaxFree], hl[kLMMaxFree * (kLMMaxFree + 1) / 2];
+    double c2 = 0.0, cx = 0.0, gl[kLMMaxFree], hl[kLMMaxFree * (kLMMaxFree + 1) / 2];
     int nh = nfree * (nfree + 1) / 2;
     for (int i = 0; i < nfree; i++) gl[i] = 0.0;
     for (int i = 0; i < nh; i++) hl[i] = 0.0;
@@ -819,6 +820,7 @@ __device__ void lm_eval(const LMArgs &a, int fit, const float *par, const int *f
         for (int q = 0; q < 8; q++) e[q] = (q == k) ? ek : e[q];
         double wi = w ? (double)w[i] : 1.0;
         double r = (double)y[i] - (double)f;
+        cx += wi * r * r;
         if (need_jac) {
             int h = 0;
             for (int q = 0; q < nfree; q++) {
@@ -830,7 +832,7 @@ __device__ void lm_eval(const LMArgs &a, int fit, const float *par, const int *f
       }
       c2 += (double)piece_tree(e);
     }
-    chi2 = (float)c2;
+    chi2 = a.chi2_fp64 ? wave_sum(cx) : (double)(float)c2;
     if (need_jac) {
         for (int k = 0; k < nfree; k++) grad[k] = wave_sum(gl[k]);
         for (int k = 0; k < nh; k++) hess[k] = wave_sum(hl[k]);
@@ -868,7 +870,7 @@ __global__ void __launch_bounds__(64) k_lm(LMArgs a) {
         ac.geom.q2 = par[18];
         ac.geom.q3 = par[19];
     }
-    float chi2;
+    double chi2;
     double grad[kLMMaxFree], hess[kLMMaxFree * (kLMMaxFree + 1) / 2];
     double H[kLMMaxFree * kLMMaxFree], scale[kLMMaxFree], delta[kLMMaxFree];
     lm_eval(ac, fit, par, fidx, nfree, true, chi2, grad, hess);
@@ -876,7 +878,7 @@ __global__ void __launch_bounds__(64) k_lm(LMArgs a) {
     int iters = 0;
     if (nfree > 0) {
         double lambda = 1e-3;
-        float prev_chi2 = chi2;
+        double prev_chi2 = chi2;
         bool failed = false;
         for (int k = 0; k < nfree; k++) scale[k] = 0.0;
         // hessian as a full matrix; the diagonal keeps its lambda shift
@@ -903,16 +905,16 @@ __global__ void __launch_bounds__(64) k_lm(LMArgs a) {
             }
             for (int k = 0; k < 20; k++) prev[k] = par[k];
             for (int k = 0; k < nfree; k++) par[fidx[k]] = (float)((double)par[fidx[k]] + delta[k]);
-            float nchi2;
+            double nchi2;
             double ngrad[kLMMaxFree], nhess[kLMMaxFree * (kLMMaxFree + 1) / 2];
             lm_eval(ac, fit, par, fidx, nfree, false, nchi2, ngrad, nhess);
             const bool decreased = nchi2 < prev_chi2;
             // cuda_check_fit_improvement (agpufitjni.cu:1453-1476): a step
             // from chi^2 = 0 never counts as failed
-            const bool step_failed = (prev_chi2 != 0.f) && !decreased;
+            const bool step_failed = (prev_chi2 != 0.0) && !decreased;
             if (!step_failed) lm_eval(ac, fit, par, fidx, nfree, true, nchi2, ngrad, nhess);
             // cuda_check_for_convergence (agpufitjni.cu:2289-2318)
-            if ((double)fabsf(nchi2 - prev_chi2) < (double)a.tol * fmax(1.0, (double)nchi2)) finished = true;
+            if (fabs(nchi2 - prev_chi2) < (double)a.tol * fmax(1.0, nchi2)) finished = true;
             else if (it == a.maxit - 1) state = kStateMaxIter;
             if (!step_failed) {
                 for (int k = 0; k < nfree; k++) grad[k] = ngrad[k];
@@ -960,13 +962,13 @@ struct FixedSlots {
 };
 
 template <int KIND, int NF>
-__device__ void lm_eval_fixed(const LMArgs &a, int fit, const float *par, bool need_jac, float &chi2, double *grad,
+__device__ void lm_eval_fixed(const LMArgs &a, int fit, const float *par, bool need_jac, double &chi2, double *grad,
                               double *hess) {
     constexpr int NH = NF * (NF + 1) / 2;
     const int lane = threadIdx.x & 63;
     const float *y = a.data + (int64_t)fit * a.npts;
     const float *w = a.weights ? a.weights + (int64_t)fit * a.npts : nullptr;
-    double c2 = 0.0, gl[NF], hl[NH];
+    double c2 = 0.0, cx = 0.0, gl[NF], hl[NH];
 #pragma unroll
     for (int i = 0; i < NF; i++) gl[i] = 0.0;
 #pragma unroll
@@ -1010,6 +1012,7 @@ __device__ void lm_eval_fixed(const LMArgs &a, int fit, const float *par, bool n
         for (int q = 0; q < 8; q++) e[q] = (q == k) ? ek : e[q];
         const double wi = w ? (double)w[i] : 1.0;
         const double r = (double)y[i] - (double)f;
+        cx += wi * r * r;
         if (need_jac) {
             int h = 0;
 #pragma unroll
@@ -1023,7 +1026,7 @@ __device__ void lm_eval_fixed(const LMArgs &a, int fit, const float *par, bool n
       }
       c2 += (double)piece_tree(e);
     }
-    chi2 = (float)c2;
+    chi2 = a.chi2_fp64 ? wave_sum(cx) : (double)(float)c2;
     if (need_jac) {
 #pragma unroll
         for (int k = 0; k < NF; k++) grad[k] = wave_sum(gl[k]);
@@ -1094,12 +1097,12 @@ __global__ void __launch_bounds__(64) k_lm_fixed(LMArgs a) {
         ac.geom.q2 = par[18];
         ac.geom.q3 = par[19];
     }
-    float chi2;
+    double chi2;
     double grad[NF], hess[NH], H[NF][NF], scale[NF];
     lm_eval_fixed<KIND, NF>(ac, fit, par, true, chi2, grad, hess);
     int state = kStateConverged, iters = 0;
     double lambda = 1e-3;
-    float prev_chi2 = chi2;
+    double prev_chi2 = chi2;
     bool failed = false;
 #pragma unroll
     for (int k = 0; k < NF; k++) scale[k] = 0.0;
@@ -1140,15 +1143,17 @@ __global__ void __launch_bounds__(64) k_lm_fixed(LMArgs a) {
             prev[k] = par[sl];
             par[sl] = (float)((double)par[sl] + delta[k]);
         }
-        float nchi2;
+        // values and derivatives at the trial point in one pass, as
+        // calc_curve_values does (agpufitjni.cu:4073-4076); the sums are
+        // dropped when the step fails
+        double nchi2;
         double ngrad[NF], nhess[NH];
-        lm_eval_fixed<KIND, NF>(ac, fit, par, false, nchi2, ngrad, nhess);
+        lm_eval_fixed<KIND, NF>(ac, fit, par, true, nchi2, ngrad, nhess);
         const bool decreased = nchi2 < prev_chi2;
         // cuda_check_fit_improvement (agpufitjni.cu:1453-1476)
-        const bool step_failed = (prev_chi2 != 0.f) && !decreased;
-        if (!step_failed) lm_eval_fixed<KIND, NF>(ac, fit, par, true, nchi2, ngrad, nhess);
+        const bool step_failed = (prev_chi2 != 0.0) && !decreased;
         // cuda_check_for_convergence (agpufitjni.cu:2289-2318)
-        if ((double)fabsf(nchi2 - prev_chi2) < (double)a.tol * fmax(1.0, (double)nchi2)) finished = true;
+        if (fabs(nchi2 - prev_chi2) < (double)a.tol * fmax(1.0, nchi2)) finished = true;
         else if (it == a.maxit - 1) state = kStateMaxIter;
         if (!step_failed) {
 #pragma unroll
@@ -1347,18 +1352,18 @@ static void launch_lm(const LMArgs &a, unsigned free_mask, hipStream_t st) {
 
 void launch_lm_acf(int nfits, int npts, const float *data, const float *weights, const float *x, const float *init,
                    const int *pfit, unsigned free_mask, const LMModelGeom &geom, float tol, int maxit,
-                   float *out_params, int *states, float *chi2, int *iters, hipStream_t st) {
-    LMArgs a{nfits, npts, 20, kModelFCS, 0, data, weights, x, init, pfit, geom, tol, maxit, out_params, states, chi2,
-             iters};
+                   int chi2_fp64, float *out_params, int *states, float *chi2, int *iters, hipStream_t st) {
+    LMArgs a{nfits, npts, 20, kModelFCS, 0, data, weights, x, init, pfit, geom, tol, maxit, chi2_fp64, out_params,
+             states, chi2, iters};
     launch_lm(a, free_mask, st);
 }
 
 void launch_lm_poly(int nfits, int npts, const float *data, const float *weights, const float *x, const float *init,
-                    const int *pfit, unsigned free_mask, int ncoef, float tol, int maxit, float *out_params,
-                    int *states, float *chi2, int *iters, hipStream_t st) {
+                    const int *pfit, unsigned free_mask, int ncoef, float tol, int maxit, int chi2_fp64,
+                    float *out_params, int *states, float *chi2, int *iters, hipStream_t st) {
     LMModelGeom g{};
-    LMArgs a{nfits, npts, 11, kModelPoly, ncoef, data, weights, x, init, pfit, g, tol, maxit, out_params, states,
-             chi2, iters};
+    LMArgs a{nfits, npts, 11, kModelPoly, ncoef, data, weights, x, init, pfit, g, tol, maxit, chi2_fp64, out_params,
+             states, chi2, iters};
     launch_lm(a, free_mask, st);
 }
 

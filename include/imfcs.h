@@ -165,6 +165,8 @@ typedef struct imfcs_config {
     int32_t do_fit;                  /* LM fit of every computed pair */
     int32_t fit_max_iterations;      /* 2000 in the plugin */
     float fit_tolerance;             /* relative chi^2 change for convergence */
+    int32_t fit_chi2_fp64;           /* 1: accept steps and test convergence on an FP64 chi^2
+                                        (tighter optimum); 0: Gpufit's FP32 chi^2 */
     imfcs_fit_model acf_model, ccf_model;
 } imfcs_config;
 
@@ -185,6 +187,11 @@ int imfcs_synchronize(imfcs_ctx *ctx);
  * stay on the device inside ctx until imfcs_get_results.  Asynchronous
  * on the context's stream. */
 int imfcs_run(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_config *cfg);
+
+/* Device time (ms) of the main multi-tau correlator kernel (levels 0..2,
+ * the pass over the corrected series) of the last imfcs_run; measured with
+ * HIP events on the context's stream. */
+int imfcs_get_correlator_kernel_ms(imfcs_ctx *ctx, float *ms);
 
 /* Copies the results of the last imfcs_run to host arrays (any may be NULL).
  * pair 0 = ACF, pair 1 = CCF (when both computed, ACF first).
