@@ -34,6 +34,21 @@ BINNINGS = (1, 2, 4)
 PIX = 0.24e-6
 PSF = 0.38e-6
 HBM_PEAK_GBS = 8000.0
+COPY_BYTES = 2 * W * H * T   # the whole uint16 stack
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_corr_main_128x128x50000_b1_traffic.md")
+
+
+def read_traffic():
+    """HBM bytes per k_corr_main launch (b = 1) measured with rocprofv3
+    --pmc FETCH_SIZE / WRITE_SIZE and scaled by a plain copy, from the
+    committed profile summary (None when absent)."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_dispatch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
 
 
 def log(*a):
@@ -86,6 +101,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=48, help="side of the oracle's pixel sample (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=1)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain device-to-device copy of the stack (known bytes) for PMC traffic scaling")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (pixels shard without "
@@ -143,6 +160,20 @@ def main():
     roof = {"kernel": "k_corr_main (b=1, ACF+CCF)", "bound": "hbm", "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "avg_ms": round(float(corr_ms), 3), "alg_bytes": alg_bytes}
+
+    traffic = read_traffic()
+    if traffic is not None:
+        roof["traffic"] = traffic
+
+    if args.copy_calibration:
+        # a plain copy of COPY_BYTES: its FETCH_SIZE / WRITE_SIZE under
+        # rocprofv3 --pmc give the counters' bytes-per-byte scale
+        lib = imfcs.lib()
+        dst = lib.imfcs_device_alloc(engines[0].ctx, ctypes.c_size_t(COPY_BYTES))
+        if dst:
+            lib.imfcs_memcpy_d2d(engines[0].ctx, dst, d_stack, ctypes.c_size_t(COPY_BYTES))
+            lib.imfcs_device_free(engines[0].ctx, dst)
+            log("copy calibration: %d bytes" % COPY_BYTES)
 
     # LM iteration statistics of every binning (untimed re-run)
     iters = {}

@@ -298,6 +298,14 @@ int imfcs_memcpy_d2h(imfcs_ctx *ctx, void *dst, const void *src, size_t bytes) {
     return 0;
 }
 
+int imfcs_memcpy_d2d(imfcs_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx) return fail(IMFCS_ERR_ARG, "null context");
+    launch_copy(dst, src, bytes, ctx->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
 int imfcs_synchronize(imfcs_ctx *ctx) {
     if (!ctx) return fail(IMFCS_ERR_ARG, "null context");
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -355,6 +363,10 @@ struct EngineArgs {
     const double *host_coef;   // [nseries][order+1] in seconds (drop-in)
     int nopit, ave;
     double frametime;
+    // bleach abscissa t = frametime * (frame + t_off), frame 0-based:
+    // 1.5 = the Java CPU path's 1-based (x + 0.5) (IFCS:9747), 0.5 = the
+    // reference GPU kernel's 0-based (i + 0.5) (agpufitjni.cu:5108)
+    double t_off;
     int p, q;
     PairDesc pd;
 };
@@ -416,7 +428,7 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTabl
             ctx->coef_scale.assign(n, 1.0);
         }
         bd.coef = ctx->coef.as<double>();
-        bd.t_off = 1.5;   // t = frametime * (x + 0.5), x 1-based (IFCS:9747)
+        bd.t_off = ea.t_off;
     } else {
         bd.tscale = 1.0;
         bd.coef = nullptr;
@@ -528,6 +540,7 @@ extern "C" int imfcs_run(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_co
     ea.nopit = std::min(g.T, 1000);   // IFCS:7323-7327
     ea.ave = g.T / ea.nopit;
     ea.frametime = c->frametime;
+    ea.t_off = 1.5;
     ea.p = c->correlator_p;
     ea.q = c->correlator_q;
     ea.pd.npix = g.npix;
@@ -799,6 +812,7 @@ extern "C" int imfcs_calc_acf(const float *pixels, double *pixels1, double *bloc
     ea.fit_trace = 0;
     ea.host_coef = hc.data();
     ea.frametime = p->frametime;
+    ea.t_off = 0.5;
     ea.p = (int)p->correlatorp;
     ea.q = (int)p->correlatorq;
     ea.pd.npix = g.npix;

@@ -983,7 +983,7 @@ __device__ void lm_eval_fixed(const LMArgs &a, int fit, const float *par, bool n
 #pragma unroll
       for (int k = 0; k < 8; k++) e[k] = 0.f;
       const int kend = min(8, (a.npts - j0 + 63) >> 6);   // wave-uniform
-#pragma unroll 1
+#pragma unroll 2
       for (int k = 0; k < kend; k++) {
         const int i = j0 + 64 * k + lane;
         if (i >= a.npts) continue;
@@ -1035,22 +1035,24 @@ __device__ void lm_eval_fixed(const LMArgs &a, int fit, const float *par, bool n
     }
 }
 
+// Cholesky solve with one reciprocal per column (the solve latency is on
+// the critical path of every LM iteration of a fit).
 template <int NF>
 __device__ __forceinline__ bool chol_solve_fixed(double (&A)[NF][NF], double (&b)[NF]) {
+    double inv[NF];
 #pragma unroll
     for (int j = 0; j < NF; j++) {
         double s = A[j][j];
 #pragma unroll
         for (int k = 0; k < j; k++) s -= A[j][k] * A[j][k];
         if (!(s > 0.0)) return false;
-        const double d = sqrt(s);
-        A[j][j] = d;
+        inv[j] = 1.0 / sqrt(s);
 #pragma unroll
         for (int i = j + 1; i < NF; i++) {
             double t = A[i][j];
 #pragma unroll
             for (int k = 0; k < j; k++) t -= A[i][k] * A[j][k];
-            A[i][j] = t / d;
+            A[i][j] = t * inv[j];
         }
     }
 #pragma unroll
@@ -1058,14 +1060,14 @@ __device__ __forceinline__ bool chol_solve_fixed(double (&A)[NF][NF], double (&b
         double t = b[i];
 #pragma unroll
         for (int k = 0; k < i; k++) t -= A[i][k] * b[k];
-        b[i] = t / A[i][i];
+        b[i] = t * inv[i];
     }
 #pragma unroll
     for (int i = NF - 1; i >= 0; i--) {
         double t = b[i];
 #pragma unroll
         for (int k = i + 1; k < NF; k++) t -= A[k][i] * b[k];
-        b[i] = t / A[i][i];
+        b[i] = t * inv[i];
     }
     return true;
 }
@@ -1226,10 +1228,26 @@ __global__ void k_pack_fit_inputs(int chanum, int npairs, const double *corav, c
     weights[dst] = 1.0f / (float)bvar[src];
 }
 
+// Plain device copy (16-byte vectors, grid-stride, byte tail).
+__global__ void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16, const uint8_t *src_tail,
+                       uint8_t *dst_tail, int ntail) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+    if (blockIdx.x == 0 && (int)threadIdx.x < ntail) dst_tail[threadIdx.x] = src_tail[threadIdx.x];
+}
+
 // ------------------------------------------------------------------
 // Launchers
 // ------------------------------------------------------------------
 static inline int cdiv(int64_t a, int b) { return (int)((a + b - 1) / b); }
+
+void launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st) {
+    const size_t n16 = bytes / 16;
+    const int ntail = (int)(bytes - n16 * 16);
+    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>((n16 + 255) / 256, 256 * 64));
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, st, (const uint4 *)src, (uint4 *)dst, n16,
+                       (const uint8_t *)src + n16 * 16, (uint8_t *)dst + n16 * 16, ntail);
+}
 
 void launch_trace_moments(const SeriesGeom &g, int nopit, int ave, double tscale, double frametime, int order,
                           int integer_trace, double *part, int nchunk, hipStream_t st) {

@@ -71,6 +71,7 @@ SIGNATURES = {
     "imfcs_device_free": (ctypes.c_int, [_VP, _VP]),
     "imfcs_memcpy_h2d": (ctypes.c_int, [_VP, _VP, _VP, _SZ]),
     "imfcs_memcpy_d2h": (ctypes.c_int, [_VP, _VP, _VP, _SZ]),
+    "imfcs_memcpy_d2d": (ctypes.c_int, [_VP, _VP, _VP, _SZ]),
     "imfcs_synchronize": (ctypes.c_int, [_VP]),
     "imfcs_run": (ctypes.c_int, [_VP, _VP, _P(PipelineConfig)]),
     "imfcs_get_results": (ctypes.c_int, [_VP, _P(_f64), _P(_f64), _P(_i32), _P(_i32), _P(_f64), _P(_f64),

@@ -181,6 +181,9 @@ void *imfcs_device_alloc(imfcs_ctx *ctx, size_t bytes);
 int imfcs_device_free(imfcs_ctx *ctx, void *ptr);
 int imfcs_memcpy_h2d(imfcs_ctx *ctx, void *dst, const void *src, size_t bytes);
 int imfcs_memcpy_d2h(imfcs_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* Device-to-device copy by a plain vector-copy kernel (also the known-size
+ * reference for HBM traffic counters). */
+int imfcs_memcpy_d2d(imfcs_ctx *ctx, void *dst, const void *src, size_t bytes);
 int imfcs_synchronize(imfcs_ctx *ctx);
 
 /* Runs the pipeline on a device-resident uint16 stack [T][H][W].  Results

@@ -27,7 +27,7 @@ def _compare_corr(g_acf, g_var, g_ind, o):
     e = rel_err(g_acf[finite], o["acf"][finite], 1e-4 * amp[finite])
     assert e.max() < 1e-4, e.max()
     same = (g_ind == o["ind"]) & finite
-    assert same.mean() >= 0.97, same.mean()
+    assert same[finite].mean() >= 0.97, same[finite].mean()
     ev = rel_err(g_var[same][:, 1:], o["var"][same][:, 1:], 1e-300)
     assert ev.max() < 1e-4, ev.max()
     return same
