@@ -218,7 +218,7 @@ def make_fit_model(ax, ay, s, sz=None, rx=0.0, ry=0.0, q2=1.0, q3=1.0, paramfit=
 def make_pipeline_config(W, H, T, binning=1, cf=(0, 0), background=0, poly_order=4, frametime=1e-3, p=16, q=8,
                          roi=None, do_acf=True, do_ccf=None, do_fit=True, acf_model=None, ccf_model=None,
                          overlap=False, firstframe=1, lastframe=None, fit_max_iterations=2000,
-                         fit_tolerance=1e-16, fit_chi2_fp64=True):
+                         fit_tolerance=1e-16, fit_chi2_fp64=True, gls=False):
     """imfcs_config of one pipeline run.  fit_chi2_fp64 selects FP64 step
     acceptance and convergence on chi^2 (the optimum the Java CPU path
     reaches); False keeps Gpufit's FP32 chi^2 of the reference GPU fit."""
@@ -241,6 +241,7 @@ def make_pipeline_config(W, H, T, binning=1, cf=(0, 0), background=0, poly_order
     c.fit_max_iterations = fit_max_iterations
     c.fit_tolerance = fit_tolerance
     c.fit_chi2_fp64 = int(bool(fit_chi2_fp64))
+    c.gls = int(bool(gls))
     if acf_model is not None:
         c.acf_model = acf_model
     if ccf_model is not None:
@@ -260,6 +261,7 @@ class Engine:
             raise ImFCSError(get_last_error())
         self._stack = None
         self._stack_bytes = 0
+        self._chanum_last = 0
 
     def close(self):
         if self.ctx:
@@ -292,6 +294,7 @@ class Engine:
 
     def run(self, cfg, d_stack=None):
         _check(self._lib.imfcs_run(self.ctx, d_stack or self._stack, ctypes.byref(cfg)))
+        self._chanum_last = cfg.correlator_p + (cfg.correlator_q - 1) * cfg.correlator_p // 2 + 1
 
     def synchronize(self):
         _check(self._lib.imfcs_synchronize(self.ctx))
@@ -302,6 +305,15 @@ class Engine:
         ms = (ctypes.c_float * 5)()
         _check(self._lib.imfcs_get_stage_ms(self.ctx, ms))
         return list(ms)
+
+    def gls_covariance(self, pair):
+        """Regularized covariance (chanum-1 x chanum-1) of one pair of the
+        last run with gls=True."""
+        r = self._lib.imfcs_get_gls_covariance
+        m = self._chanum_last - 1
+        out = np.zeros((m, m))
+        _check(r(self.ctx, int(pair), _dp(out)))
+        return out
 
     def correlator_kernel_ms(self):
         """Device time (ms) of the last run's main correlator kernel."""

@@ -179,12 +179,13 @@ struct imfcs_ctx {
     std::vector<float> h_xs, h_init;
     std::vector<int> h_pfit;
     DevBuf part, coef, ctot, rec, F, bpart, ind, bis, corav, bvar, fdata, fw, fx, finit, fpfit, fout, fstate, fchi2,
-        fiter, cpart, hpart, l3, lvd, ctab, csum, slots, X;
+        fiter, cpart, hpart, l3, lvd, ctab, csum, slots, X, glsblk, glsW, glsy, glsok, glscov;
     CorrTables host_ct;
     std::vector<int32_t> host_slots;   // slot_of then slot origins (source of the async copy)
     std::vector<double> host_coef;     // drop-in bleach coefficients per slot (source of the async copy)   // source of the async copy into ctab
     // last run
     int npix = 0, npairs = 0, nseries = 0, nslots = 0, chanum = 0, order = 0, ow = 0, oh = 0, did_fit = 0, nfree = 0;
+    int did_gls = 0;
     int force_v1 = 0;   // IMFCS_CORRELATOR=generic selects the one-thread-per-level kernel
     double chi2_norm = 1.0;
     std::vector<double> coef_scale;   // u-basis -> seconds conversion per power
@@ -251,7 +252,7 @@ static void release_all(imfcs_ctx *ctx) {
                       &ctx->bis, &ctx->corav, &ctx->bvar, &ctx->fdata, &ctx->fw, &ctx->fx, &ctx->finit,
                       &ctx->fpfit, &ctx->fout, &ctx->fstate, &ctx->fchi2, &ctx->fiter, &ctx->cpart,
                       &ctx->hpart, &ctx->l3, &ctx->lvd, &ctx->ctab, &ctx->csum, &ctx->slots,
-                      &ctx->X};
+                      &ctx->X, &ctx->glsblk, &ctx->glsW, &ctx->glsy, &ctx->glsok, &ctx->glscov};
     for (DevBuf *b : bufs) b->release();
 }
 
@@ -359,6 +360,7 @@ int build_slots(const SeriesGeom &g, std::vector<int32_t> &buf) {
 struct EngineArgs {
     SeriesGeom g;
     int order;             // polynomial order (0 = none)
+    int gls;               // 1: regularized covariance of every pair (calculateCF GLS branch)
     int fit_trace;         // 1: trace + fit on device; 0: coefficients given in host_coef
     const double *host_coef;   // [nseries][order+1] in seconds (drop-in)
     int nopit, ave;
@@ -461,6 +463,18 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTabl
     // correlation
     HIP_TRY(ctx->corav.ensure(sizeof(double) * (size_t)np * ct.chanum));
     HIP_TRY(ctx->bvar.ensure(sizeof(double) * (size_t)np * ct.chanum));
+    const int m = ct.chanum - 1, nblk = (int)ct.mtab_last;
+    ctx->did_gls = 0;
+    if (ea.gls) {
+        if (!correlate_fast_supported(ct) || ctx->force_v1)
+            return fail(IMFCS_ERR_ARG, "GLS needs the p = 16 correlator layout with blocknum >= q");
+        if (m > kGlsMaxM) return fail(IMFCS_ERR_ARG, "GLS supports at most 104 fitted channels");
+        HIP_TRY(ctx->glsblk.ensure(sizeof(float) * (size_t)np * nblk * m));
+        HIP_TRY(ctx->glsW.ensure(sizeof(double) * (size_t)np * m * m));
+        HIP_TRY(ctx->glscov.ensure(sizeof(double) * (size_t)np * m * m));
+        HIP_TRY(ctx->glsy.ensure(sizeof(double) * (size_t)np * m));
+        HIP_TRY(ctx->glsok.ensure(sizeof(int32_t) * (size_t)np));
+    }
     if (correlate_fast_supported(ct) && !ctx->force_v1) {
         const int clen = corr_fast_chunk_len(ct, np);
         const int nch = (g.T + clen - 1) / clen;
@@ -473,8 +487,17 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTabl
         launch_correlate_fast(g, bd, ct, dct, ea.pd, ctx->F.as<double>(), ctx->X.as<float>(), ctx->ind.as<int32_t>(),
                               ctx->cpart.as<float>(), ctx->hpart.as<double>(), ctx->l3.as<float>(),
                               ctx->lvd.as<double>(), clen,
-                              ctx->corav.as<double>(), ctx->bvar.as<double>(), st, ctx->evk[0], ctx->evk[1]);
+                              ctx->corav.as<double>(), ctx->bvar.as<double>(), st, ctx->evk[0], ctx->evk[1],
+                              ea.gls ? ctx->glsblk.as<float>() : nullptr, nblk);
         ctx->evk_valid = 1;
+        if (ea.gls) {
+            // mcov replaces the correlation as Java's GLS output (IFCS:9638-9639)
+            GlsArgs A{dct, g, ea.pd, bd, ctx->F.as<double>(), ctx->ind.as<int32_t>(), ctx->glsblk.as<float>(), nblk,
+                      ctx->corav.as<double>(), ctx->glsW.as<double>(), ctx->glsy.as<double>(),
+                      ctx->glsok.as<int32_t>(), ctx->glscov.as<double>()};
+            launch_gls_cov(A, np, st);
+            ctx->did_gls = 1;
+        }
     } else {
         HIP_TRY(hipEventRecord(ctx->evk[0], st));
         launch_correlate(g, bd, ct, dct, ea.pd, ctx->F.as<double>(), ctx->X.as<float>(), ctx->ind.as<int32_t>(),
@@ -540,6 +563,7 @@ extern "C" int imfcs_run(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_co
     ea.nopit = std::min(g.T, 1000);   // IFCS:7323-7327
     ea.ave = g.T / ea.nopit;
     ea.frametime = c->frametime;
+    ea.gls = c->gls ? 1 : 0;
     ea.t_off = 1.5;
     ea.p = c->correlator_p;
     ea.q = c->correlator_q;
@@ -567,7 +591,8 @@ extern "C" int imfcs_run(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_co
         HIP_TRY(ctx->fstate.ensure(sizeof(int) * np));
         HIP_TRY(ctx->fchi2.ensure(sizeof(float) * np));
         HIP_TRY(ctx->fiter.ensure(sizeof(int) * np));
-        launch_pack_fit_inputs(ct, np, ctx->corav.as<double>(), ctx->bvar.as<double>(), ctx->fdata.as<float>(),
+        if (!ctx->did_gls)
+            launch_pack_fit_inputs(ct, np, ctx->corav.as<double>(), ctx->bvar.as<double>(), ctx->fdata.as<float>(),
                                ctx->fw.as<float>(), st);
         // lag times of channels 1..chanum-1 (IFCS:15122-15125) and initial
         // parameters with the fixed geometry in slots 11..19 (IFCS:15090-15110)
@@ -601,13 +626,23 @@ extern "C" int imfcs_run(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_co
         HIP_TRY(hipEventRecord(ctx->ev_h2d, st));
         unsigned mask = 0;
         for (int k = 0; k < 11; k++) mask |= (pfit[k] ? 1u : 0u) << k;
-        launch_lm_acf(np, npts, ctx->fdata.as<float>(), ctx->fw.as<float>(), ctx->fx.as<float>(),
-                      ctx->finit.as<float>(), ctx->fpfit.as<int>(), mask, LMModelGeom{}, c->fit_tolerance,
-                      c->fit_max_iterations, c->fit_chi2_fp64 ? 1 : 0, ctx->fout.as<float>(),
-                      ctx->fstate.as<int>(), ctx->fchi2.as<float>(), ctx->fiter.as<int>(), st);
         ctx->nfree = nfree;
-        // chi2 normalization of GPU_ACF_Fit (IFCS:15234)
-        ctx->chi2_norm = (double)((npts - 1) - nfree - 1);
+        if (ctx->did_gls) {
+            if (mask != 0x13u) return fail(IMFCS_ERR_ARG, "the GLS fit supports the free set N, D, G_inf");
+            launch_lm_gls(np, npts, ctx->fx.as<float>(), ctx->finit.as<float>(), ctx->fpfit.as<int>(),
+                          c->fit_tolerance, c->fit_max_iterations, ctx->glsW.as<double>(), ctx->glsy.as<double>(),
+                          ctx->glsok.as<int32_t>(), ctx->fout.as<float>(), ctx->fstate.as<int>(),
+                          ctx->fchi2.as<float>(), ctx->fiter.as<int>(), st);
+            // chi2 of GLSFit: sum of whitened residuals^2 / (chanum - nfree - 1) (IFCS:10733-10735)
+            ctx->chi2_norm = (double)(ct.chanum - nfree - 1);
+        } else {
+            launch_lm_acf(np, npts, ctx->fdata.as<float>(), ctx->fw.as<float>(), ctx->fx.as<float>(),
+                          ctx->finit.as<float>(), ctx->fpfit.as<int>(), mask, LMModelGeom{}, c->fit_tolerance,
+                          c->fit_max_iterations, c->fit_chi2_fp64 ? 1 : 0, ctx->fout.as<float>(),
+                          ctx->fstate.as<int>(), ctx->fchi2.as<float>(), ctx->fiter.as<int>(), st);
+            // chi2 normalization of GPU_ACF_Fit (IFCS:15234)
+            ctx->chi2_norm = (double)((npts - 1) - nfree - 1);
+        }
     }
     HIP_TRY(hipEventRecord(ctx->ev[5], st));
     HIP_TRY(hipGetLastError());
@@ -622,6 +657,17 @@ extern "C" int imfcs_get_stage_ms(imfcs_ctx *ctx, float *ms) {
         HIP_TRY(hipEventElapsedTime(&t, ctx->ev[i], ctx->ev[i + 1]));
         ms[i] = t;
     }
+    return 0;
+}
+
+extern "C" int imfcs_get_gls_covariance(imfcs_ctx *ctx, int pair, double *cov) {
+    if (!ctx || !cov) return fail(IMFCS_ERR_ARG, "null argument");
+    if (!ctx->did_gls) return fail(IMFCS_ERR_ARG, "the last run computed no GLS covariance");
+    if (pair < 0 || pair >= ctx->npairs) return fail(IMFCS_ERR_ARG, "pair index out of range");
+    const size_t m = ctx->chanum - 1;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipMemcpy(cov, ctx->glscov.as<double>() + (size_t)pair * m * m, sizeof(double) * m * m,
+                      hipMemcpyDeviceToHost));
     return 0;
 }
 

@@ -42,6 +42,10 @@ struct MainArgs {
     float *l3;      // level-3 values [numbin_3][nseries]
     double *lvd;    // levels >= 4, level c at lvd + loff[c]: [numbin_c][nseries]
     int64_t loff[kMaxLevel];
+    // GLS: every block sum of the blocked variance, [pair][block][channel-1]
+    // (channels 1..chanum-1); null when no covariance is requested
+    float *gls;
+    int gls_nblk;
 };
 
 struct Tile {
@@ -219,6 +223,9 @@ __device__ void run_role(const MainArgs &A, Tile &tl, Loader &ld, int lane, int 
                         if (inblk) {
                             acc.a1[i] += S;
                             acc.a2[i] = fmaf(S, S, acc.a2[i]);
+                            const int x = ct.first_chan[LEVEL] + (LEVEL == 0 ? DMIN : DMIN - 9) + i;
+                            if (A.gls && x > 0)
+                                A.gls[((int64_t)pair * A.gls_nblk + (ye / B - 1)) * (ct.chanum - 1) + (x - 1)] = S;
                         }
                         acc.cross[i] += acc.sab[i];
                         acc.sab[i] = 0.f;
@@ -298,7 +305,7 @@ __global__ void k_corr_rebin(const float *l3, double *lvd, int64_t off_src, int6
 // one call.
 template <int G>
 __device__ void high_level(const L3Ctx &L, bool same, const double *alpha, const double *beta, int B, int pnum,
-                           int ybeg, int yend, double *cross, double *a1, double *a2) {
+                           int ybeg, int yend, double *cross, double *a1, double *a2, float *gls_row, int gls_ld) {
     constexpr int WN = G + 16;
     double w[WN], ga[G], sab[8], sb[8];
     double sa = 0.0;
@@ -336,6 +343,7 @@ __device__ void high_level(const L3Ctx &L, bool same, const double *alpha, const
                 if (inblk) {
                     a1[i] += S;
                     a2[i] += S * S;
+                    if (gls_row) gls_row[(int64_t)(ye / B - 1) * gls_ld + i] = (float)S;
                 }
                 cross[i] += sab[i];
                 sab[i] = sb[i] = 0.0;
@@ -380,13 +388,14 @@ __global__ void __launch_bounds__(64) k_corr_high(MainArgs A, double *hpart /* [
     const int len = A.chunk_len >> c;
     const int ybeg = chunk * len;
     const int yend = (chunk == A.nchunk - 1) ? ct.numbin[c] : ybeg + len;
+    float *grow = A.gls ? A.gls + (int64_t)pidx * A.gls_nblk * (ct.chanum - 1) + (x0 - 1) : nullptr;
     int G = min(16, 1 << max(ct.q - 1 - c, 0));
     switch (G) {
-        case 16: high_level<16>(L, sa == sb, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2); break;
-        case 8: high_level<8>(L, sa == sb, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2); break;
-        case 4: high_level<4>(L, sa == sb, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2); break;
-        case 2: high_level<2>(L, sa == sb, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2); break;
-        default: high_level<1>(L, sa == sb, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2); break;
+        case 16: high_level<16>(L, sa == sb, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ct.chanum - 1); break;
+        case 8: high_level<8>(L, sa == sb, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ct.chanum - 1); break;
+        case 4: high_level<4>(L, sa == sb, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ct.chanum - 1); break;
+        case 2: high_level<2>(L, sa == sb, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ct.chanum - 1); break;
+        default: high_level<1>(L, sa == sb, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ct.chanum - 1); break;
     }
     double *out = hpart + (((int64_t)chunk * A.pd.npairs + pidx) * nhigh + (x0 - kMainChan)) * 3;
     for (int i = 0; i < 8; i++) {
@@ -458,7 +467,8 @@ int corr_fast_chunk_len(const CorrTables &ct, int npairs) {
 void launch_correlate_fast(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const CorrTables *dct,
                            const PairDesc &pd, const double *F, const float *X, const int32_t *ind, float *part,
                            double *hpart, float *l3, double *lvd, int chunk_len, double *corav, double *bvar,
-                           hipStream_t st, hipEvent_t ev_main_begin, hipEvent_t ev_main_end) {
+                           hipStream_t st, hipEvent_t ev_main_begin, hipEvent_t ev_main_end, float *gls_blk,
+                           int gls_nblk) {
     MainArgs A;
     A.g = g;
     A.bd = bd;
@@ -474,6 +484,8 @@ void launch_correlate_fast(const SeriesGeom &g, const BleachDesc &bd, const Corr
     A.part = part;
     A.l3 = l3;
     A.lvd = lvd;
+    A.gls = gls_blk;
+    A.gls_nblk = gls_nblk;
     int64_t off = 0;
     for (int c = 0; c < kMaxLevel; c++) {
         A.loff[c] = off;

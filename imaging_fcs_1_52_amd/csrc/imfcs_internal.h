@@ -98,6 +98,28 @@ struct LMModelGeom {
     float ax, ay, s, sz, rx, ry, obsvol, q2, q3;   // micrometres
 };
 
+// GLS covariance of every pair (imfcs_gls.hip).
+constexpr int kGlsMaxM = 104;   // fitted channels up to q = 12 (p = 16)
+struct GlsArgs {
+    const CorrTables *ctp;
+    SeriesGeom g;
+    PairDesc pd;
+    BleachDesc bd;
+    const double *F;
+    const int32_t *ind;
+    const float *blk;     // [pair][nblk][m] block sums of the blocked variance
+    int nblk;
+    double *mcov;         // [pair][chanum] (channel 0 = 0), the GLS data
+    double *W;            // [pair][m][m] inverse Cholesky factor (lower)
+    double *yw;           // [pair][m] whitened data W mcov
+    int32_t *ok;          // [pair] 1 = factorized
+    double *cov;          // [pair][m][m] regularized covariance (optional)
+};
+void launch_gls_cov(const GlsArgs &A, int npairs, hipStream_t st);
+void launch_lm_gls(int nfits, int m, const float *x, const float *init, const int *pfit, float tol, int maxit,
+                   const double *W, const double *yw, const int32_t *ok, float *out_params, int *states, float *chi2,
+                   int *iters, hipStream_t st);
+
 // Launchers (imfcs_kernels.hip)
 void launch_trace_moments(const SeriesGeom &g, int nopit, int ave, double tscale, double frametime,
                           int order, int integer_trace, double *part, int nchunk, hipStream_t st);
@@ -129,7 +151,8 @@ int corr_fast_chunk_len(const CorrTables &ct, int npairs);
 void launch_correlate_fast(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const CorrTables *dct,
                            const PairDesc &pd, const double *F, const float *X, const int32_t *ind, float *part,
                            double *hpart, float *l3, double *lvd, int chunk_len, double *corav, double *bvar,
-                           hipStream_t st, hipEvent_t ev_main_begin = nullptr, hipEvent_t ev_main_end = nullptr);
+                           hipStream_t st, hipEvent_t ev_main_begin = nullptr, hipEvent_t ev_main_end = nullptr,
+                           float *gls_blk = nullptr, int gls_nblk = 0);
 void launch_lm_acf(int nfits, int npts, const float *data, const float *weights, const float *x,
                    const float *init, const int *pfit, unsigned free_mask, const LMModelGeom &geom, float tol, int maxit,
                    int chi2_fp64, float *out_params, int *states, float *chi2, int *iters, hipStream_t st);

@@ -1072,37 +1072,43 @@ __device__ __forceinline__ bool chol_solve_fixed(double (&A)[NF][NF], double (&b
     return true;
 }
 
-template <int KIND, int NF>
-__global__ void __launch_bounds__(64) k_lm_fixed(LMArgs a) {
-    constexpr int NP = (KIND == kModelFCS) ? 20 : 11;
+// FCS parameters: SI at the boundary (D m^2/s, v m/s, geometry m), micro-
+// metres and seconds inside the model; the fixed geometry travels in
+// parameter slots 11..19 (the ACF_1D layout of GPU_ACF_Fit, IFCS:15090-15110).
+__device__ __forceinline__ void fcs_params_in(float *par, LMModelGeom &geom) {
+    par[1] *= 1e12f;
+    par[6] *= 1e12f;
+    par[8] *= 1e12f;
+    par[2] *= 1e6f;
+    par[3] *= 1e6f;
+    geom.ax = par[11] * 1e6f;
+    geom.ay = par[12] * 1e6f;
+    geom.s = par[13] * 1e6f;
+    geom.sz = par[14] * 1e6f;
+    geom.rx = par[15] * 1e6f;
+    geom.ry = par[16] * 1e6f;
+    geom.obsvol = par[17] * 1e12f;
+    geom.q2 = par[18];
+    geom.q3 = par[19];
+}
+__device__ __forceinline__ void fcs_params_out(float *par) {
+    par[1] *= 1e-12f;
+    par[6] *= 1e-12f;
+    par[8] *= 1e-12f;
+    par[2] *= 1e-6f;
+    par[3] *= 1e-6f;
+}
+
+// Gpufit's LM iteration over the fixed free slots.  eval(par, chi2, grad,
+// hess) returns the chi-square, gradient and packed normal matrix of the
+// current parameters.
+template <int KIND, int NF, class Eval>
+__device__ void lm_fixed_iterate(const LMArgs &a, float *par, Eval &eval, double &chi2, int &state, int &iters) {
     constexpr int NH = NF * (NF + 1) / 2;
-    const int fit = blockIdx.x;
-    if (fit >= a.nfits) return;
-    const int lane = threadIdx.x & 63;
-    float par[NP];
-#pragma unroll
-    for (int k = 0; k < NP; k++) par[k] = a.init[(int64_t)fit * NP + k];
-    LMArgs ac = a;
-    if (KIND == kModelFCS) {
-        par[1] *= 1e12f;
-        par[6] *= 1e12f;
-        par[8] *= 1e12f;
-        par[2] *= 1e6f;
-        par[3] *= 1e6f;
-        ac.geom.ax = par[11] * 1e6f;
-        ac.geom.ay = par[12] * 1e6f;
-        ac.geom.s = par[13] * 1e6f;
-        ac.geom.sz = par[14] * 1e6f;
-        ac.geom.rx = par[15] * 1e6f;
-        ac.geom.ry = par[16] * 1e6f;
-        ac.geom.obsvol = par[17] * 1e12f;
-        ac.geom.q2 = par[18];
-        ac.geom.q3 = par[19];
-    }
-    double chi2;
     double grad[NF], hess[NH], H[NF][NF], scale[NF];
-    lm_eval_fixed<KIND, NF>(ac, fit, par, true, chi2, grad, hess);
-    int state = kStateConverged, iters = 0;
+    eval(par, chi2, grad, hess);
+    state = kStateConverged;
+    iters = 0;
     double lambda = 1e-3;
     double prev_chi2 = chi2;
     bool failed = false;
@@ -1139,8 +1145,6 @@ __global__ void __launch_bounds__(64) k_lm_fixed(LMArgs a) {
         float prev[NF];
 #pragma unroll
         for (int k = 0; k < NF; k++) {
-            constexpr int dummy = 0;
-            (void)dummy;
             const int sl = FixedSlots<KIND, NF>::slot(k);
             prev[k] = par[sl];
             par[sl] = (float)((double)par[sl] + delta[k]);
@@ -1150,7 +1154,7 @@ __global__ void __launch_bounds__(64) k_lm_fixed(LMArgs a) {
         // dropped when the step fails
         double nchi2;
         double ngrad[NF], nhess[NH];
-        lm_eval_fixed<KIND, NF>(ac, fit, par, true, nchi2, ngrad, nhess);
+        eval(par, nchi2, ngrad, nhess);
         const bool decreased = nchi2 < prev_chi2;
         // cuda_check_fit_improvement (agpufitjni.cu:1453-1476)
         const bool step_failed = (prev_chi2 != 0.0) && !decreased;
@@ -1178,13 +1182,115 @@ __global__ void __launch_bounds__(64) k_lm_fixed(LMArgs a) {
         }
     }
     chi2 = prev_chi2;
-    if (KIND == kModelFCS) {
-        par[1] *= 1e-12f;
-        par[6] *= 1e-12f;
-        par[8] *= 1e-12f;
-        par[2] *= 1e-6f;
-        par[3] *= 1e-6f;
+}
+
+template <int KIND, int NF>
+__global__ void __launch_bounds__(64) k_lm_fixed(LMArgs a) {
+    constexpr int NP = (KIND == kModelFCS) ? 20 : 11;
+    const int fit = blockIdx.x;
+    if (fit >= a.nfits) return;
+    const int lane = threadIdx.x & 63;
+    float par[NP];
+#pragma unroll
+    for (int k = 0; k < NP; k++) par[k] = a.init[(int64_t)fit * NP + k];
+    LMArgs ac = a;
+    if (KIND == kModelFCS) fcs_params_in(par, ac.geom);
+    auto eval = [&](const float *p, double &c2, double *g, double *h) {
+        lm_eval_fixed<KIND, NF>(ac, fit, p, true, c2, g, h);
+    };
+    double chi2;
+    int state, iters;
+    lm_fixed_iterate<KIND, NF>(a, par, eval, chi2, state, iters);
+    if (KIND == kModelFCS) fcs_params_out(par);
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NP; k++) a.out_params[(int64_t)fit * NP + k] = par[k];
+        a.states[fit] = state;
+        a.chi2[fit] = (float)chi2;
+        a.iters[fit] = iters;
     }
+}
+
+// GLS fit of the FCS model with N, D, G_inf free (GLSFit, IFCS:10622-10789):
+// the model values and Jacobian at the m lag times are whitened by W = L^-1
+// (dataTransform / GLS_fitFunction, IFCS:13569-13729) and fitted to the
+// whitened data yw = W mcov with unit weights; chi-square in FP64.  W lives
+// in LDS (m x m doubles) for the fit's lifetime.
+__global__ void __launch_bounds__(64) k_lm_gls(LMArgs a, const double *W, const double *yw, const int32_t *okv,
+                                               int m) {
+    constexpr int NF = 3, NH = 6, NP = 20;
+    extern __shared__ double gls_lds[];
+    double *Wl = gls_lds;              // [m][m]
+    double *fv = gls_lds + m * m;      // [m]
+    double *jv = fv + m;               // [m][NF]
+    const int fit = blockIdx.x;
+    if (fit >= a.nfits) return;
+    const int lane = threadIdx.x & 63;
+    float par[NP];
+#pragma unroll
+    for (int k = 0; k < NP; k++) par[k] = a.init[(int64_t)fit * NP + k];
+    if (!okv[fit]) {
+        // the Cholesky factorization failed: Java's fit throws (no result)
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < NP; k++) a.out_params[(int64_t)fit * NP + k] = par[k];
+            a.states[fit] = kStateSingular;
+            a.chi2[fit] = __builtin_nanf("");
+            a.iters[fit] = 0;
+        }
+        return;
+    }
+    const double *Wg = W + (int64_t)fit * m * m;
+    for (int i = lane; i < m * m; i += 64) Wl[i] = Wg[i];
+    const double *ywf = yw + (int64_t)fit * m;
+    LMArgs ac = a;
+    fcs_params_in(par, ac.geom);
+    const bool c2on = (par[5] != 0.f) || a.pfit[6];
+    const bool c3on = (par[7] != 0.f) || a.pfit[8];
+    __syncthreads();
+    auto eval = [&](const float *p, double &c2, double *g, double *h) {
+        for (int i = lane; i < m; i += 64) {
+            float gfull[11];
+            const float f = fcs3p(ac.geom, a.x[i], p, true, c2on, c3on, gfull);
+            fv[i] = (double)f;
+#pragma unroll
+            for (int q = 0; q < NF; q++) jv[i * NF + q] = (double)gfull[FixedSlots<kModelFCS, NF>::slot(q)];
+        }
+        __syncthreads();
+        double cx = 0.0, gl[NF], hl[NH];
+#pragma unroll
+        for (int q = 0; q < NF; q++) gl[q] = 0.0;
+#pragma unroll
+        for (int q = 0; q < NH; q++) hl[q] = 0.0;
+        for (int r = lane; r < m; r += 64) {
+            double fw = 0.0, jw[NF] = {0.0, 0.0, 0.0};
+            for (int k = 0; k <= r; k++) {
+                const double w = Wl[r * m + k];
+                fw = fma(w, fv[k], fw);
+#pragma unroll
+                for (int q = 0; q < NF; q++) jw[q] = fma(w, jv[k * NF + q], jw[q]);
+            }
+            const double res = ywf[r] - fw;
+            cx = fma(res, res, cx);
+            int hh = 0;
+#pragma unroll
+            for (int q = 0; q < NF; q++) {
+                gl[q] = fma(res, jw[q], gl[q]);
+#pragma unroll
+                for (int l = 0; l <= q; l++, hh++) hl[hh] = fma(jw[q], jw[l], hl[hh]);
+            }
+        }
+        __syncthreads();
+        c2 = wave_sum(cx);
+#pragma unroll
+        for (int q = 0; q < NF; q++) g[q] = wave_sum(gl[q]);
+#pragma unroll
+        for (int q = 0; q < NH; q++) h[q] = wave_sum(hl[q]);
+    };
+    double chi2;
+    int state, iters;
+    lm_fixed_iterate<kModelFCS, NF>(a, par, eval, chi2, state, iters);
+    fcs_params_out(par);
     if (lane == 0) {
 #pragma unroll
         for (int k = 0; k < NP; k++) a.out_params[(int64_t)fit * NP + k] = par[k];
@@ -1366,6 +1472,17 @@ static void launch_lm(const LMArgs &a, unsigned free_mask, hipStream_t st) {
         }
     }
     hipLaunchKernelGGL(k_lm, dim3(a.nfits), dim3(64), 0, st, a);
+}
+
+void launch_lm_gls(int nfits, int m, const float *x, const float *init, const int *pfit, float tol, int maxit,
+                   const double *W, const double *yw, const int32_t *ok, float *out_params, int *states, float *chi2,
+                   int *iters, hipStream_t st) {
+    LMArgs a{nfits, m, 20, kModelFCS, 0, nullptr, nullptr, x, init, pfit, LMModelGeom{}, tol, maxit, 1, out_params,
+             states, chi2, iters};
+    const size_t lds = sizeof(double) * ((size_t)m * m + (size_t)m * 4);
+    if (lds > 65536) (void)hipFuncSetAttribute((const void *)k_lm_gls, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)lds);
+    hipLaunchKernelGGL(k_lm_gls, dim3(nfits), dim3(64), lds, st, a, W, yw, ok, m);
 }
 
 void launch_lm_acf(int nfits, int npts, const float *data, const float *weights, const float *x, const float *init,

@@ -44,7 +44,7 @@ class PipelineConfig(ctypes.Structure):
                 ("correlator_p", _i32), ("correlator_q", _i32), ("frametime", _f64), ("poly_order", _i32),
                 ("roi_x", _i32), ("roi_y", _i32), ("roi_w", _i32), ("roi_h", _i32), ("do_acf", _i32),
                 ("do_ccf", _i32), ("do_fit", _i32), ("fit_max_iterations", _i32), ("fit_tolerance", _f32),
-                ("fit_chi2_fp64", _i32), ("acf_model", FitModelGeom), ("ccf_model", FitModelGeom)]
+                ("fit_chi2_fp64", _i32), ("gls", _i32), ("acf_model", FitModelGeom), ("ccf_model", FitModelGeom)]
 
 
 # every C-ABI entry point of include/imfcs.h with its ctypes signature
@@ -78,6 +78,7 @@ SIGNATURES = {
                                          _P(_i32), _P(_i32), _P(_f64)]),
     "imfcs_get_stage_ms": (ctypes.c_int, [_VP, _P(_f32)]),
     "imfcs_get_correlator_kernel_ms": (ctypes.c_int, [_VP, _P(_f32)]),
+    "imfcs_get_gls_covariance": (ctypes.c_int, [_VP, ctypes.c_int, _P(_f64)]),
 }
 
 JNI_EXPORTS = ["fit", "getLastError", "isCudaAvailableInt", "getCudaVersionAsArray", "resetGPU",

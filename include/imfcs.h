@@ -167,6 +167,10 @@ typedef struct imfcs_config {
     float fit_tolerance;             /* relative chi^2 change for convergence */
     int32_t fit_chi2_fp64;           /* 1: accept steps and test convergence on an FP64 chi^2
                                         (tighter optimum); 0: Gpufit's FP32 chi^2 */
+    int32_t gls;                     /* 1: regularized channel covariance of every pair
+                                        (calculateCF GLS branch, IFCS:9524-9639); the acf
+                                        output becomes its mean curve and the fit is the
+                                        GLS fit (GLSFit, IFCS:10622-10789) */
     imfcs_fit_model acf_model, ccf_model;
 } imfcs_config;
 
@@ -190,6 +194,10 @@ int imfcs_synchronize(imfcs_ctx *ctx);
  * stay on the device inside ctx until imfcs_get_results.  Asynchronous
  * on the context's stream. */
 int imfcs_run(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_config *cfg);
+
+/* Regularized covariance (chanum-1)^2, FP64, row-major, of one pair of the
+ * last imfcs_run with cfg.gls = 1 (covmats of IFCS:9601-9613). */
+int imfcs_get_gls_covariance(imfcs_ctx *ctx, int pair, double *cov);
 
 /* Device time (ms) of the main multi-tau correlator kernel (levels 0..2,
  * the pass over the corrected series) of the last imfcs_run; measured with
