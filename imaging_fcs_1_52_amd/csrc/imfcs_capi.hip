@@ -179,7 +179,8 @@ struct imfcs_ctx {
     std::vector<float> h_xs, h_init;
     std::vector<int> h_pfit;
     DevBuf part, coef, ctot, rec, F, bpart, ind, bis, corav, bvar, fdata, fw, fx, finit, fpfit, fout, fstate, fchi2,
-        fiter, cpart, hpart, l3, lvd, ctab, csum, slots, X, glsblk, glsW, glsy, glsok, glscov;
+        fiter, cpart, hpart, l3, lvd, ctab, csum, slots, X, glsblk, glsW, glsy, glsok, glscov, dropin_in, nbm, nbc;
+    SeriesGeom last_g{};   // geometry (with device slot tables) of the last engine run
     CorrTables host_ct;
     std::vector<int32_t> host_slots;   // slot_of then slot origins (source of the async copy)
     std::vector<double> host_coef;     // drop-in bleach coefficients per slot (source of the async copy)   // source of the async copy into ctab
@@ -252,7 +253,8 @@ static void release_all(imfcs_ctx *ctx) {
                       &ctx->bis, &ctx->corav, &ctx->bvar, &ctx->fdata, &ctx->fw, &ctx->fx, &ctx->finit,
                       &ctx->fpfit, &ctx->fout, &ctx->fstate, &ctx->fchi2, &ctx->fiter, &ctx->cpart,
                       &ctx->hpart, &ctx->l3, &ctx->lvd, &ctx->ctab, &ctx->csum, &ctx->slots,
-                      &ctx->X, &ctx->glsblk, &ctx->glsW, &ctx->glsy, &ctx->glsok, &ctx->glscov};
+                      &ctx->X, &ctx->glsblk, &ctx->glsW, &ctx->glsy, &ctx->glsok, &ctx->glscov,
+                      &ctx->dropin_in, &ctx->nbm, &ctx->nbc};
     for (DevBuf *b : bufs) b->release();
 }
 
@@ -507,6 +509,7 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTabl
     }
     HIP_TRY(hipEventRecord(ctx->ev[4], st));
     HIP_TRY(hipGetLastError());
+    ctx->last_g = g;
     return 0;
 }
 
@@ -807,14 +810,13 @@ extern "C" int imfcs_calc_acf(const float *pixels, double *pixels1, double *bloc
     (void)samp;
     (void)lag;
     if (!pixels || !pixels1 || !blockvararray || !blocked1D || !p) return fail(IMFCS_ERR_ARG, "null argument");
-    if (p->isNBcalculation) return fail(IMFCS_ERR_ARG, "N&B calculation is not implemented");
+    if (p->isNBcalculation && (!NBmean || !NBcov)) return fail(IMFCS_ERR_ARG, "null N&B output");
     std::lock_guard<std::mutex> lk(g_dropin_mu);
     imfcs_ctx *ctx = dropin_ctx();
     if (!ctx) return IMFCS_ERR_DEVICE;
     int T = p->lastframe - p->firstframe + 1;
     size_t nin = (size_t)p->w_temp * p->h_temp * T;
-    // input lives in the generic scratch buffer `rec` is needed later, so use fdata
-    DevBuf input;
+    DevBuf &input = ctx->dropin_in;
     HIP_TRY(input.ensure(nin * 4));
     HIP_TRY(hipMemcpyAsync(input.p, pixels, nin * 4, hipMemcpyHostToDevice, ctx->stream));
     EngineArgs ea{};
@@ -837,10 +839,8 @@ extern "C" int imfcs_calc_acf(const float *pixels, double *pixels1, double *bloc
     g.cfy = p->cfYDistance;
     bool ccf = (p->cfXDistance != 0 || p->cfYDistance != 0);
     g.nseries = g.npix * (ccf ? 2 : 1);
-    if ((g.ow - 1) * g.pbx + g.cfx >= p->w_temp || (g.oh - 1) * g.pby + g.cfy >= p->h_temp) {
-        input.release();
+    if ((g.ow - 1) * g.pbx + g.cfx >= p->w_temp || (g.oh - 1) * g.pby + g.cfy >= p->h_temp)
         return fail(IMFCS_ERR_ARG, "output grid exceeds the binned input");
-    }
     int order = p->bleachcorr_gpu ? p->bleachcorr_order - 1 : 0;
     std::vector<double> hc;
     if (order > 0) {
@@ -869,11 +869,22 @@ extern "C" int imfcs_calc_acf(const float *pixels, double *pixels1, double *bloc
     HostTables ht;
     CorrTables ct;
     int rc = run_engine(ctx, ea, ht, ct);
+    if (rc == 0 && p->isNBcalculation) {
+        HIP_TRY(ctx->nbm.ensure(sizeof(double) * (size_t)g.npix));
+        HIP_TRY(ctx->nbc.ensure(sizeof(double) * (size_t)g.npix));
+        launch_nb_extract(ctx->ctab.as<CorrTables>(), ctx->last_g, ea.pd, ctx->F.as<double>(), ctx->corav.as<double>(),
+                          ctx->nbm.as<double>(), ctx->nbc.as<double>(), ctx->stream);
+        HIP_TRY(hipGetLastError());
+    }
     if (rc == 0) {
         size_t np = g.npix, nc = ct.chanum;
         std::vector<double> cav(np * nc), bv(np * nc);
         std::vector<int32_t> iv(np), bi(np);
         hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess && p->isNBcalculation) {
+            e = hipMemcpy(NBmean, ctx->nbm.p, sizeof(double) * np, hipMemcpyDeviceToHost);
+            if (e == hipSuccess) e = hipMemcpy(NBcov, ctx->nbc.p, sizeof(double) * np, hipMemcpyDeviceToHost);
+        }
         if (e == hipSuccess) e = hipMemcpy(cav.data(), ctx->corav.p, sizeof(double) * np * nc, hipMemcpyDeviceToHost);
         if (e == hipSuccess) e = hipMemcpy(bv.data(), ctx->bvar.p, sizeof(double) * np * nc, hipMemcpyDeviceToHost);
         if (e == hipSuccess) e = hipMemcpy(iv.data(), ctx->ind.p, sizeof(int32_t) * np, hipMemcpyDeviceToHost);
@@ -895,9 +906,6 @@ extern "C" int imfcs_calc_acf(const float *pixels, double *pixels1, double *bloc
         }
     }
     (void)hipStreamSynchronize(ctx->stream);
-    input.release();
-    (void)NBmean;
-    (void)NBcov;
     return rc;
 }
 

@@ -164,6 +164,8 @@ void launch_binning_f32(const float *in, float *out, int win, int hin, int wt, i
 void launch_bleach_average_f32(const float *in, float *out, int wt, int ht, int nopit, int ave,
                                hipStream_t st);
 void launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st);
+void launch_nb_extract(const CorrTables *dct, const SeriesGeom &g, const PairDesc &pd, const double *F,
+                       const double *corav, double *nbmean, double *nbcov, hipStream_t st);
 void launch_pack_fit_inputs(const CorrTables &ct, int npairs, const double *corav, const double *bvar,
                             float *data, float *weights, hipStream_t st);
 

@@ -1334,6 +1334,25 @@ __global__ void k_pack_fit_inputs(int chanum, int npairs, const double *corav, c
     weights[dst] = 1.0f / (float)bvar[src];
 }
 
+// N&B of the drop-in calcACF (isNBcalculation, agpufitjni.cu:5167-5179):
+// the mean <n(t)> over frames 0..T-2 and the lag-1 covariance
+// <n(t) n(t+1)> - <n(t)><n(t+1)>, i.e. channel 1's direct monitor and its
+// correlation numerator: nbcov = G(lag 1) * abar * bbar.
+__global__ void k_nb_extract(const CorrTables *__restrict__ ctp, SeriesGeom g, PairDesc pd, const double *F,
+                             const double *corav, double *nbmean, double *nbcov) {
+    const CorrTables &ct = *ctp;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= pd.npairs) return;
+    int sa, sb;
+    pair_series(pd, p, sa, sb);
+    const double *Fa = F + (int64_t)g.slot_of[sa] * ct.npos, *Fb = F + (int64_t)g.slot_of[sb] * ct.npos;
+    const double n = (double)(ct.numbin[0] - ct.del[1]);
+    const double am = Fa[ct.pos_a_end[1]] / n;
+    const double bm = (Fb[ct.pos_b_end[1]] - Fb[ct.pos_b_beg[1]]) / n;
+    nbmean[p] = am;
+    nbcov[p] = corav[(int64_t)p * ct.chanum + 1] * am * bm;
+}
+
 // Plain device copy (16-byte vectors, grid-stride, byte tail).
 __global__ void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16, const uint8_t *src_tail,
                        uint8_t *dst_tail, int ntail) {
@@ -1346,6 +1365,11 @@ __global__ void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, s
 // Launchers
 // ------------------------------------------------------------------
 static inline int cdiv(int64_t a, int b) { return (int)((a + b - 1) / b); }
+
+void launch_nb_extract(const CorrTables *dct, const SeriesGeom &g, const PairDesc &pd, const double *F,
+                       const double *corav, double *nbmean, double *nbcov, hipStream_t st) {
+    hipLaunchKernelGGL(k_nb_extract, dim3(cdiv(pd.npairs, 64)), dim3(64), 0, st, dct, g, pd, F, corav, nbmean, nbcov);
+}
 
 void launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st) {
     const size_t n16 = bytes / 16;

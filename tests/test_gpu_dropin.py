@@ -195,6 +195,38 @@ def test_calc_acf_through_jni_matches_oracle(poly_order):
     assert worst < 1e-4, worst
 
 
+def test_calc_acf_nb_through_jni_matches_g1_statistics():
+    """N&B (isNBcalculation): mean <n(t)> and lag-1 covariance
+    <n(t) n(t+1)> - <n(t)><n(t+1)> of every pixel over frames 1..T-1, the
+    statistics of the plugin's G1 N&B mode (IFCS:7872-7888), bleach
+    corrected as in calcACF."""
+    W, H, T = 6, 5, 8192
+    stack = imfcs.simulate_2d(W, H, T, 50, tau_bleach_s=15.0, seed=61)
+    bg = int(stack.min())
+    a = _params(W, H, T, 1, (0, 0), 4, bg=bg)
+    a.isNBcalculation = 1
+    pixels = plugin_glue.get_pixels(stack, a)
+    bc, _ = plugin_glue.bleach_correction(pixels, a)
+    lag, samp, _, _ = imfcs.correlator_tables(16, 8, T, 1e-3)
+    n = a.width * a.height * a.chanum
+    env = FakeEnv()
+    outs = [np.zeros(n) for _ in range(3)]
+    nbm, nbc = np.zeros(a.width * a.height), np.zeros(a.width * a.height)
+    env.call("calcACF", env.handle(pixels), env.handle(outs[0]), env.handle(outs[1]), env.handle(nbm),
+             env.handle(nbc), env.handle(outs[2]), env.handle(bc), env.handle(samp), env.handle(lag.astype(np.int32)),
+             _java_params(env, a))
+    assert not env.exceptions and not env.trapped
+    nc = a.bleachcorr_order
+    for y in range(a.height):
+        for x in range(a.width):
+            k = y * a.w_temp + x
+            d = _oracle_series(stack, a, x, y, bg, bc[k * nc:(k + 1) * nc])[1:]
+            m1, m2 = d[:-1].mean(), d[1:].mean()
+            cov = (d[:-1] * d[1:]).mean() - m1 * m2
+            assert abs(nbm[y * a.width + x] - m1) <= 1e-9 * abs(m1)
+            assert abs(nbc[y * a.width + x] - cov) <= 1e-4 * abs(cov)
+
+
 def test_plugin_glue_acf_all_matches_native_pipeline(engine):
     """GPU_Calculate_ACF_All through the drop-in layer and the native
     pipeline give the same CCF for the same stack (no bleach correction:
