@@ -180,8 +180,10 @@ def main():
     for b, c, e in zip(BINNINGS, cfgs, engines):
         e.run(c, d_stack)
         r = e.results(c)
-        it = np.asarray(r["iters"]).ravel()
-        stt = np.asarray(r["states"]).ravel()
+        # ACF fits only (the CCF pair is correlated, not fitted), pixels in the ROI
+        fitted = np.isfinite(np.asarray(r["fitres"])[0][..., 0])
+        it = np.asarray(r["iters"])[0][fitted]
+        stt = np.asarray(r["states"])[0][fitted]
         iters["b%d" % b] = {"median": float(np.median(it)), "p99": float(np.percentile(it, 99)),
                             "max": int(it.max()), "converged_frac": round(float((stt == 0).mean()), 4)}
 

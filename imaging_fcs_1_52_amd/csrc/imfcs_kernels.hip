@@ -621,42 +621,73 @@ struct Comp {
     float plat, dDplat, pspim, dDpspim, dvx, dvy, acf;
 };
 
+// exp and 1/x from the hardware exp2 / reciprocal (1 ulp each); the model
+// is evaluated ~100x per point and fit, so the full-precision library
+// sequences (~10 instructions each) are the bulk of an LM iteration.
+__device__ __forceinline__ float fexp(float v) { return __builtin_amdgcn_exp2f(v * 1.4426950408889634f); }
+__device__ __forceinline__ float frcp(float v) { return __builtin_amdgcn_rcpf(v); }
+
+// One lateral dimension of the pixel-integrated Gaussian (IFCS:13163-13190
+// for x, the same with a_y, r_y, v_y for y): l = p0/sqrt(pi) pexp + perf,
+// dDpexp (the exponential part of d l / dD, exponents combined) and
+// ev = erf(p2) + 2 erf(p3) - erf(p1) (the flow derivative factor).
+struct DimTerms {
+    float e1, e2, e3, pexp, l, dDpexp;
+};
+
+__device__ __forceinline__ void fcs_dim(float a, float r, float v, float x, float p0t, float ip0, float ip02,
+                                        bool grad, DimTerms &o) {
+    const float isqrpi = 0.56418958354775628f;
+    if (r == 0.f && v == 0.f) {
+        // centred (ACF without flow): p1 = p2 = a, p3 = 0, so erf(p1) = erf(p2),
+        // erf(p3) = 0 and exp(-p3^2 / p0^2) = exp((p7 - p4) / p0^2) = 1 exactly
+        float u = a * ip0;
+        float e = erff(u);
+        o.e1 = o.e2 = e;
+        o.e3 = 0.f;
+        o.pexp = 2.f * fexp(-u * u) - 2.f;
+        o.l = p0t * isqrpi * o.pexp + 2.f * (a * e);
+        if (grad) o.dDpexp = 2.f * (2.f * fexp(-(a * a) * ip02) - 2.f);
+        return;
+    }
+    float p1t = a + r - v * x, p2t = a - r + v * x, p3t = r - v * x;
+    o.e1 = erff(p1t * ip0);
+    o.e2 = erff(p2t * ip0);
+    o.e3 = erff(p3t * ip0);
+    o.pexp = fexp(-(p1t * ip0) * (p1t * ip0)) + fexp(-(p2t * ip0) * (p2t * ip0)) - 2.f * fexp(-(p3t * ip0) * (p3t * ip0));
+    float perf = p1t * o.e1 + p2t * o.e2 - 2.f * p3t * o.e3;
+    o.l = p0t * isqrpi * o.pexp + perf;
+    if (grad) {
+        // dDpexp = 2 exp(-p4/p0^2)(exp(p5/p0^2) + exp(p6/p0^2) - 2 exp(p7/p0^2)), exponents combined
+        float p4 = 2.f * a * a + 3.f * r * r - 6.f * x * r * v + 3.f * (x * v) * (x * v);
+        float p5 = p3t * p3t + p1t * p1t, p6 = p3t * p3t + p2t * p2t;
+        float p7 = 2.f * (a * a + r * r - 2.f * x * r * v + (x * v) * (x * v));
+        o.dDpexp = 2.f * (fexp((p5 - p4) * ip02) + fexp((p6 - p4) * ip02) - 2.f * fexp((p7 - p4) * ip02));
+    }
+}
+
 __device__ __forceinline__ void fcs_component(const LMModelGeom &m, float x, float D, float vx, float vy, bool grad,
                                               Comp &o) {
-    const float sqrpi = 1.7724538509055159f;
+    const float isqrpi = 0.56418958354775628f;
     float p0t = sqrtf(4.f * D * x + m.s * m.s);
-    float ip0 = 1.f / p0t, ip02 = ip0 * ip0;
-    float p1xt = m.ax + m.rx - vx * x, p2xt = m.ax - m.rx + vx * x, p3xt = m.rx - vx * x;
-    float p1yt = m.ay + m.ry - vy * x, p2yt = m.ay - m.ry + vy * x, p3yt = m.ry - vy * x;
-    float e1x = erff(p1xt * ip0), e2x = erff(p2xt * ip0), e3x = erff(p3xt * ip0);
-    float e1y = erff(p1yt * ip0), e2y = erff(p2yt * ip0), e3y = erff(p3yt * ip0);
-    float pexpx = expf(-(p1xt * ip0) * (p1xt * ip0)) + expf(-(p2xt * ip0) * (p2xt * ip0)) -
-                  2.f * expf(-(p3xt * ip0) * (p3xt * ip0));
-    float pexpy = expf(-(p1yt * ip0) * (p1yt * ip0)) + expf(-(p2yt * ip0) * (p2yt * ip0)) -
-                  2.f * expf(-(p3yt * ip0) * (p3yt * ip0));
-    float perfx = p1xt * e1x + p2xt * e2x - 2.f * p3xt * e3x;
-    float perfy = p1yt * e1y + p2yt * e2y - 2.f * p3yt * e3y;
-    float lx = p0t / sqrpi * pexpx + perfx, ly = p0t / sqrpi * pexpy + perfy;
-    float norm = 4.f * (m.ax * m.ay) * (m.ax * m.ay) / m.obsvol;
-    o.plat = lx * ly / norm;
-    float szi2 = 1.f / (m.sz * m.sz);
+    float ip0 = frcp(p0t), ip02 = ip0 * ip0;
+    DimTerms tx, ty;
+    fcs_dim(m.ax, m.rx, vx, x, p0t, ip0, ip02, grad, tx);
+    if (m.ay == m.ax && m.ry == m.rx && vy == vx)
+        ty = tx;   // square pixel, symmetric shift and flow: identical y factor
+    else
+        fcs_dim(m.ay, m.ry, vy, x, p0t, ip0, ip02, grad, ty);
+    float inorm = m.obsvol * frcp(4.f * (m.ax * m.ay) * (m.ax * m.ay));
+    o.plat = tx.l * ty.l * inorm;
+    float szi2 = frcp(m.sz * m.sz);
     float q = 1.f + 4.f * D * x * szi2;
     o.pspim = rsqrtf(q);
     o.acf = o.plat * o.pspim;
     if (grad) {
-        // dDpexp = 2 exp(-p4/p0^2)(exp(p5/p0^2) + exp(p6/p0^2) - 2 exp(p7/p0^2)), exponents combined
-        float p4x = 2.f * m.ax * m.ax + 3.f * m.rx * m.rx - 6.f * x * m.rx * vx + 3.f * (x * vx) * (x * vx);
-        float p5x = p3xt * p3xt + p1xt * p1xt, p6x = p3xt * p3xt + p2xt * p2xt;
-        float p7x = 2.f * (m.ax * m.ax + m.rx * m.rx - 2.f * x * m.rx * vx + (x * vx) * (x * vx));
-        float p4y = 2.f * m.ay * m.ay + 3.f * m.ry * m.ry - 6.f * x * m.ry * vy + 3.f * (x * vy) * (x * vy);
-        float p5y = p3yt * p3yt + p1yt * p1yt, p6y = p3yt * p3yt + p2yt * p2yt;
-        float p7y = 2.f * (m.ay * m.ay + m.ry * m.ry - 2.f * x * m.ry * vy + (x * vy) * (x * vy));
-        float dDpexpx = 2.f * (expf((p5x - p4x) * ip02) + expf((p6x - p4x) * ip02) - 2.f * expf((p7x - p4x) * ip02));
-        float dDpexpy = 2.f * (expf((p5y - p4y) * ip02) + expf((p6y - p4y) * ip02) - 2.f * expf((p7y - p4y) * ip02));
-        o.dDplat = (ip0 / sqrpi) * (dDpexpy * x * lx + dDpexpx * x * ly) / norm;
-        o.dDpspim = -4.f * x / (2.f * m.sz * m.sz * q * sqrtf(q));
-        o.dvx = ly * ((e2x + 2.f * e3x - e1x) * x);
-        o.dvy = lx * ((e2y + 2.f * e3y - e1y) * x);
+        o.dDplat = (ip0 * isqrpi) * (ty.dDpexp * x * tx.l + tx.dDpexp * x * ty.l) * inorm;
+        o.dDpspim = -4.f * x * frcp(2.f * m.sz * m.sz * q * sqrtf(q));
+        o.dvx = ty.l * ((tx.e2 + 2.f * tx.e3 - tx.e1) * x);
+        o.dvy = tx.l * ((ty.e2 + 2.f * ty.e3 - ty.e1) * x);
     }
 }
 
@@ -701,10 +732,30 @@ __device__ float fcs3p(const LMModelGeom &m, float x, const float *pq, bool grad
     return pacf;
 }
 
+// One DPP move of a double (both halves); lanes the row mask excludes, or
+// whose source lane is out of the row, read 0.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a wave (all lanes active), returned to every
+// lane.  DPP steps: pairs and quads (quad_perm), 8 and 16 lanes (half-row
+// and row mirror), then rows 0+1 and 2+3 (row_bcast:15 into rows 1 and 3)
+// and the two halves (row_bcast:31 into rows 2 and 3); lane 63 holds the
+// total, which is read back as a wave-uniform value.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    v += dpp_f64<0xB1, 0xf>(v);    // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E, 0xf>(v);    // quad_perm [2,3,0,1]
+    v += dpp_f64<0x141, 0xf>(v);   // row_half_mirror
+    v += dpp_f64<0x140, 0xf>(v);   // row_mirror
+    v += dpp_f64<0x142, 0xa>(v);   // row_bcast:15
+    v += dpp_f64<0x143, 0xc>(v);   // row_bcast:31
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double(hi, lo);
 }
 
 // FP32 chi-square with the reduction order of Gpufit: points are grouped in
@@ -1024,7 +1075,7 @@ __device__ void lm_eval_fixed(const LMArgs &a, int fit, const float *par, bool n
             }
         }
       }
-      c2 += (double)piece_tree(e);
+      if (!a.chi2_fp64) c2 += (double)piece_tree(e);
     }
     chi2 = a.chi2_fp64 ? wave_sum(cx) : (double)(float)c2;
     if (need_jac) {
@@ -1203,6 +1254,77 @@ __global__ void __launch_bounds__(64) k_lm_fixed(LMArgs a) {
     lm_fixed_iterate<KIND, NF>(a, par, eval, chi2, state, iters);
     if (KIND == kModelFCS) fcs_params_out(par);
     if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NP; k++) a.out_params[(int64_t)fit * NP + k] = par[k];
+        a.states[fit] = state;
+        a.chi2[fit] = (float)chi2;
+        a.iters[fit] = iters;
+    }
+}
+
+// The same LM fit of the FCS model with NF fixed free slots for curves of
+// 65..128 points and an FP64 chi-square: two wavefronts per fit, one point
+// per lane, so one model evaluation per iteration instead of two passes of
+// one wave (an iteration of a lone long-running fit is latency-bound).  The
+// per-wave sums meet in LDS (double-buffered, one barrier per evaluation);
+// both waves then take identical LM steps from identical sums.
+template <int NF>
+__global__ void __launch_bounds__(128) k_lm_fcs_2w(LMArgs a) {
+    constexpr int NP = 20, NH = NF * (NF + 1) / 2, NS = 1 + NF + NH;
+    __shared__ double red[2][2][NS];   // [buffer][wave][sum]
+    const int fit = blockIdx.x;
+    if (fit >= a.nfits) return;
+    const int tid = threadIdx.x, wv = tid >> 6;
+    float par[NP];
+#pragma unroll
+    for (int k = 0; k < NP; k++) par[k] = a.init[(int64_t)fit * NP + k];
+    LMArgs ac = a;
+    fcs_params_in(par, ac.geom);
+    const bool c2on = (par[5] != 0.f) || a.pfit[6];
+    const bool c3on = (par[7] != 0.f) || a.pfit[8];
+    const bool act = tid < a.npts;
+    const float xi = act ? a.x[tid] : 0.f;
+    const float yi = act ? a.data[(int64_t)fit * a.npts + tid] : 0.f;
+    const double wi = !act ? 0.0 : (a.weights ? (double)a.weights[(int64_t)fit * a.npts + tid] : 1.0);
+    int buf = 0;
+    auto eval = [&](const float *p, double &c2, double *g, double *h) {
+        double sv[NS];
+#pragma unroll
+        for (int k = 0; k < NS; k++) sv[k] = 0.0;
+        if (act) {
+            float gfull[11];
+            const float f = fcs3p(ac.geom, xi, p, true, c2on, c3on, gfull);
+            const double r = (double)yi - (double)f;
+            sv[0] = wi * r * r;
+            int hh = 1 + NF;
+#pragma unroll
+            for (int q = 0; q < NF; q++) {
+                const double jq = (double)gfull[FixedSlots<kModelFCS, NF>::slot(q)];
+                sv[1 + q] = wi * r * jq;
+#pragma unroll
+                for (int l = 0; l <= q; l++, hh++)
+                    sv[hh] = wi * jq * (double)gfull[FixedSlots<kModelFCS, NF>::slot(l)];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NS; k++) sv[k] = wave_sum(sv[k]);
+        if ((tid & 63) == 0) {
+#pragma unroll
+            for (int k = 0; k < NS; k++) red[buf][wv][k] = sv[k];
+        }
+        __syncthreads();
+        c2 = red[buf][0][0] + red[buf][1][0];
+#pragma unroll
+        for (int q = 0; q < NF; q++) g[q] = red[buf][0][1 + q] + red[buf][1][1 + q];
+#pragma unroll
+        for (int q = 0; q < NH; q++) h[q] = red[buf][0][1 + NF + q] + red[buf][1][1 + NF + q];
+        buf ^= 1;
+    };
+    double chi2;
+    int state, iters;
+    lm_fixed_iterate<kModelFCS, NF>(a, par, eval, chi2, state, iters);
+    fcs_params_out(par);
+    if (tid == 0) {
 #pragma unroll
         for (int k = 0; k < NP; k++) a.out_params[(int64_t)fit * NP + k] = par[k];
         a.states[fit] = state;
@@ -1477,7 +1599,10 @@ void launch_correlate(const SeriesGeom &g, const BleachDesc &bd, const CorrTable
 static void launch_lm(const LMArgs &a, unsigned free_mask, hipStream_t st) {
     if (a.nfits <= 0) return;
     if (a.kind == kModelFCS && free_mask == 0x13u) {   // N, D, G_inf
-        hipLaunchKernelGGL((k_lm_fixed<kModelFCS, 3>), dim3(a.nfits), dim3(64), 0, st, a);
+        if (a.chi2_fp64 && a.npts > 64 && a.npts <= 128)
+            hipLaunchKernelGGL((k_lm_fcs_2w<3>), dim3(a.nfits), dim3(128), 0, st, a);
+        else
+            hipLaunchKernelGGL((k_lm_fixed<kModelFCS, 3>), dim3(a.nfits), dim3(64), 0, st, a);
         return;
     }
     if (a.kind == kModelPoly) {
