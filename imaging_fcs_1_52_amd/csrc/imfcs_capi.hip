@@ -57,6 +57,26 @@ struct DevBuf {
     T *as() const { return (T *)p; }
 };
 
+// Host-to-device copy of a small per-run parameter block, skipped when the
+// device buffer still holds the same bytes from an earlier run (repeated
+// runs of one configuration then enqueue no copies).  The shadow is the
+// copy's source and is only rewritten after the previous run's copies have
+// completed (the run start waits for them).
+struct CachedUpload {
+    std::vector<unsigned char> host;
+    void *dev = nullptr;
+    void invalidate() { dev = nullptr; }
+    hipError_t put(DevBuf &buf, const void *src, size_t n, hipStream_t st) {
+        const bool grows = n > buf.n;
+        hipError_t e = buf.ensure(n);
+        if (e != hipSuccess) return e;
+        if (!grows && dev == buf.p && host.size() == n && std::memcmp(host.data(), src, n) == 0) return hipSuccess;
+        host.assign((const unsigned char *)src, (const unsigned char *)src + n);
+        dev = buf.p;
+        return hipMemcpyAsync(buf.p, host.data(), n, hipMemcpyHostToDevice, st);
+    }
+};
+
 // Correlator tables exactly as setParameters builds them (IFCS:7391-7424).
 struct HostTables {
     int chanum = 0;
@@ -175,14 +195,15 @@ struct imfcs_ctx {
     hipEvent_t ev_h2d = nullptr;   // after the last host-to-device copy of a run
     hipEvent_t evk[2] = {};        // around the main correlator kernel
     int evk_valid = 0;
-    // host sources of a run's asynchronous copies; reused only after ev_h2d
+    // host staging of a run's fit parameters and the cached uploads of the
+    // per-run parameter blocks
     std::vector<float> h_xs, h_init;
     std::vector<int> h_pfit;
+    CachedUpload up_ctab, up_slots, up_fx, up_finit, up_fpfit;
     DevBuf part, coef, ctot, rec, F, bpart, ind, bis, corav, bvar, fdata, fw, fx, finit, fpfit, fout, fstate, fchi2,
         fiter, cpart, hpart, l3, lvd, ctab, csum, slots, X, glsblk, glsW, glsy, glsok, glscov, dropin_in, nbm, nbc;
     SeriesGeom last_g{};   // geometry (with device slot tables) of the last engine run
-    CorrTables host_ct;
-    std::vector<int32_t> host_slots;   // slot_of then slot origins (source of the async copy)
+    std::vector<int32_t> host_slots;   // slot_of then slot origins
     std::vector<double> host_coef;     // drop-in bleach coefficients per slot (source of the async copy)   // source of the async copy into ctab
     // last run
     int npix = 0, npairs = 0, nseries = 0, nslots = 0, chanum = 0, order = 0, ow = 0, oh = 0, did_fit = 0, nfree = 0;
@@ -385,14 +406,10 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTabl
     if (rc) return rc;
     // the host sources of the previous run's copies are reused below
     HIP_TRY(hipEventSynchronize(ctx->ev_h2d));
-    ctx->host_ct = ct;
-    HIP_TRY(ctx->ctab.ensure(sizeof(CorrTables)));
-    HIP_TRY(hipMemcpyAsync(ctx->ctab.p, &ctx->host_ct, sizeof(CorrTables), hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx->up_ctab.put(ctx->ctab, &ct, sizeof(CorrTables), st));
     const CorrTables *dct = ctx->ctab.as<CorrTables>();
     g.nslots = build_slots(g, ctx->host_slots);
-    HIP_TRY(ctx->slots.ensure(sizeof(int32_t) * ctx->host_slots.size()));
-    HIP_TRY(hipMemcpyAsync(ctx->slots.p, ctx->host_slots.data(), sizeof(int32_t) * ctx->host_slots.size(),
-                           hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx->up_slots.put(ctx->slots, ctx->host_slots.data(), sizeof(int32_t) * ctx->host_slots.size(), st));
     g.slot_of = ctx->slots.as<int32_t>();
     g.slot_org = ctx->slots.as<int32_t>() + g.nseries;
     ctx->nslots = g.nslots;
@@ -587,9 +604,6 @@ extern "C" int imfcs_run(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_co
         const int np = ea.pd.npairs, npts = ct.chanum - 1;
         HIP_TRY(ctx->fdata.ensure(sizeof(float) * (size_t)np * npts));
         HIP_TRY(ctx->fw.ensure(sizeof(float) * (size_t)np * npts));
-        HIP_TRY(ctx->fx.ensure(sizeof(float) * npts));
-        HIP_TRY(ctx->finit.ensure(sizeof(float) * (size_t)np * 20));
-        HIP_TRY(ctx->fpfit.ensure(sizeof(int) * 20));
         HIP_TRY(ctx->fout.ensure(sizeof(float) * (size_t)np * 20));
         HIP_TRY(ctx->fstate.ensure(sizeof(int) * np));
         HIP_TRY(ctx->fchi2.ensure(sizeof(float) * np));
@@ -598,20 +612,23 @@ extern "C" int imfcs_run(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_co
             launch_pack_fit_inputs(ct, np, ctx->corav.as<double>(), ctx->bvar.as<double>(), ctx->fdata.as<float>(),
                                ctx->fw.as<float>(), st);
         // lag times of channels 1..chanum-1 (IFCS:15122-15125) and initial
-        // parameters with the fixed geometry in slots 11..19 (IFCS:15090-15110)
+        // parameters with the fixed geometry in slots 11..19 (IFCS:15090-15110):
+        // every ACF fit starts from the ACF model's row, every CCF fit from
+        // the CCF model's, so two rows travel and the kernel picks one
         std::vector<float> &xs = ctx->h_xs, &init = ctx->h_init;
         std::vector<int> &pfit = ctx->h_pfit;
         xs.assign(npts, 0.f);
-        init.assign((size_t)np * 20, 0.f);
+        init.assign(2 * 20, 0.f);
         pfit.assign(20, 0);
         for (int i = 0; i < npts; i++) xs[i] = (float)ht.lagtime[i + 1];
         const imfcs_fit_model *models[2] = {&c->acf_model, &c->ccf_model};
         for (int k = 0; k < 11; k++) pfit[k] = c->acf_model.paramfit[k] ? 1 : 0;
         int nfree = 0;
         for (int k = 0; k < 11; k++) nfree += pfit[k];
-        for (int p = 0; p < np; p++) {
-            const imfcs_fit_model *m = models[(c->do_acf && p < g.npix) ? 0 : 1];
-            float *q = &init[(size_t)p * 20];
+        const int init_split = c->do_acf ? g.npix : 0;
+        for (int r = 0; r < 2; r++) {
+            const imfcs_fit_model *m = models[r];
+            float *q = &init[(size_t)r * 20];
             for (int k = 0; k < 11; k++) q[k] = (float)m->paraminit[k];
             q[11] = (float)m->ax;
             q[12] = (float)m->ay;
@@ -623,9 +640,9 @@ extern "C" int imfcs_run(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_co
             q[18] = (float)m->q2;
             q[19] = (float)m->q3;
         }
-        HIP_TRY(hipMemcpyAsync(ctx->fx.p, xs.data(), sizeof(float) * npts, hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(ctx->finit.p, init.data(), sizeof(float) * init.size(), hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(ctx->fpfit.p, pfit.data(), sizeof(int) * 20, hipMemcpyHostToDevice, st));
+        HIP_TRY(ctx->up_fx.put(ctx->fx, xs.data(), sizeof(float) * npts, st));
+        HIP_TRY(ctx->up_finit.put(ctx->finit, init.data(), sizeof(float) * init.size(), st));
+        HIP_TRY(ctx->up_fpfit.put(ctx->fpfit, pfit.data(), sizeof(int) * 20, st));
         HIP_TRY(hipEventRecord(ctx->ev_h2d, st));
         unsigned mask = 0;
         for (int k = 0; k < 11; k++) mask |= (pfit[k] ? 1u : 0u) << k;
@@ -635,14 +652,14 @@ extern "C" int imfcs_run(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_co
             launch_lm_gls(np, npts, ctx->fx.as<float>(), ctx->finit.as<float>(), ctx->fpfit.as<int>(),
                           c->fit_tolerance, c->fit_max_iterations, ctx->glsW.as<double>(), ctx->glsy.as<double>(),
                           ctx->glsok.as<int32_t>(), ctx->fout.as<float>(), ctx->fstate.as<int>(),
-                          ctx->fchi2.as<float>(), ctx->fiter.as<int>(), st);
+                          ctx->fchi2.as<float>(), ctx->fiter.as<int>(), st, init_split);
             // chi2 of GLSFit: sum of whitened residuals^2 / (chanum - nfree - 1) (IFCS:10733-10735)
             ctx->chi2_norm = (double)(ct.chanum - nfree - 1);
         } else {
             launch_lm_acf(np, npts, ctx->fdata.as<float>(), ctx->fw.as<float>(), ctx->fx.as<float>(),
                           ctx->finit.as<float>(), ctx->fpfit.as<int>(), mask, LMModelGeom{}, c->fit_tolerance,
                           c->fit_max_iterations, c->fit_chi2_fp64 ? 1 : 0, ctx->fout.as<float>(),
-                          ctx->fstate.as<int>(), ctx->fchi2.as<float>(), ctx->fiter.as<int>(), st);
+                          ctx->fstate.as<int>(), ctx->fchi2.as<float>(), ctx->fiter.as<int>(), st, init_split);
             // chi2 normalization of GPU_ACF_Fit (IFCS:15234)
             ctx->chi2_norm = (double)((npts - 1) - nfree - 1);
         }
@@ -937,6 +954,9 @@ extern "C" int imfcs_fit(int n_fits, int n_points, const float *data, const floa
     HIP_TRY(ctx->fx.ensure((size_t)n_points * 4));
     HIP_TRY(ctx->finit.ensure((size_t)n_fits * npar * 4));
     HIP_TRY(ctx->fpfit.ensure(npar * sizeof(int)));
+    ctx->up_fx.invalidate();
+    ctx->up_finit.invalidate();
+    ctx->up_fpfit.invalidate();
     HIP_TRY(ctx->fout.ensure((size_t)n_fits * npar * 4));
     HIP_TRY(ctx->fstate.ensure((size_t)n_fits * 4));
     HIP_TRY(ctx->fchi2.ensure((size_t)n_fits * 4));

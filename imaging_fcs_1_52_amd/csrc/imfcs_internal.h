@@ -116,9 +116,11 @@ struct GlsArgs {
     double *cov;          // [pair][m][m] regularized covariance (optional)
 };
 void launch_gls_cov(const GlsArgs &A, int npairs, hipStream_t st);
+// init: one parameter row per fit, or, with init_split >= 0, two rows (fits
+// below init_split start from row 0, the others from row 1).
 void launch_lm_gls(int nfits, int m, const float *x, const float *init, const int *pfit, float tol, int maxit,
                    const double *W, const double *yw, const int32_t *ok, float *out_params, int *states, float *chi2,
-                   int *iters, hipStream_t st);
+                   int *iters, hipStream_t st, int init_split = -1);
 
 // Launchers (imfcs_kernels.hip)
 void launch_trace_moments(const SeriesGeom &g, int nopit, int ave, double tscale, double frametime,
@@ -155,7 +157,8 @@ void launch_correlate_fast(const SeriesGeom &g, const BleachDesc &bd, const Corr
                            float *gls_blk = nullptr, int gls_nblk = 0);
 void launch_lm_acf(int nfits, int npts, const float *data, const float *weights, const float *x,
                    const float *init, const int *pfit, unsigned free_mask, const LMModelGeom &geom, float tol, int maxit,
-                   int chi2_fp64, float *out_params, int *states, float *chi2, int *iters, hipStream_t st);
+                   int chi2_fp64, float *out_params, int *states, float *chi2, int *iters, hipStream_t st,
+                   int init_split = -1);
 void launch_lm_poly(int nfits, int npts, const float *data, const float *weights, const float *x,
                     const float *init, const int *pfit, unsigned free_mask, int ncoef, float tol, int maxit,
                     int chi2_fp64, float *out_params, int *states, float *chi2, int *iters, hipStream_t st);

@@ -819,7 +819,14 @@ struct LMArgs {
     int *states;
     float *chi2;
     int *iters;
+    int init_two_rows;   // 0: init holds a row per fit; 1: two rows, split at init_split
+    int init_split;
 };
+
+__device__ __forceinline__ const float *init_row(const LMArgs &a, int fit) {
+    if (a.init_two_rows) return a.init + (fit < a.init_split ? 0 : a.nparams);
+    return a.init + (int64_t)fit * a.nparams;
+}
 
 // Evaluates chi^2 and (optionally) gradient / hessian sums for the fit, with
 // every lane handling points lane, lane+64, ...  Results are wave totals.
@@ -898,7 +905,7 @@ __global__ void __launch_bounds__(64) k_lm(LMArgs a) {
     float par[20], prev[20];
     int fidx[kLMMaxFree], nfree = 0;
     for (int k = 0; k < a.nparams; k++) {
-        par[k] = a.init[(int64_t)fit * a.nparams + k];
+        par[k] = init_row(a, fit)[k];
         if (a.pfit[k] && nfree < kLMMaxFree && (a.kind == kModelPoly ? k < a.ncoef : k < 11)) fidx[nfree++] = k;
     }
     for (int k = a.nparams; k < 20; k++) par[k] = 0.f;
@@ -1243,7 +1250,7 @@ __global__ void __launch_bounds__(64) k_lm_fixed(LMArgs a) {
     const int lane = threadIdx.x & 63;
     float par[NP];
 #pragma unroll
-    for (int k = 0; k < NP; k++) par[k] = a.init[(int64_t)fit * NP + k];
+    for (int k = 0; k < NP; k++) par[k] = init_row(a, fit)[k];
     LMArgs ac = a;
     if (KIND == kModelFCS) fcs_params_in(par, ac.geom);
     auto eval = [&](const float *p, double &c2, double *g, double *h) {
@@ -1277,7 +1284,7 @@ __global__ void __launch_bounds__(128) k_lm_fcs_2w(LMArgs a) {
     const int tid = threadIdx.x, wv = tid >> 6;
     float par[NP];
 #pragma unroll
-    for (int k = 0; k < NP; k++) par[k] = a.init[(int64_t)fit * NP + k];
+    for (int k = 0; k < NP; k++) par[k] = init_row(a, fit)[k];
     LMArgs ac = a;
     fcs_params_in(par, ac.geom);
     const bool c2on = (par[5] != 0.f) || a.pfit[6];
@@ -1350,7 +1357,7 @@ __global__ void __launch_bounds__(64) k_lm_gls(LMArgs a, const double *W, const 
     const int lane = threadIdx.x & 63;
     float par[NP];
 #pragma unroll
-    for (int k = 0; k < NP; k++) par[k] = a.init[(int64_t)fit * NP + k];
+    for (int k = 0; k < NP; k++) par[k] = init_row(a, fit)[k];
     if (!okv[fit]) {
         // the Cholesky factorization failed: Java's fit throws (no result)
         if (lane == 0) {
@@ -1625,9 +1632,9 @@ static void launch_lm(const LMArgs &a, unsigned free_mask, hipStream_t st) {
 
 void launch_lm_gls(int nfits, int m, const float *x, const float *init, const int *pfit, float tol, int maxit,
                    const double *W, const double *yw, const int32_t *ok, float *out_params, int *states, float *chi2,
-                   int *iters, hipStream_t st) {
+                   int *iters, hipStream_t st, int init_split) {
     LMArgs a{nfits, m, 20, kModelFCS, 0, nullptr, nullptr, x, init, pfit, LMModelGeom{}, tol, maxit, 1, out_params,
-             states, chi2, iters};
+             states, chi2, iters, init_split >= 0 ? 1 : 0, init_split};
     const size_t lds = sizeof(double) * ((size_t)m * m + (size_t)m * 4);
     if (lds > 65536) (void)hipFuncSetAttribute((const void *)k_lm_gls, hipFuncAttributeMaxDynamicSharedMemorySize,
                                                (int)lds);
@@ -1636,9 +1643,10 @@ void launch_lm_gls(int nfits, int m, const float *x, const float *init, const in
 
 void launch_lm_acf(int nfits, int npts, const float *data, const float *weights, const float *x, const float *init,
                    const int *pfit, unsigned free_mask, const LMModelGeom &geom, float tol, int maxit,
-                   int chi2_fp64, float *out_params, int *states, float *chi2, int *iters, hipStream_t st) {
+                   int chi2_fp64, float *out_params, int *states, float *chi2, int *iters, hipStream_t st,
+                   int init_split) {
     LMArgs a{nfits, npts, 20, kModelFCS, 0, data, weights, x, init, pfit, geom, tol, maxit, chi2_fp64, out_params,
-             states, chi2, iters};
+             states, chi2, iters, init_split >= 0 ? 1 : 0, init_split};
     launch_lm(a, free_mask, st);
 }
 
@@ -1647,7 +1655,7 @@ void launch_lm_poly(int nfits, int npts, const float *data, const float *weights
                     float *out_params, int *states, float *chi2, int *iters, hipStream_t st) {
     LMModelGeom g{};
     LMArgs a{nfits, npts, 11, kModelPoly, ncoef, data, weights, x, init, pfit, g, tol, maxit, chi2_fp64, out_params,
-             states, chi2, iters};
+             states, chi2, iters, 0, 0};
     launch_lm(a, free_mask, st);
 }
 
