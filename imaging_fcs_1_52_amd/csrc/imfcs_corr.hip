@@ -277,14 +277,18 @@ __global__ void __launch_bounds__(640) k_corr_main(MainArgs A) {
 // ------------------------------------------------------------------
 // Levels 3..q-1 from the level-3 series.
 // ------------------------------------------------------------------
+// Values of one level >= 3: level 3 from the FP32 8-frame sums, higher
+// levels from the FP64 rebinned series.  Branch-free (clamped index, zero
+// past the end) so that a group's loads issue back to back.
+template <bool L3>
 struct L3Ctx {
     const float *l3;    // level 3 (FP32 sums of 8 frames)
     const double *lv;   // this level's values when c >= 4 (FP64)
     int nseries, sa, sb, c, numbin;
     __device__ __forceinline__ double val(int s, int y) const {
-        if (y >= numbin) return 0.0;
-        const int64_t i = (int64_t)y * nseries + s;
-        return c == 3 ? (double)l3[i] : lv[i];
+        const int64_t i = (int64_t)min(y, numbin - 1) * nseries + s;
+        const double v = L3 ? (double)l3[i] : lv[i];
+        return (y < numbin) ? v : 0.0;
     }
 };
 
@@ -303,8 +307,8 @@ __global__ void k_corr_rebin(const float *l3, double *lvd, int64_t off_src, int6
 // Values [ybeg, yend) of one level; yend - ybeg and ybeg are multiples of G
 // and of the block length B, so every block of the blocked variance lies in
 // one call.
-template <int G>
-__device__ void high_level(const L3Ctx &L, bool same, const double *alpha, const double *beta, int B, int pnum,
+template <int G, bool L3>
+__device__ void high_level(const L3Ctx<L3> &L, const double *alpha, const double *beta, int B, int pnum,
                            int ybeg, int yend, double *cross, double *a1, double *a2, float *gls_row, int gls_ld) {
     constexpr int WN = G + 16;
     double w[WN], ga[G], sab[8], sb[8];
@@ -318,7 +322,7 @@ __device__ void high_level(const L3Ctx &L, bool same, const double *alpha, const
     for (int k = 0; k < WN; k++) w[k] = L.val(L.sb, ybeg + k);
     for (int y0 = ybeg; y0 < yend; y0 += G) {
 #pragma unroll
-        for (int k = 0; k < G; k++) ga[k] = same ? w[k] : L.val(L.sa, y0 + k);
+        for (int k = 0; k < G; k++) ga[k] = L.val(L.sa, y0 + k);   // (= w[k] for an ACF; loaded unconditionally so the loads batch)
         double gs = 0.0;
 #pragma unroll
         for (int k = 0; k < G; k++) gs += ga[k];
@@ -369,7 +373,6 @@ __global__ void __launch_bounds__(64) k_corr_high(MainArgs A, double *hpart /* [
     if (pidx >= A.pd.npairs || c >= ct.nlevels) return;
     int sa, sb;
     pair_series(A.pd, pidx, sa, sb);
-    L3Ctx L{A.l3, A.lvd + A.loff[c], A.g.nseries, sa, sb, c, ct.numbin[c]};
     const double *Fa = A.F + (int64_t)A.g.slot_of[sa] * ct.npos, *Fb = A.F + (int64_t)A.g.slot_of[sb] * ct.npos;
     const double mua = level_shift(A, sa, c);
     const double mub = level_shift(A, sb, c);
@@ -389,13 +392,26 @@ __global__ void __launch_bounds__(64) k_corr_high(MainArgs A, double *hpart /* [
     const int ybeg = chunk * len;
     const int yend = (chunk == A.nchunk - 1) ? ct.numbin[c] : ybeg + len;
     float *grow = A.gls ? A.gls + (int64_t)pidx * A.gls_nblk * (ct.chanum - 1) + (x0 - 1) : nullptr;
-    int G = min(16, 1 << max(ct.q - 1 - c, 0));
-    switch (G) {
-        case 16: high_level<16>(L, sa == sb, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ct.chanum - 1); break;
-        case 8: high_level<8>(L, sa == sb, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ct.chanum - 1); break;
-        case 4: high_level<4>(L, sa == sb, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ct.chanum - 1); break;
-        case 2: high_level<2>(L, sa == sb, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ct.chanum - 1); break;
-        default: high_level<1>(L, sa == sb, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ct.chanum - 1); break;
+    const int G = min(16, 1 << max(ct.q - 1 - c, 0));
+    const int ld = ct.chanum - 1;
+    if (c == 3) {
+        const L3Ctx<true> L{A.l3, nullptr, A.g.nseries, sa, sb, c, ct.numbin[c]};
+        switch (G) {
+            case 16: high_level<16, true>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
+            case 8: high_level<8, true>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
+            case 4: high_level<4, true>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
+            case 2: high_level<2, true>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
+            default: high_level<1, true>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
+        }
+    } else {
+        const L3Ctx<false> L{nullptr, A.lvd + A.loff[c], A.g.nseries, sa, sb, c, ct.numbin[c]};
+        switch (G) {
+            case 16: high_level<16, false>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
+            case 8: high_level<8, false>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
+            case 4: high_level<4, false>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
+            case 2: high_level<2, false>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
+            default: high_level<1, false>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
+        }
     }
     double *out = hpart + (((int64_t)chunk * A.pd.npairs + pidx) * nhigh + (x0 - kMainChan)) * 3;
     for (int i = 0; i < 8; i++) {
