@@ -368,18 +368,24 @@ __global__ void k_blocking_finalize(const CorrTables *__restrict__ ctp, SeriesGe
     double nb = (double)n0;
     double S1[kMaxBlock], S2[kMaxBlock], pend[kMaxBlock];
 #pragma unroll
-    for (int k = 0; k < kMaxBlock; k++) {
-        S1[k] = S2[k] = pend[k] = 0.0;
-        if (k < K && k < kBlkJ) {
-            for (int c = 0; c < nsplit; c++) {
-                int64_t base = ((int64_t)c * pd.npairs + p) * kBlkJ * 2;
-                S1[k] += bpart[base + 2 * k];
-                S2[k] += bpart[base + 2 * k + 1];
-            }
+    for (int k = 0; k < kMaxBlock; k++) S1[k] = S2[k] = pend[k] = 0.0;
+    // split partials (split-major, so a split's 2 kBlkJ values are contiguous);
+    // unrolled so that several splits' loads are in flight together
+#pragma unroll 4
+    for (int c = 0; c < nsplit; c++) {
+        const double *bp = bpart + ((int64_t)c * pd.npairs + p) * kBlkJ * 2;
+#pragma unroll
+        for (int k = 0; k < kBlkJ; k++) {
+            S1[k] += bp[2 * k];
+            S2[k] += bp[2 * k + 1];
         }
     }
+#pragma unroll
+    for (int k = 0; k < kBlkJ; k++)
+        if (k >= K) S1[k] = S2[k] = 0.0;
     if (K > kBlkJ) {
         const int nfull = n0 >> kBlkJ;
+#pragma unroll 4
         for (int t = 0; t < nfull; t++) {
             double carry = csum[(int64_t)t * pd.npairs + p];
             bool open = true;
