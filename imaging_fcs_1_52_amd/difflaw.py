@@ -1,0 +1,99 @@
+"""FCS diffusion law over pixel binnings on the native pipeline.
+
+The plugin's diffusion law (dlfit, IFCS:11007-11231, "All" mode with
+non-overlapping binning, and the conversion of determineDiffusionLaw,
+IFCS:8190-8215) runs the whole correlate-and-fit pipeline once per binning
+u = 1..numbin and reduces each binning to one point:
+
+    Aeff(u) = obsvolFCS_ST2D1p(2) at pixel size u * a     [um^2]
+    D(u)    = mean of the fitted D over converged pixels  [um^2/s]
+    tau(u)  = Aeff / D,   err(u) = Aeff / D^2 * sd(D)
+
+fitDiffLaw (IFCS:8253-8296) then fits the line tau = t0 + Aeff / D_eff over
+a range of points with weights 1 / err^2 (LineFit, IFCS:10792-10843, whose
+least-squares optimum is the closed-form weighted line below).
+
+Here every binning is one native pipeline run on the device-resident
+stack, one engine (HIP stream) per binning, so the binnings run
+concurrently.
+"""
+import numpy as np
+
+from . import Engine, make_fit_model, make_pipeline_config, obsvol
+
+
+def diffusion_law(stack, numbin, pixel_size_m, psf_m, background=None, poly_order=4, frametime=1e-3, sz=None,
+                  engines=None, d_stack=None):
+    """Diffusion-law points of binnings 1..numbin.
+
+    stack: uint16 [T][H][W] (host array; uploaded once unless d_stack, a
+    device pointer already holding it, is given).  Returns a dict with
+    'binning', 'aeff' (um^2), 'd_mean' (um^2/s), 'd_var' ((um^2/s)^2),
+    'tau' (s), 'err' (s), 'n_fitted' and, per binning, the fitted D map
+    ('d_maps', um^2/s, NaN where not converged)."""
+    stack = np.asarray(stack)
+    T, H, W = stack.shape
+    if numbin < 1 or numbin > min(W, H):
+        raise ValueError("numbin must be between 1 and the image side")
+    bg = int(stack.min()) if background is None else int(background)
+    bins = list(range(1, numbin + 1))
+    own = engines is None
+    if own:
+        engines = [Engine(0) for _ in bins]
+    if len(engines) < len(bins):
+        raise ValueError("one engine per binning is needed")
+    try:
+        dev = d_stack if d_stack is not None else engines[0].upload(stack)
+        cfgs = []
+        for u, eng in zip(bins, engines):
+            ax = u * pixel_size_m
+            model = make_fit_model(ax, ax, psf_m, sz=sz)
+            cfg = make_pipeline_config(W, H, T, binning=u, cf=(0, 0), background=bg, poly_order=poly_order,
+                                       frametime=frametime, do_fit=True, acf_model=model)
+            eng.run(cfg, dev)
+            cfgs.append(cfg)
+        out = {k: [] for k in ("binning", "aeff", "d_mean", "d_var", "tau", "err", "n_fitted", "d_maps")}
+        for u, eng, cfg in zip(bins, engines, cfgs):
+            eng.synchronize()
+            r = eng.results(cfg)
+            ok = (np.asarray(r["states"])[0] == 0) & np.isfinite(np.asarray(r["fitres"])[0][..., 1])
+            d = np.asarray(r["fitres"])[0][..., 1] * 1e12
+            dv = d[ok]
+            n = int(ok.sum())
+            aeff = obsvol(u * pixel_size_m, u * pixel_size_m, psf_m) * 1e12
+            dm = float(dv.mean()) if n else float("nan")
+            var = float((dv * dv).mean() - dm * dm) if n else float("nan")
+            out["binning"].append(u)
+            out["aeff"].append(aeff)
+            out["d_mean"].append(dm)
+            out["d_var"].append(var)
+            out["tau"].append(aeff / dm)
+            out["err"].append(aeff / dm ** 2 * np.sqrt(max(var, 0.0)))
+            out["n_fitted"].append(n)
+            out["d_maps"].append(np.where(ok, d, np.nan))
+        for k in ("binning", "aeff", "d_mean", "d_var", "tau", "err", "n_fitted"):
+            out[k] = np.asarray(out[k])
+        return out
+    finally:
+        if own:
+            for e in engines:
+                e.close()
+
+
+def fit_diffusion_law(aeff, tau, err, first=1, last=5):
+    """fitDiffLaw (IFCS:8253-8296): weighted line tau = intercept + slope *
+    Aeff over the points first..last (1-based, inclusive) with weights
+    1 / err^2.  Returns (intercept [s], slope [s/um^2]); 1 / slope is the
+    effective D."""
+    lo, hi = first - 1, last - 1
+    if lo >= hi or lo < 0 or hi >= len(aeff):
+        raise ValueError("fit start/end not correctly set")
+    x = np.asarray(aeff[lo:hi + 1], np.float64)
+    y = np.asarray(tau[lo:hi + 1], np.float64)
+    w = 1.0 / np.asarray(err[lo:hi + 1], np.float64) ** 2
+    sw, sx, sy = w.sum(), (w * x).sum(), (w * y).sum()
+    sxx, sxy = (w * x * x).sum(), (w * x * y).sum()
+    den = sw * sxx - sx * sx
+    slope = (sw * sxy - sx * sy) / den
+    intercept = (sy - slope * sx) / sw
+    return intercept, slope
