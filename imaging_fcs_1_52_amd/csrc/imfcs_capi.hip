@@ -409,6 +409,16 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTabl
     HIP_TRY(ctx->up_ctab.put(ctx->ctab, &ct, sizeof(CorrTables), st));
     const CorrTables *dct = ctx->ctab.as<CorrTables>();
     g.nslots = build_slots(g, ctx->host_slots);
+    // vectorized window rows: every slot origin, the row stride and the
+    // frame size are multiples of the window width (uint16 source)
+    g.vec = 0;
+    if (g.src == kSrcU16 && (g.bx == 2 || g.bx == 4) && g.stride % g.bx == 0 && g.plane % g.bx == 0 &&
+        (uintptr_t)g.data % (2 * g.bx) == 0) {
+        bool aligned = true;
+        const int32_t *org = ctx->host_slots.data() + g.nseries;
+        for (int k = 0; k < g.nslots && aligned; k++) aligned = (org[2 * k] % g.bx) == 0;
+        if (aligned) g.vec = g.bx;
+    }
     HIP_TRY(ctx->up_slots.put(ctx->slots, ctx->host_slots.data(), sizeof(int32_t) * ctx->host_slots.size(), st));
     g.slot_of = ctx->slots.as<int32_t>();
     g.slot_org = ctx->slots.as<int32_t>() + g.nseries;

@@ -47,6 +47,33 @@ __device__ __forceinline__ void raw_ints16(const SeriesGeom &g, int ox, int oy, 
     const uint16_t *q = (const uint16_t *)g.data + (int64_t)(g.f0 + f0) * g.plane;
 #pragma unroll
     for (int j = 0; j < 16; j++) acc[j] = -g.bckg_total;
+    if (g.vec == 4 && g.bx == 4) {
+        // aligned rows of 4 pixels: one 8-byte load per row and frame
+        const uint2 *q4 = (const uint2 *)q;
+        for (int k = 0; k < g.by; k++) {
+            const uint32_t d = (lo + (uint32_t)(k * g.stride)) >> 2;
+            uint2 r[16];
+#pragma unroll
+            for (int j = 0; j < 16; j++) r[j] = q4[d + (uint32_t)min(j, nv - 1) * (pl >> 2)];
+#pragma unroll
+            for (int j = 0; j < 16; j++)
+                acc[j] += (int)((r[j].x & 0xffffu) + (r[j].x >> 16) + (r[j].y & 0xffffu) + (r[j].y >> 16));
+        }
+        return;
+    }
+    if (g.vec == 2 && g.bx == 2) {
+        // aligned pixel pairs: one 4-byte load per row and frame
+        const uint32_t *q2 = (const uint32_t *)q;
+        for (int k = 0; k < g.by; k++) {
+            const uint32_t d = (lo + (uint32_t)(k * g.stride)) >> 1;
+            uint32_t r[16];
+#pragma unroll
+            for (int j = 0; j < 16; j++) r[j] = q2[d + (uint32_t)min(j, nv - 1) * (pl >> 1)];
+#pragma unroll
+            for (int j = 0; j < 16; j++) acc[j] += (int)((r[j] & 0xffffu) + (r[j] >> 16));
+        }
+        return;
+    }
     for (int k = 0; k < g.by; k++)
         for (int i = 0; i < g.bx; i++) {
             const uint32_t d = lo + (uint32_t)(k * g.stride + i);
