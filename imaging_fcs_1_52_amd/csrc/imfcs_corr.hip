@@ -292,15 +292,47 @@ struct L3Ctx {
     }
 };
 
-// Level c >= 4 from level c - 1 by the pairwise-sum rebinning
-// (IFCS:9477-9478): v_c[y] = v_{c-1}[2y] + v_{c-1}[2y+1], in FP64.
-__global__ void k_corr_rebin(const float *l3, double *lvd, int64_t off_src, int64_t off_dst, int from_l3,
-                             int nseries, int numbin) {
+// Levels c0+1 .. c0+4 from level c0 by the pairwise-sum rebinning
+// (IFCS:9477-9478): v_c[y] = v_{c-1}[2y] + v_{c-1}[2y+1], in FP64.  One
+// thread per (series, 16 consecutive level-c0 values): it reads the 16
+// values once and writes the 8 + 4 + 2 + 1 values they produce on the next
+// four levels (levels past nlev, and values past each level's numbin, are
+// not written).
+struct RebinArgs {
+    const float *src32;    // level c0 as FP32 (level 3), or null
+    const double *src64;   // level c0 as FP64 (lvd + loff[c0]) otherwise
+    double *dst[4];        // lvd + loff[c0 + 1 + k]
+    int numbin_src;
+    int numbin[4];         // numbin of levels c0+1 .. c0+4 (0 = level absent)
+    int nseries;
+};
+
+__global__ void k_corr_rebin4(RebinArgs R) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)numbin * nseries) return;
-    const int64_t y = i / nseries, s = i - y * nseries;
-    const int64_t i0 = (2 * y) * nseries + s, i1 = (2 * y + 1) * nseries + s;
-    lvd[off_dst + i] = from_l3 ? (double)l3[i0] + (double)l3[i1] : lvd[off_src + i0] + lvd[off_src + i1];
+    const int64_t ngrp = (R.numbin_src + 15) / 16;
+    if (i >= ngrp * R.nseries) return;
+    const int64_t grp = i / R.nseries, s = i - grp * R.nseries;
+    double v[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const int64_t y = min(grp * 16 + k, (int64_t)R.numbin_src - 1);
+        v[k] = R.src32 ? (double)R.src32[y * R.nseries + s] : R.src64[y * R.nseries + s];
+    }
+    int n = 16;
+#pragma unroll
+    for (int lv = 0; lv < 4; lv++) {
+        n >>= 1;
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (k < n) v[k] = v[2 * k] + v[2 * k + 1];
+        if (R.numbin[lv] > 0) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int64_t y = grp * n + k;
+                if (k < n && y < R.numbin[lv]) R.dst[lv][y * R.nseries + s] = v[k];
+            }
+        }
+    }
 }
 
 // Levels >= 3 hold few, large (rebinned) values: FP64 throughout.
@@ -512,10 +544,19 @@ void launch_correlate_fast(const SeriesGeom &g, const BleachDesc &bd, const Corr
     if (ev_main_begin) (void)hipEventRecord(ev_main_begin, st);
     hipLaunchKernelGGL(k_corr_main, dim3(tiles, A.nchunk), dim3(threads), 0, st, A);
     if (ev_main_end) (void)hipEventRecord(ev_main_end, st);
-    for (int c = 4; c < ct.nlevels; c++) {
-        const int64_t n = (int64_t)ct.numbin[c] * g.nseries;
-        hipLaunchKernelGGL(k_corr_rebin, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (const float *)l3, lvd,
-                           A.loff[c - 1], A.loff[c], c == 4 ? 1 : 0, g.nseries, ct.numbin[c]);
+    for (int c0 = 3; c0 + 1 < ct.nlevels; c0 += 4) {
+        RebinArgs R{};
+        R.src32 = c0 == 3 ? l3 : nullptr;
+        R.src64 = c0 == 3 ? nullptr : lvd + A.loff[c0];
+        R.numbin_src = ct.numbin[c0];
+        for (int k = 0; k < 4; k++) {
+            const int c = c0 + 1 + k;
+            R.dst[k] = c < ct.nlevels ? lvd + A.loff[c] : nullptr;
+            R.numbin[k] = c < ct.nlevels ? ct.numbin[c] : 0;
+        }
+        R.nseries = g.nseries;
+        const int64_t n = (int64_t)((ct.numbin[c0] + 15) / 16) * g.nseries;
+        hipLaunchKernelGGL(k_corr_rebin4, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, R);
     }
     const int nhigh = ct.chanum - kMainChan;
     hipLaunchKernelGGL(k_corr_high, dim3((pd.npairs + 63) / 64, ct.nlevels - 3, A.nchunk), dim3(64), 0, st, A, hpart,
