@@ -343,11 +343,11 @@ template <int G, bool L3>
 __device__ void high_level(const L3Ctx<L3> &L, const double *alpha, const double *beta, int B, int pnum,
                            int ybeg, int yend, double *cross, double *a1, double *a2, float *gls_row, int gls_ld) {
     constexpr int WN = G + 16;
-    double w[WN], ga[G], sab[8], sb[8];
-    double sa = 0.0;
+    double w[WN], ga[G], sab[8], head[8];
+    double sa = 0.0, sbk = 0.0;   // block sums of a_y and of b_y over the block's products
 #pragma unroll
     for (int i = 0; i < 8; i++) {
-        sab[i] = sb[i] = 0.0;
+        sab[i] = head[i] = 0.0;
         cross[i] = a1[i] = a2[i] = 0.0;
     }
 #pragma unroll
@@ -355,36 +355,56 @@ __device__ void high_level(const L3Ctx<L3> &L, const double *alpha, const double
     for (int y0 = ybeg; y0 < yend; y0 += G) {
 #pragma unroll
         for (int k = 0; k < G; k++) ga[k] = L.val(L.sa, y0 + k);   // (= w[k] for an ACF; loaded unconditionally so the loads batch)
-        double gs = 0.0;
+        if ((y0 & (B - 1)) == 0) {
+            // block start (blocks hold whole groups: B >= G): head[i] = sum of
+            // the block's first 9 + i b-values
+            double pre = 0.0;
 #pragma unroll
-        for (int k = 0; k < G; k++) gs += ga[k];
+            for (int k = 0; k < 9; k++) pre += w[k];
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                if (i > 0) pre += w[8 + i];
+                head[i] = pre;
+            }
+        }
+        double gs = 0.0, gb = 0.0;
+#pragma unroll
+        for (int k = 0; k < G; k++) {
+            gs += ga[k];
+            gb += w[k];
+        }
         sa += gs;
+        sbk += gb;
 #pragma unroll
         for (int i = 0; i < 8; i++) {
-            double sv = 0.0, t = 0.0;
+            double sv = 0.0;
 #pragma unroll
-            for (int k = 0; k < G; k++) {
-                sv = fma(ga[k], w[k + 9 + i], sv);
-                t += w[k + 9 + i];
-            }
+            for (int k = 0; k < G; k++) sv = fma(ga[k], w[k + 9 + i], sv);
             sab[i] += sv;
-            sb[i] += t;
         }
         const int ye = y0 + G;
         if ((ye & (B - 1)) == 0) {
             const bool inblk = (ye / B - 1) < pnum;
+            // sum_{y in block} b_{y+d} = block sum - head(d) + tail(d), the tail
+            // being the first d b-values after the block (w[G..G+15])
+            double tail = 0.0;
+#pragma unroll
+            for (int k = 0; k < 9; k++) tail += w[G + k];
 #pragma unroll
             for (int i = 0; i < 8; i++) {
-                double S = sab[i] - beta[i] * sa - alpha[i] * sb[i] + (double)B * alpha[i] * beta[i];
+                if (i > 0) tail += w[G + 8 + i];
+                const double sbd = sbk - head[i] + tail;
+                double S = sab[i] - beta[i] * sa - alpha[i] * sbd + (double)B * alpha[i] * beta[i];
                 if (inblk) {
                     a1[i] += S;
                     a2[i] += S * S;
                     if (gls_row) gls_row[(int64_t)(ye / B - 1) * gls_ld + i] = (float)S;
                 }
                 cross[i] += sab[i];
-                sab[i] = sb[i] = 0.0;
+                sab[i] = 0.0;
             }
             sa = 0.0;
+            sbk = 0.0;
         }
 #pragma unroll
         for (int k = 0; k < 16; k++) w[k] = w[k + G];
@@ -396,7 +416,7 @@ __device__ void high_level(const L3Ctx<L3> &L, const double *alpha, const double
 }
 
 // One thread per (pair, level >= 3, time chunk of the main kernel).
-__global__ void __launch_bounds__(64) k_corr_high(MainArgs A, double *hpart /* [chunk][pair][nhigh][3] */,
+__global__ void __launch_bounds__(64, 2) k_corr_high(MainArgs A, double *hpart /* [chunk][pair][nhigh][3] */,
                                                   int nhigh) {
     const int pidx = blockIdx.x * 64 + threadIdx.x;
     const int c = 3 + blockIdx.y;
