@@ -122,7 +122,9 @@ def main():
         n_fitted_pixels += ow.value * oh.value
 
     def step():
-        for e, c in zip(engines, cfgs):
+        # the coarse binnings are enqueued first: their pipelines are short
+        # and end in a few long LM fits, which then overlap the b=1 pipeline
+        for e, c in reversed(list(zip(engines, cfgs))):
             e.run(c, d_stack)
         stage, kern = [], []
         for e in engines:
