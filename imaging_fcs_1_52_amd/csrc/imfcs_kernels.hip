@@ -641,6 +641,30 @@ struct DimTerms {
     float e1, e2, e3, pexp, l, dDpexp;
 };
 
+// erf(u) from exp(-u^2), which the model needs anyway.  |u| >= 0.5:
+// Abramowitz & Stegun, Handbook of Mathematical Functions (1964), formula
+// 7.1.26, erf(u) = 1 - (a1 t + ... + a5 t^5) exp(-u^2), t = 1 / (1 + p u),
+// absolute error below 1.5e-7 (relative below 1e-6 there); |u| < 0.5: the
+// Maclaurin series 2/sqrt(pi) sum (-1)^n u^(2n+1) / (n! (2n+1)) to n = 5
+// (truncation below 3e-8 relative).  Both are evaluated and one selected.
+__device__ __forceinline__ float erf_as(float u, float expmu2) {
+    const float au = fabsf(u);
+    const float t = frcp(fmaf(0.3275911f, au, 1.f));
+    float poly = fmaf(t, 1.061405429f, -1.453152027f);
+    poly = fmaf(t, poly, 1.421413741f);
+    poly = fmaf(t, poly, -0.284496736f);
+    poly = fmaf(t, poly, 0.254829592f);
+    const float e_as = fmaf(-poly * t, expmu2, 1.f);
+    const float s = au * au;
+    float ser = fmaf(s, -1.f / 1320.f, 1.f / 216.f);
+    ser = fmaf(s, ser, -1.f / 42.f);
+    ser = fmaf(s, ser, 1.f / 10.f);
+    ser = fmaf(s, ser, -1.f / 3.f);
+    ser = fmaf(s, ser, 1.f);
+    const float e_ser = 1.1283791670955126f * au * ser;
+    return copysignf(au < 0.5f ? e_ser : e_as, u);
+}
+
 __device__ __forceinline__ void fcs_dim(float a, float r, float v, float x, float p0t, float ip0, float ip02,
                                         bool grad, DimTerms &o) {
     const float isqrpi = 0.56418958354775628f;
@@ -648,19 +672,22 @@ __device__ __forceinline__ void fcs_dim(float a, float r, float v, float x, floa
         // centred (ACF without flow): p1 = p2 = a, p3 = 0, so erf(p1) = erf(p2),
         // erf(p3) = 0 and exp(-p3^2 / p0^2) = exp((p7 - p4) / p0^2) = 1 exactly
         float u = a * ip0;
-        float e = erff(u);
+        const float E = fexp(-u * u);
+        float e = erf_as(u, E);
         o.e1 = o.e2 = e;
         o.e3 = 0.f;
-        o.pexp = 2.f * fexp(-u * u) - 2.f;
+        o.pexp = 2.f * E - 2.f;
         o.l = p0t * isqrpi * o.pexp + 2.f * (a * e);
         if (grad) o.dDpexp = 2.f * (2.f * fexp(-(a * a) * ip02) - 2.f);
         return;
     }
     float p1t = a + r - v * x, p2t = a - r + v * x, p3t = r - v * x;
-    o.e1 = erff(p1t * ip0);
-    o.e2 = erff(p2t * ip0);
-    o.e3 = erff(p3t * ip0);
-    o.pexp = fexp(-(p1t * ip0) * (p1t * ip0)) + fexp(-(p2t * ip0) * (p2t * ip0)) - 2.f * fexp(-(p3t * ip0) * (p3t * ip0));
+    const float u1 = p1t * ip0, u2 = p2t * ip0, u3 = p3t * ip0;
+    const float E1 = fexp(-u1 * u1), E2 = fexp(-u2 * u2), E3 = fexp(-u3 * u3);
+    o.e1 = erf_as(u1, E1);
+    o.e2 = erf_as(u2, E2);
+    o.e3 = erf_as(u3, E3);
+    o.pexp = E1 + E2 - 2.f * E3;
     float perf = p1t * o.e1 + p2t * o.e2 - 2.f * p3t * o.e3;
     o.l = p0t * isqrpi * o.pexp + perf;
     if (grad) {
@@ -705,32 +732,36 @@ __device__ float fcs3p(const LMModelGeom &m, float x, const float *pq, bool grad
     if (c2on) fcs_component(m, x, pq[6], pq[2], pq[3], grad, c2);
     if (c3on) fcs_component(m, x, pq[8], pq[2], pq[3], grad, c3);
     float q2 = m.q2, q3 = m.q3, F2 = pq[5], F3 = pq[7];
-    float et = (pq[10] != 0.f) ? expf(-x / pq[10]) : 0.f;
-    float triplet = 1.f + pq[9] / (1.f - pq[9]) * et;
+    // reciprocals (1 ulp) instead of full-precision divisions
+    const float rT = (pq[10] != 0.f) ? frcp(pq[10]) : 0.f;
+    float et = (pq[10] != 0.f) ? fexp(-x * rT) : 0.f;
+    const float r1F = frcp(1.f - pq[9]);
+    float triplet = 1.f + pq[9] * r1F * et;
     float den = 1.f - F2 - F3 + q2 * F2 + q3 * F3;
-    float invN = 1.f / pq[0];
+    const float rden = frcp(den);
+    float invN = frcp(pq[0]);
     float mix = (1.f - F2 - F3) * c1.acf + q2 * q2 * F2 * c2.acf + q3 * q3 * F3 * c3.acf;
-    float pacf = invN * mix / (den * den) * triplet + pq[4];
+    float pacf = invN * mix * (rden * rden) * triplet + pq[4];
     if (grad) {
-        float pf1 = (1.f - F2 - F3) / den, pf2 = q2 * q2 * F2 / den, pf3 = q3 * q3 * F3 / den;
-        float dfnom = den * den * den;
+        float pf1 = (1.f - F2 - F3) * rden, pf2 = q2 * q2 * F2 * rden, pf3 = q3 * q3 * F3 * rden;
+        const float rdfnom = rden * rden * rden;
         float df21 = 1.f - F2 - F3 + q2 * F2 - q3 * F3 + 2.f * q2 * F3 - 2.f * q2;
         float df22 = q2 * q2 * (1.f + F2 - F3 - q2 * F2 + q3 * F3);
         float df23 = 2.f * F3 * q3 * q3 * (1.f - q2);
         float df31 = 1.f - F2 - F3 - q2 * F2 + 2.f * q3 * F2 - 2.f * q3 + q3 * F3;
         float df32 = 2.f * F2 * q2 * q2 * (1.f - q3);
         float df33 = q3 * q3 * (1.f - F2 + F3 + q2 * F2 - q3 * F3);
-        float norm = 4.f * (m.ax * m.ay) * (m.ax * m.ay) / m.obsvol;
-        float dtF = et * (1.f / (1.f - pq[9]) + pq[9] / ((1.f - pq[9]) * (1.f - pq[9])));
-        float dtT = (pq[10] != 0.f) ? et * (pq[9] * x) / ((1.f - pq[9]) * pq[10] * pq[10]) : 0.f;
+        const float inorm = m.obsvol * frcp(4.f * (m.ax * m.ay) * (m.ax * m.ay));
+        float dtF = et * (r1F + pq[9] * (r1F * r1F));
+        float dtT = (pq[10] != 0.f) ? et * (pq[9] * x) * r1F * (rT * rT) : 0.f;
         gr[0] = -invN * invN * (pf1 * c1.acf + pf2 * c2.acf + pf3 * c3.acf) * triplet;
         gr[1] = invN * pf1 * (c1.plat * c1.dDpspim + c1.pspim * c1.dDplat);   // no triplet, IFCS:13310
-        gr[2] = invN * (pf1 * c1.dvx * c1.pspim + pf2 * c2.dvx * c2.pspim + pf3 * c3.dvx * c3.pspim) / norm * triplet;
-        gr[3] = invN * (pf1 * c1.dvy * c1.pspim + pf2 * c2.dvy * c2.pspim + pf3 * c3.dvy * c3.pspim) / norm * triplet;
+        gr[2] = invN * (pf1 * c1.dvx * c1.pspim + pf2 * c2.dvx * c2.pspim + pf3 * c3.dvx * c3.pspim) * inorm * triplet;
+        gr[3] = invN * (pf1 * c1.dvy * c1.pspim + pf2 * c2.dvy * c2.pspim + pf3 * c3.dvy * c3.pspim) * inorm * triplet;
         gr[4] = 1.f;
-        gr[5] = invN / dfnom * (df21 * c1.acf + df22 * c2.acf + df23 * c3.acf) * triplet;
+        gr[5] = invN * rdfnom * (df21 * c1.acf + df22 * c2.acf + df23 * c3.acf) * triplet;
         gr[6] = invN * pf2 * (c2.plat * c2.dDpspim + c2.pspim * c2.dDplat) * triplet;
-        gr[7] = invN / dfnom * (df31 * c1.acf + df32 * c2.acf + df33 * c3.acf) * triplet;
+        gr[7] = invN * rdfnom * (df31 * c1.acf + df32 * c2.acf + df33 * c3.acf) * triplet;
         gr[8] = invN * pf3 * (c3.plat * c3.dDpspim + c3.pspim * c3.dDplat) * triplet;
         gr[9] = dtF * pacf;
         gr[10] = dtT * pacf;
@@ -762,6 +793,22 @@ __device__ __forceinline__ double wave_sum(double v) {
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
     const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
     return __hiloint2double(hi, lo);
+}
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_f32(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xf, false));
+}
+
+// FP32 version of wave_sum (same DPP steps).
+__device__ __forceinline__ float wave_sum_f32(float v) {
+    v += dpp_f32<0xB1, 0xf>(v);
+    v += dpp_f32<0x4E, 0xf>(v);
+    v += dpp_f32<0x141, 0xf>(v);
+    v += dpp_f32<0x140, 0xf>(v);
+    v += dpp_f32<0x142, 0xa>(v);
+    v += dpp_f32<0x143, 0xc>(v);
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
 // FP32 chi-square with the reduction order of Gpufit: points are grouped in
@@ -1110,7 +1157,7 @@ __device__ __forceinline__ bool chol_solve_fixed(double (&A)[NF][NF], double (&b
 #pragma unroll
         for (int k = 0; k < j; k++) s -= A[j][k] * A[j][k];
         if (!(s > 0.0)) return false;
-        inv[j] = 1.0 / sqrt(s);
+        inv[j] = rsqrt(s);
 #pragma unroll
         for (int i = j + 1; i < NF; i++) {
             double t = A[i][j];
@@ -1283,8 +1330,12 @@ __global__ void __launch_bounds__(64) k_lm_fixed(LMArgs a) {
 // both waves then take identical LM steps from identical sums.
 template <int NF>
 __global__ void __launch_bounds__(128) k_lm_fcs_2w(LMArgs a) {
-    constexpr int NP = 20, NH = NF * (NF + 1) / 2, NS = 1 + NF + NH;
-    __shared__ double red[2][2][NS];   // [buffer][wave][sum]
+    constexpr int NP = 20, NH = NF * (NF + 1) / 2, NS = NF + NH;
+    // per buffer and wave: the FP64 chi-square and the FP32 gradient and
+    // normal-matrix sums (FP32 as in Gpufit; only the chi-square, which
+    // decides step acceptance and convergence, needs FP64)
+    __shared__ double red_c2[2][2];
+    __shared__ float red[2][2][NS];
     const int fit = blockIdx.x;
     if (fit >= a.nfits) return;
     const int tid = threadIdx.x, wv = tid >> 6;
@@ -1300,37 +1351,41 @@ __global__ void __launch_bounds__(128) k_lm_fcs_2w(LMArgs a) {
     const float yi = act ? a.data[(int64_t)fit * a.npts + tid] : 0.f;
     const double wi = !act ? 0.0 : (a.weights ? (double)a.weights[(int64_t)fit * a.npts + tid] : 1.0);
     int buf = 0;
+    const float wf = (float)wi;
     auto eval = [&](const float *p, double &c2, double *g, double *h) {
-        double sv[NS];
+        double cx = 0.0;
+        float sv[NS];
 #pragma unroll
-        for (int k = 0; k < NS; k++) sv[k] = 0.0;
+        for (int k = 0; k < NS; k++) sv[k] = 0.f;
         if (act) {
             float gfull[11];
             const float f = fcs3p(ac.geom, xi, p, true, c2on, c3on, gfull);
-            const double r = (double)yi - (double)f;
-            sv[0] = wi * r * r;
-            int hh = 1 + NF;
+            const double r = (double)yi - (double)f;   // exact
+            cx = wi * r * r;
+            const float wr = wf * (float)r;
+            int hh = NF;
 #pragma unroll
             for (int q = 0; q < NF; q++) {
-                const double jq = (double)gfull[FixedSlots<kModelFCS, NF>::slot(q)];
-                sv[1 + q] = wi * r * jq;
+                const float jq = gfull[FixedSlots<kModelFCS, NF>::slot(q)];
+                sv[q] = wr * jq;
 #pragma unroll
-                for (int l = 0; l <= q; l++, hh++)
-                    sv[hh] = wi * jq * (double)gfull[FixedSlots<kModelFCS, NF>::slot(l)];
+                for (int l = 0; l <= q; l++, hh++) sv[hh] = wf * jq * gfull[FixedSlots<kModelFCS, NF>::slot(l)];
             }
         }
+        cx = wave_sum(cx);
 #pragma unroll
-        for (int k = 0; k < NS; k++) sv[k] = wave_sum(sv[k]);
+        for (int k = 0; k < NS; k++) sv[k] = wave_sum_f32(sv[k]);
         if ((tid & 63) == 0) {
+            red_c2[buf][wv] = cx;
 #pragma unroll
             for (int k = 0; k < NS; k++) red[buf][wv][k] = sv[k];
         }
         __syncthreads();
-        c2 = red[buf][0][0] + red[buf][1][0];
+        c2 = red_c2[buf][0] + red_c2[buf][1];
 #pragma unroll
-        for (int q = 0; q < NF; q++) g[q] = red[buf][0][1 + q] + red[buf][1][1 + q];
+        for (int q = 0; q < NF; q++) g[q] = (double)(red[buf][0][q] + red[buf][1][q]);
 #pragma unroll
-        for (int q = 0; q < NH; q++) h[q] = red[buf][0][1 + NF + q] + red[buf][1][1 + NF + q];
+        for (int q = 0; q < NH; q++) h[q] = (double)(red[buf][0][NF + q] + red[buf][1][NF + q]);
         buf ^= 1;
     };
     double chi2;
