@@ -182,12 +182,16 @@ def main():
     for b, c, e in zip(BINNINGS, cfgs, engines):
         e.run(c, d_stack)
         r = e.results(c)
-        # ACF fits only (the CCF pair is correlated, not fitted), pixels in the ROI
-        fitted = np.isfinite(np.asarray(r["fitres"])[0][..., 0])
-        it = np.asarray(r["iters"])[0][fitted]
-        stt = np.asarray(r["states"])[0][fitted]
+        # ACF fits (pair 0).  Pixels whose bleach polynomial crosses zero get
+        # NaN correlations (the reference's correction takes sqrt(f/c0) of a
+        # negative ratio) and their fits stop at once; iteration counts are
+        # over the converged fits
+        acf_ok = np.isfinite(np.asarray(r["acf"])[0]).all(-1)
+        stt = np.asarray(r["states"])[0]
+        it = np.asarray(r["iters"])[0][stt == 0]
         iters["b%d" % b] = {"median": float(np.median(it)), "p99": float(np.percentile(it, 99)),
-                            "max": int(it.max()), "converged_frac": round(float((stt == 0).mean()), 4)}
+                            "max": int(it.max()), "finite_acf_frac": round(float(acf_ok.mean()), 4),
+                            "converged_frac": round(float((stt == 0).mean()), 4)}
 
     cpu = None
     if args.cpu_sample > 0:
