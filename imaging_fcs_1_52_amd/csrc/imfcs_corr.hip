@@ -339,7 +339,7 @@ __global__ void k_corr_rebin4(RebinArgs R) {
 // Values [ybeg, yend) of one level; yend - ybeg and ybeg are multiples of G
 // and of the block length B, so every block of the blocked variance lies in
 // one call.
-template <int G, bool L3>
+template <int G, bool L3, bool SAME>
 __device__ void high_level(const L3Ctx<L3> &L, const double *alpha, const double *beta, int B, int pnum,
                            int ybeg, int yend, double *cross, double *a1, double *a2, float *gls_row, int gls_ld) {
     constexpr int WN = G + 16;
@@ -354,7 +354,7 @@ __device__ void high_level(const L3Ctx<L3> &L, const double *alpha, const double
     for (int k = 0; k < WN; k++) w[k] = L.val(L.sb, ybeg + k);
     for (int y0 = ybeg; y0 < yend; y0 += G) {
 #pragma unroll
-        for (int k = 0; k < G; k++) ga[k] = L.val(L.sa, y0 + k);   // (= w[k] for an ACF; loaded unconditionally so the loads batch)
+        for (int k = 0; k < G; k++) ga[k] = SAME ? w[k] : L.val(L.sa, y0 + k);
         if ((y0 & (B - 1)) == 0) {
             // block start (blocks hold whole groups: B >= G): head[i] = sum of
             // the block's first 9 + i b-values
@@ -415,14 +415,45 @@ __device__ void high_level(const L3Ctx<L3> &L, const double *alpha, const double
     for (int i = 0; i < 8; i++) cross[i] += sab[i];
 }
 
-// One thread per (pair, level >= 3, time chunk of the main kernel).
+template <bool SAME>
+__device__ __forceinline__ void high_dispatch(const MainArgs &A, int c, int G, int sa, int sb, const double *alpha,
+                                              const double *beta, int B, int pnum, int ybeg, int yend, double *cross,
+                                              double *a1, double *a2, float *grow, int ld) {
+    const CorrTables &ct = *A.ctp;
+    if (c == 3) {
+        const L3Ctx<true> L{A.l3, nullptr, A.g.nseries, sa, sb, c, ct.numbin[c]};
+        switch (G) {
+            case 16: high_level<16, true, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
+            case 8: high_level<8, true, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
+            case 4: high_level<4, true, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
+            case 2: high_level<2, true, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
+            default: high_level<1, true, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
+        }
+    } else {
+        const L3Ctx<false> L{nullptr, A.lvd + A.loff[c], A.g.nseries, sa, sb, c, ct.numbin[c]};
+        switch (G) {
+            case 16: high_level<16, false, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
+            case 8: high_level<8, false, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
+            case 4: high_level<4, false, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
+            case 2: high_level<2, false, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
+            default: high_level<1, false, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
+        }
+    }
+}
+
+// One thread per (pair, level >= 3, time chunk of the main kernel).  The
+// ACF pairs and the CCF pairs fill separate workgroups (blocks [0, nb_acf)
+// and the rest), so whether a and b are one series is uniform per wave and
+// an ACF loads its series once.
 __global__ void __launch_bounds__(64, 2) k_corr_high(MainArgs A, double *hpart /* [chunk][pair][nhigh][3] */,
-                                                  int nhigh) {
-    const int pidx = blockIdx.x * 64 + threadIdx.x;
+                                                  int nhigh, int nb_acf) {
+    const bool acf_block = (int)blockIdx.x < nb_acf;
+    const int p = (acf_block ? (int)blockIdx.x : (int)blockIdx.x - nb_acf) * 64 + threadIdx.x;
+    const int pidx = acf_block ? p : (A.pd.acf_first ? A.pd.npix : 0) + p;
     const int c = 3 + blockIdx.y;
     const int chunk = blockIdx.z;
     const CorrTables &ct = *A.ctp;
-    if (pidx >= A.pd.npairs || c >= ct.nlevels) return;
+    if (p >= A.pd.npix || pidx >= A.pd.npairs || c >= ct.nlevels) return;
     int sa, sb;
     pair_series(A.pd, pidx, sa, sb);
     const double *Fa = A.F + (int64_t)A.g.slot_of[sa] * ct.npos, *Fb = A.F + (int64_t)A.g.slot_of[sb] * ct.npos;
@@ -446,25 +477,11 @@ __global__ void __launch_bounds__(64, 2) k_corr_high(MainArgs A, double *hpart /
     float *grow = A.gls ? A.gls + (int64_t)pidx * A.gls_nblk * (ct.chanum - 1) + (x0 - 1) : nullptr;
     const int G = min(16, 1 << max(ct.q - 1 - c, 0));
     const int ld = ct.chanum - 1;
-    if (c == 3) {
-        const L3Ctx<true> L{A.l3, nullptr, A.g.nseries, sa, sb, c, ct.numbin[c]};
-        switch (G) {
-            case 16: high_level<16, true>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
-            case 8: high_level<8, true>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
-            case 4: high_level<4, true>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
-            case 2: high_level<2, true>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
-            default: high_level<1, true>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
-        }
-    } else {
-        const L3Ctx<false> L{nullptr, A.lvd + A.loff[c], A.g.nseries, sa, sb, c, ct.numbin[c]};
-        switch (G) {
-            case 16: high_level<16, false>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
-            case 8: high_level<8, false>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
-            case 4: high_level<4, false>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
-            case 2: high_level<2, false>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
-            default: high_level<1, false>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
-        }
-    }
+    // (wave-uniform: a workgroup holds only ACF or only CCF pairs)
+    if (acf_block && A.pd.acf_first)
+        high_dispatch<true>(A, c, G, sa, sb, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld);
+    else
+        high_dispatch<false>(A, c, G, sa, sb, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld);
     double *out = hpart + (((int64_t)chunk * A.pd.npairs + pidx) * nhigh + (x0 - kMainChan)) * 3;
     for (int i = 0; i < 8; i++) {
         out[i * 3 + 0] = cross[i];
@@ -579,8 +596,10 @@ void launch_correlate_fast(const SeriesGeom &g, const BleachDesc &bd, const Corr
         hipLaunchKernelGGL(k_corr_rebin4, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, R);
     }
     const int nhigh = ct.chanum - kMainChan;
-    hipLaunchKernelGGL(k_corr_high, dim3((pd.npairs + 63) / 64, ct.nlevels - 3, A.nchunk), dim3(64), 0, st, A, hpart,
-                       nhigh);
+    const int nbp = (pd.npix + 63) / 64;   // workgroups per pair type
+    const int nb_acf = pd.acf_first ? nbp : 0;
+    const int nb_all = nb_acf + (pd.has_ccf ? nbp : 0);
+    hipLaunchKernelGGL(k_corr_high, dim3(nb_all, ct.nlevels - 3, A.nchunk), dim3(64), 0, st, A, hpart, nhigh, nb_acf);
     hipLaunchKernelGGL(k_corr_finalize, dim3(pd.npairs), dim3(128), 0, st, A, (const double *)hpart, nhigh, corav,
                        bvar);
 }
