@@ -200,6 +200,7 @@ struct imfcs_ctx {
     std::vector<float> h_xs, h_init;
     std::vector<int> h_pfit;
     CachedUpload up_ctab, up_slots, up_fx, up_finit, up_fpfit;
+    DevBuf swacf, swvar;   // sliding-window averages
     DevBuf part, coef, ctot, rec, F, bpart, ind, bis, corav, bvar, fdata, fw, fx, finit, fpfit, fout, fstate, fchi2,
         fiter, cpart, hpart, l3, lvd, ctab, csum, slots, X, glsblk, glsW, glsy, glsok, glscov, dropin_in, nbm, nbc;
     SeriesGeom last_g{};   // geometry (with device slot tables) of the last engine run
@@ -230,6 +231,16 @@ int imfcs_get_versions(int *runtime_version, int *driver_version) {
     return 0;
 }
 
+// Correlator depth q of a run: with sliding windows, setParameters lowers it
+// to what a window of L frames supports, lagnum = floor((ln(L / (20 + p)) +
+// 1) / ln 2) (swMinFrameReq = 20, IFCS:203, 7331-7341).
+static int effective_q(const imfcs_config *c) {
+    if (c->bleach_mode != IMFCS_BLEACH_SLIDING_WINDOW) return c->correlator_q;
+    const double L = (double)c->sliding_window_length;
+    const int lagnum = (int)std::floor((std::log(L / (20.0 + c->correlator_p)) + 1.0) / std::log(2.0));
+    return std::min(c->correlator_q, lagnum);
+}
+
 int imfcs_plan(const imfcs_config *c, int32_t *out_w, int32_t *out_h, int32_t *chanum) {
     if (!c) return fail(IMFCS_ERR_ARG, "null config");
     if (c->pixbin_x <= 0 || c->pixbin_y <= 0 || c->binning_x <= 0 || c->binning_y <= 0)
@@ -241,7 +252,7 @@ int imfcs_plan(const imfcs_config *c, int32_t *out_w, int32_t *out_h, int32_t *c
     int ey = (int)std::floor((double)(c->roi_y + c->roi_h - c->binning_y) / c->pixbin_y);
     if (out_w) *out_w = ex - sx + 1;
     if (out_h) *out_h = ey - sy + 1;
-    if (chanum) *chanum = c->correlator_p + (c->correlator_q - 1) * c->correlator_p / 2 + 1;
+    if (chanum) *chanum = c->correlator_p + (effective_q(c) - 1) * c->correlator_p / 2 + 1;
     return 0;
 }
 
@@ -275,7 +286,7 @@ static void release_all(imfcs_ctx *ctx) {
                       &ctx->fpfit, &ctx->fout, &ctx->fstate, &ctx->fchi2, &ctx->fiter, &ctx->cpart,
                       &ctx->hpart, &ctx->l3, &ctx->lvd, &ctx->ctab, &ctx->csum, &ctx->slots,
                       &ctx->X, &ctx->glsblk, &ctx->glsW, &ctx->glsy, &ctx->glsok, &ctx->glscov,
-                      &ctx->dropin_in, &ctx->nbm, &ctx->nbc};
+                      &ctx->dropin_in, &ctx->nbm, &ctx->nbc, &ctx->swacf, &ctx->swvar};
     for (DevBuf *b : bufs) b->release();
 }
 
@@ -596,7 +607,7 @@ extern "C" int imfcs_run(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_co
     ea.gls = c->gls ? 1 : 0;
     ea.t_off = 1.5;
     ea.p = c->correlator_p;
-    ea.q = c->correlator_q;
+    ea.q = effective_q(c);
     ea.pd.npix = g.npix;
     ea.pd.acf_first = c->do_acf ? 1 : 0;
     ea.pd.has_ccf = c->do_ccf ? 1 : 0;
@@ -606,9 +617,42 @@ extern "C" int imfcs_run(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_co
     ctx->oh = oh;
     HostTables ht;
     CorrTables ct;
-    rc = run_engine(ctx, ea, ht, ct);
-    if (rc) return rc;
     hipStream_t st = ctx->stream;
+    if (c->bleach_mode == IMFCS_BLEACH_SLIDING_WINDOW) {
+        // Sliding windows (IFCS:9146-9172): each window of L frames is
+        // correlated on its own, without bleach correction (the intensities
+        // of the window, getIntensity with no matching mode), and the ACF
+        // and blocked variance are averaged over the windows, each term
+        // divided by the window count; the blocking index reported is the
+        // last window's.
+        const int L = c->sliding_window_length;
+        if (c->poly_order != 0 || c->gls)
+            return fail(IMFCS_ERR_ARG, "sliding windows exclude the polynomial correction and GLS");
+        if (L < 2 || L > g.T) return fail(IMFCS_ERR_ARG, "sliding window length outside the frame range");
+        if (ea.q < 2) return fail(IMFCS_ERR_ARG, "sliding window too short for the correlator");
+        const int numofsw = g.T / L;
+        const int first = g.f0;
+        ea.order = 0;
+        ea.g.T = L;
+        ea.nopit = std::min(L, 1000);
+        ea.ave = L / ea.nopit;
+        for (int i = 0; i < numofsw; i++) {
+            ea.g.f0 = first + i * L;
+            rc = run_engine(ctx, ea, ht, ct);
+            if (rc) return rc;
+            const int64_t n = (int64_t)ea.pd.npairs * ct.chanum;
+            HIP_TRY(ctx->swacf.ensure(sizeof(double) * n));
+            HIP_TRY(ctx->swvar.ensure(sizeof(double) * n));
+            launch_accum_div(ctx->swacf.as<double>(), ctx->corav.as<double>(), n, (double)numofsw, i == 0, st);
+            launch_accum_div(ctx->swvar.as<double>(), ctx->bvar.as<double>(), n, (double)numofsw, i == 0, st);
+        }
+        const int64_t n = (int64_t)ea.pd.npairs * ct.chanum;
+        launch_copy(ctx->corav.as<double>(), ctx->swacf.as<double>(), sizeof(double) * n, st);
+        launch_copy(ctx->bvar.as<double>(), ctx->swvar.as<double>(), sizeof(double) * n, st);
+    } else {
+        rc = run_engine(ctx, ea, ht, ct);
+        if (rc) return rc;
+    }
     ctx->did_fit = c->do_fit;
     if (c->do_fit) {
         const int np = ea.pd.npairs, npts = ct.chanum - 1;

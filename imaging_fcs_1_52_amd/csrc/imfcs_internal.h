@@ -170,6 +170,8 @@ void launch_binning_f32(const float *in, float *out, int win, int hin, int wt, i
 void launch_bleach_average_f32(const float *in, float *out, int wt, int ht, int nopit, int ave,
                                hipStream_t st);
 void launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st);
+// dst[i] = (first ? 0 : dst[i]) + src[i] / div
+void launch_accum_div(double *dst, const double *src, int64_t n, double div, int first, hipStream_t st);
 void launch_nb_extract(const CorrTables *dct, const SeriesGeom &g, const PairDesc &pd, const double *F,
                        const double *corav, double *nbmean, double *nbcov, hipStream_t st);
 void launch_pack_fit_inputs(const CorrTables &ct, int npairs, const double *corav, const double *bvar,

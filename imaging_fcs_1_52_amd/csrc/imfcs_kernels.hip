@@ -1561,6 +1561,16 @@ void launch_nb_extract(const CorrTables *dct, const SeriesGeom &g, const PairDes
     hipLaunchKernelGGL(k_nb_extract, dim3(cdiv(pd.npairs, 64)), dim3(64), 0, st, dct, g, pd, F, corav, nbmean, nbcov);
 }
 
+__global__ void k_accum_div(double *dst, const double *src, int64_t n, double div, int first) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (first ? 0.0 : dst[i]) + src[i] / div;
+}
+
+void launch_accum_div(double *dst, const double *src, int64_t n, double div, int first, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_accum_div, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dst, src, n, div, first);
+}
+
 void launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st) {
     const size_t n16 = bytes / 16;
     const int ntail = (int)(bytes - n16 * 16);

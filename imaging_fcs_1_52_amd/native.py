@@ -44,7 +44,10 @@ class PipelineConfig(ctypes.Structure):
                 ("correlator_p", _i32), ("correlator_q", _i32), ("frametime", _f64), ("poly_order", _i32),
                 ("roi_x", _i32), ("roi_y", _i32), ("roi_w", _i32), ("roi_h", _i32), ("do_acf", _i32),
                 ("do_ccf", _i32), ("do_fit", _i32), ("fit_max_iterations", _i32), ("fit_tolerance", _f32),
-                ("fit_chi2_fp64", _i32), ("gls", _i32), ("acf_model", FitModelGeom), ("ccf_model", FitModelGeom)]
+                ("fit_chi2_fp64", _i32), ("gls", _i32), ("acf_model", FitModelGeom), ("ccf_model", FitModelGeom),
+                ("bleach_mode", _i32), ("sliding_window_length", _i32)]
+
+BLEACH_POLYNOMIAL, BLEACH_SLIDING_WINDOW = 0, 1
 
 
 # every C-ABI entry point of include/imfcs.h with its ctypes signature

@@ -172,7 +172,19 @@ typedef struct imfcs_config {
                                         output becomes its mean curve and the fit is the
                                         GLS fit (GLSFit, IFCS:10622-10789) */
     imfcs_fit_model acf_model, ccf_model;
+    int32_t bleach_mode;             /* IMFCS_BLEACH_*: POLYNOMIAL (of poly_order; 0 = no
+                                        correction) or SLIDING_WINDOW (IFCS:9146-9172) */
+    int32_t sliding_window_length;   /* frames per window for SLIDING_WINDOW */
 } imfcs_config;
+
+/* Bleach correction modes of the native pipeline (cbBleachCor, IFCS:9145). */
+enum {
+    IMFCS_BLEACH_POLYNOMIAL = 0,
+    /* correlations of consecutive windows of sliding_window_length frames,
+       uncorrected, averaged (ACF, blocked variance) over the windows; the
+       correlator depth is reduced to what a window supports (IFCS:7331-7341) */
+    IMFCS_BLEACH_SLIDING_WINDOW = 1
+};
 
 /* Output geometry for a config: out_w x out_h pixels, chanum channels. */
 int imfcs_plan(const imfcs_config *cfg, int32_t *out_w, int32_t *out_h, int32_t *chanum);

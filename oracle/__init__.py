@@ -165,6 +165,38 @@ def calculate_cf(a, b, ind, p=16, q=8, frametime=0.001, gls=False):
     return corav, bvar, bsd, cov
 
 
+def sliding_window_q(L, p=16, q=8):
+    """Correlator depth of a sliding-window run: setParameters replaces q by
+    lagnum = floor((ln(L / (swMinFrameReq + p)) + 1) / ln 2) when that is
+    smaller, swMinFrameReq = 20 (IFCS:203, 7331-7341)."""
+    lagnum = int(np.floor((np.log(L / (20.0 + p)) + 1.0) / np.log(2.0)))
+    return min(q, lagnum)
+
+
+def sliding_window_cf(stack, px, py, bx, by, bckg, first, last, L, p=16, q=8, frametime=0.001, px2=None,
+                      py2=None):
+    """correlate() with "Sliding Window" bleach correction (IFCS:9146-9172):
+    the numofsw = floor(num / L) windows are read without correction
+    (getIntensity has no branch for this mode), blocked and correlated on
+    their own with the window's tables, and corav, blockvar, blocksd are
+    summed as term / numofsw.  Returns (acf, var, sd, ind, blockIndS, q) with
+    the last window's blocking result."""
+    qq = sliding_window_q(L, p, q)
+    nsw = (last - first + 1) // L
+    acf = var = sd = 0.0
+    ind = bis = 0
+    for i in range(nsw):
+        f0, f1 = i * L + first, (i + 1) * L + first - 1
+        a = get_intensity(stack, px, py, bx, by, bckg, f0, f1)
+        b = a if px2 is None else get_intensity(stack, px2, py2, bx, by, bckg, f0, f1)
+        ind, bis = block_transform(a, b, p, qq, frametime)
+        corav, bvar, bsd, _ = calculate_cf(a, b, ind, p, qq, frametime)
+        acf = acf + corav / nsw
+        var = var + bvar / nsw
+        sd = sd + bsd / nsw
+    return acf, var, sd, ind, bis, qq
+
+
 def make_model(ax, ay, s, sz=None, rx=0.0, ry=0.0, q2=1.0, q3=1.0, paramfit=None, paraminit=None):
     """Build FcsModel in SI units.  Defaults follow the fit panel
     (IFCS:1959-1968): N=1, D=1 um^2/s, everything else 0; N, D, G free."""

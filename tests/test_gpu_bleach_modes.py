@@ -1,0 +1,53 @@
+"""GPU parity of the sliding-window bleach mode (IFCS:9146-9172) against the
+oracle's restatement of correlate()'s window loop: averaged ACF within 1e-4
+relative (floor 1e-4 of the amplitude), averaged blocked variance within
+1e-4 relative for pixels whose every window blocks alike (at least 95 % of
+them), fitted N and D within 1e-3 relative where both sides fitted."""
+import numpy as np
+import pytest
+
+import imaging_fcs_1_52_amd as imfcs
+import oracle
+from parity import rel_err
+
+pytestmark = pytest.mark.gpu
+
+PIX, PSF = 0.24e-6, 0.38e-6
+
+
+@pytest.mark.parametrize("L", [2000, 1000])   # correlator depth 7 (main kernel path) and 6 (generic path)
+def test_sliding_window_matches_oracle(engine, L):
+    stack = imfcs.simulate_2d(8, 6, 12000, 50, D_um2s=1.0, tau_bleach_s=10.0, seed=1234 + L)
+    T, H, W = stack.shape
+    bg = int(stack.min())
+    acf_m = imfcs.make_fit_model(PIX, PIX, PSF)
+    cfg = imfcs.make_pipeline_config(W, H, T, binning=1, background=bg, poly_order=0, do_fit=True,
+                                     acf_model=acf_m, sliding_window=L)
+    engine.upload(stack)
+    engine.run(cfg)
+    g = engine.results(cfg)
+    q = oracle.sliding_window_q(L)
+    n = 16 + (q - 1) * 8 + 1
+    assert g["acf"].shape[-1] == n
+    _, _, _, lagtime = oracle.tables(16, q, L, 1e-3)
+    om = oracle.make_model(PIX, PIX, PSF)
+    o_acf, o_var = np.zeros((H, W, n)), np.zeros((H, W, n))
+    o_fit, o_ok = np.full((H, W, 11), np.nan), np.zeros((H, W), bool)
+    for y in range(H):
+        for x in range(W):
+            a, v, _, _, _, qq = oracle.sliding_window_cf(stack, x, y, 1, 1, bg, 1, T, L)
+            assert qq == q
+            o_acf[y, x], o_var[y, x] = a, v
+            fr, _, ok, _, _ = oracle.standard_fit(om, lagtime, a, v)
+            o_fit[y, x], o_ok[y, x] = fr, ok
+    ga, gv = g["acf"][0], g["var"][0]
+    amp = np.abs(o_acf[..., 1:]).max(-1, keepdims=True)
+    e = rel_err(ga, o_acf, 1e-4 * amp)
+    assert e.max() < 1e-4, e.max()
+    ev = rel_err(gv[..., 1:], o_var[..., 1:], 1e-300).max(-1)
+    assert (ev < 1e-4).mean() >= 0.95, np.sort(ev)[-5:]
+    ok = (g["states"][0] == 0) & o_ok
+    assert ok.mean() > 0.8
+    for k in (0, 1):
+        err = rel_err(g["fitres"][0][..., k][ok], o_fit[..., k][ok], 1e-30)
+        assert err.max() < 1e-3, (k, err.max())
