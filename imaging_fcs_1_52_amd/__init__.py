@@ -218,12 +218,13 @@ def make_fit_model(ax, ay, s, sz=None, rx=0.0, ry=0.0, q2=1.0, q3=1.0, paramfit=
 def make_pipeline_config(W, H, T, binning=1, cf=(0, 0), background=0, poly_order=4, frametime=1e-3, p=16, q=8,
                          roi=None, do_acf=True, do_ccf=None, do_fit=True, acf_model=None, ccf_model=None,
                          overlap=False, firstframe=1, lastframe=None, fit_max_iterations=2000,
-                         fit_tolerance=1e-16, fit_chi2_fp64=True, gls=False, sliding_window=0):
+                         fit_tolerance=1e-16, fit_chi2_fp64=True, gls=False, sliding_window=0, lin_segment=0):
     """imfcs_config of one pipeline run.  fit_chi2_fp64 selects FP64 step
     acceptance and convergence on chi^2 (the optimum the Java CPU path
     reaches); False keeps Gpufit's FP32 chi^2 of the reference GPU fit.
     sliding_window > 0 selects the sliding-window bleach mode with windows of
-    that many frames (poly_order must then be 0)."""
+    that many frames, lin_segment > 0 the linear-segment correction with
+    T // lin_segment segments (poly_order must then be 0)."""
     c = PipelineConfig()
     c.width, c.height, c.frames = W, H, T
     c.firstframe = firstframe
@@ -244,8 +245,8 @@ def make_pipeline_config(W, H, T, binning=1, cf=(0, 0), background=0, poly_order
     c.fit_tolerance = fit_tolerance
     c.fit_chi2_fp64 = int(bool(fit_chi2_fp64))
     c.gls = int(bool(gls))
-    c.bleach_mode = 1 if sliding_window else 0
-    c.sliding_window_length = int(sliding_window)
+    c.bleach_mode = 1 if sliding_window else (2 if lin_segment else 0)
+    c.sliding_window_length = int(sliding_window or lin_segment)
     if acf_model is not None:
         c.acf_model = acf_model
     if ccf_model is not None:
@@ -335,7 +336,11 @@ class Engine:
         ind, bis = np.zeros(npairs, np.int32), np.zeros(npairs, np.int32)
         fitres, chi2 = np.full((npairs, 11), np.nan), np.full(npairs, np.nan)
         states, iters = np.full(npairs, -1, np.int32), np.zeros(npairs, np.int32)
-        coef = np.zeros((nseries, max(cfg.poly_order, 0) + 1))
+        if cfg.bleach_mode == 2:   # bcInt0 and the segment means
+            ncoef = (cfg.lastframe - cfg.firstframe + 1) // cfg.sliding_window_length + 1
+        else:
+            ncoef = max(cfg.poly_order, 0) + 1
+        coef = np.zeros((nseries, ncoef))
         _check(self._lib.imfcs_get_results(self.ctx, _dp(acf), _dp(var), _ip(ind), _ip(bis), _dp(fitres),
                                            _dp(chi2), _ip(states), _ip(iters), _dp(coef)))
         npair_types = int(cfg.do_acf) + int(cfg.do_ccf)

@@ -405,6 +405,7 @@ struct EngineArgs {
     double t_off;
     int p, q;
     PairDesc pd;
+    int seg_swl;           // > 0: "Lin Segment" correction with this sliding window length
 };
 
 int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTables &ct) {
@@ -443,7 +444,21 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTabl
     bd.order = ea.order;
     bd.frametime = ea.frametime;
     HIP_TRY(hipEventRecord(ctx->ev[0], st));
-    if (ea.order > 0) {
+    if (ea.seg_swl > 0) {
+        // "Lin Segment" (IFCS:9767-9805): bcNopit = num / L segments of
+        // bcAve = num / bcNopit frames
+        const int nseg = g.T / ea.seg_swl, seg_len = nseg > 0 ? g.T / nseg : 0;
+        if (nseg < 2 || seg_len < 2) return fail(IMFCS_ERR_ARG, "linear segments need two segments of >= 2 frames");
+        HIP_TRY(ctx->coef.ensure(sizeof(double) * (size_t)ns * (nseg + 1)));
+        launch_seg_means(g, seg_len, nseg, ctx->coef.as<double>(), st);
+        bd.mode = 1;
+        bd.order = nseg;
+        bd.seg_len = seg_len;
+        bd.coef = ctx->coef.as<double>();
+        bd.tscale = 1.0;
+        ctx->order = nseg;
+        ctx->coef_scale.assign(nseg + 1, 1.0);
+    } else if (ea.order > 0) {
         if (ea.order > kMaxOrder) return fail(IMFCS_ERR_ARG, "polynomial order above 8");
         HIP_TRY(ctx->coef.ensure(sizeof(double) * (size_t)ns * n));
         if (ea.fit_trace) {
@@ -618,6 +633,12 @@ extern "C" int imfcs_run(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_co
     HostTables ht;
     CorrTables ct;
     hipStream_t st = ctx->stream;
+    if (c->bleach_mode == IMFCS_BLEACH_LIN_SEGMENT) {
+        if (c->poly_order != 0) return fail(IMFCS_ERR_ARG, "linear segments exclude the polynomial correction");
+        ea.seg_swl = c->sliding_window_length;
+    } else if (c->bleach_mode != IMFCS_BLEACH_POLYNOMIAL && c->bleach_mode != IMFCS_BLEACH_SLIDING_WINDOW) {
+        return fail(IMFCS_ERR_ARG, "unknown bleach correction mode");
+    }
     if (c->bleach_mode == IMFCS_BLEACH_SLIDING_WINDOW) {
         // Sliding windows (IFCS:9146-9172): each window of L frames is
         // correlated on its own, without bleach correction (the intensities

@@ -142,6 +142,40 @@ struct BleachP {
     }
 };
 
+// "Lin Segment" bleach correction (IFCS:9767-9805) of one slot: bcTrace[k]
+// are the segment means, c0 = bcInt0 their extrapolation to the start; the
+// reference function g is linear between segment centres, extrapolated
+// before the first centre and, with the reference's index nf = x - (n-1)
+// len + len/2 + len, after the last.  x is Java's 1-based frame index.
+struct BleachSeg {
+    const double *t;
+    double c0;
+    int n, len, h;
+    __device__ void load(const BleachDesc &bd, int slot) {
+        n = bd.order;
+        len = bd.seg_len;
+        h = len / 2;
+        const double *c = bd.coef + (int64_t)slot * (n + 1);
+        c0 = c[0];
+        t = c + 1;
+    }
+    __device__ __forceinline__ double apply(double v, int f) const {
+        const int x = f + 1;
+        double g;
+        if (x < h) {
+            g = t[0] + (t[0] - t[1]) / len * (double)(h - x);
+        } else if (x < (n - 1) * len + h) {
+            const int xs = (x - h) / len + 1, y = (x - h) - (xs - 1) * len;
+            g = t[xs - 1] + (t[xs] - t[xs - 1]) * y / len;
+        } else {
+            const int nf = x - (n - 1) * len + h + len;
+            g = t[n - 2] + (t[n - 1] - t[n - 2]) * nf / len;
+        }
+        const double r = sqrt(g / c0);
+        return v / r + c0 * (1.0 - r);
+    }
+};
+
 // Compile-time degree used for a run's polynomial order.
 inline int bleach_maxo(int order) { return order <= 0 ? 0 : (order <= 4 ? 4 : kMaxOrder); }
 

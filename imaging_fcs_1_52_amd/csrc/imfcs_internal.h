@@ -54,9 +54,13 @@ __host__ __device__ inline double slot_mu(const double *coef, int order, int slo
 // u = (frame + t_off) * frametime / tscale.  coef[s][k] are the
 // coefficients of f(u); order 0 means "no correction".
 struct BleachDesc {
-    const double *coef;   // [nseries][order+1]
+    const double *coef;   // [nslots][order+1]
     int32_t order;
     double frametime, tscale, t_off;
+    // 1: linear segments (coef[slot] = {bcInt0, bcTrace[0..order-1]},
+    // segments of seg_len frames); 0: polynomial of the given order
+    int32_t mode;
+    int32_t seg_len;
 };
 
 // Multi-tau correlator layout derived from the lag/samp tables.
@@ -132,6 +136,9 @@ void launch_polyfit(const double *part, int nchunk, int nseries, int nopit, int 
                     double frametime, int order, double *coef, hipStream_t st);
 void launch_apply(const SeriesGeom &g, const BleachDesc &bd, const CorrTables *dct, double *ctot, double *rec,
                   float *X, int nchunk, int chunk_len, hipStream_t st);
+// Linear-segment bleach correction: per slot the segment means bcTrace and
+// the extrapolated start bcInt0, into coef[slot][0..nseg].
+void launch_seg_means(const SeriesGeom &g, int seg_len, int nseg, double *coef, hipStream_t st);
 void launch_prefix_combine(const CorrTables *dct, int nslots, const double *ctot, const double *rec,
                            int nchunk, int chunk_len, double *F, hipStream_t st);
 // Work split of the blocking pass: products in chunks of 256 (levels 0..7

@@ -119,7 +119,7 @@ __global__ void k_polyfit(const double *part, int nchunk, int nseries, int order
 // IFCS:9487-9494, 9262-9270).  One thread per (slot, frame chunk); frames
 // are read in groups of 16 so the loads of a group are in flight together.
 // ------------------------------------------------------------------
-template <int MAXO>
+template <class BL>
 __global__ void __launch_bounds__(64) k_apply(SeriesGeom g, BleachDesc bd, const CorrTables *__restrict__ ctp,
                                               int chunk_len, double *ctot, double *rec, float *X) {
     const CorrTables &ct = *ctp;
@@ -128,7 +128,7 @@ __global__ void __launch_bounds__(64) k_apply(SeriesGeom g, BleachDesc bd, const
     if (k >= g.nslots) return;
     int ox, oy;
     slot_origin(g, k, ox, oy);
-    BleachP<MAXO> bl;
+    BL bl;
     bl.load(bd, k);
     const double mu = slot_mu(bd.coef, bd.order, k);
     const int f0 = chunk * chunk_len, f1 = min(g.T, f0 + chunk_len);
@@ -1617,13 +1617,56 @@ void launch_polyfit(const double *part, int nchunk, int nseries, int nopit, int 
 void launch_apply(const SeriesGeom &g, const BleachDesc &bd, const CorrTables *dct, double *ctot, double *rec,
                   float *X, int nchunk, int chunk_len, hipStream_t st) {
     dim3 grid(cdiv(g.nslots, 64), nchunk);
+    if (bd.mode == 1) {
+        hipLaunchKernelGGL(k_apply<BleachSeg>, grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec, X);
+        return;
+    }
     switch (bleach_maxo(bd.order)) {
-        case 0: hipLaunchKernelGGL(k_apply<0>, grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec, X); break;
-        case 4: hipLaunchKernelGGL(k_apply<4>, grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec, X); break;
+        case 0:
+            hipLaunchKernelGGL(k_apply<BleachP<0>>, grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec, X);
+            break;
+        case 4:
+            hipLaunchKernelGGL(k_apply<BleachP<4>>, grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec, X);
+            break;
         default:
-            hipLaunchKernelGGL(k_apply<kMaxOrder>, grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec, X);
+            hipLaunchKernelGGL(k_apply<BleachP<kMaxOrder>>, grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec,
+                               X);
             break;
     }
+}
+
+// Segment means of getIntensity's "Lin Segment" branch (IFCS:9779-9787):
+// segment k sums the 1-based frames 1 + k len .. (k+1) len - 1 (len - 1
+// frames) and divides by len; one thread per (slot, segment), exact integer
+// window values summed in FP64.
+__global__ void k_seg_means(SeriesGeom g, int seg_len, int nseg, double *coef) {
+    const int s = blockIdx.x * 64 + threadIdx.x, k = blockIdx.y;
+    if (s >= g.nslots) return;
+    int ox, oy;
+    slot_origin(g, s, ox, oy);
+    const int fa = k * seg_len, fe = (k + 1) * seg_len - 1;   // 0-based [fa, fe)
+    double sum = 0.0;
+    for (int f = fa; f < fe; f += 16) {
+        int v[16];
+        raw_ints16(g, ox, oy, f, fe - 1, v);
+#pragma unroll
+        for (int i = 0; i < 16; i++)
+            if (f + i < fe) sum += (double)v[i];
+    }
+    coef[(int64_t)s * (nseg + 1) + 1 + k] = sum / seg_len;
+}
+
+// bcInt0 = bcTrace[0] + (bcTrace[0] - bcTrace[1]) / 2 (IFCS:9789)
+__global__ void k_seg_int0(int nslots, int nseg, double *coef) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    double *c = coef + (int64_t)s * (nseg + 1);
+    c[0] = c[1] + (c[1] - c[2]) / 2;
+}
+
+void launch_seg_means(const SeriesGeom &g, int seg_len, int nseg, double *coef, hipStream_t st) {
+    hipLaunchKernelGGL(k_seg_means, dim3(cdiv(g.nslots, 64), nseg), dim3(64), 0, st, g, seg_len, nseg, coef);
+    hipLaunchKernelGGL(k_seg_int0, dim3(cdiv(g.nslots, 64)), dim3(64), 0, st, g.nslots, nseg, coef);
 }
 
 void launch_prefix_combine(const CorrTables *dct, int nslots, const double *ctot, const double *rec, int nchunk,

@@ -173,8 +173,10 @@ typedef struct imfcs_config {
                                         GLS fit (GLSFit, IFCS:10622-10789) */
     imfcs_fit_model acf_model, ccf_model;
     int32_t bleach_mode;             /* IMFCS_BLEACH_*: POLYNOMIAL (of poly_order; 0 = no
-                                        correction) or SLIDING_WINDOW (IFCS:9146-9172) */
-    int32_t sliding_window_length;   /* frames per window for SLIDING_WINDOW */
+                                        correction), SLIDING_WINDOW (IFCS:9146-9172) or
+                                        LIN_SEGMENT (IFCS:9767-9805) */
+    int32_t sliding_window_length;   /* frames per window (SLIDING_WINDOW), or the length
+                                        that sets the segment count (LIN_SEGMENT) */
 } imfcs_config;
 
 /* Bleach correction modes of the native pipeline (cbBleachCor, IFCS:9145). */
@@ -183,7 +185,10 @@ enum {
     /* correlations of consecutive windows of sliding_window_length frames,
        uncorrected, averaged (ACF, blocked variance) over the windows; the
        correlator depth is reduced to what a window supports (IFCS:7331-7341) */
-    IMFCS_BLEACH_SLIDING_WINDOW = 1
+    IMFCS_BLEACH_SLIDING_WINDOW = 1,
+    /* piecewise-linear bleach curve through the means of num / L segments;
+       coefficients returned per series: bcInt0 then the segment means */
+    IMFCS_BLEACH_LIN_SEGMENT = 2
 };
 
 /* Output geometry for a config: out_w x out_h pixels, chanum channels. */

@@ -69,6 +69,8 @@ def lib():
         L.orc_polyfit.restype = ctypes.c_int
         L.orc_polyfit.argtypes = [_D, _D, ctypes.c_int, ctypes.c_int, _D]
         L.orc_bleach_apply.argtypes = [_D, ctypes.c_int, _D, ctypes.c_int, ctypes.c_double]
+        L.orc_bleach_linseg.restype = ctypes.c_int
+        L.orc_bleach_linseg.argtypes = [_D, ctypes.c_int, ctypes.c_int, _D]
         L.orc_block_transform.restype = ctypes.c_int
         L.orc_block_transform.argtypes = [_D, _D, ctypes.c_int, ctypes.c_int, _I, _D, _D,
                                           ctypes.c_double, ctypes.c_int, _I]
@@ -139,6 +141,18 @@ def bleach_apply(intdat, coef, frametime):
     coef = np.ascontiguousarray(coef, np.float64)
     lib().orc_bleach_apply(_dp(d), len(d) - 1, _dp(coef), len(coef) - 1, frametime)
     return d
+
+
+def bleach_linseg(intdat, swl):
+    """"Lin Segment" correction of a 1-based trace (IFCS:9767-9805).
+    Returns (corrected trace, [bcInt0, segment means...])."""
+    d = np.array(intdat, np.float64)
+    num = len(d) - 1
+    seg = np.zeros(num // swl + 1)
+    n = lib().orc_bleach_linseg(_dp(d), num, swl, _dp(seg))
+    if n == 0:
+        raise ValueError("linear segments need two segments of >= 2 frames")
+    return d, seg
 
 
 def block_transform(a, b, p=16, q=8, frametime=0.001):

@@ -588,6 +588,48 @@ void orc_bleach_apply(double *intdat, int num, const double *res, int maxord,
 }
 
 /* ------------------------------------------------------------------ */
+/* "Lin Segment" bleach correction, getIntensity IFCS:9767-9805        */
+/* ------------------------------------------------------------------ */
+static double linseg_apply(double v, double g, double int0)
+{
+    return v / sqrt(g / int0) + int0 * (1 - sqrt(g / int0));
+}
+
+int orc_bleach_linseg(double *intdat, int num, int swl, double *seg)
+{
+    int bcNopit = num / swl;                       /* number of linear segments */
+    if (bcNopit < 2) return 0;
+    int bcAve = num / bcNopit;                     /* frames per segment */
+    if (bcAve < 2) return 0;
+    int h = bcAve / 2;
+    double *bc = (double *)malloc(sizeof(double) * bcNopit);
+    for (int x = 0; x < bcNopit; x++) {            /* segment means: len - 1 frames over len */
+        double sum = 0;
+        for (int y = 1 + x * bcAve; y < (x + 1) * bcAve; y++) sum += intdat[y];
+        bc[x] = sum / bcAve;
+    }
+    double int0 = bc[0] + (bc[0] - bc[1]) / 2;     /* extrapolated start */
+    for (int x = 1; x < h; x++)
+        intdat[x] = linseg_apply(intdat[x], bc[0] + (bc[0] - bc[1]) / bcAve * (h - x), int0);
+    for (int x = 1; x < bcNopit; x++)
+        for (int y = 0; y < bcAve; y++) {
+            int nf = (x - 1) * bcAve + h + y;
+            intdat[nf] = linseg_apply(intdat[nf], bc[x - 1] + (bc[x] - bc[x - 1]) * y / bcAve, int0);
+        }
+    for (int x = (bcNopit - 1) * bcAve + h; x <= num; x++) {
+        int nf = x - (bcNopit - 1) * bcAve + h + bcAve;   /* the reference's index */
+        intdat[x] = linseg_apply(intdat[x], bc[bcNopit - 2] + (bc[bcNopit - 1] - bc[bcNopit - 2]) * nf / bcAve,
+                                 int0);
+    }
+    if (seg) {
+        seg[0] = int0;
+        for (int x = 0; x < bcNopit; x++) seg[1 + x] = bc[x];
+    }
+    free(bc);
+    return bcNopit;
+}
+
+/* ------------------------------------------------------------------ */
 /* a7: blockTransform (IFCS:9217-9422), blocklag = 1                  */
 /* ------------------------------------------------------------------ */
 int orc_block_transform(const double *a, const double *b, int num,

@@ -57,6 +57,12 @@ int orc_polyfit(const double *trace, const double *intTime, int nopit,
 void orc_bleach_apply(double *intdat, int num, const double *coef,
                       int order, double frametime);
 
+/* getIntensity's "Lin Segment" correction (IFCS:9767-9805) of the 1-based
+ * trace intdat[1..num] with sliding window length swl; seg (optional,
+ * bcNopit + 1 values) receives bcInt0 then the segment means.  Returns the
+ * segment count, or 0 if fewer than two segments of >= 2 frames. */
+int orc_bleach_linseg(double *intdat, int num, int swl, double *seg);
+
 /* blockTransform (IFCS:9217-9422).  a, b: 1-based traces (num+1).
  * Returns ind; *blockIndS receives the success flag. */
 int orc_block_transform(const double *a, const double *b, int num,

@@ -51,3 +51,53 @@ def test_sliding_window_matches_oracle(engine, L):
     for k in (0, 1):
         err = rel_err(g["fitres"][0][..., k][ok], o_fit[..., k][ok], 1e-30)
         assert err.max() < 1e-3, (k, err.max())
+
+
+def test_linear_segments_match_oracle(engine):
+    """"Lin Segment" correction (IFCS:9767-9805) on the GPU against the
+    oracle: segment means and bcInt0 to 1e-12 relative, ACF within 1e-4,
+    blocked variance within 1e-4 where the blocking index agrees, fitted N
+    and D within 1e-3."""
+    stack = imfcs.simulate_2d(8, 6, 12000, 60, D_um2s=1.0, tau_bleach_s=8.0, seed=4321)
+    T, H, W = stack.shape
+    bg = int(stack.min())
+    L = 600
+    acf_m = imfcs.make_fit_model(PIX, PIX, PSF)
+    cfg = imfcs.make_pipeline_config(W, H, T, binning=1, background=bg, poly_order=0, do_fit=True,
+                                     acf_model=acf_m, lin_segment=L)
+    engine.upload(stack)
+    engine.run(cfg)
+    g = engine.results(cfg)
+    _, _, _, lagtime = oracle.tables(16, 8, T, 1e-3)
+    om = oracle.make_model(PIX, PIX, PSF)
+    n = lagtime.size
+    o_acf, o_var = np.zeros((H, W, n)), np.zeros((H, W, n))
+    o_ind = np.zeros((H, W), int)
+    o_fit, o_ok = np.full((H, W, 11), np.nan), np.zeros((H, W), bool)
+    for y in range(H):
+        for x in range(W):
+            d = oracle.get_intensity(stack, x, y, 1, 1, bg, 1, T)
+            d, seg = oracle.bleach_linseg(d, L)
+            np.testing.assert_allclose(g["coef"][y * W + x], seg, rtol=1e-12)
+            ind, _ = oracle.block_transform(d, d)
+            a, v, _, _ = oracle.calculate_cf(d, d, ind)
+            o_acf[y, x], o_var[y, x], o_ind[y, x] = a, v, ind
+            fr, _, ok, _, _ = oracle.standard_fit(om, lagtime, a, v)
+            o_fit[y, x], o_ok[y, x] = fr, ok
+    # the reference's extrapolation after the last segment centre can turn
+    # negative under strong bleaching: those pixels are NaN on both sides
+    fin = np.isfinite(o_acf).all(-1)
+    assert np.array_equal(np.isfinite(g["acf"][0]).all(-1), fin)
+    assert fin.mean() > 0.5
+    amp = np.abs(o_acf[..., 1:]).max(-1, keepdims=True)
+    e = rel_err(g["acf"][0][fin], o_acf[fin], 1e-4 * amp[fin])
+    assert e.max() < 1e-4, e.max()
+    same = (g["ind"][0] == o_ind) & fin
+    assert same[fin].mean() >= 0.95
+    ev = rel_err(g["var"][0][same][:, 1:], o_var[same][:, 1:], 1e-300)
+    assert ev.max() < 1e-4, ev.max()
+    ok = (g["states"][0] == 0) & o_ok & same
+    assert ok[fin].mean() > 0.8
+    for k in (0, 1):
+        err = rel_err(g["fitres"][0][..., k][ok], o_fit[..., k][ok], 1e-30)
+        assert err.max() < 1e-3, (k, err.max())

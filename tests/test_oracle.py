@@ -343,3 +343,35 @@ def test_gls_with_diagonal_covariance_equals_weighted_fit():
     f2, c2, ok2 = oracle.gls_fit(m, lagtime, acf_n, np.diag(var[1:]))
     assert ok1 and ok2
     np.testing.assert_allclose(f1[[0, 1, 4]], f2[[0, 1, 4]], rtol=1e-6)
+
+
+def test_linseg_constant_trace_is_unchanged():
+    """"Lin Segment" (IFCS:9767-9805): with no bleaching every segment mean
+    is the same, the reference curve is flat at bcInt0 and the correction
+    I/sqrt(g/c0) + c0(1 - sqrt(g/c0)) is the identity, in all three ranges
+    (before the first segment centre, between centres, after the last)."""
+    d = np.full(12001, 57.0)
+    d[0] = 0.0
+    out, seg = oracle.bleach_linseg(d, 600)
+    assert len(seg) == 21
+    np.testing.assert_allclose(out[1:], d[1:], rtol=1e-13)
+    # segment means divide len - 1 frames by len (the reference's sum bounds)
+    np.testing.assert_allclose(seg[1:], 57.0 * 599 / 600, rtol=1e-13)
+
+
+def test_linseg_flattens_a_linear_decay():
+    """A linear bleach between the first and the last segment centre is
+    followed by the piecewise-linear curve, so the corrected trace there is
+    flat up to the segment-mean offset: its slope drops by orders of
+    magnitude."""
+    num = 12000
+    x = np.arange(num + 1, dtype=np.float64)
+    d = 1000.0 - 0.03 * x
+    d[0] = 0.0
+    out, seg = oracle.bleach_linseg(d, 600)
+    bc = len(seg) - 1
+    ave = num // bc
+    mid = slice(ave, (bc - 1) * ave)
+    slope_in = np.polyfit(x[mid], d[mid], 1)[0]
+    slope_out = np.polyfit(x[mid], out[mid], 1)[0]
+    assert abs(slope_out) < 1e-3 * abs(slope_in)
