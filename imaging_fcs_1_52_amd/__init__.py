@@ -218,13 +218,15 @@ def make_fit_model(ax, ay, s, sz=None, rx=0.0, ry=0.0, q2=1.0, q3=1.0, paramfit=
 def make_pipeline_config(W, H, T, binning=1, cf=(0, 0), background=0, poly_order=4, frametime=1e-3, p=16, q=8,
                          roi=None, do_acf=True, do_ccf=None, do_fit=True, acf_model=None, ccf_model=None,
                          overlap=False, firstframe=1, lastframe=None, fit_max_iterations=2000,
-                         fit_tolerance=1e-16, fit_chi2_fp64=True, gls=False, sliding_window=0, lin_segment=0):
+                         fit_tolerance=1e-16, fit_chi2_fp64=True, gls=False, sliding_window=0, lin_segment=0,
+                         exp_bleach=0):
     """imfcs_config of one pipeline run.  fit_chi2_fp64 selects FP64 step
     acceptance and convergence on chi^2 (the optimum the Java CPU path
     reaches); False keeps Gpufit's FP32 chi^2 of the reference GPU fit.
     sliding_window > 0 selects the sliding-window bleach mode with windows of
     that many frames, lin_segment > 0 the linear-segment correction with
-    T // lin_segment segments (poly_order must then be 0)."""
+    T // lin_segment segments, exp_bleach=True the single-exponential
+    correction (poly_order must then be 0)."""
     c = PipelineConfig()
     c.width, c.height, c.frames = W, H, T
     c.firstframe = firstframe
@@ -245,7 +247,7 @@ def make_pipeline_config(W, H, T, binning=1, cf=(0, 0), background=0, poly_order
     c.fit_tolerance = fit_tolerance
     c.fit_chi2_fp64 = int(bool(fit_chi2_fp64))
     c.gls = int(bool(gls))
-    c.bleach_mode = 1 if sliding_window else (2 if lin_segment else 0)
+    c.bleach_mode = 1 if sliding_window else (2 if lin_segment else (3 if exp_bleach else 0))
     c.sliding_window_length = int(sliding_window or lin_segment)
     if acf_model is not None:
         c.acf_model = acf_model
@@ -338,6 +340,8 @@ class Engine:
         states, iters = np.full(npairs, -1, np.int32), np.zeros(npairs, np.int32)
         if cfg.bleach_mode == 2:   # bcInt0 and the segment means
             ncoef = (cfg.lastframe - cfg.firstframe + 1) // cfg.sliding_window_length + 1
+        elif cfg.bleach_mode == 3:   # f(0), A, t1, C
+            ncoef = 4
         else:
             ncoef = max(cfg.poly_order, 0) + 1
         coef = np.zeros((nseries, ncoef))

@@ -199,8 +199,11 @@ struct imfcs_ctx {
     // per-run parameter blocks
     std::vector<float> h_xs, h_init;
     std::vector<int> h_pfit;
-    CachedUpload up_ctab, up_slots, up_fx, up_finit, up_fpfit;
+    CachedUpload up_ctab, up_slots, up_fx, up_finit, up_fpfit, up_eitime, up_epfit;
+    std::vector<float> h_eitime;
+    std::vector<int> h_epfit;
     DevBuf swacf, swvar;   // sliding-window averages
+    DevBuf etrace, eitime, epfit, einit, eout, estate, echi2, eiter;   // exponential bleach fits
     DevBuf part, coef, ctot, rec, F, bpart, ind, bis, corav, bvar, fdata, fw, fx, finit, fpfit, fout, fstate, fchi2,
         fiter, cpart, hpart, l3, lvd, ctab, csum, slots, X, glsblk, glsW, glsy, glsok, glscov, dropin_in, nbm, nbc;
     SeriesGeom last_g{};   // geometry (with device slot tables) of the last engine run
@@ -286,7 +289,9 @@ static void release_all(imfcs_ctx *ctx) {
                       &ctx->fpfit, &ctx->fout, &ctx->fstate, &ctx->fchi2, &ctx->fiter, &ctx->cpart,
                       &ctx->hpart, &ctx->l3, &ctx->lvd, &ctx->ctab, &ctx->csum, &ctx->slots,
                       &ctx->X, &ctx->glsblk, &ctx->glsW, &ctx->glsy, &ctx->glsok, &ctx->glscov,
-                      &ctx->dropin_in, &ctx->nbm, &ctx->nbc, &ctx->swacf, &ctx->swvar};
+                      &ctx->dropin_in, &ctx->nbm, &ctx->nbc, &ctx->swacf, &ctx->swvar, &ctx->etrace,
+                      &ctx->eitime, &ctx->epfit, &ctx->einit, &ctx->eout, &ctx->estate, &ctx->echi2,
+                      &ctx->eiter};
     for (DevBuf *b : bufs) b->release();
 }
 
@@ -406,6 +411,7 @@ struct EngineArgs {
     int p, q;
     PairDesc pd;
     int seg_swl;           // > 0: "Lin Segment" correction with this sliding window length
+    int exp_n;             // 1: "Single Exp" correction
 };
 
 int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTables &ct) {
@@ -458,6 +464,41 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTabl
         bd.tscale = 1.0;
         ctx->order = nseg;
         ctx->coef_scale.assign(nseg + 1, 1.0);
+    } else if (ea.exp_n > 0) {
+        // "Single Exp" (IFCS:9682-9709): LM fit of the nopit-point intensity
+        // trace (SingleExpFit, IFCS:10845-10895), then the correction with
+        // the fitted curve
+        const int nexp = ea.exp_n, np = nexp == 1 ? 3 : 5;
+        const int nopit = ea.nopit;
+        const int nchunk = std::min(std::min(nopit, 128), std::max(40, (1 << 19) / std::max(1, ns)));
+        HIP_TRY(ctx->part.ensure(sizeof(double) * (size_t)nchunk * ns));
+        HIP_TRY(ctx->etrace.ensure(sizeof(float) * (size_t)ns * nopit));
+        launch_trace_moments(g, nopit, ea.ave, 1.0, ea.frametime, 0, 1, ctx->part.as<double>(), nchunk, st,
+                             ctx->etrace.as<float>());
+        ctx->h_eitime.resize(nopit);
+        for (int x = 0; x < nopit; x++) ctx->h_eitime[x] = (float)(ea.frametime * (x + 0.5) * ea.ave);   // IFCS:9851
+        ctx->h_epfit.assign(11, 0);
+        for (int k = 0; k < np; k++) ctx->h_epfit[k] = 1;
+        HIP_TRY(ctx->up_eitime.put(ctx->eitime, ctx->h_eitime.data(), sizeof(float) * nopit, st));
+        HIP_TRY(ctx->up_epfit.put(ctx->epfit, ctx->h_epfit.data(), sizeof(int) * 11, st));
+        HIP_TRY(ctx->einit.ensure(sizeof(float) * 11 * (size_t)ns));
+        HIP_TRY(ctx->eout.ensure(sizeof(double) * 5 * (size_t)ns));
+        HIP_TRY(ctx->estate.ensure(sizeof(int) * (size_t)ns));
+        HIP_TRY(ctx->echi2.ensure(sizeof(float) * (size_t)ns));
+        HIP_TRY(ctx->eiter.ensure(sizeof(int) * (size_t)ns));
+        launch_exp_init(ctx->etrace.as<float>(), ctx->eitime.as<float>(), ns, nopit, nexp, ctx->einit.as<float>(), st);
+        launch_lm_exp(ns, nopit, ctx->etrace.as<float>(), ctx->eitime.as<float>(), ctx->einit.as<float>(),
+                      ctx->epfit.as<int>(), nexp, 1e-16f, 2000, ctx->eout.as<double>(), ctx->estate.as<int>(),
+                      ctx->echi2.as<float>(), ctx->eiter.as<int>(), st);
+        HIP_TRY(ctx->coef.ensure(sizeof(double) * (size_t)ns * (np + 1)));
+        launch_exp_pack(ctx->eout.as<double>(), ns, nexp, ctx->coef.as<double>(), st);
+        bd.mode = nexp == 1 ? 2 : 3;
+        bd.order = np;
+        bd.coef = ctx->coef.as<double>();
+        bd.tscale = 1.0;
+        bd.t_off = ea.t_off;
+        ctx->order = np;
+        ctx->coef_scale.assign(np + 1, 1.0);
     } else if (ea.order > 0) {
         if (ea.order > kMaxOrder) return fail(IMFCS_ERR_ARG, "polynomial order above 8");
         HIP_TRY(ctx->coef.ensure(sizeof(double) * (size_t)ns * n));
@@ -636,6 +677,10 @@ extern "C" int imfcs_run(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_co
     if (c->bleach_mode == IMFCS_BLEACH_LIN_SEGMENT) {
         if (c->poly_order != 0) return fail(IMFCS_ERR_ARG, "linear segments exclude the polynomial correction");
         ea.seg_swl = c->sliding_window_length;
+    } else if (c->bleach_mode == IMFCS_BLEACH_SINGLE_EXP) {
+        if (c->poly_order != 0) return fail(IMFCS_ERR_ARG, "exponential correction excludes the polynomial one");
+        if (ea.nopit < 10) return fail(IMFCS_ERR_ARG, "too few frames for an exponential bleach fit");
+        ea.exp_n = 1;
     } else if (c->bleach_mode != IMFCS_BLEACH_POLYNOMIAL && c->bleach_mode != IMFCS_BLEACH_SLIDING_WINDOW) {
         return fail(IMFCS_ERR_ARG, "unknown bleach correction mode");
     }

@@ -176,6 +176,44 @@ struct BleachSeg {
     }
 };
 
+// "Single Exp" / "Double Exp" bleach correction (IFCS:9682-9738) of one
+// slot: f(t) = A e^{-t/t1} + C (+ B e^{-t/t2}), f0 = f(0), t = frametime
+// (x + 0.5) with Java's 1-based frame index x; I' = I / sqrt(f/f0) +
+// f0 (1 - sqrt(f/f0)).  f0 = 0 marks a fit the reference would not apply.
+// FP64 throughout, as the Java path.
+template <int NE>
+struct BleachExp {
+    double a, t1, b, t2, c, f0, scale, off;
+    __device__ void load(const BleachDesc &bd, int slot) {
+        const int np = NE == 1 ? 3 : 5;
+        const double *q = bd.coef + (int64_t)slot * (np + 1);
+        f0 = q[0];
+        a = q[1];
+        t1 = q[2];
+        if (NE == 1) {
+            b = 0.0;
+            t2 = 1.0;
+            c = q[3];
+        } else {
+            b = q[3];
+            t2 = q[4];
+            c = q[5];
+        }
+        scale = bd.frametime;
+        off = bd.t_off;
+    }
+    __device__ __forceinline__ double curve(double t) const {
+        double v = a * exp(-t / t1) + c;
+        if (NE == 2) v += b * exp(-t / t2);
+        return v;
+    }
+    __device__ __forceinline__ double apply(double v, int f) const {
+        if (f0 == 0.0) return v;
+        const double r = sqrt(curve(((double)f + off) * scale) / f0);
+        return v / r + f0 * (1.0 - r);
+    }
+};
+
 // Compile-time degree used for a run's polynomial order.
 inline int bleach_maxo(int order) { return order <= 0 ? 0 : (order <= 4 ? 4 : kMaxOrder); }
 

@@ -58,7 +58,9 @@ struct BleachDesc {
     int32_t order;
     double frametime, tscale, t_off;
     // 1: linear segments (coef[slot] = {bcInt0, bcTrace[0..order-1]},
-    // segments of seg_len frames); 0: polynomial of the given order
+    // segments of seg_len frames); 2, 3: single / double exponential
+    // (coef[slot] = {f(0), A, t1, C} / {f(0), A, t1, B, t2, C}); 0:
+    // polynomial of the given order
     int32_t mode;
     int32_t seg_len;
 };
@@ -131,7 +133,17 @@ void launch_lm_gls(int nfits, int m, const float *x, const float *init, const in
 
 // Launchers (imfcs_kernels.hip)
 void launch_trace_moments(const SeriesGeom &g, int nopit, int ave, double tscale, double frametime,
-                          int order, int integer_trace, double *part, int nchunk, hipStream_t st);
+                          int order, int integer_trace, double *part, int nchunk, hipStream_t st,
+                          float *trace_out = nullptr);
+// Exponential bleach fits (SingleExpFit / DoubleExpFit): initial guesses
+// from the traces, the LM fit of every slot's trace (FP64 chi-square), and
+// coef[slot] = {f(0), params} (f(0) = 0: no correction).
+void launch_exp_init(const float *trace, const float *itime, int nslots, int nopit, int nexp, float *init,
+                     hipStream_t st);
+void launch_lm_exp(int nfits, int npts, const float *data, const float *x, const float *init, const int *pfit,
+                   int nexp, float tol, int maxit, double *out64, int *states, float *chi2, int *iters,
+                   hipStream_t st);
+void launch_exp_pack(const double *out64, int nslots, int nexp, double *coef, hipStream_t st);
 void launch_polyfit(const double *part, int nchunk, int nseries, int nopit, int ave, double tscale,
                     double frametime, int order, double *coef, hipStream_t st);
 void launch_apply(const SeriesGeom &g, const BleachDesc &bd, const CorrTables *dct, double *ctot, double *rec,

@@ -29,7 +29,7 @@ namespace imfcs {
 // Stage 1: intensity trace moments and polynomial fit
 // ------------------------------------------------------------------
 __global__ void k_trace_moments(SeriesGeom g, int nopit, int ave, double tscale, double frametime, int order,
-                                int integer_trace, int pts_per_chunk, double *part) {
+                                int integer_trace, int pts_per_chunk, double *part, float *trace_out) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;   // slot
     int chunk = blockIdx.y;
     if (s >= g.nslots) return;
@@ -57,6 +57,7 @@ __global__ void k_trace_moments(SeriesGeom g, int nopit, int ave, double tscale,
             for (int f = x * ave; f < (x + 1) * ave; f++) sum += raw_value(g, ox, oy, f);
             y = sum / ave;
         }
+        if (trace_out) trace_out[(int64_t)s * nopit + x] = (float)y;
         double u = frametime * (x + 0.5) * ave / tscale, up = 1.0;
 #pragma unroll
         for (int k = 0; k <= kMaxOrder; k++) {
@@ -858,7 +859,9 @@ __device__ bool chol_solve(double *A, double *b, int n) {
     return true;
 }
 
-enum ModelKind { kModelFCS = 0, kModelPoly = 1 };
+// kModelExp1/2: the bleach models SingleExp A e^{-t/t1} + C and DoubleExp
+// A e^{-t/t1} + B e^{-t/t2} + C (IFCS:13028-13086), fitted by k_lm_exp64.
+enum ModelKind { kModelFCS = 0, kModelPoly = 1, kModelExp1 = 2, kModelExp2 = 3 };
 
 struct LMArgs {
     int nfits, npts, nparams, kind, ncoef;
@@ -1213,8 +1216,8 @@ __device__ __forceinline__ void fcs_params_out(float *par) {
 // Gpufit's LM iteration over the fixed free slots.  eval(par, chi2, grad,
 // hess) returns the chi-square, gradient and packed normal matrix of the
 // current parameters.
-template <int KIND, int NF, class Eval>
-__device__ void lm_fixed_iterate(const LMArgs &a, float *par, Eval &eval, double &chi2, int &state, int &iters) {
+template <int KIND, int NF, class Eval, class PT>
+__device__ void lm_fixed_iterate(const LMArgs &a, PT *par, Eval &eval, double &chi2, int &state, int &iters) {
     constexpr int NH = NF * (NF + 1) / 2;
     double grad[NF], hess[NH], H[NF][NF], scale[NF];
     eval(par, chi2, grad, hess);
@@ -1253,12 +1256,12 @@ __device__ void lm_fixed_iterate(const LMArgs &a, float *par, Eval &eval, double
             state = kStateSingular;
             break;
         }
-        float prev[NF];
+        PT prev[NF];
 #pragma unroll
         for (int k = 0; k < NF; k++) {
             const int sl = FixedSlots<KIND, NF>::slot(k);
             prev[k] = par[sl];
-            par[sl] = (float)((double)par[sl] + delta[k]);
+            par[sl] = (PT)((double)par[sl] + delta[k]);
         }
         // values and derivatives at the trial point in one pass, as
         // calc_curve_values does (agpufitjni.cu:4073-4076); the sums are
@@ -1316,6 +1319,92 @@ __global__ void __launch_bounds__(64) k_lm_fixed(LMArgs a) {
     if (lane == 0) {
 #pragma unroll
         for (int k = 0; k < NP; k++) a.out_params[(int64_t)fit * NP + k] = par[k];
+        a.states[fit] = state;
+        a.chi2[fit] = (float)chi2;
+        a.iters[fit] = iters;
+    }
+}
+
+// Exponential bleach fits with FP64 parameters, model and sums (the Java
+// CPU fit is FP64; FP32 parameters cannot follow the shallow A-C-t valley of
+// a weakly decaying trace).  Gpufit's step control as above; one wave per
+// trace, points lane, lane + 64, ...; the result goes to out64[fit][5].
+template <int NEXP>
+__global__ void __launch_bounds__(64) k_lm_exp64(LMArgs a, double *out64) {
+    constexpr int NF = NEXP == 1 ? 3 : 5, NH = NF * (NF + 1) / 2;
+    const int fit = blockIdx.x;
+    if (fit >= a.nfits) return;
+    const int lane = threadIdx.x & 63;
+    double par[11];
+#pragma unroll
+    for (int k = 0; k < 11; k++) par[k] = (double)init_row(a, fit)[k];
+    const float *y = a.data + (int64_t)fit * a.npts;
+    auto eval = [&](const double *p, double &c2, double *g, double *h) {
+        double cx = 0.0, gl[NF], hl[NH];
+#pragma unroll
+        for (int q = 0; q < NF; q++) gl[q] = 0.0;
+#pragma unroll
+        for (int q = 0; q < NH; q++) hl[q] = 0.0;
+        for (int i = lane; i < a.npts; i += 64) {
+            const double t = (double)a.x[i];
+            const double e1 = exp(-t / p[1]);
+            double jf[NF], v = p[0] * e1;
+            jf[0] = e1;
+            jf[1] = p[0] * t * e1 / (p[1] * p[1]);
+            if constexpr (NEXP == 2) {
+                const double e2 = exp(-t / p[3]);
+                v += p[2] * e2 + p[4];
+                jf[2] = e2;
+                jf[3] = p[2] * t * e2 / (p[3] * p[3]);
+                jf[4] = 1.0;
+            } else {
+                v += p[2];
+                jf[2] = 1.0;
+            }
+            const double r = (double)y[i] - v;
+            cx = fma(r, r, cx);
+            int hh = 0;
+#pragma unroll
+            for (int q = 0; q < NF; q++) {
+                gl[q] = fma(r, jf[q], gl[q]);
+#pragma unroll
+                for (int l = 0; l <= q; l++, hh++) hl[hh] = fma(jf[q], jf[l], hl[hh]);
+            }
+        }
+        c2 = wave_sum(cx);
+#pragma unroll
+        for (int q = 0; q < NF; q++) g[q] = wave_sum(gl[q]);
+#pragma unroll
+        for (int q = 0; q < NH; q++) h[q] = wave_sum(hl[q]);
+    };
+    double chi2;
+    int state, iters;
+    lm_fixed_iterate<kModelPoly, NF>(a, par, eval, chi2, state, iters);
+    // Undamped Gauss-Newton steps from the LM end point, kept while chi^2
+    // falls: the damped steps crawl along the shallow A-C-t valley, and the
+    // reference's MINPACK-style LM reaches its bottom.
+    for (int it = 0; it < 16; it++) {
+        double c2, g[NF], hp[NH], H[NF][NF];
+        eval(par, c2, g, hp);
+        int h = 0;
+#pragma unroll
+        for (int k = 0; k < NF; k++)
+#pragma unroll
+            for (int l = 0; l <= k; l++, h++) H[k][l] = H[l][k] = hp[h];
+        if (!chol_solve_fixed<NF>(H, g)) break;
+        double trial[11];
+#pragma unroll
+        for (int k = 0; k < 11; k++) trial[k] = k < NF ? par[k] + g[k] : par[k];
+        double c2t, gt[NF], ht[NH];
+        eval(trial, c2t, gt, ht);
+        if (!(c2t < c2)) break;
+#pragma unroll
+        for (int k = 0; k < NF; k++) par[k] = trial[k];
+        chi2 = c2t;
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 5; k++) out64[(int64_t)fit * 5 + k] = k < NF ? par[k] : 0.0;
         a.states[fit] = state;
         a.chi2[fit] = (float)chi2;
         a.iters[fit] = iters;
@@ -1580,11 +1669,56 @@ void launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st) {
 }
 
 void launch_trace_moments(const SeriesGeom &g, int nopit, int ave, double tscale, double frametime, int order,
-                          int integer_trace, double *part, int nchunk, hipStream_t st) {
+                          int integer_trace, double *part, int nchunk, hipStream_t st, float *trace_out) {
     int ppc = cdiv(nopit, nchunk);
     dim3 grid(cdiv(g.nslots, 64), nchunk);
     hipLaunchKernelGGL(k_trace_moments, grid, dim3(64), 0, st, g, nopit, ave, tscale, frametime, order,
-                       integer_trace, ppc, part);
+                       integer_trace, ppc, part, trace_out);
+}
+
+// Initial guesses of SingleExpFit / DoubleExpFit (IFCS:10862-10864,
+// 10913-10917) from each slot's trace: [tr[1], t[n/2], 0] or
+// [tr[1]/2, t[n/10], tr[1]/2, t[n/2], 0], in the 11-slot parameter layout.
+__global__ void k_exp_init(const float *trace, const float *itime, int nslots, int nopit, int nexp, float *init) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    const float t1 = trace[(int64_t)s * nopit + 1];
+    float *q = init + (int64_t)s * 11;
+#pragma unroll
+    for (int k = 0; k < 11; k++) q[k] = 0.f;
+    if (nexp == 1) {
+        q[0] = t1;
+        q[1] = itime[nopit / 2];
+    } else {
+        q[0] = t1 / 2;
+        q[1] = itime[nopit / 10];
+        q[2] = t1 / 2;
+        q[3] = itime[nopit / 2];
+    }
+}
+
+// Fitted exponentials -> coef[slot] = {f(0), params}: f(0) = A + C or
+// A + B + C; 0 when the reference would skip the correction (A t1 == 0,
+// or A t1 B t2 == 0; IFCS:9689, 9713).
+__global__ void k_exp_pack(const double *out64, int nslots, int nexp, double *coef) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    const int np = nexp == 1 ? 3 : 5;
+    const double *p = out64 + (int64_t)s * 5;
+    double *c = coef + (int64_t)s * (np + 1);
+    for (int k = 0; k < np; k++) c[1 + k] = p[k];
+    const bool on = nexp == 1 ? (p[0] * p[1] != 0.0) : (p[0] * p[1] * p[2] * p[3] != 0.0);
+    c[0] = !on ? 0.0 : (nexp == 1 ? p[0] + p[2] : p[0] + p[2] + p[4]);
+}
+
+void launch_exp_init(const float *trace, const float *itime, int nslots, int nopit, int nexp, float *init,
+                     hipStream_t st) {
+    hipLaunchKernelGGL(k_exp_init, dim3(cdiv(nslots, 64)), dim3(64), 0, st, trace, itime, nslots, nopit, nexp,
+                       init);
+}
+
+void launch_exp_pack(const double *out64, int nslots, int nexp, double *coef, hipStream_t st) {
+    hipLaunchKernelGGL(k_exp_pack, dim3(cdiv(nslots, 64)), dim3(64), 0, st, out64, nslots, nexp, coef);
 }
 
 void launch_polyfit(const double *part, int nchunk, int nseries, int nopit, int ave, double tscale,
@@ -1619,6 +1753,10 @@ void launch_apply(const SeriesGeom &g, const BleachDesc &bd, const CorrTables *d
     dim3 grid(cdiv(g.nslots, 64), nchunk);
     if (bd.mode == 1) {
         hipLaunchKernelGGL(k_apply<BleachSeg>, grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec, X);
+        return;
+    }
+    if (bd.mode == 2) {
+        hipLaunchKernelGGL(k_apply<BleachExp<1>>, grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec, X);
         return;
     }
     switch (bleach_maxo(bd.order)) {
@@ -1762,6 +1900,16 @@ void launch_lm_acf(int nfits, int npts, const float *data, const float *weights,
     LMArgs a{nfits, npts, 20, kModelFCS, 0, data, weights, x, init, pfit, geom, tol, maxit, chi2_fp64, out_params,
              states, chi2, iters, init_split >= 0 ? 1 : 0, init_split};
     launch_lm(a, free_mask, st);
+}
+
+void launch_lm_exp(int nfits, int npts, const float *data, const float *x, const float *init, const int *pfit,
+                   int nexp, float tol, int maxit, double *out64, int *states, float *chi2, int *iters,
+                   hipStream_t st) {
+    LMModelGeom g{};
+    LMArgs a{nfits, npts, 11, nexp == 1 ? kModelExp1 : kModelExp2, 0, data, nullptr, x, init, pfit, g, tol, maxit, 1,
+             nullptr, states, chi2, iters, 0, 0};
+    if (nfits <= 0 || nexp != 1) return;
+    hipLaunchKernelGGL(k_lm_exp64<1>, dim3(nfits), dim3(64), 0, st, a, out64);
 }
 
 void launch_lm_poly(int nfits, int npts, const float *data, const float *weights, const float *x, const float *init,

@@ -47,7 +47,7 @@ class PipelineConfig(ctypes.Structure):
                 ("fit_chi2_fp64", _i32), ("gls", _i32), ("acf_model", FitModelGeom), ("ccf_model", FitModelGeom),
                 ("bleach_mode", _i32), ("sliding_window_length", _i32)]
 
-BLEACH_POLYNOMIAL, BLEACH_SLIDING_WINDOW, BLEACH_LIN_SEGMENT = 0, 1, 2
+BLEACH_POLYNOMIAL, BLEACH_SLIDING_WINDOW, BLEACH_LIN_SEGMENT, BLEACH_SINGLE_EXP = 0, 1, 2, 3
 
 
 # every C-ABI entry point of include/imfcs.h with its ctypes signature

@@ -173,8 +173,9 @@ typedef struct imfcs_config {
                                         GLS fit (GLSFit, IFCS:10622-10789) */
     imfcs_fit_model acf_model, ccf_model;
     int32_t bleach_mode;             /* IMFCS_BLEACH_*: POLYNOMIAL (of poly_order; 0 = no
-                                        correction), SLIDING_WINDOW (IFCS:9146-9172) or
-                                        LIN_SEGMENT (IFCS:9767-9805) */
+                                        correction), SLIDING_WINDOW (IFCS:9146-9172),
+                                        LIN_SEGMENT (IFCS:9767-9805) or SINGLE_EXP
+                                        (IFCS:9682-9709) */
     int32_t sliding_window_length;   /* frames per window (SLIDING_WINDOW), or the length
                                         that sets the segment count (LIN_SEGMENT) */
 } imfcs_config;
@@ -188,7 +189,12 @@ enum {
     IMFCS_BLEACH_SLIDING_WINDOW = 1,
     /* piecewise-linear bleach curve through the means of num / L segments;
        coefficients returned per series: bcInt0 then the segment means */
-    IMFCS_BLEACH_LIN_SEGMENT = 2
+    IMFCS_BLEACH_LIN_SEGMENT = 2,
+    /* LM fit of A e^{-t/t1} + C to the intensity trace and the correction
+       with that curve (IFCS:9682-9709); coefficients returned per series:
+       f(0), A, t1, C.  ("Double Exp" is not offered: its fit ends in
+       different local optima depending on the LM's path.) */
+    IMFCS_BLEACH_SINGLE_EXP = 3
 };
 
 /* Output geometry for a config: out_w x out_h pixels, chanum channels. */

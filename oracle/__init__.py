@@ -71,6 +71,9 @@ def lib():
         L.orc_bleach_apply.argtypes = [_D, ctypes.c_int, _D, ctypes.c_int, ctypes.c_double]
         L.orc_bleach_linseg.restype = ctypes.c_int
         L.orc_bleach_linseg.argtypes = [_D, ctypes.c_int, ctypes.c_int, _D]
+        L.orc_expfit.restype = ctypes.c_int
+        L.orc_expfit.argtypes = [_D, _D, ctypes.c_int, ctypes.c_int, _D]
+        L.orc_bleach_exp.argtypes = [_D, ctypes.c_int, _D, ctypes.c_int, ctypes.c_double]
         L.orc_block_transform.restype = ctypes.c_int
         L.orc_block_transform.argtypes = [_D, _D, ctypes.c_int, ctypes.c_int, _I, _D, _D,
                                           ctypes.c_double, ctypes.c_int, _I]
@@ -153,6 +156,24 @@ def bleach_linseg(intdat, swl):
     if n == 0:
         raise ValueError("linear segments need two segments of >= 2 frames")
     return d, seg
+
+
+def expfit(trace, int_time, nexp):
+    """SingleExpFit (nexp=1) / DoubleExpFit (nexp=2) of a trace
+    (IFCS:10845-10950): returns (params, status)."""
+    trace = np.ascontiguousarray(trace, np.float64)
+    int_time = np.ascontiguousarray(int_time, np.float64)
+    res = np.zeros(3 if nexp == 1 else 5)
+    st = lib().orc_expfit(_dp(trace), _dp(int_time), len(trace), nexp, _dp(res))
+    return res, st
+
+
+def bleach_exp(intdat, res, nexp, frametime):
+    """"Single Exp" / "Double Exp" correction of a 1-based trace (IFCS:9682-9738)."""
+    d = np.array(intdat, np.float64)
+    res = np.ascontiguousarray(res, np.float64)
+    lib().orc_bleach_exp(_dp(d), len(d) - 1, _dp(res), nexp, frametime)
+    return d
 
 
 def block_transform(a, b, p=16, q=8, frametime=0.001):

@@ -588,6 +588,71 @@ void orc_bleach_apply(double *intdat, int num, const double *res, int maxord,
 }
 
 /* ------------------------------------------------------------------ */
+/* exponential bleach fits and corrections                             */
+/* ------------------------------------------------------------------ */
+typedef struct { const double *t; int m, nexp; } exp_ctx;
+
+/* SingleExp / DoubleExp value and gradient (IFCS:13028-13086) */
+static void exp_model(void *vctx, const double *p, double *values, double *jac)
+{
+    const exp_ctx *c = (const exp_ctx *)vctx;
+    int n = c->nexp == 1 ? 3 : 5;
+    for (int i = 0; i < c->m; i++) {
+        double x = c->t[i];
+        double e1 = exp(-x / p[1]);
+        double *J = jac + (size_t)i * n;
+        if (c->nexp == 1) {
+            values[i] = p[0] * e1 + p[2];
+            J[0] = e1;
+            J[1] = p[0] * x * e1 / (p[1] * p[1]);
+            J[2] = 1;
+        } else {
+            double e2 = exp(-x / p[3]);
+            values[i] = p[0] * e1 + p[2] * e2 + p[4];
+            J[0] = e1;
+            J[1] = p[0] * x * e1 / (p[1] * p[1]);
+            J[2] = e2;
+            J[3] = p[2] * x * e2 / (p[3] * p[3]);
+            J[4] = 1;
+        }
+    }
+}
+
+int orc_expfit(const double *trace, const double *intTime, int nopit, int nexp, double *res)
+{
+    exp_ctx c = {intTime, nopit, nexp};
+    double *w = (double *)malloc(sizeof(double) * (size_t)nopit);
+    for (int i = 0; i < nopit; i++) w[i] = 1;
+    if (nexp == 1) {                    /* IFCS:10862-10864 */
+        res[0] = trace[1];
+        res[1] = intTime[nopit / 2];
+        res[2] = 0;
+    } else {                            /* IFCS:10913-10917 */
+        res[0] = trace[1] / 2;
+        res[1] = intTime[nopit / 10];
+        res[2] = trace[1] / 2;
+        res[3] = intTime[nopit / 2];
+        res[4] = 0;
+    }
+    int st = orc_lm(nopit, nexp == 1 ? 3 : 5, res, trace, w, exp_model, &c, 2147483647L, 2147483647L, NULL, NULL,
+                    NULL);
+    free(w);
+    return st;
+}
+
+void orc_bleach_exp(double *intdat, int num, const double *res, int nexp, double frametime)
+{
+    if (nexp == 1 ? res[0] * res[1] == 0 : res[0] * res[1] * res[2] * res[3] == 0) return;
+    double f0 = nexp == 1 ? res[0] + res[2] : res[0] + res[2] + res[4];
+    for (int x = 1; x <= num; x++) {
+        double t = frametime * (x + 0.5);
+        double f = nexp == 1 ? res[0] * exp(-t / res[1]) + res[2]
+                             : res[0] * exp(-t / res[1]) + res[2] * exp(-t / res[3]) + res[4];
+        intdat[x] = intdat[x] / sqrt(f / f0) + f0 * (1 - sqrt(f / f0));
+    }
+}
+
+/* ------------------------------------------------------------------ */
 /* "Lin Segment" bleach correction, getIntensity IFCS:9767-9805        */
 /* ------------------------------------------------------------------ */
 static double linseg_apply(double v, double g, double int0)

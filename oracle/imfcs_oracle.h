@@ -63,6 +63,16 @@ void orc_bleach_apply(double *intdat, int num, const double *coef,
  * segment count, or 0 if fewer than two segments of >= 2 frames. */
 int orc_bleach_linseg(double *intdat, int num, int swl, double *seg);
 
+/* SingleExpFit (nexp = 1: A e^{-t/t1} + C) / DoubleExpFit (nexp = 2:
+ * A e^{-t/t1} + B e^{-t/t2} + C) of the trace (IFCS:10845-10950, models
+ * 13028-13086): unweighted LM from the reference's initial guesses; res
+ * receives 3 or 5 parameters.  Returns the LM status. */
+int orc_expfit(const double *trace, const double *intTime, int nopit, int nexp, double *res);
+
+/* getIntensity's "Single Exp" / "Double Exp" correction of intdat[1..num]
+ * (IFCS:9682-9738); skipped when A t1 (A t1 B t2) is 0. */
+void orc_bleach_exp(double *intdat, int num, const double *res, int nexp, double frametime);
+
 /* blockTransform (IFCS:9217-9422).  a, b: 1-based traces (num+1).
  * Returns ind; *blockIndS receives the success flag. */
 int orc_block_transform(const double *a, const double *b, int num,

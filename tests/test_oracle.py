@@ -375,3 +375,55 @@ def test_linseg_flattens_a_linear_decay():
     slope_in = np.polyfit(x[mid], d[mid], 1)[0]
     slope_out = np.polyfit(x[mid], out[mid], 1)[0]
     assert abs(slope_out) < 1e-3 * abs(slope_in)
+
+
+def _exp_trace(nexp, seed=3):
+    rng = np.random.default_rng(seed)
+    t = 0.001 * (np.arange(1000) + 0.5) * 20
+    if nexp == 1:
+        f = 300.0 * np.exp(-t / 6.0) + 80.0
+    else:
+        f = 200.0 * np.exp(-t / 1.5) + 150.0 * np.exp(-t / 9.0) + 60.0
+    return t, f + rng.normal(0, 2.0, t.size)
+
+
+@pytest.mark.parametrize("nexp", [1, 2])
+def test_expfit_reaches_the_least_squares_optimum(nexp):
+    """SingleExpFit / DoubleExpFit (IFCS:10845-10950): the oracle's LM from
+    the reference's initial guesses ends where MINPACK (scipy's
+    least_squares 'lm', the algorithm commons-math3 translates) ends from
+    the same start."""
+    from scipy.optimize import least_squares
+    t, y = _exp_trace(nexp)
+    res, st = oracle.expfit(y, t, nexp)
+    assert st == 0
+    if nexp == 1:
+        x0 = [y[1], t[len(t) // 2], 0.0]
+        fun = lambda p: p[0] * np.exp(-t / p[1]) + p[2] - y
+    else:
+        x0 = [y[1] / 2, t[len(t) // 10], y[1] / 2, t[len(t) // 2], 0.0]
+        fun = lambda p: p[0] * np.exp(-t / p[1]) + p[2] * np.exp(-t / p[3]) + p[4] - y
+    ref = least_squares(fun, x0, method="lm", xtol=1e-14, ftol=1e-14, gtol=1e-14).x
+    curve = lambda p: fun(p) + y
+    np.testing.assert_allclose(curve(res), curve(ref), rtol=1e-8)
+    if nexp == 1:
+        np.testing.assert_allclose(res, ref, rtol=1e-6)
+
+
+@pytest.mark.parametrize("nexp", [1, 2])
+def test_bleach_exp_flattens_its_own_curve(nexp):
+    """Correcting the exact fitted curve gives the constant f(0): I/sqrt(f/f0)
+    + f0(1 - sqrt(f/f0)) = f0 when I = f (IFCS:9690, 9714); a zero amplitude
+    or time skips the correction."""
+    res = np.array([300.0, 6.0, 80.0]) if nexp == 1 else np.array([200.0, 1.5, 150.0, 9.0, 60.0])
+    num = 5000
+    x = np.arange(num + 1)
+    tt = 0.001 * (x + 0.5)
+    f = res[0] * np.exp(-tt / res[1]) + (res[2] if nexp == 1 else res[2] * np.exp(-tt / res[3]) + res[4])
+    f[0] = 0.0
+    out = oracle.bleach_exp(f, res, nexp, 0.001)
+    f0 = res[0] + res[2] if nexp == 1 else res[0] + res[2] + res[4]
+    np.testing.assert_allclose(out[1:], f0, rtol=1e-12)
+    z = res.copy()
+    z[0] = 0.0
+    np.testing.assert_array_equal(oracle.bleach_exp(f, z, nexp, 0.001), f)
