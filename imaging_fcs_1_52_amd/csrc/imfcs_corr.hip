@@ -52,6 +52,11 @@ struct Tile {
     float l0[kTileF][kNS];
     float l1[kTileF / 2][kNS];
     float l2[kTileF / 4][kNS];
+    // sums of each 16-frame block's staged values on levels 0, 1 and 2
+    // (16, 8 and 4 values): the group sums of the role waves
+    float gs0[kTileF / 16][kNS];
+    float gs1[kTileF / 16][kNS];
+    float gs2[kTileF / 16][kNS];
 };
 
 // Shift of series s at level c (2^c frames per value).
@@ -87,6 +92,8 @@ struct Loader {
             const float x = xq[(uint32_t)min(k, max(T - 1 - fbase, 0)) * (uint32_t)ld + (uint32_t)slot];
             xr[k] = (f < T) ? x : 0.f;
         }
+        double g0 = 0.0;
+        float g1 = 0.f, g2 = 0.f;
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             double v[8];
@@ -96,21 +103,29 @@ struct Loader {
                 tl.l0[16 * fb + 8 * h + k][sl] = xr[8 * h + k];
             }
 #pragma unroll
-            for (int k = 0; k < 4; k++)
-                tl.l1[8 * fb + 4 * h + k][sl] = (fbase + 8 * h + 2 * k + 1 < T) ? (float)(v[2 * k] + v[2 * k + 1]) : 0.f;
+            for (int k = 0; k < 4; k++) {
+                const float u = (fbase + 8 * h + 2 * k + 1 < T) ? (float)(v[2 * k] + v[2 * k + 1]) : 0.f;
+                tl.l1[8 * fb + 4 * h + k][sl] = u;
+                g1 += u;
+            }
 #pragma unroll
-            for (int k = 0; k < 2; k++)
-                tl.l2[4 * fb + 2 * h + k][sl] =
-                    (fbase + 8 * h + 4 * k + 3 < T) ? (float)((v[4 * k] + v[4 * k + 1]) + (v[4 * k + 2] + v[4 * k + 3]))
-                                                    : 0.f;
+            for (int k = 0; k < 2; k++) {
+                const float u = (fbase + 8 * h + 4 * k + 3 < T)
+                                    ? (float)((v[4 * k] + v[4 * k + 1]) + (v[4 * k + 2] + v[4 * k + 3]))
+                                    : 0.f;
+                tl.l2[4 * fb + 2 * h + k][sl] = u;
+                g2 += u;
+            }
+            const double s8 = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+            g0 += s8;
             if (own && fbase < chunk_end) {
                 int y3 = fbase / 8 + h;
-                if (y3 < A.ctp->numbin[3]) {
-                    double s8 = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
-                    A.l3[(int64_t)y3 * A.g.nseries + s] = (float)s8;
-                }
+                if (y3 < A.ctp->numbin[3]) A.l3[(int64_t)y3 * A.g.nseries + s] = (float)s8;
             }
         }
+        tl.gs0[fb][sl] = (float)g0;
+        tl.gs1[fb][sl] = g1;
+        tl.gs2[fb][sl] = g2;
     }
 };
 
@@ -119,9 +134,17 @@ struct Loader {
 // ~1e-6 relative; chunks are combined in FP64 by k_corr_finalize.
 template <int NL>
 struct LagAcc {
-    float sab[NL], sb[NL], alpha[NL], beta[NL];
+    float sab[NL], head[NL], alpha[NL], beta[NL];
     float a1[NL], a2[NL], cross[NL];
 };
+
+// Sum of the 16 values of group q of a tile on level LEVEL, series s.
+template <int LEVEL>
+__device__ __forceinline__ float group_sum(const Tile &tl, int q, int s) {
+    if (LEVEL == 0) return tl.gs0[q][s];
+    if (LEVEL == 1) return tl.gs1[2 * q][s] + tl.gs1[2 * q + 1][s];
+    return (tl.gs2[0][s] + tl.gs2[1][s]) + (tl.gs2[2][s] + tl.gs2[3][s]);
+}
 
 // One role: NL delays starting at DMIN on level LEVEL (group size 16).
 template <int NL, int DMIN, int LEVEL, bool CCF>
@@ -137,7 +160,10 @@ __device__ void run_role(const MainArgs &A, Tile &tl, Loader &ld, int lane, int 
     const int sb = ccf ? A.g.npix + sa : sa;
     const int asl = lane, bsl = ccf ? 64 + lane : lane;
     LagAcc<NL> acc;
-    float sa_blk = 0.f;
+    // block sums of a and b over the current blocking block; the b sum at
+    // delay d is sb_blk + sum_{k<d} b[ye+k] - sum_{k<d} b[ys+k] for the block
+    // [ys, ye), the head term kept per delay from the previous block end
+    float sa_blk = 0.f, sb_blk = 0.f;
     int B = 1, pnum = 0;
     {
         const double *Fa = A.F + (int64_t)A.g.slot_of[sa] * ct.npos, *Fb = A.F + (int64_t)A.g.slot_of[sb] * ct.npos;
@@ -150,7 +176,7 @@ __device__ void run_role(const MainArgs &A, Tile &tl, Loader &ld, int lane, int 
             double n = (double)(ct.numbin[LEVEL] - ct.del[x]);
             acc.alpha[i] = (float)(Fa[ct.pos_a_end[x]] / n - mua);
             acc.beta[i] = (float)((Fb[ct.pos_b_end[x]] - Fb[ct.pos_b_beg[x]]) / n - mub);
-            acc.sab[i] = acc.sb[i] = 0.f;
+            acc.sab[i] = acc.head[i] = 0.f;
             acc.a1[i] = acc.a2[i] = acc.cross[i] = 0.f;
         }
         int indp = live ? A.ind[pair] : ct.q - 1;
@@ -164,6 +190,7 @@ __device__ void run_role(const MainArgs &A, Tile &tl, Loader &ld, int lane, int 
     const int yend = last_chunk ? ct.numbin[LEVEL] : (cend >> LEVEL);
     const int ntiles = (cend - cstart + kTileF - 1) / kTileF + 2;
     float w[32], ga_cur[CCF ? 16 : 1], ga_nxt[CCF ? 16 : 1];
+    float gsa_cur = 0.f, gsa_nxt = 0.f, gsb_cur = 0.f, gsb_nxt = 0.f;
 #pragma unroll
     for (int k = 0; k < 32; k++) w[k] = 0.f;
     if (CCF) {
@@ -183,42 +210,61 @@ __device__ void run_role(const MainArgs &A, Tile &tl, Loader &ld, int lane, int 
                 w[k] = w[k + 16];
                 if (CCF) ga_cur[k] = ga_nxt[k];
             }
+            gsb_cur = gsb_nxt;
+            if (CCF) gsa_cur = gsa_nxt;
 #pragma unroll
             for (int k = 0; k < 16; k++) {
                 const float *row = (LEVEL == 0) ? tl.l0[16 * q + k] : (LEVEL == 1) ? tl.l1[16 * q + k] : tl.l2[16 * q + k];
                 w[16 + k] = row[bsl];
                 if (CCF) ga_nxt[k] = row[asl];
             }
+            gsb_nxt = group_sum<LEVEL>(tl, q, bsl);
+            if (CCF) gsa_nxt = group_sum<LEVEL>(tl, q, asl);
             ynext += 16;
             const int y0 = ynext - 32;   // group whose window is complete
             if (live && y0 >= ystart && y0 < yend) {
-                float gs = 0.f;
                 const float *ga = CCF ? ga_cur : w;
+                if (y0 == ystart) {
+                    // the chunk starts a block: head terms sum_{k<d} b[ystart+k]
+                    float h = 0.f;
 #pragma unroll
-                for (int k = 0; k < 16; k++) gs += ga[k];
-                sa_blk += gs;
-                float tsum = 0.f;
-#pragma unroll
-                for (int k = 0; k < 16; k++) tsum += w[k + DMIN];
+                    for (int k = 0; k < DMIN + NL; k++) {
+                        if (k >= DMIN) acc.head[k - DMIN] = h;
+                        if (k + 1 < DMIN + NL) h += w[k];
+                    }
+                }
+                sa_blk += CCF ? gsa_cur : gsb_cur;
+                if (CCF) sb_blk += gsb_cur;
 #pragma unroll
                 for (int i = 0; i < NL; i++) {
                     const int d = DMIN + i;
-                    if (i > 0) tsum = tsum - w[d - 1] + w[15 + d];
-                    float s0 = 0.f, s1 = 0.f;
+                    float s0 = acc.sab[i], s1 = 0.f;
 #pragma unroll
                     for (int k = 0; k < 16; k += 2) {
                         s0 = fmaf(ga[k], w[k + d], s0);
                         s1 = fmaf(ga[k + 1], w[k + 1 + d], s1);
                     }
-                    acc.sab[i] += s0 + s1;
-                    acc.sb[i] += tsum;
+                    acc.sab[i] = s0 + s1;
                 }
                 const int ye = y0 + 16;
                 if ((ye & (B - 1)) == 0) {
                     const bool inblk = (ye / B - 1) < pnum;
+                    // tail terms sum_{k<d} b[ye+k]: the next group's values
+                    float tail[NL];
+                    {
+                        float t = 0.f;
+#pragma unroll
+                        for (int k = 0; k < DMIN + NL; k++) {
+                            if (k >= DMIN) tail[k - DMIN] = t;
+                            if (k + 1 < DMIN + NL) t += w[16 + k];
+                        }
+                    }
+                    const float sbb = CCF ? sb_blk : sa_blk;
 #pragma unroll
                     for (int i = 0; i < NL; i++) {
-                        float S = acc.sab[i] - acc.beta[i] * sa_blk - acc.alpha[i] * acc.sb[i] +
+                        const float sb = sbb + (tail[i] - acc.head[i]);
+                        acc.head[i] = tail[i];
+                        float S = acc.sab[i] - acc.beta[i] * sa_blk - acc.alpha[i] * sb +
                                   (float)B * acc.alpha[i] * acc.beta[i];
                         if (inblk) {
                             acc.a1[i] += S;
@@ -229,9 +275,9 @@ __device__ void run_role(const MainArgs &A, Tile &tl, Loader &ld, int lane, int 
                         }
                         acc.cross[i] += acc.sab[i];
                         acc.sab[i] = 0.f;
-                        acc.sb[i] = 0.f;
                     }
                     sa_blk = 0.f;
+                    sb_blk = 0.f;
                 }
             }
         }
