@@ -367,7 +367,8 @@ namespace {
 // or a power of two below 64) so that the 64 lanes of a wave read whole,
 // aligned row segments of the stack: unaligned uint16 rows cost ~10x in
 // load throughput.  Padding slots repeat the last window of their row.
-// buf = slot_of[nseries] followed by the window origins [nslots][2].
+// buf = slot_of[nseries], the window origins [nslots][2], then the owner
+// flags [nseries] (first series of each slot).
 int build_slots(const SeriesGeom &g, std::vector<int32_t> &buf) {
     const bool ccf = g.nseries > g.npix;
     const bool on_grid = ccf && (g.cfx % g.pbx == 0) && (g.cfy % g.pby == 0);
@@ -378,7 +379,7 @@ int build_slots(const SeriesGeom &g, std::vector<int32_t> &buf) {
     if (sw > 64) swp = (sw + 63) / 64 * 64;
     const int ngrid = (ccf && !on_grid) ? 2 : 1;
     const int nslots = ngrid * sh * swp;
-    buf.assign((size_t)g.nseries + 2 * (size_t)nslots, 0);
+    buf.assign(2 * (size_t)g.nseries + 2 * (size_t)nslots, 0);
     int32_t *org = buf.data() + g.nseries;
     for (int gi = 0; gi < ngrid; gi++)
         for (int y = 0; y < sh; y++)
@@ -392,6 +393,12 @@ int build_slots(const SeriesGeom &g, std::vector<int32_t> &buf) {
         const int x = k % g.ow, y = k / g.ow;
         buf[k] = y * swp + x;
         if (ccf) buf[g.npix + k] = on_grid ? (y + dy) * swp + (x + dx) : (sh + y) * swp + x;
+    }
+    std::vector<char> seen(nslots, 0);
+    int32_t *own = org + 2 * (size_t)nslots;
+    for (int s = 0; s < g.nseries; s++) {
+        own[s] = !seen[buf[s]];
+        seen[buf[s]] = 1;
     }
     return nslots;
 }
@@ -440,6 +447,7 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTabl
     HIP_TRY(ctx->up_slots.put(ctx->slots, ctx->host_slots.data(), sizeof(int32_t) * ctx->host_slots.size(), st));
     g.slot_of = ctx->slots.as<int32_t>();
     g.slot_org = ctx->slots.as<int32_t>() + g.nseries;
+    g.slot_own = g.slot_org + 2 * (size_t)g.nslots;
     ctx->nslots = g.nslots;
     const int ns = g.nslots, np = ea.pd.npairs, n = ea.order + 1;
     ctx->npairs = np;
@@ -578,8 +586,8 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTabl
         HIP_TRY(ctx->hpart.ensure(sizeof(double) * (size_t)nch * np * (ct.chanum - 33) * 3));
         size_t nlv = 0;
         for (int c = 4; c < ct.nlevels; c++) nlv += (size_t)ct.numbin[c];
-        HIP_TRY(ctx->l3.ensure(sizeof(float) * (size_t)ct.numbin[3] * g.nseries));
-        HIP_TRY(ctx->lvd.ensure(sizeof(double) * std::max<size_t>(nlv, 1) * g.nseries));
+        HIP_TRY(ctx->l3.ensure(sizeof(float) * (size_t)ct.numbin[3] * g.nslots));
+        HIP_TRY(ctx->lvd.ensure(sizeof(double) * std::max<size_t>(nlv, 1) * g.nslots));
         launch_correlate_fast(g, bd, ct, dct, ea.pd, ctx->F.as<double>(), ctx->X.as<float>(), ctx->ind.as<int32_t>(),
                               ctx->cpart.as<float>(), ctx->hpart.as<double>(), ctx->l3.as<float>(),
                               ctx->lvd.as<double>(), clen,

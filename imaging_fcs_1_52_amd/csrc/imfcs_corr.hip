@@ -39,8 +39,8 @@ struct MainArgs {
     int chunk_len, nchunk;
     int has_acf, has_ccf;
     float *part;    // [chunk][pair][kMainChan][3] : cross, A1, A2
-    float *l3;      // level-3 values [numbin_3][nseries]
-    double *lvd;    // levels >= 4, level c at lvd + loff[c]: [numbin_c][nseries]
+    float *l3;      // level-3 values [numbin_3][nslots]
+    double *lvd;    // levels >= 4, level c at lvd + loff[c]: [numbin_c][nslots]
     int64_t loff[kMaxLevel];
     // GLS: every block sum of the blocked variance, [pair][block][channel-1]
     // (channels 1..chanum-1); null when no covariance is requested
@@ -66,7 +66,7 @@ __device__ __forceinline__ double level_shift(const MainArgs &A, int s, int c) {
 
 // Per-lane loader: one series, 16 consecutive frames of every tile.
 struct Loader {
-    bool active;
+    bool active, own;   // own: this series writes its slot's level-3 values
     int sl, s, fb, slot;
     __device__ void init(const MainArgs &A, Tile &tl, int wave, int lane, int p0, int nloader_waves) {
         (void)tl;
@@ -78,8 +78,9 @@ struct Loader {
         if (p >= A.g.npix) p = A.g.npix - 1;   // padding lanes duplicate the last pixel
         s = half ? A.g.npix + p : p;
         slot = A.g.slot_of[s];
+        own = A.g.slot_own[s] != 0;
     }
-    __device__ void load(const MainArgs &A, Tile &tl, int f0, int chunk_end, bool own) {
+    __device__ void load(const MainArgs &A, Tile &tl, int f0, int chunk_end) {
         if (!active) return;
         const int T = A.g.T;
         const int fbase = f0 + 16 * fb;
@@ -120,7 +121,7 @@ struct Loader {
             g0 += s8;
             if (own && fbase < chunk_end) {
                 int y3 = fbase / 8 + h;
-                if (y3 < A.ctp->numbin[3]) A.l3[(int64_t)y3 * A.g.nseries + s] = (float)s8;
+                if (y3 < A.ctp->numbin[3]) A.l3[(int64_t)y3 * A.g.nslots + slot] = (float)s8;
             }
         }
         tl.gs0[fb][sl] = (float)g0;
@@ -200,7 +201,7 @@ __device__ void run_role(const MainArgs &A, Tile &tl, Loader &ld, int lane, int 
     int ynext = ystart;   // level index of the next value to load
     for (int t = 0; t < ntiles; t++) {
         const int f0 = cstart + t * kTileF;
-        ld.load(A, tl, f0, cend, true);
+        ld.load(A, tl, f0, cend);
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < GPT; q++) {
@@ -330,9 +331,9 @@ template <bool L3>
 struct L3Ctx {
     const float *l3;    // level 3 (FP32 sums of 8 frames)
     const double *lv;   // this level's values when c >= 4 (FP64)
-    int nseries, sa, sb, c, numbin;
+    int nslots, sa, sb, c, numbin;   // sa, sb: slots of the pair's series
     __device__ __forceinline__ double val(int s, int y) const {
-        const int64_t i = (int64_t)min(y, numbin - 1) * nseries + s;
+        const int64_t i = (int64_t)min(y, numbin - 1) * nslots + s;
         const double v = L3 ? (double)l3[i] : lv[i];
         return (y < numbin) ? v : 0.0;
     }
@@ -340,7 +341,7 @@ struct L3Ctx {
 
 // Levels c0+1 .. c0+4 from level c0 by the pairwise-sum rebinning
 // (IFCS:9477-9478): v_c[y] = v_{c-1}[2y] + v_{c-1}[2y+1], in FP64.  One
-// thread per (series, 16 consecutive level-c0 values): it reads the 16
+// thread per (slot, 16 consecutive level-c0 values): it reads the 16
 // values once and writes the 8 + 4 + 2 + 1 values they produce on the next
 // four levels (levels past nlev, and values past each level's numbin, are
 // not written).
@@ -350,19 +351,19 @@ struct RebinArgs {
     double *dst[4];        // lvd + loff[c0 + 1 + k]
     int numbin_src;
     int numbin[4];         // numbin of levels c0+1 .. c0+4 (0 = level absent)
-    int nseries;
+    int nslots;
 };
 
 __global__ void k_corr_rebin4(RebinArgs R) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t ngrp = (R.numbin_src + 15) / 16;
-    if (i >= ngrp * R.nseries) return;
-    const int64_t grp = i / R.nseries, s = i - grp * R.nseries;
+    if (i >= ngrp * R.nslots) return;
+    const int64_t grp = i / R.nslots, s = i - grp * R.nslots;
     double v[16];
 #pragma unroll
     for (int k = 0; k < 16; k++) {
         const int64_t y = min(grp * 16 + k, (int64_t)R.numbin_src - 1);
-        v[k] = R.src32 ? (double)R.src32[y * R.nseries + s] : R.src64[y * R.nseries + s];
+        v[k] = R.src32 ? (double)R.src32[y * R.nslots + s] : R.src64[y * R.nslots + s];
     }
     int n = 16;
 #pragma unroll
@@ -375,7 +376,7 @@ __global__ void k_corr_rebin4(RebinArgs R) {
 #pragma unroll
             for (int k = 0; k < 8; k++) {
                 const int64_t y = grp * n + k;
-                if (k < n && y < R.numbin[lv]) R.dst[lv][y * R.nseries + s] = v[k];
+                if (k < n && y < R.numbin[lv]) R.dst[lv][y * R.nslots + s] = v[k];
             }
         }
     }
@@ -467,7 +468,7 @@ __device__ __forceinline__ void high_dispatch(const MainArgs &A, int c, int G, i
                                               double *a1, double *a2, float *grow, int ld) {
     const CorrTables &ct = *A.ctp;
     if (c == 3) {
-        const L3Ctx<true> L{A.l3, nullptr, A.g.nseries, sa, sb, c, ct.numbin[c]};
+        const L3Ctx<true> L{A.l3, nullptr, A.g.nslots, sa, sb, c, ct.numbin[c]};
         switch (G) {
             case 16: high_level<16, true, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
             case 8: high_level<8, true, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
@@ -476,7 +477,7 @@ __device__ __forceinline__ void high_dispatch(const MainArgs &A, int c, int G, i
             default: high_level<1, true, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
         }
     } else {
-        const L3Ctx<false> L{nullptr, A.lvd + A.loff[c], A.g.nseries, sa, sb, c, ct.numbin[c]};
+        const L3Ctx<false> L{nullptr, A.lvd + A.loff[c], A.g.nslots, sa, sb, c, ct.numbin[c]};
         switch (G) {
             case 16: high_level<16, false, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
             case 8: high_level<8, false, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
@@ -502,7 +503,8 @@ __global__ void __launch_bounds__(64, 2) k_corr_high(MainArgs A, double *hpart /
     if (p >= A.pd.npix || pidx >= A.pd.npairs || c >= ct.nlevels) return;
     int sa, sb;
     pair_series(A.pd, pidx, sa, sb);
-    const double *Fa = A.F + (int64_t)A.g.slot_of[sa] * ct.npos, *Fb = A.F + (int64_t)A.g.slot_of[sb] * ct.npos;
+    const int ka = A.g.slot_of[sa], kb = A.g.slot_of[sb];   // coarse levels are stored per slot
+    const double *Fa = A.F + (int64_t)ka * ct.npos, *Fb = A.F + (int64_t)kb * ct.npos;
     const double mua = level_shift(A, sa, c);
     const double mub = level_shift(A, sb, c);
     const int x0 = ct.first_chan[c];
@@ -525,9 +527,9 @@ __global__ void __launch_bounds__(64, 2) k_corr_high(MainArgs A, double *hpart /
     const int ld = ct.chanum - 1;
     // (wave-uniform: a workgroup holds only ACF or only CCF pairs)
     if (acf_block && A.pd.acf_first)
-        high_dispatch<true>(A, c, G, sa, sb, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld);
+        high_dispatch<true>(A, c, G, ka, kb, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld);
     else
-        high_dispatch<false>(A, c, G, sa, sb, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld);
+        high_dispatch<false>(A, c, G, ka, kb, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld);
     double *out = hpart + (((int64_t)chunk * A.pd.npairs + pidx) * nhigh + (x0 - kMainChan)) * 3;
     for (int i = 0; i < 8; i++) {
         out[i * 3 + 0] = cross[i];
@@ -620,7 +622,7 @@ void launch_correlate_fast(const SeriesGeom &g, const BleachDesc &bd, const Corr
     int64_t off = 0;
     for (int c = 0; c < kMaxLevel; c++) {
         A.loff[c] = off;
-        if (c >= 4 && c < ct.nlevels) off += (int64_t)ct.numbin[c] * g.nseries;
+        if (c >= 4 && c < ct.nlevels) off += (int64_t)ct.numbin[c] * g.nslots;
     }
     const int tiles = (g.npix + 63) / 64;
     const int threads = pd.has_ccf ? 640 : 320;
@@ -637,8 +639,8 @@ void launch_correlate_fast(const SeriesGeom &g, const BleachDesc &bd, const Corr
             R.dst[k] = c < ct.nlevels ? lvd + A.loff[c] : nullptr;
             R.numbin[k] = c < ct.nlevels ? ct.numbin[c] : 0;
         }
-        R.nseries = g.nseries;
-        const int64_t n = (int64_t)((ct.numbin[c0] + 15) / 16) * g.nseries;
+        R.nslots = g.nslots;
+        const int64_t n = (int64_t)((ct.numbin[c0] + 15) / 16) * g.nslots;
         hipLaunchKernelGGL(k_corr_rebin4, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, R);
     }
     const int nhigh = ct.chanum - kMainChan;

@@ -36,6 +36,9 @@ struct SeriesGeom {
     int32_t nslots;
     const int32_t *slot_of;    // [nseries] device
     const int32_t *slot_org;   // [nslots][2] device
+    // [nseries] device: 1 for the first series of each slot, which writes
+    // the slot's coarse correlator levels (stored once per slot)
+    const int32_t *slot_own;
     // 2 or 4: every window row starts on a multiple of bx elements (uint16
     // source), so a row of bx = vec pixels is one 4- or 8-byte load; else 0
     int32_t vec;
