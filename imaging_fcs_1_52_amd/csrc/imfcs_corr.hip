@@ -2,10 +2,13 @@
 // LDS-staged and wave-specialised for CDNA4.
 //
 // k_corr_main: one workgroup per (64 output pixels, time chunk).  Loader
-//   waves read the bleach-corrected FP32 deviations X of the a and b series
-//   and stage 64 frames in LDS, together with the pair-summed level-1 and
-//   level-2 values; the level-3 (8-frame) sums go to global memory for
-//   k_corr_high.  Five role waves per correlation type then consume the tile:
+//   waves (one per staged half: a series, partner b series) read the
+//   bleach-corrected FP32 deviations X and stage 64 frames in LDS, together
+//   with the pair-summed level-1 and level-2 values and 16-frame group sums;
+//   the level-3 (8-frame) sums go to global memory for k_corr_high.  The
+//   LDS tile is double-buffered: the loaders fetch tile t+2 and store tile
+//   t+1 while the role waves work on tile t (one barrier per tile).  Five
+//   role waves per correlation type consume the tile:
 //     role 0..2: level 0, delays 0-5 / 6-11 / 12-16
 //     role 3   : level 1, delays 9-16
 //     role 4   : level 2, delays 9-16
@@ -64,71 +67,109 @@ __device__ __forceinline__ double level_shift(const MainArgs &A, int s, int c) {
     return slot_mu(A.bd.coef, A.bd.order, A.g.slot_of[s]) * (double)(1 << c);
 }
 
-// Per-lane loader: one series, 16 consecutive frames of every tile.
-struct Loader {
-    bool active, own;   // own: this series writes its slot's level-3 values
-    int sl, s, fb, slot;
-    __device__ void init(const MainArgs &A, Tile &tl, int wave, int lane, int p0, int nloader_waves) {
-        (void)tl;
-        active = wave < nloader_waves;
-        int half = (nloader_waves == 8) ? (wave & 1) : 0;
-        fb = (nloader_waves == 8) ? (wave >> 1) : wave;
+// Loader wave: one staged series per lane, all 64 frames of a tile.  The
+// next tile is fetched into registers while the role waves work on the
+// current one, then written to the other LDS buffer.
+struct TileLoader {
+    bool own;   // this series writes its slot's level-3 values
+    int sl, slot;
+    int nb3;    // level-3 values (held in a register: the level-3 stores
+                // would otherwise force a reload and a wait on every store)
+    float xr[kTileF];
+    __device__ void init(const MainArgs &A, int half, int lane, int p0) {
         sl = half * 64 + lane;
         int p = p0 + lane;
         if (p >= A.g.npix) p = A.g.npix - 1;   // padding lanes duplicate the last pixel
-        s = half ? A.g.npix + p : p;
+        const int s = half ? A.g.npix + p : p;
         slot = A.g.slot_of[s];
         own = A.g.slot_own[s] != 0;
+        nb3 = A.ctp->numbin[3];
     }
-    __device__ void load(const MainArgs &A, Tile &tl, int f0, int chunk_end) {
-        if (!active) return;
+    // frames f0 .. f0+63 (rows past the end of the series read the last
+    // row; store() zeroes them).  Nothing consumes the values before the
+    // next tile's store, so all 64 loads are in flight together.
+    __device__ void fetch(const MainArgs &A, int f0) {
         const int T = A.g.T;
-        const int fbase = f0 + 16 * fb;
         const int64_t ld = A.g.nslots;
-        float xr[16];
-        const float *xq = A.X + (int64_t)min(fbase, T - 1) * ld;   // wave-uniform
 #pragma unroll
-        for (int k = 0; k < 16; k++) {
-            const int f = fbase + k;
-            const float x = xq[(uint32_t)min(k, max(T - 1 - fbase, 0)) * (uint32_t)ld + (uint32_t)slot];
-            xr[k] = (f < T) ? x : 0.f;
+        for (int k = 0; k < kTileF; k++) {
+            const float *row = A.X + (int64_t)min(f0 + k, T - 1) * ld;   // wave-uniform
+            xr[k] = row[(uint32_t)slot];
         }
-        double g0 = 0.0;
-        float g1 = 0.f, g2 = 0.f;
+    }
+    // level 0..2 rows and group sums into LDS, level-3 sums of the chunk's
+    // own frames to global memory
+    __device__ void store(const MainArgs &A, Tile &tl, int f0, int chunk_end) const {
+        const int T = A.g.T;
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-            double v[8];
+        for (int fb = 0; fb < kTileF / 16; fb++) {
+            const int fbase = f0 + 16 * fb;
+            double g0 = 0.0;
+            float g1 = 0.f, g2 = 0.f;
 #pragma unroll
-            for (int k = 0; k < 8; k++) {
-                v[k] = (double)xr[8 * h + k];
-                tl.l0[16 * fb + 8 * h + k][sl] = xr[8 * h + k];
-            }
+            for (int h = 0; h < 2; h++) {
+                double v[8];
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const float u = (fbase + 8 * h + 2 * k + 1 < T) ? (float)(v[2 * k] + v[2 * k + 1]) : 0.f;
-                tl.l1[8 * fb + 4 * h + k][sl] = u;
-                g1 += u;
-            }
+                for (int k = 0; k < 8; k++) {
+                    const float x = (fbase + 8 * h + k < T) ? xr[16 * fb + 8 * h + k] : 0.f;
+                    v[k] = (double)x;
+                    tl.l0[16 * fb + 8 * h + k][sl] = x;
+                }
 #pragma unroll
-            for (int k = 0; k < 2; k++) {
-                const float u = (fbase + 8 * h + 4 * k + 3 < T)
-                                    ? (float)((v[4 * k] + v[4 * k + 1]) + (v[4 * k + 2] + v[4 * k + 3]))
-                                    : 0.f;
-                tl.l2[4 * fb + 2 * h + k][sl] = u;
-                g2 += u;
+                for (int k = 0; k < 4; k++) {
+                    const float u = (fbase + 8 * h + 2 * k + 1 < T) ? (float)(v[2 * k] + v[2 * k + 1]) : 0.f;
+                    tl.l1[8 * fb + 4 * h + k][sl] = u;
+                    g1 += u;
+                }
+#pragma unroll
+                for (int k = 0; k < 2; k++) {
+                    const float u = (fbase + 8 * h + 4 * k + 3 < T)
+                                        ? (float)((v[4 * k] + v[4 * k + 1]) + (v[4 * k + 2] + v[4 * k + 3]))
+                                        : 0.f;
+                    tl.l2[4 * fb + 2 * h + k][sl] = u;
+                    g2 += u;
+                }
+                const double s8 = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+                g0 += s8;
+                if (own && fbase < chunk_end) {
+                    const int y3 = fbase / 8 + h;
+                    if (y3 < nb3) A.l3[(int64_t)y3 * A.g.nslots + slot] = (float)s8;
+                }
             }
-            const double s8 = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
-            g0 += s8;
-            if (own && fbase < chunk_end) {
-                int y3 = fbase / 8 + h;
-                if (y3 < A.ctp->numbin[3]) A.l3[(int64_t)y3 * A.g.nslots + slot] = (float)s8;
-            }
+            tl.gs0[fb][sl] = (float)g0;
+            tl.gs1[fb][sl] = g1;
+            tl.gs2[fb][sl] = g2;
         }
-        tl.gs0[fb][sl] = (float)g0;
-        tl.gs1[fb][sl] = g1;
-        tl.gs2[fb][sl] = g2;
     }
 };
+
+// Tiles of one workgroup's chunk: its own frames plus two tiles beyond, so
+// that the lag windows of the last groups are complete.
+__device__ __forceinline__ int chunk_tiles(const MainArgs &A, int chunk, int &cstart, int &cend) {
+    cstart = chunk * A.chunk_len;
+    cend = min(A.g.T, cstart + A.chunk_len);
+    return (cend - cstart + kTileF - 1) / kTileF + 2;
+}
+
+// Loader waves: tile t+1 goes to LDS buffer (t+1)&1 while the roles read
+// buffer t&1; one barrier per tile (the same count as the role waves).
+__device__ void run_loader(const MainArgs &A, Tile *tl2, int half, int lane, int p0, int chunk) {
+    TileLoader L;
+    L.init(A, half, lane, p0);
+    int cstart, cend;
+    const int ntiles = chunk_tiles(A, chunk, cstart, cend);
+    L.fetch(A, cstart);
+    L.store(A, tl2[0], cstart, cend);
+    L.fetch(A, cstart + kTileF);   // ntiles >= 3
+    __syncthreads();
+    for (int t = 0; t < ntiles; t++) {
+        if (t + 1 < ntiles) {
+            L.store(A, tl2[(t + 1) & 1], cstart + (t + 1) * kTileF, cend);
+            if (t + 2 < ntiles) L.fetch(A, cstart + (t + 2) * kTileF);
+        }
+        __syncthreads();
+    }
+}
 
 // Per-lag sums of one chunk.  A chunk spans at most 2^(blocknum-1) frames,
 // i.e. at most 32 blocks of the blocked variance, so FP32 sums stay within
@@ -149,7 +190,7 @@ __device__ __forceinline__ float group_sum(const Tile &tl, int q, int s) {
 
 // One role: NL delays starting at DMIN on level LEVEL (group size 16).
 template <int NL, int DMIN, int LEVEL, bool CCF>
-__device__ void run_role(const MainArgs &A, Tile &tl, Loader &ld, int lane, int p0, int chunk) {
+__device__ void run_role(const MainArgs &A, const Tile *tl2, int lane, int p0, int chunk) {
     constexpr int type = CCF ? 1 : 0;
     constexpr int GPT = kTileF / (16 << LEVEL);   // groups per tile
     const CorrTables &ct = *A.ctp;
@@ -165,7 +206,7 @@ __device__ void run_role(const MainArgs &A, Tile &tl, Loader &ld, int lane, int 
     // delay d is sb_blk + sum_{k<d} b[ye+k] - sum_{k<d} b[ys+k] for the block
     // [ys, ye), the head term kept per delay from the previous block end
     float sa_blk = 0.f, sb_blk = 0.f;
-    int B = 1, pnum = 0;
+    int B = 1, lb = 0, pnum = 0;   // block length B = 2^lb values
     {
         const double *Fa = A.F + (int64_t)A.g.slot_of[sa] * ct.npos, *Fb = A.F + (int64_t)A.g.slot_of[sb] * ct.npos;
         double mua = level_shift(A, sa, LEVEL);
@@ -181,15 +222,15 @@ __device__ void run_role(const MainArgs &A, Tile &tl, Loader &ld, int lane, int 
             acc.a1[i] = acc.a2[i] = acc.cross[i] = 0.f;
         }
         int indp = live ? A.ind[pair] : ct.q - 1;
-        B = 1 << (indp - LEVEL);
+        lb = indp - LEVEL;
+        B = 1 << lb;
         pnum = (int)floor(ct.mtab_last / pow(2.0, fmax(indp - ct.log2_samp_last, 0.0)));
     }
-    const int cstart = chunk * A.chunk_len;
-    const int cend = min(A.g.T, cstart + A.chunk_len);
+    int cstart, cend;
+    const int ntiles = chunk_tiles(A, chunk, cstart, cend);
     const bool last_chunk = (chunk == A.nchunk - 1);
     const int ystart = cstart >> LEVEL;
     const int yend = last_chunk ? ct.numbin[LEVEL] : (cend >> LEVEL);
-    const int ntiles = (cend - cstart + kTileF - 1) / kTileF + 2;
     float w[32], ga_cur[CCF ? 16 : 1], ga_nxt[CCF ? 16 : 1];
     float gsa_cur = 0.f, gsa_nxt = 0.f, gsb_cur = 0.f, gsb_nxt = 0.f;
 #pragma unroll
@@ -198,12 +239,19 @@ __device__ void run_role(const MainArgs &A, Tile &tl, Loader &ld, int lane, int 
 #pragma unroll
         for (int k = 0; k < 16; k++) ga_cur[k] = ga_nxt[k] = 0.f;
     }
+    // GLS block sums of this role's channels: row [pair][block], column
+    // channel - 1 (channel 0 has none); hoisted so that the stores inside
+    // the loop force no reloads of the tables
+    const int xch0 = ct.first_chan[LEVEL] + (LEVEL == 0 ? DMIN : DMIN - 9);
+    const int gls_ld = ct.chanum - 1;
+    float *const gls_row = A.gls ? A.gls + (int64_t)pair * A.gls_nblk * gls_ld : nullptr;
     int ynext = ystart;   // level index of the next value to load
     for (int t = 0; t < ntiles; t++) {
-        const int f0 = cstart + t * kTileF;
-        ld.load(A, tl, f0, cend);
-        __syncthreads();
-#pragma unroll
+        const Tile &tl = tl2[t & 1];
+        // unrolled by two: the window shift becomes register renaming, with
+        // half the code of a full unroll (the role loops of one workgroup
+        // share the CU's instruction cache)
+#pragma unroll 2
         for (int q = 0; q < GPT; q++) {
             // shift in 16 new level values
 #pragma unroll
@@ -249,7 +297,8 @@ __device__ void run_role(const MainArgs &A, Tile &tl, Loader &ld, int lane, int 
                 }
                 const int ye = y0 + 16;
                 if ((ye & (B - 1)) == 0) {
-                    const bool inblk = (ye / B - 1) < pnum;
+                    const int blk = (ye >> lb) - 1;   // ye >= 16 > 0
+                    const bool inblk = blk < pnum;
                     // tail terms sum_{k<d} b[ye+k]: the next group's values
                     float tail[NL];
                     {
@@ -270,9 +319,7 @@ __device__ void run_role(const MainArgs &A, Tile &tl, Loader &ld, int lane, int 
                         if (inblk) {
                             acc.a1[i] += S;
                             acc.a2[i] = fmaf(S, S, acc.a2[i]);
-                            const int x = ct.first_chan[LEVEL] + (LEVEL == 0 ? DMIN : DMIN - 9) + i;
-                            if (A.gls && x > 0)
-                                A.gls[((int64_t)pair * A.gls_nblk + (ye / B - 1)) * (ct.chanum - 1) + (x - 1)] = S;
+                            if (gls_row && xch0 + i > 0) gls_row[(int64_t)blk * gls_ld + xch0 + i - 1] = S;
                         }
                         acc.cross[i] += acc.sab[i];
                         acc.sab[i] = 0.f;
@@ -296,28 +343,38 @@ __device__ void run_role(const MainArgs &A, Tile &tl, Loader &ld, int lane, int 
     }
 }
 
-__global__ void __launch_bounds__(640) k_corr_main(MainArgs A) {
-    __shared__ Tile tl;
+// 12 waves with partner series (10 roles + 2 loaders), 6 without (5 + 1).
+// Waves go to the four SIMDs round-robin, so wave w shares its SIMD with
+// w +- 4 and w +- 8; the role order below gives every SIMD about the same
+// work per tile (two level-0 roles and a level-2 role on two SIMDs; a
+// level-0 role, a level-1 role and a loader on the other two).  Each role
+// has one call site, so that every role is inlined.
+__global__ void __launch_bounds__(768) k_corr_main(MainArgs A) {
+    __shared__ Tile tl2[2];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int p0 = blockIdx.x * 64;
     const int chunk = blockIdx.y;
-    const int nwaves = blockDim.x >> 6;
-    const int nloader = (nwaves == 10) ? 8 : 4;
-    Loader ld;
-    ld.init(A, tl, wave, lane, p0, nloader);
+    const bool pairs2 = (blockDim.x >> 6) == 12;   // partner series staged
+    const int nrole = pairs2 ? 10 : 5;
+    if (wave >= nrole) {
+        run_loader(A, tl2, wave - nrole, lane, p0, chunk);
+        return;
+    }
+    // without partners: waves 0..4 take roles 0, 4, 2, 6, 8
+    const int role = pairs2 ? wave : (wave == 1 ? 4 : wave == 3 ? 6 : wave == 4 ? 8 : wave);
     __syncthreads();
-    switch (wave) {
-        case 0: run_role<6, 0, 0, false>(A, tl, ld, lane, p0, chunk); break;
-        case 1: run_role<6, 6, 0, false>(A, tl, ld, lane, p0, chunk); break;
-        case 2: run_role<5, 12, 0, false>(A, tl, ld, lane, p0, chunk); break;
-        case 3: run_role<8, 9, 1, false>(A, tl, ld, lane, p0, chunk); break;
-        case 4: run_role<8, 9, 2, false>(A, tl, ld, lane, p0, chunk); break;
-        case 5: run_role<6, 0, 0, true>(A, tl, ld, lane, p0, chunk); break;
-        case 6: run_role<6, 6, 0, true>(A, tl, ld, lane, p0, chunk); break;
-        case 7: run_role<5, 12, 0, true>(A, tl, ld, lane, p0, chunk); break;
-        case 8: run_role<8, 9, 1, true>(A, tl, ld, lane, p0, chunk); break;
-        default: run_role<8, 9, 2, true>(A, tl, ld, lane, p0, chunk); break;
+    switch (role) {
+        case 0: run_role<6, 0, 0, false>(A, tl2, lane, p0, chunk); break;
+        case 1: run_role<6, 0, 0, true>(A, tl2, lane, p0, chunk); break;
+        case 2: run_role<5, 12, 0, false>(A, tl2, lane, p0, chunk); break;
+        case 3: run_role<5, 12, 0, true>(A, tl2, lane, p0, chunk); break;
+        case 4: run_role<6, 6, 0, false>(A, tl2, lane, p0, chunk); break;
+        case 5: run_role<6, 6, 0, true>(A, tl2, lane, p0, chunk); break;
+        case 6: run_role<8, 9, 1, false>(A, tl2, lane, p0, chunk); break;
+        case 7: run_role<8, 9, 1, true>(A, tl2, lane, p0, chunk); break;
+        case 8: run_role<8, 9, 2, false>(A, tl2, lane, p0, chunk); break;
+        default: run_role<8, 9, 2, true>(A, tl2, lane, p0, chunk); break;
     }
 }
 
@@ -625,7 +682,7 @@ void launch_correlate_fast(const SeriesGeom &g, const BleachDesc &bd, const Corr
         if (c >= 4 && c < ct.nlevels) off += (int64_t)ct.numbin[c] * g.nslots;
     }
     const int tiles = (g.npix + 63) / 64;
-    const int threads = pd.has_ccf ? 640 : 320;
+    const int threads = pd.has_ccf ? 768 : 384;
     if (ev_main_begin) (void)hipEventRecord(ev_main_begin, st);
     hipLaunchKernelGGL(k_corr_main, dim3(tiles, A.nchunk), dim3(threads), 0, st, A);
     if (ev_main_end) (void)hipEventRecord(ev_main_end, st);
