@@ -587,10 +587,10 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTabl
         size_t nlv = 0;
         for (int c = 4; c < ct.nlevels; c++) nlv += (size_t)ct.numbin[c];
         HIP_TRY(ctx->l3.ensure(sizeof(float) * (size_t)ct.numbin[3] * g.nslots));
-        HIP_TRY(ctx->lvd.ensure(sizeof(double) * std::max<size_t>(nlv, 1) * g.nslots));
+        HIP_TRY(ctx->lvd.ensure(sizeof(float) * std::max<size_t>(nlv, 1) * g.nslots));
         launch_correlate_fast(g, bd, ct, dct, ea.pd, ctx->F.as<double>(), ctx->X.as<float>(), ctx->ind.as<int32_t>(),
                               ctx->cpart.as<float>(), ctx->hpart.as<double>(), ctx->l3.as<float>(),
-                              ctx->lvd.as<double>(), clen,
+                              ctx->lvd.as<float>(), clen,
                               ctx->corav.as<double>(), ctx->bvar.as<double>(), st, ctx->evk[0], ctx->evk[1],
                               ea.gls ? ctx->glsblk.as<float>() : nullptr, nblk);
         ctx->evk_valid = 1;

@@ -43,7 +43,7 @@ struct MainArgs {
     int has_acf, has_ccf;
     float *part;    // [chunk][pair][kMainChan][3] : cross, A1, A2
     float *l3;      // level-3 values [numbin_3][nslots]
-    double *lvd;    // levels >= 4, level c at lvd + loff[c]: [numbin_c][nslots]
+    float *lvd;     // levels >= 4, level c at lvd + loff[c]: [numbin_c][nslots]
     int64_t loff[kMaxLevel];
     // GLS: every block sum of the blocked variance, [pair][block][channel-1]
     // (channels 1..chanum-1); null when no covariance is requested
@@ -381,31 +381,29 @@ __global__ void __launch_bounds__(768) k_corr_main(MainArgs A) {
 // ------------------------------------------------------------------
 // Levels 3..q-1 from the level-3 series.
 // ------------------------------------------------------------------
-// Values of one level >= 3: level 3 from the FP32 8-frame sums, higher
-// levels from the FP64 rebinned series.  Branch-free (clamped index, zero
-// past the end) so that a group's loads issue back to back.
-template <bool L3>
-struct L3Ctx {
-    const float *l3;    // level 3 (FP32 sums of 8 frames)
-    const double *lv;   // this level's values when c >= 4 (FP64)
+// Values of one level >= 3 (stored as FP32 sums of 2^c frames, computed
+// from there on in FP64).  Branch-free (clamped index, zero past the end)
+// so that a group's loads issue back to back.
+struct LevelCtx {
+    const float *lv;   // this level's values [numbin][nslots]
     int nslots, sa, sb, c, numbin;   // sa, sb: slots of the pair's series
     __device__ __forceinline__ double val(int s, int y) const {
         const int64_t i = (int64_t)min(y, numbin - 1) * nslots + s;
-        const double v = L3 ? (double)l3[i] : lv[i];
+        const double v = (double)lv[i];
         return (y < numbin) ? v : 0.0;
     }
 };
 
 // Levels c0+1 .. c0+4 from level c0 by the pairwise-sum rebinning
-// (IFCS:9477-9478): v_c[y] = v_{c-1}[2y] + v_{c-1}[2y+1], in FP64.  One
+// (IFCS:9477-9478): v_c[y] = v_{c-1}[2y] + v_{c-1}[2y+1], summed in FP64
+// and stored as FP32 (as level 3).  One
 // thread per (slot, 16 consecutive level-c0 values): it reads the 16
 // values once and writes the 8 + 4 + 2 + 1 values they produce on the next
 // four levels (levels past nlev, and values past each level's numbin, are
 // not written).
 struct RebinArgs {
-    const float *src32;    // level c0 as FP32 (level 3), or null
-    const double *src64;   // level c0 as FP64 (lvd + loff[c0]) otherwise
-    double *dst[4];        // lvd + loff[c0 + 1 + k]
+    const float *src;      // level c0 (l3, or lvd + loff[c0])
+    float *dst[4];         // lvd + loff[c0 + 1 + k]
     int numbin_src;
     int numbin[4];         // numbin of levels c0+1 .. c0+4 (0 = level absent)
     int nslots;
@@ -420,7 +418,7 @@ __global__ void k_corr_rebin4(RebinArgs R) {
 #pragma unroll
     for (int k = 0; k < 16; k++) {
         const int64_t y = min(grp * 16 + k, (int64_t)R.numbin_src - 1);
-        v[k] = R.src32 ? (double)R.src32[y * R.nslots + s] : R.src64[y * R.nslots + s];
+        v[k] = (double)R.src[y * R.nslots + s];
     }
     int n = 16;
 #pragma unroll
@@ -433,18 +431,18 @@ __global__ void k_corr_rebin4(RebinArgs R) {
 #pragma unroll
             for (int k = 0; k < 8; k++) {
                 const int64_t y = grp * n + k;
-                if (k < n && y < R.numbin[lv]) R.dst[lv][y * R.nslots + s] = v[k];
+                if (k < n && y < R.numbin[lv]) R.dst[lv][y * R.nslots + s] = (float)v[k];
             }
         }
     }
 }
 
-// Levels >= 3 hold few, large (rebinned) values: FP64 throughout.
+// Levels >= 3 hold few, large (rebinned) values: products and sums in FP64.
 // Values [ybeg, yend) of one level; yend - ybeg and ybeg are multiples of G
 // and of the block length B, so every block of the blocked variance lies in
 // one call.
-template <int G, bool L3, bool SAME>
-__device__ void high_level(const L3Ctx<L3> &L, const double *alpha, const double *beta, int B, int pnum,
+template <int G, bool SAME>
+__device__ void high_level(const LevelCtx &L, const double *alpha, const double *beta, int B, int pnum,
                            int ybeg, int yend, double *cross, double *a1, double *a2, float *gls_row, int gls_ld) {
     constexpr int WN = G + 16;
     double w[WN], ga[G], sab[8], head[8];
@@ -524,24 +522,13 @@ __device__ __forceinline__ void high_dispatch(const MainArgs &A, int c, int G, i
                                               const double *beta, int B, int pnum, int ybeg, int yend, double *cross,
                                               double *a1, double *a2, float *grow, int ld) {
     const CorrTables &ct = *A.ctp;
-    if (c == 3) {
-        const L3Ctx<true> L{A.l3, nullptr, A.g.nslots, sa, sb, c, ct.numbin[c]};
-        switch (G) {
-            case 16: high_level<16, true, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
-            case 8: high_level<8, true, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
-            case 4: high_level<4, true, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
-            case 2: high_level<2, true, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
-            default: high_level<1, true, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
-        }
-    } else {
-        const L3Ctx<false> L{nullptr, A.lvd + A.loff[c], A.g.nslots, sa, sb, c, ct.numbin[c]};
-        switch (G) {
-            case 16: high_level<16, false, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
-            case 8: high_level<8, false, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
-            case 4: high_level<4, false, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
-            case 2: high_level<2, false, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
-            default: high_level<1, false, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
-        }
+    const LevelCtx L{c == 3 ? A.l3 : A.lvd + A.loff[c], A.g.nslots, sa, sb, c, ct.numbin[c]};
+    switch (G) {
+        case 16: high_level<16, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
+        case 8: high_level<8, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
+        case 4: high_level<4, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
+        case 2: high_level<2, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
+        default: high_level<1, SAME>(L, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2, grow, ld); break;
     }
 }
 
@@ -656,7 +643,7 @@ int corr_fast_chunk_len(const CorrTables &ct, int npairs) {
 
 void launch_correlate_fast(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const CorrTables *dct,
                            const PairDesc &pd, const double *F, const float *X, const int32_t *ind, float *part,
-                           double *hpart, float *l3, double *lvd, int chunk_len, double *corav, double *bvar,
+                           double *hpart, float *l3, float *lvd, int chunk_len, double *corav, double *bvar,
                            hipStream_t st, hipEvent_t ev_main_begin, hipEvent_t ev_main_end, float *gls_blk,
                            int gls_nblk) {
     MainArgs A;
@@ -688,8 +675,7 @@ void launch_correlate_fast(const SeriesGeom &g, const BleachDesc &bd, const Corr
     if (ev_main_end) (void)hipEventRecord(ev_main_end, st);
     for (int c0 = 3; c0 + 1 < ct.nlevels; c0 += 4) {
         RebinArgs R{};
-        R.src32 = c0 == 3 ? l3 : nullptr;
-        R.src64 = c0 == 3 ? nullptr : lvd + A.loff[c0];
+        R.src = c0 == 3 ? l3 : lvd + A.loff[c0];
         R.numbin_src = ct.numbin[c0];
         for (int k = 0; k < 4; k++) {
             const int c = c0 + 1 + k;

@@ -177,7 +177,7 @@ bool correlate_fast_supported(const CorrTables &ct);
 int corr_fast_chunk_len(const CorrTables &ct, int npairs);
 void launch_correlate_fast(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const CorrTables *dct,
                            const PairDesc &pd, const double *F, const float *X, const int32_t *ind, float *part,
-                           double *hpart, float *l3, double *lvd, int chunk_len, double *corav, double *bvar,
+                           double *hpart, float *l3, float *lvd, int chunk_len, double *corav, double *bvar,
                            hipStream_t st, hipEvent_t ev_main_begin = nullptr, hipEvent_t ev_main_end = nullptr,
                            float *gls_blk = nullptr, int gls_nblk = 0);
 void launch_lm_acf(int nfits, int npts, const float *data, const float *weights, const float *x,
