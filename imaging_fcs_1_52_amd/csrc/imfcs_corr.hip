@@ -387,10 +387,10 @@ __global__ void __launch_bounds__(768) k_corr_main(MainArgs A) {
 struct LevelCtx {
     const float *lv;   // this level's values [numbin][nslots]
     int nslots, sa, sb, c, numbin;   // sa, sb: slots of the pair's series
-    __device__ __forceinline__ double val(int s, int y) const {
+    __device__ __forceinline__ float val(int s, int y) const {
         const int64_t i = (int64_t)min(y, numbin - 1) * nslots + s;
-        const double v = (double)lv[i];
-        return (y < numbin) ? v : 0.0;
+        const float v = lv[i];
+        return (y < numbin) ? v : 0.f;
     }
 };
 
@@ -437,15 +437,17 @@ __global__ void k_corr_rebin4(RebinArgs R) {
     }
 }
 
-// Levels >= 3 hold few, large (rebinned) values: products and sums in FP64.
 // Values [ybeg, yend) of one level; yend - ybeg and ybeg are multiples of G
 // and of the block length B, so every block of the blocked variance lies in
-// one call.
+// one call.  The window holds the stored FP32 values; each group's 16
+// products per delay are summed in FP32 (about 1e-7 relative), the block
+// and chunk sums in FP64.
 template <int G, bool SAME>
 __device__ void high_level(const LevelCtx &L, const double *alpha, const double *beta, int B, int pnum,
                            int ybeg, int yend, double *cross, double *a1, double *a2, float *gls_row, int gls_ld) {
     constexpr int WN = G + 16;
-    double w[WN], ga[G], sab[8], head[8];
+    float w[WN], ga[G];
+    double sab[8], head[8];
     double sa = 0.0, sbk = 0.0;   // block sums of a_y and of b_y over the block's products
 #pragma unroll
     for (int i = 0; i < 8; i++) {
@@ -454,35 +456,49 @@ __device__ void high_level(const LevelCtx &L, const double *alpha, const double 
     }
 #pragma unroll
     for (int k = 0; k < WN; k++) w[k] = L.val(L.sb, ybeg + k);
-    for (int y0 = ybeg; y0 < yend; y0 += G) {
+    if (!SAME) {
 #pragma unroll
-        for (int k = 0; k < G; k++) ga[k] = SAME ? w[k] : L.val(L.sa, y0 + k);
+        for (int k = 0; k < G; k++) ga[k] = L.val(L.sa, ybeg + k);
+    }
+    for (int y0 = ybeg; y0 < yend; y0 += G) {
+        // the next group's values, in flight while this group is reduced
+        // (past yend they are values of the next chunk or zero: in bounds)
+        float nb[G], na[SAME ? 1 : G];
+#pragma unroll
+        for (int k = 0; k < G; k++) {
+            nb[k] = L.val(L.sb, y0 + WN + k);
+            if (!SAME) na[k] = L.val(L.sa, y0 + G + k);
+        }
+        if (SAME) {
+#pragma unroll
+            for (int k = 0; k < G; k++) ga[k] = w[k];
+        }
         if ((y0 & (B - 1)) == 0) {
             // block start (blocks hold whole groups: B >= G): head[i] = sum of
             // the block's first 9 + i b-values
             double pre = 0.0;
 #pragma unroll
-            for (int k = 0; k < 9; k++) pre += w[k];
+            for (int k = 0; k < 9; k++) pre += (double)w[k];
 #pragma unroll
             for (int i = 0; i < 8; i++) {
-                if (i > 0) pre += w[8 + i];
+                if (i > 0) pre += (double)w[8 + i];
                 head[i] = pre;
             }
         }
-        double gs = 0.0, gb = 0.0;
+        float gs = 0.f, gb = 0.f;
 #pragma unroll
         for (int k = 0; k < G; k++) {
             gs += ga[k];
             gb += w[k];
         }
-        sa += gs;
-        sbk += gb;
+        sa += (double)gs;
+        sbk += (double)gb;
 #pragma unroll
         for (int i = 0; i < 8; i++) {
-            double sv = 0.0;
+            float sv = 0.f;
 #pragma unroll
-            for (int k = 0; k < G; k++) sv = fma(ga[k], w[k + 9 + i], sv);
-            sab[i] += sv;
+            for (int k = 0; k < G; k++) sv = fmaf(ga[k], w[k + 9 + i], sv);
+            sab[i] += (double)sv;
         }
         const int ye = y0 + G;
         if ((ye & (B - 1)) == 0) {
@@ -491,10 +507,10 @@ __device__ void high_level(const LevelCtx &L, const double *alpha, const double 
             // being the first d b-values after the block (w[G..G+15])
             double tail = 0.0;
 #pragma unroll
-            for (int k = 0; k < 9; k++) tail += w[G + k];
+            for (int k = 0; k < 9; k++) tail += (double)w[G + k];
 #pragma unroll
             for (int i = 0; i < 8; i++) {
-                if (i > 0) tail += w[G + 8 + i];
+                if (i > 0) tail += (double)w[G + 8 + i];
                 const double sbd = sbk - head[i] + tail;
                 double S = sab[i] - beta[i] * sa - alpha[i] * sbd + (double)B * alpha[i] * beta[i];
                 if (inblk) {
@@ -511,7 +527,10 @@ __device__ void high_level(const LevelCtx &L, const double *alpha, const double 
 #pragma unroll
         for (int k = 0; k < 16; k++) w[k] = w[k + G];
 #pragma unroll
-        for (int k = 0; k < G; k++) w[16 + k] = L.val(L.sb, y0 + WN + k);
+        for (int k = 0; k < G; k++) {
+            w[16 + k] = nb[k];
+            if (!SAME) ga[k] = na[k];
+        }
     }
 #pragma unroll
     for (int i = 0; i < 8; i++) cross[i] += sab[i];
