@@ -25,6 +25,14 @@ __device__ __forceinline__ void slot_origin(const SeriesGeom &g, int slot, int &
 }
 
 // Exact integer window sum minus background (IFCS:9668-9678).
+// Rows of a window that raw_ints16 reads in one batch, else 0: two-row
+// windows (four-row windows measured no faster batched: their passes run
+// near the memory bandwidth either way).
+__host__ __device__ inline int raw_batch_rows(const SeriesGeom &g) {
+    const bool vec = g.src == kSrcU16 && ((g.vec == 4 && g.bx == 4) || (g.vec == 2 && g.bx == 2));
+    return (vec && g.by == 2) ? 2 : 0;
+}
+
 __device__ __forceinline__ int raw_u16(const SeriesGeom &g, int ox, int oy, int f) {
     const uint16_t *base = (const uint16_t *)g.data + (int64_t)(g.f0 + f) * g.plane +
                            (int64_t)oy * g.stride + ox;
@@ -37,9 +45,13 @@ __device__ __forceinline__ int raw_u16(const SeriesGeom &g, int ox, int oy, int 
 
 // Window sums (uint16 source, minus background) of 16 consecutive frames
 // f0 .. f0+15; frames past fmax repeat frame fmax.  The frame base address
-// is wave-uniform (f0 is), so each load needs one 32-bit lane offset; the
-// window loops are outermost so the 16 loads of a window element are in
-// flight together.
+// is wave-uniform (f0 is), so each load needs one 32-bit lane offset.  With
+// aligned rows of 2 or 4 pixels and windows of NB = 2 or 4 rows, the loads
+// of all rows are issued before any is summed (one round trip per group,
+// at the cost of 32..128 registers: separate instantiations, NB =
+// raw_batch_rows(g)); with NB = 0 the window loops are outermost so the 16
+// loads of a window element are in flight together.
+template <int NB>
 __device__ __forceinline__ void raw_ints16(const SeriesGeom &g, int ox, int oy, int f0, int fmax, int (&acc)[16]) {
     const int nv = min(16, fmax - f0 + 1);
     const uint32_t pl = (uint32_t)g.plane;
@@ -47,6 +59,37 @@ __device__ __forceinline__ void raw_ints16(const SeriesGeom &g, int ox, int oy, 
     const uint16_t *q = (const uint16_t *)g.data + (int64_t)(g.f0 + f0) * g.plane;
 #pragma unroll
     for (int j = 0; j < 16; j++) acc[j] = -g.bckg_total;
+    if (NB > 0 && g.vec == 4) {
+        const uint2 *q4 = (const uint2 *)q;
+        uint2 r[NB > 0 ? NB : 1][16];
+#pragma unroll
+        for (int k = 0; k < NB; k++) {
+            const uint32_t d = (lo + (uint32_t)(k * g.stride)) >> 2;
+#pragma unroll
+            for (int j = 0; j < 16; j++) r[k][j] = q4[d + (uint32_t)min(j, nv - 1) * (pl >> 2)];
+        }
+#pragma unroll
+        for (int k = 0; k < NB; k++)
+#pragma unroll
+            for (int j = 0; j < 16; j++)
+                acc[j] += (int)((r[k][j].x & 0xffffu) + (r[k][j].x >> 16) + (r[k][j].y & 0xffffu) + (r[k][j].y >> 16));
+        return;
+    }
+    if (NB > 0) {   // g.vec == 2
+        const uint32_t *q2 = (const uint32_t *)q;
+        uint32_t r[NB > 0 ? NB : 1][16];
+#pragma unroll
+        for (int k = 0; k < NB; k++) {
+            const uint32_t d = (lo + (uint32_t)(k * g.stride)) >> 1;
+#pragma unroll
+            for (int j = 0; j < 16; j++) r[k][j] = q2[d + (uint32_t)min(j, nv - 1) * (pl >> 1)];
+        }
+#pragma unroll
+        for (int k = 0; k < NB; k++)
+#pragma unroll
+            for (int j = 0; j < 16; j++) acc[j] += (int)((r[k][j] & 0xffffu) + (r[k][j] >> 16));
+        return;
+    }
     if (g.vec == 4 && g.bx == 4) {
         // aligned rows of 4 pixels: one 8-byte load per row and frame
         const uint2 *q4 = (const uint2 *)q;
@@ -85,10 +128,11 @@ __device__ __forceinline__ void raw_ints16(const SeriesGeom &g, int ox, int oy, 
         }
 }
 
+template <int NB>
 __device__ __forceinline__ void raw_values16(const SeriesGeom &g, int ox, int oy, int f0, int fmax, double (&v)[16]) {
     if (g.src == kSrcU16) {
         int acc[16];
-        raw_ints16(g, ox, oy, f0, fmax, acc);
+        raw_ints16<NB>(g, ox, oy, f0, fmax, acc);
 #pragma unroll
         for (int j = 0; j < 16; j++) v[j] = (double)acc[j];
     } else {

@@ -28,8 +28,10 @@ namespace imfcs {
 // ------------------------------------------------------------------
 // Stage 1: intensity trace moments and polynomial fit
 // ------------------------------------------------------------------
-__global__ void k_trace_moments(SeriesGeom g, int nopit, int ave, double tscale, double frametime, int order,
-                                int integer_trace, int pts_per_chunk, double *part, float *trace_out) {
+template <int NB>
+__global__ void __launch_bounds__(64) k_trace_moments(SeriesGeom g, int nopit, int ave, double tscale,
+                                                      double frametime, int order, int integer_trace,
+                                                      int pts_per_chunk, double *part, float *trace_out) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;   // slot
     int chunk = blockIdx.y;
     if (s >= g.nslots) return;
@@ -47,7 +49,7 @@ __global__ void k_trace_moments(SeriesGeom g, int nopit, int ave, double tscale,
             const int fe = (x + 1) * ave;
             for (int f = x * ave; f < fe; f += 16) {
                 int v[16];
-                raw_ints16(g, ox, oy, f, fe - 1, v);
+                raw_ints16<NB>(g, ox, oy, f, fe - 1, v);
 #pragma unroll
                 for (int i = 0; i < 16; i++) isum += (f + i < fe) ? v[i] : 0;
             }
@@ -120,7 +122,7 @@ __global__ void k_polyfit(const double *part, int nchunk, int nseries, int order
 // IFCS:9487-9494, 9262-9270).  One thread per (slot, frame chunk); frames
 // are read in groups of 16 so the loads of a group are in flight together.
 // ------------------------------------------------------------------
-template <class BL>
+template <class BL, int NB>
 __global__ void __launch_bounds__(64) k_apply(SeriesGeom g, BleachDesc bd, const CorrTables *__restrict__ ctp,
                                               int chunk_len, double *ctot, double *rec, float *X) {
     const CorrTables &ct = *ctp;
@@ -141,7 +143,7 @@ __global__ void __launch_bounds__(64) k_apply(SeriesGeom g, BleachDesc bd, const
     const uint32_t ld = (uint32_t)g.nslots;
     for (int fb = f0; fb < f1; fb += 16) {
         double v[16];
-        raw_values16(g, ox, oy, fb, f1 - 1, v);
+        raw_values16<NB>(g, ox, oy, fb, f1 - 1, v);
         float *xw = X + (int64_t)fb * g.nslots;   // wave-uniform
 #pragma unroll
         for (int i = 0; i < 16; i++) {
@@ -1672,8 +1674,16 @@ void launch_trace_moments(const SeriesGeom &g, int nopit, int ave, double tscale
                           int integer_trace, double *part, int nchunk, hipStream_t st, float *trace_out) {
     int ppc = cdiv(nopit, nchunk);
     dim3 grid(cdiv(g.nslots, 64), nchunk);
-    hipLaunchKernelGGL(k_trace_moments, grid, dim3(64), 0, st, g, nopit, ave, tscale, frametime, order,
-                       integer_trace, ppc, part, trace_out);
+    switch (raw_batch_rows(g)) {
+        case 2:
+            hipLaunchKernelGGL(k_trace_moments<2>, grid, dim3(64), 0, st, g, nopit, ave, tscale, frametime, order,
+                               integer_trace, ppc, part, trace_out);
+            break;
+        default:
+            hipLaunchKernelGGL(k_trace_moments<0>, grid, dim3(64), 0, st, g, nopit, ave, tscale, frametime, order,
+                               integer_trace, ppc, part, trace_out);
+            break;
+    }
 }
 
 // Initial guesses of SingleExpFit / DoubleExpFit (IFCS:10862-10864,
@@ -1748,28 +1758,40 @@ void launch_polyfit(const double *part, int nchunk, int nseries, int nopit, int 
     hipLaunchKernelGGL(k_polyfit, dim3(cdiv(nseries, 64)), dim3(64), 0, st, part, nchunk, nseries, order, Lf, coef);
 }
 
-void launch_apply(const SeriesGeom &g, const BleachDesc &bd, const CorrTables *dct, double *ctot, double *rec,
-                  float *X, int nchunk, int chunk_len, hipStream_t st) {
+template <int NB>
+static void launch_apply_t(const SeriesGeom &g, const BleachDesc &bd, const CorrTables *dct, double *ctot,
+                           double *rec, float *X, int nchunk, int chunk_len, hipStream_t st) {
     dim3 grid(cdiv(g.nslots, 64), nchunk);
     if (bd.mode == 1) {
-        hipLaunchKernelGGL(k_apply<BleachSeg>, grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec, X);
+        hipLaunchKernelGGL((k_apply<BleachSeg, NB>), grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec, X);
         return;
     }
     if (bd.mode == 2) {
-        hipLaunchKernelGGL(k_apply<BleachExp<1>>, grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec, X);
+        hipLaunchKernelGGL((k_apply<BleachExp<1>, NB>), grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec,
+                           X);
         return;
     }
     switch (bleach_maxo(bd.order)) {
         case 0:
-            hipLaunchKernelGGL(k_apply<BleachP<0>>, grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec, X);
-            break;
-        case 4:
-            hipLaunchKernelGGL(k_apply<BleachP<4>>, grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec, X);
-            break;
-        default:
-            hipLaunchKernelGGL(k_apply<BleachP<kMaxOrder>>, grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec,
+            hipLaunchKernelGGL((k_apply<BleachP<0>, NB>), grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec,
                                X);
             break;
+        case 4:
+            hipLaunchKernelGGL((k_apply<BleachP<4>, NB>), grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec,
+                               X);
+            break;
+        default:
+            hipLaunchKernelGGL((k_apply<BleachP<kMaxOrder>, NB>), grid, dim3(64), 0, st, g, bd, dct, chunk_len,
+                               ctot, rec, X);
+            break;
+    }
+}
+
+void launch_apply(const SeriesGeom &g, const BleachDesc &bd, const CorrTables *dct, double *ctot, double *rec,
+                  float *X, int nchunk, int chunk_len, hipStream_t st) {
+    switch (raw_batch_rows(g)) {
+        case 2: launch_apply_t<2>(g, bd, dct, ctot, rec, X, nchunk, chunk_len, st); break;
+        default: launch_apply_t<0>(g, bd, dct, ctot, rec, X, nchunk, chunk_len, st); break;
     }
 }
 
@@ -1786,7 +1808,7 @@ __global__ void k_seg_means(SeriesGeom g, int seg_len, int nseg, double *coef) {
     double sum = 0.0;
     for (int f = fa; f < fe; f += 16) {
         int v[16];
-        raw_ints16(g, ox, oy, f, fe - 1, v);
+        raw_ints16<0>(g, ox, oy, f, fe - 1, v);
 #pragma unroll
         for (int i = 0; i < 16; i++)
             if (f + i < fe) sum += (double)v[i];
