@@ -30,6 +30,8 @@ namespace imfcs {
 constexpr int kTileF = 64;     // frames per LDS tile
 constexpr int kMainChan = 33;  // channels of levels 0..2 (p = 16)
 constexpr int kNS = 128;       // staged series per tile (a and partner b)
+// third word of a raw (unswizzled, 32-bit) buffer descriptor on gfx9
+constexpr int kBufferWord3 = 0x00020000;
 
 struct MainArgs {
     SeriesGeom g;
@@ -87,10 +89,22 @@ struct TileLoader {
     }
     // frames f0 .. f0+63 (rows past the end of the series read the last
     // row; store() zeroes them).  Nothing consumes the values before the
-    // next tile's store, so all 64 loads are in flight together.
+    // next tile's store, so all 64 loads are in flight together.  A loader
+    // wave issues every instruction of its tile, so the interior tiles use
+    // buffer loads: one lane offset for all rows and one scalar offset per
+    // row, instead of a 64-bit address per load.
     __device__ void fetch(const MainArgs &A, int f0) {
         const int T = A.g.T;
         const int64_t ld = A.g.nslots;
+        if (f0 + kTileF <= T) {   // wave-uniform
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                (void *)(A.X + (int64_t)f0 * ld), (short)0, (int)(kTileF * ld * 4), kBufferWord3);
+            const int lo = slot * 4, rowb = (int)ld * 4;
+#pragma unroll
+            for (int k = 0; k < kTileF; k++)   // (the builtin returns the 32 bits as an integer)
+                xr[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, lo, k * rowb, 0));
+            return;
+        }
 #pragma unroll
         for (int k = 0; k < kTileF; k++) {
             const float *row = A.X + (int64_t)min(f0 + k, T - 1) * ld;   // wave-uniform
@@ -98,8 +112,17 @@ struct TileLoader {
         }
     }
     // level 0..2 rows and group sums into LDS, level-3 sums of the chunk's
-    // own frames to global memory
+    // own frames to global memory; values past the end of the series are
+    // zero (and incomplete pairs on levels 1, 2), checked only in the tiles
+    // that reach the end
     __device__ void store(const MainArgs &A, Tile &tl, int f0, int chunk_end) const {
+        if (f0 + kTileF <= A.g.T)   // wave-uniform
+            store_t<false>(A, tl, f0, chunk_end);
+        else
+            store_t<true>(A, tl, f0, chunk_end);
+    }
+    template <bool TAIL>
+    __device__ __forceinline__ void store_t(const MainArgs &A, Tile &tl, int f0, int chunk_end) const {
         const int T = A.g.T;
 #pragma unroll
         for (int fb = 0; fb < kTileF / 16; fb++) {
@@ -111,19 +134,20 @@ struct TileLoader {
                 double v[8];
 #pragma unroll
                 for (int k = 0; k < 8; k++) {
-                    const float x = (fbase + 8 * h + k < T) ? xr[16 * fb + 8 * h + k] : 0.f;
+                    const float x = (!TAIL || fbase + 8 * h + k < T) ? xr[16 * fb + 8 * h + k] : 0.f;
                     v[k] = (double)x;
                     tl.l0[16 * fb + 8 * h + k][sl] = x;
                 }
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
-                    const float u = (fbase + 8 * h + 2 * k + 1 < T) ? (float)(v[2 * k] + v[2 * k + 1]) : 0.f;
+                    const float u =
+                        (!TAIL || fbase + 8 * h + 2 * k + 1 < T) ? (float)(v[2 * k] + v[2 * k + 1]) : 0.f;
                     tl.l1[8 * fb + 4 * h + k][sl] = u;
                     g1 += u;
                 }
 #pragma unroll
                 for (int k = 0; k < 2; k++) {
-                    const float u = (fbase + 8 * h + 4 * k + 3 < T)
+                    const float u = (!TAIL || fbase + 8 * h + 4 * k + 3 < T)
                                         ? (float)((v[4 * k] + v[4 * k + 1]) + (v[4 * k + 2] + v[4 * k + 3]))
                                         : 0.f;
                     tl.l2[4 * fb + 2 * h + k][sl] = u;
@@ -287,13 +311,12 @@ __device__ void run_role(const MainArgs &A, const Tile *tl2, int lane, int p0, i
 #pragma unroll
                 for (int i = 0; i < NL; i++) {
                     const int d = DMIN + i;
-                    float s0 = acc.sab[i], s1 = 0.f;
+                    // one FMA chain per delay: the role's delays are
+                    // independent chains, and its SIMD runs two more waves
+                    float s0 = acc.sab[i];
 #pragma unroll
-                    for (int k = 0; k < 16; k += 2) {
-                        s0 = fmaf(ga[k], w[k + d], s0);
-                        s1 = fmaf(ga[k + 1], w[k + 1 + d], s1);
-                    }
-                    acc.sab[i] = s0 + s1;
+                    for (int k = 0; k < 16; k++) s0 = fmaf(ga[k], w[k + d], s0);
+                    acc.sab[i] = s0;
                 }
                 const int ye = y0 + 16;
                 if ((ye & (B - 1)) == 0) {
