@@ -121,48 +121,57 @@ struct TileLoader {
         else
             store_t<true>(A, tl, f0, chunk_end);
     }
+    // Pair sums in FP32 (each level-1 value is the correctly rounded sum
+    // of its two frames; levels 2 and 3 are sums of rounded pairs, within
+    // 2 ulp of the exact sums): the loader wave issues every instruction
+    // of its tile, so it avoids FP64 conversions.  Interior 16-frame blocks
+    // have one group sum on all three levels; in the last tile each level
+    // sums its own (masked) values.
     template <bool TAIL>
     __device__ __forceinline__ void store_t(const MainArgs &A, Tile &tl, int f0, int chunk_end) const {
         const int T = A.g.T;
 #pragma unroll
         for (int fb = 0; fb < kTileF / 16; fb++) {
             const int fbase = f0 + 16 * fb;
-            double g0 = 0.0;
-            float g1 = 0.f, g2 = 0.f;
+            float x[16], u1[8], u2[4];
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
-                double v[8];
-#pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    const float x = (!TAIL || fbase + 8 * h + k < T) ? xr[16 * fb + 8 * h + k] : 0.f;
-                    v[k] = (double)x;
-                    tl.l0[16 * fb + 8 * h + k][sl] = x;
-                }
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const float u =
-                        (!TAIL || fbase + 8 * h + 2 * k + 1 < T) ? (float)(v[2 * k] + v[2 * k + 1]) : 0.f;
-                    tl.l1[8 * fb + 4 * h + k][sl] = u;
-                    g1 += u;
-                }
-#pragma unroll
-                for (int k = 0; k < 2; k++) {
-                    const float u = (!TAIL || fbase + 8 * h + 4 * k + 3 < T)
-                                        ? (float)((v[4 * k] + v[4 * k + 1]) + (v[4 * k + 2] + v[4 * k + 3]))
-                                        : 0.f;
-                    tl.l2[4 * fb + 2 * h + k][sl] = u;
-                    g2 += u;
-                }
-                const double s8 = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
-                g0 += s8;
-                if (own && fbase < chunk_end) {
-                    const int y3 = fbase / 8 + h;
-                    if (y3 < nb3) A.l3[(int64_t)y3 * A.g.nslots + slot] = (float)s8;
-                }
+            for (int k = 0; k < 16; k++) {
+                x[k] = (!TAIL || fbase + k < T) ? xr[16 * fb + k] : 0.f;
+                tl.l0[16 * fb + k][sl] = x[k];
             }
-            tl.gs0[fb][sl] = (float)g0;
-            tl.gs1[fb][sl] = g1;
-            tl.gs2[fb][sl] = g2;
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                u1[k] = x[2 * k] + x[2 * k + 1];
+                tl.l1[8 * fb + k][sl] = (!TAIL || fbase + 2 * k + 1 < T) ? u1[k] : 0.f;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                u2[k] = u1[2 * k] + u1[2 * k + 1];
+                tl.l2[4 * fb + k][sl] = (!TAIL || fbase + 4 * k + 3 < T) ? u2[k] : 0.f;
+            }
+            const float s8a = u2[0] + u2[1], s8b = u2[2] + u2[3];
+            if (own && fbase < chunk_end) {
+                const int y3 = fbase / 8;   // complete 8-frame sums only (y3 < T / 8)
+                if (y3 < nb3) A.l3[(int64_t)y3 * A.g.nslots + slot] = s8a;
+                if (y3 + 1 < nb3) A.l3[(int64_t)(y3 + 1) * A.g.nslots + slot] = s8b;
+            }
+            if (!TAIL) {
+                const float g = s8a + s8b;
+                tl.gs0[fb][sl] = g;
+                tl.gs1[fb][sl] = g;
+                tl.gs2[fb][sl] = g;
+            } else {
+                float g0 = 0.f, g1 = 0.f, g2 = 0.f;
+#pragma unroll
+                for (int k = 0; k < 16; k++) g0 += x[k];
+#pragma unroll
+                for (int k = 0; k < 8; k++) g1 += (fbase + 2 * k + 1 < T) ? u1[k] : 0.f;
+#pragma unroll
+                for (int k = 0; k < 4; k++) g2 += (fbase + 4 * k + 3 < T) ? u2[k] : 0.f;
+                tl.gs0[fb][sl] = g0;
+                tl.gs1[fb][sl] = g1;
+                tl.gs2[fb][sl] = g2;
+            }
         }
     }
 };
