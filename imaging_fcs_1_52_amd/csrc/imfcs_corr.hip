@@ -9,9 +9,9 @@
 //   LDS tile is double-buffered: the loaders fetch tile t+2 and store tile
 //   t+1 while the role waves work on tile t (one barrier per tile).  Five
 //   role waves per correlation type consume the tile:
-//     role 0..2: level 0, delays 0-5 / 6-11 / 12-16
-//     role 3   : level 1, delays 9-16
-//     role 4   : level 2, delays 9-16
+//     level 0, delays 0-4 / 5-9 / 10-16
+//     level 1, delays 9-16
+//     level 2, delays 9-16
 //   Each lane keeps a 32-value register window of its level and processes
 //   products in groups of 16 a-indices; block sums of the blocked variance
 //   (2^(ind-level) products, multiples of 16) are closed at group ends.
@@ -378,9 +378,10 @@ __device__ void run_role(const MainArgs &A, const Tile *tl2, int lane, int p0, i
 // 12 waves with partner series (10 roles + 2 loaders), 6 without (5 + 1).
 // Waves go to the four SIMDs round-robin, so wave w shares its SIMD with
 // w +- 4 and w +- 8; the role order below gives every SIMD about the same
-// work per tile (two level-0 roles and a level-2 role on two SIMDs; a
-// level-0 role, a level-1 role and a loader on the other two).  Each role
-// has one call site, so that every role is inlined.
+// work per tile: two 5-delay level-0 roles and a level-2 role on two SIMDs;
+// the 7-delay level-0 role, a level-1 role and a loader (few arithmetic
+// instructions) on the other two.  Each role has one call site, so that
+// every role is inlined.
 __global__ void __launch_bounds__(768) k_corr_main(MainArgs A) {
     __shared__ Tile tl2[2];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -397,12 +398,12 @@ __global__ void __launch_bounds__(768) k_corr_main(MainArgs A) {
     const int role = pairs2 ? wave : (wave == 1 ? 4 : wave == 3 ? 6 : wave == 4 ? 8 : wave);
     __syncthreads();
     switch (role) {
-        case 0: run_role<6, 0, 0, false>(A, tl2, lane, p0, chunk); break;
-        case 1: run_role<6, 0, 0, true>(A, tl2, lane, p0, chunk); break;
-        case 2: run_role<5, 12, 0, false>(A, tl2, lane, p0, chunk); break;
-        case 3: run_role<5, 12, 0, true>(A, tl2, lane, p0, chunk); break;
-        case 4: run_role<6, 6, 0, false>(A, tl2, lane, p0, chunk); break;
-        case 5: run_role<6, 6, 0, true>(A, tl2, lane, p0, chunk); break;
+        case 0: run_role<5, 0, 0, false>(A, tl2, lane, p0, chunk); break;
+        case 1: run_role<5, 0, 0, true>(A, tl2, lane, p0, chunk); break;
+        case 2: run_role<7, 10, 0, false>(A, tl2, lane, p0, chunk); break;
+        case 3: run_role<7, 10, 0, true>(A, tl2, lane, p0, chunk); break;
+        case 4: run_role<5, 5, 0, false>(A, tl2, lane, p0, chunk); break;
+        case 5: run_role<5, 5, 0, true>(A, tl2, lane, p0, chunk); break;
         case 6: run_role<8, 9, 1, false>(A, tl2, lane, p0, chunk); break;
         case 7: run_role<8, 9, 1, true>(A, tl2, lane, p0, chunk); break;
         case 8: run_role<8, 9, 2, false>(A, tl2, lane, p0, chunk); break;
