@@ -320,12 +320,13 @@ __device__ void run_role(const MainArgs &A, const Tile *tl2, int lane, int p0, i
 #pragma unroll
                 for (int i = 0; i < NL; i++) {
                     const int d = DMIN + i;
-                    // one FMA chain per delay: the role's delays are
-                    // independent chains, and its SIMD runs two more waves
-                    float s0 = acc.sab[i];
+                    // the group's 16 products summed on their own, then added
+                    // to the block sum once (one rounding at the block sum's
+                    // magnitude per group)
+                    float s0 = 0.f;
 #pragma unroll
                     for (int k = 0; k < 16; k++) s0 = fmaf(ga[k], w[k + d], s0);
-                    acc.sab[i] = s0;
+                    acc.sab[i] += s0;
                 }
                 const int ye = y0 + 16;
                 if ((ye & (B - 1)) == 0) {
