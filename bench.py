@@ -154,9 +154,9 @@ def main():
     # roofline of the main correlator kernel at b = 1 (HIP events on its
     # stream, averaged over the timed steps): it reads the corrected FP32
     # series of every slot once (4 B per slot-frame) and writes the level-3
-    # sums of every series (4 B per 8 frames)
+    # sums of every slot once (4 B per 8 frames)
     nslots_b1 = W * H   # 128 x 128 slot grid (127 output columns + the x+1 partner column)
-    alg_bytes = 4.0 * nslots_b1 * T + 4.0 * 2 * 127 * 128 * (T // 8)
+    alg_bytes = 4.0 * nslots_b1 * T + 4.0 * nslots_b1 * (T // 8)
     corr_ms = kern_ms[0]
     achieved = alg_bytes / (corr_ms * 1e-3) / 1e9
     roof = {"kernel": "k_corr_main (b=1, ACF+CCF)", "bound": "hbm", "achieved": round(achieved, 1),
