@@ -7,8 +7,9 @@ One step = for each binning b in (1, 2, 4): fused b x b binning, order-4
 polynomial bleach correction, ACF and CCF to the x+b neighbour with blocked
 variances, and the LM fit of N, D, G_inf for every ACF and CCF pixel.  The
 three binnings run as three pipelines on three HIP streams (one engine each,
-all reading the one device-resident stack), so the tail of one pipeline's
-fit overlaps the others' work.  The stack is device-resident before timing
+all reading the one device-resident stack, started by imfcs_run_multi: one
+pass over the stack sums every pixel over each bleach-trace point for all
+three), so the tail of one pipeline's fit overlaps the others' work.  The stack is device-resident before timing
 starts.  ``value`` counts ACF-fitted output pixels (the CCF work is inside
 the timed region but not counted).
 
@@ -123,9 +124,10 @@ def main():
 
     def step():
         # the coarse binnings are enqueued first: their pipelines are short
-        # and end in a few long LM fits, which then overlap the b=1 pipeline
-        for e, c in reversed(list(zip(engines, cfgs))):
-            e.run(c, d_stack)
+        # and end in a few long LM fits, which then overlap the b=1 pipeline.
+        # One pass over the stack gives every binning its bleach trace
+        # (imfcs_run_multi)
+        imfcs.Engine.run_multi(engines[::-1], cfgs[::-1], d_stack)
         stage, kern = [], []
         for e in engines:
             e.synchronize()

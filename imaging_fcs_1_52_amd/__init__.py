@@ -306,6 +306,21 @@ class Engine:
     def synchronize(self):
         _check(self._lib.imfcs_synchronize(self.ctx))
 
+    @staticmethod
+    def run_multi(engines, cfgs, d_stack):
+        """imfcs_run_multi: cfgs[i] on engines[i] over one device stack (the
+        binnings of a diffusion law); with the polynomial correction the
+        bleach traces of all configurations come from one pass over the
+        stack.  Asynchronous, as run()."""
+        n = len(engines)
+        if len(cfgs) != n or n == 0:
+            raise ValueError("one configuration per engine is needed")
+        ctxs = (ctypes.c_void_p * n)(*[e.ctx for e in engines])
+        arr = (PipelineConfig * n)(*cfgs)
+        _check(lib().imfcs_run_multi(ctxs, n, d_stack, arr))
+        for e, c in zip(engines, cfgs):
+            e._chanum_last = c.correlator_p + (c.correlator_q - 1) * c.correlator_p // 2 + 1
+
     def stage_ms(self):
         """Device times (ms) of the last run's stages: trace + polynomial fit,
         bleach correction + window means, blocking, correlation, LM fit."""

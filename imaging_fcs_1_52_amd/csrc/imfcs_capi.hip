@@ -203,6 +203,8 @@ struct imfcs_ctx {
     std::vector<float> h_eitime;
     std::vector<int> h_epfit;
     DevBuf swacf, swvar;   // sliding-window averages
+    DevBuf psums;          // per-pixel bleach-point sums shared by imfcs_run_multi
+    hipEvent_t ev_psums = nullptr;
     DevBuf etrace, eitime, epfit, einit, eout, estate, echi2, eiter;   // exponential bleach fits
     DevBuf part, coef, ctot, rec, F, bpart, ind, bis, corav, bvar, fdata, fw, fx, finit, fpfit, fout, fstate, fchi2,
         fiter, cpart, hpart, l3, lvd, ctab, csum, slots, X, glsblk, glsW, glsy, glsok, glscov, dropin_in, nbm, nbc;
@@ -277,6 +279,7 @@ imfcs_ctx *imfcs_create(int device) {
     }
     for (auto &e : ctx->ev) (void)hipEventCreate(&e);
     (void)hipEventCreateWithFlags(&ctx->ev_h2d, hipEventDisableTiming);
+    (void)hipEventCreateWithFlags(&ctx->ev_psums, hipEventDisableTiming);
     for (auto &e : ctx->evk) (void)hipEventCreate(&e);
     const char *sel = std::getenv("IMFCS_CORRELATOR");
     ctx->force_v1 = (sel && std::strcmp(sel, "generic") == 0) ? 1 : 0;
@@ -291,7 +294,7 @@ static void release_all(imfcs_ctx *ctx) {
                       &ctx->X, &ctx->glsblk, &ctx->glsW, &ctx->glsy, &ctx->glsok, &ctx->glscov,
                       &ctx->dropin_in, &ctx->nbm, &ctx->nbc, &ctx->swacf, &ctx->swvar, &ctx->etrace,
                       &ctx->eitime, &ctx->epfit, &ctx->einit, &ctx->eout, &ctx->estate, &ctx->echi2,
-                      &ctx->eiter};
+                      &ctx->eiter, &ctx->psums};
     for (DevBuf *b : bufs) b->release();
 }
 
@@ -302,6 +305,7 @@ void imfcs_destroy(imfcs_ctx *ctx) {
     release_all(ctx);
     for (auto &e : ctx->ev) (void)hipEventDestroy(e);
     if (ctx->ev_h2d) (void)hipEventDestroy(ctx->ev_h2d);
+    if (ctx->ev_psums) (void)hipEventDestroy(ctx->ev_psums);
     for (auto &e : ctx->evk) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -419,6 +423,9 @@ struct EngineArgs {
     PairDesc pd;
     int seg_swl;           // > 0: "Lin Segment" correction with this sliding window length
     int exp_n;             // 1: "Single Exp" correction
+    // per-pixel point sums of the stack (P[nopit][plane], this run's first
+    // frame, nopit and ave) to take the trace from instead of the stack
+    const int32_t *psums;
 };
 
 int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTables &ct) {
@@ -481,8 +488,12 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTabl
         const int nchunk = std::min(std::min(nopit, 128), std::max(40, (1 << 19) / std::max(1, ns)));
         HIP_TRY(ctx->part.ensure(sizeof(double) * (size_t)nchunk * ns));
         HIP_TRY(ctx->etrace.ensure(sizeof(float) * (size_t)ns * nopit));
-        launch_trace_moments(g, nopit, ea.ave, 1.0, ea.frametime, 0, 1, ctx->part.as<double>(), nchunk, st,
-                             ctx->etrace.as<float>());
+        if (ea.psums)
+            launch_trace_points(g, ea.psums, nopit, ea.ave, 1.0, ea.frametime, 0, ctx->part.as<double>(), nchunk, st,
+                                ctx->etrace.as<float>());
+        else
+            launch_trace_moments(g, nopit, ea.ave, 1.0, ea.frametime, 0, 1, ctx->part.as<double>(), nchunk, st,
+                                 ctx->etrace.as<float>());
         ctx->h_eitime.resize(nopit);
         for (int x = 0; x < nopit; x++) ctx->h_eitime[x] = (float)(ea.frametime * (x + 0.5) * ea.ave);   // IFCS:9851
         ctx->h_epfit.assign(11, 0);
@@ -515,8 +526,12 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTabl
             double tscale = ea.frametime * (ea.nopit - 0.5) * ea.ave;
             int nchunk = std::min(std::min(ea.nopit, 128), std::max(40, (1 << 19) / std::max(1, ns)));
             HIP_TRY(ctx->part.ensure(sizeof(double) * (size_t)nchunk * ns * n));
-            launch_trace_moments(g, ea.nopit, ea.ave, tscale, ea.frametime, ea.order, 1, ctx->part.as<double>(),
-                                 nchunk, st);
+            if (ea.psums)
+                launch_trace_points(g, ea.psums, ea.nopit, ea.ave, tscale, ea.frametime, ea.order,
+                                    ctx->part.as<double>(), nchunk, st);
+            else
+                launch_trace_moments(g, ea.nopit, ea.ave, tscale, ea.frametime, ea.order, 1, ctx->part.as<double>(),
+                                     nchunk, st);
             launch_polyfit(ctx->part.as<double>(), nchunk, ns, ea.nopit, ea.ave, tscale, ea.frametime, ea.order,
                            ctx->coef.as<double>(), st);
             bd.tscale = tscale;
@@ -633,7 +648,7 @@ int check_geom(const imfcs_config *c, int ow, int oh) {
 
 }  // namespace
 
-extern "C" int imfcs_run(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_config *c) {
+static int run_config(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_config *c, const int32_t *psums) {
     if (!ctx || !d_stack || !c) return fail(IMFCS_ERR_ARG, "null argument");
     if (!c->do_acf && !c->do_ccf) return fail(IMFCS_ERR_ARG, "neither ACF nor CCF requested");
     HIP_TRY(hipSetDevice(ctx->device));
@@ -676,6 +691,7 @@ extern "C" int imfcs_run(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_co
     ea.pd.acf_first = c->do_acf ? 1 : 0;
     ea.pd.has_ccf = c->do_ccf ? 1 : 0;
     ea.pd.npairs = g.npix * ((c->do_acf ? 1 : 0) + (c->do_ccf ? 1 : 0));
+    ea.psums = psums;
     ctx->npix = g.npix;
     ctx->ow = ow;
     ctx->oh = oh;
@@ -794,6 +810,58 @@ extern "C" int imfcs_run(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_co
     }
     HIP_TRY(hipEventRecord(ctx->ev[5], st));
     HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int imfcs_run(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_config *c) {
+    return run_config(ctx, d_stack, c, nullptr);
+}
+
+// Several configurations of one stack (binnings of a diffusion law, or the
+// benchmark's three binnings), one context (stream) each.  When every
+// configuration takes the polynomial bleach correction over the same frames
+// and full frame size, one pass over the stack writes the per-pixel sums of
+// each bleach-trace point (on the first context's stream) and every
+// pipeline takes its trace from them; the other streams wait for that pass.
+extern "C" int imfcs_run_multi(imfcs_ctx *const *ctxs, int n, const uint16_t *d_stack, const imfcs_config *cfgs) {
+    if (!ctxs || !cfgs || !d_stack || n < 1) return fail(IMFCS_ERR_ARG, "null argument");
+    for (int i = 0; i < n; i++)
+        if (!ctxs[i]) return fail(IMFCS_ERR_ARG, "null context");
+    bool share = true;
+    for (int i = 0; i < n && share; i++) {
+        const imfcs_config &c = cfgs[i];
+        share = c.bleach_mode == IMFCS_BLEACH_POLYNOMIAL && c.poly_order > 0 && c.width == cfgs[0].width &&
+                c.height == cfgs[0].height && c.firstframe == cfgs[0].firstframe &&
+                c.lastframe == cfgs[0].lastframe && ctxs[i]->device == ctxs[0]->device;
+        for (int j = 0; j < i && share; j++) share = ctxs[j] != ctxs[i];
+    }
+    if (!share) {
+        for (int i = 0; i < n; i++) {
+            const int rc = run_config(ctxs[i], d_stack, &cfgs[i], nullptr);
+            if (rc) return rc;
+        }
+        return 0;
+    }
+    imfcs_ctx *c0 = ctxs[0];
+    const imfcs_config &k = cfgs[0];
+    const int T = k.lastframe - k.firstframe + 1;
+    if (T < 1 || k.firstframe < 1 || k.width < 1 || k.height < 1) return fail(IMFCS_ERR_ARG, "bad frame range");
+    const int nopit = std::min(T, 1000), ave = T / nopit;   // as imfcs_run (IFCS:7323-7327)
+    const int64_t plane = (int64_t)k.width * k.height;
+    HIP_TRY(hipSetDevice(c0->device));
+    // the previous runs of every context may still read the sums: wait for them
+    for (int i = 1; i < n; i++) HIP_TRY(hipStreamWaitEvent(c0->stream, ctxs[i]->ev[5], 0));
+    HIP_TRY(c0->psums.ensure(sizeof(int32_t) * (size_t)nopit * plane));
+    launch_point_sums(d_stack, plane, k.firstframe - 1, nopit, ave, c0->psums.as<int32_t>(), c0->stream);
+    HIP_TRY(hipEventRecord(c0->ev_psums, c0->stream));
+    for (int i = 1; i < n; i++) HIP_TRY(hipStreamWaitEvent(ctxs[i]->stream, c0->ev_psums, 0));
+    // enqueued in the given order: a pipeline is ~25 launches (≈0.4 ms of
+    // host time), so later configurations start later (enqueueing them from
+    // parallel host threads measured no faster end to end)
+    for (int i = 0; i < n; i++) {
+        const int rc = run_config(ctxs[i], d_stack, &cfgs[i], c0->psums.as<int32_t>());
+        if (rc) return rc;
+    }
     return 0;
 }
 

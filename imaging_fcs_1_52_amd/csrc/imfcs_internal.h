@@ -138,6 +138,13 @@ void launch_lm_gls(int nfits, int m, const float *x, const float *init, const in
 void launch_trace_moments(const SeriesGeom &g, int nopit, int ave, double tscale, double frametime,
                           int order, int integer_trace, double *part, int nchunk, hipStream_t st,
                           float *trace_out = nullptr);
+// Per-pixel sums of every bleach-trace point over the whole uint16 frame,
+// P[nopit][plane] (int32), and the trace moments of each slot taken from them.
+void launch_point_sums(const uint16_t *data, int64_t plane, int f0, int nopit, int ave, int32_t *P,
+                       hipStream_t st);
+void launch_trace_points(const SeriesGeom &g, const int32_t *P, int nopit, int ave, double tscale,
+                         double frametime, int order, double *part, int nchunk, hipStream_t st,
+                         float *trace_out = nullptr);
 // Exponential bleach fits (SingleExpFit / DoubleExpFit): initial guesses
 // from the traces, the LM fit of every slot's trace (FP64 chi-square), and
 // coef[slot] = {f(0), params} (f(0) = 0: no correction).

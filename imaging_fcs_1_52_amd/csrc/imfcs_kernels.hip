@@ -73,6 +73,93 @@ __global__ void __launch_bounds__(64) k_trace_moments(SeriesGeom g, int nopit, i
         if (k <= order) part[base + k] = acc[k];
 }
 
+// Per-pixel integer sums of every bleach-trace point (frames x*ave ..
+// (x+1)*ave - 1 after the first frame) over the whole uint16 frame, for
+// several binnings of one stack: the trace of a window is the sum of its
+// pixels' point sums (the Java integer division, IFCS:9852, comes after the
+// sum), so one pass over the stack serves every binning.
+// P[x][plane], int32; VEC consecutive pixels per lane (VEC = 4: one 8-byte
+// load per frame).
+template <int VEC>
+__global__ void __launch_bounds__(256) k_point_sums(const uint16_t *__restrict__ data, int64_t plane, int f0,
+                                                    int nopit, int ave, int pts_per_thread, int32_t *P) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;   // pixel group
+    if (q * VEC >= plane) return;
+    const int x0 = blockIdx.y * pts_per_thread, x1 = min(nopit, x0 + pts_per_thread);
+    for (int x = x0; x < x1; x++) {
+        int acc[VEC];
+#pragma unroll
+        for (int v = 0; v < VEC; v++) acc[v] = 0;
+        const uint16_t *base = data + ((int64_t)f0 + (int64_t)x * ave) * plane + q * VEC;   // wave-uniform frame
+        for (int f = 0; f < ave; f += 16) {
+            const int nv = min(16, ave - f);
+            if (VEC == 4) {
+                uint2 r[16];
+#pragma unroll
+                for (int j = 0; j < 16; j++) r[j] = *(const uint2 *)(base + (int64_t)(f + min(j, nv - 1)) * plane);
+#pragma unroll
+                for (int j = 0; j < 16; j++)
+                    if (j < nv) {
+                        acc[0] += (int)(r[j].x & 0xffffu);
+                        acc[VEC > 1 ? 1 : 0] += (int)(r[j].x >> 16);
+                        acc[VEC > 2 ? 2 : 0] += (int)(r[j].y & 0xffffu);
+                        acc[VEC > 3 ? 3 : 0] += (int)(r[j].y >> 16);
+                    }
+            } else {
+                uint16_t r[16];
+#pragma unroll
+                for (int j = 0; j < 16; j++) r[j] = base[(int64_t)(f + min(j, nv - 1)) * plane];
+#pragma unroll
+                for (int j = 0; j < 16; j++)
+                    if (j < nv) acc[0] += r[j];
+            }
+        }
+        int32_t *out = P + (int64_t)x * plane + q * VEC;
+        if (VEC == 4) {
+            *(int4 *)out = make_int4(acc[0], acc[VEC > 1 ? 1 : 0], acc[VEC > 2 ? 2 : 0], acc[VEC > 3 ? 3 : 0]);
+        } else {
+            out[0] = acc[0];
+        }
+    }
+}
+
+// k_trace_moments from the point sums: the integer trace of each slot's
+// window, y = (sum of P over the window - ave * background) / ave with
+// Java's integer division, and its moments; the same points per chunk
+// and the same summation order as k_trace_moments (identical results).
+__global__ void __launch_bounds__(64) k_trace_points(SeriesGeom g, const int32_t *__restrict__ P, int nopit,
+                                                     int ave, double tscale, double frametime, int order,
+                                                     int pts_per_chunk, double *part, float *trace_out) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;   // slot
+    int chunk = blockIdx.y;
+    if (s >= g.nslots) return;
+    int ox, oy;
+    slot_origin(g, s, ox, oy);
+    const int x0 = chunk * pts_per_chunk, x1 = min(nopit, x0 + pts_per_chunk);
+    const int64_t lo = (int64_t)oy * g.stride + ox;
+    double acc[kMaxOrder + 1];
+#pragma unroll
+    for (int k = 0; k <= kMaxOrder; k++) acc[k] = 0.0;
+    for (int x = x0; x < x1; x++) {
+        const int32_t *px = P + (int64_t)x * g.plane + lo;
+        int isum = -ave * g.bckg_total;
+        for (int k = 0; k < g.by; k++)
+            for (int i = 0; i < g.bx; i++) isum += px[(int64_t)k * g.stride + i];
+        const double y = (double)(isum / ave);
+        if (trace_out) trace_out[(int64_t)s * nopit + x] = (float)y;
+        double u = frametime * (x + 0.5) * ave / tscale, up = 1.0;
+#pragma unroll
+        for (int k = 0; k <= kMaxOrder; k++) {
+            if (k <= order) acc[k] += y * up;
+            up *= u;
+        }
+    }
+    int64_t base = ((int64_t)chunk * g.nslots + s) * (order + 1);
+#pragma unroll
+    for (int k = 0; k <= kMaxOrder; k++)
+        if (k <= order) part[base + k] = acc[k];
+}
+
 struct PolyL {
     double L[(kMaxOrder + 1) * (kMaxOrder + 1)];
 };
@@ -1729,6 +1816,26 @@ void launch_exp_init(const float *trace, const float *itime, int nslots, int nop
 
 void launch_exp_pack(const double *out64, int nslots, int nexp, double *coef, hipStream_t st) {
     hipLaunchKernelGGL(k_exp_pack, dim3(cdiv(nslots, 64)), dim3(64), 0, st, out64, nslots, nexp, coef);
+}
+
+void launch_point_sums(const uint16_t *data, int64_t plane, int f0, int nopit, int ave, int32_t *P,
+                       hipStream_t st) {
+    const int ppt = 8;
+    const bool vec4 = plane % 4 == 0 && (uintptr_t)data % 8 == 0;
+    const int64_t groups = vec4 ? plane / 4 : plane;
+    dim3 grid((unsigned)((groups + 255) / 256), (unsigned)cdiv(nopit, ppt));
+    if (vec4)
+        hipLaunchKernelGGL(k_point_sums<4>, grid, dim3(256), 0, st, data, plane, f0, nopit, ave, ppt, P);
+    else
+        hipLaunchKernelGGL(k_point_sums<1>, grid, dim3(256), 0, st, data, plane, f0, nopit, ave, ppt, P);
+}
+
+void launch_trace_points(const SeriesGeom &g, const int32_t *P, int nopit, int ave, double tscale,
+                         double frametime, int order, double *part, int nchunk, hipStream_t st, float *trace_out) {
+    int ppc = cdiv(nopit, nchunk);
+    dim3 grid(cdiv(g.nslots, 64), nchunk);
+    hipLaunchKernelGGL(k_trace_points, grid, dim3(64), 0, st, g, P, nopit, ave, tscale, frametime, order, ppc, part,
+                       trace_out);
 }
 
 void launch_polyfit(const double *part, int nchunk, int nseries, int nopit, int ave, double tscale,

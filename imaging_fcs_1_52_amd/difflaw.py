@@ -15,7 +15,8 @@ least-squares optimum is the closed-form weighted line below).
 
 Here every binning is one native pipeline run on the device-resident
 stack, one engine (HIP stream) per binning, so the binnings run
-concurrently.
+concurrently (imfcs_run_multi: with the polynomial correction their bleach
+traces come from one pass over the stack).
 """
 import numpy as np
 
@@ -45,13 +46,12 @@ def diffusion_law(stack, numbin, pixel_size_m, psf_m, background=None, poly_orde
     try:
         dev = d_stack if d_stack is not None else engines[0].upload(stack)
         cfgs = []
-        for u, eng in zip(bins, engines):
+        for u in bins:
             ax = u * pixel_size_m
             model = make_fit_model(ax, ax, psf_m, sz=sz)
-            cfg = make_pipeline_config(W, H, T, binning=u, cf=(0, 0), background=bg, poly_order=poly_order,
-                                       frametime=frametime, do_fit=True, acf_model=model)
-            eng.run(cfg, dev)
-            cfgs.append(cfg)
+            cfgs.append(make_pipeline_config(W, H, T, binning=u, cf=(0, 0), background=bg, poly_order=poly_order,
+                                             frametime=frametime, do_fit=True, acf_model=model))
+        Engine.run_multi(engines[:len(bins)], cfgs, dev)
         out = {k: [] for k in ("binning", "aeff", "d_mean", "d_var", "tau", "err", "n_fitted", "d_maps")}
         for u, eng, cfg in zip(bins, engines, cfgs):
             eng.synchronize()

@@ -77,6 +77,7 @@ SIGNATURES = {
     "imfcs_memcpy_d2d": (ctypes.c_int, [_VP, _VP, _VP, _SZ]),
     "imfcs_synchronize": (ctypes.c_int, [_VP]),
     "imfcs_run": (ctypes.c_int, [_VP, _VP, _P(PipelineConfig)]),
+    "imfcs_run_multi": (ctypes.c_int, [_P(_VP), ctypes.c_int, _VP, _P(PipelineConfig)]),
     "imfcs_get_results": (ctypes.c_int, [_VP, _P(_f64), _P(_f64), _P(_i32), _P(_i32), _P(_f64), _P(_f64),
                                          _P(_i32), _P(_i32), _P(_f64)]),
     "imfcs_get_stage_ms": (ctypes.c_int, [_VP, _P(_f32)]),

@@ -218,6 +218,19 @@ int imfcs_synchronize(imfcs_ctx *ctx);
  * on the context's stream. */
 int imfcs_run(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_config *cfg);
 
+/* n configurations of one device stack, cfgs[i] on ctxs[i] (distinct
+ * contexts, one stream each) -- the binnings of a diffusion law
+ * (determineDiffusionLaw, IFCS:8190-8215, runs one pipeline per binning).
+ * No single reference call corresponds: it replaces n calls of the
+ * per-binning path.  When all take the polynomial bleach correction over
+ * the same frame range and frame size, one pass over the stack sums every
+ * pixel over each bleach-trace point (on ctxs[0]'s stream) and every
+ * pipeline takes its intensity trace from those sums instead of the stack
+ * (identical traces: Java's integer division comes after the window sum);
+ * the other streams wait for that pass.  Otherwise equal to n imfcs_run
+ * calls.  Asynchronous; results per context as after imfcs_run. */
+int imfcs_run_multi(imfcs_ctx *const *ctxs, int n, const uint16_t *d_stack, const imfcs_config *cfgs);
+
 /* Regularized covariance (chanum-1)^2, FP64, row-major, of one pair of the
  * last imfcs_run with cfg.gls = 1 (covmats of IFCS:9601-9613). */
 int imfcs_get_gls_covariance(imfcs_ctx *ctx, int pair, double *cov);

@@ -1,0 +1,59 @@
+"""imfcs_run_multi against separate imfcs_run calls on the same stack: the
+bleach traces taken from the shared per-pixel point sums must give
+identical results (the integer window sums are exact and the moments are
+summed in the same order), for several binnings, partners and background
+levels; a set with a non-polynomial configuration falls back to separate
+runs."""
+import numpy as np
+import pytest
+
+import imaging_fcs_1_52_amd as imfcs
+from parity import models
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("acf", "var", "ind", "blockIndS", "fitres", "chi2", "states", "iters", "coef")
+
+
+def _cfg(W, H, T, b, cf, bg, order):
+    acf, ccf, _, _ = models(b, cf)
+    return imfcs.make_pipeline_config(W, H, T, binning=b, cf=cf, background=bg, poly_order=order, do_fit=True,
+                                      acf_model=acf, ccf_model=ccf)
+
+
+@pytest.mark.parametrize("T", [6000, 4321])
+def test_run_multi_matches_separate_runs(engine, T):
+    W, H = 24, 16
+    print(f"run_multi T={T}", flush=True)
+    stack = imfcs.simulate_2d(W, H, T, 150, D_um2s=1.0, tau_bleach_s=10.0, seed=7 + T)
+    d = engine.upload(stack)
+    bg = int(stack.min())
+    cfgs = [_cfg(W, H, T, 4, (4, 0), bg, 4), _cfg(W, H, T, 2, (0, 2), bg - 3, 4), _cfg(W, H, T, 1, (1, 0), bg, 3),
+            _cfg(W, H, T, 3, (0, 0), bg + 1, 4)]
+    sep = []
+    for c in cfgs:
+        engine.run(c, d)
+        engine.synchronize()
+        sep.append(engine.results(c))
+    engines = [imfcs.Engine(0) for _ in cfgs]
+    try:
+        imfcs.Engine.run_multi(engines, cfgs, d)
+        for e, c, r in zip(engines, cfgs, sep):
+            e.synchronize()
+            m = e.results(c)
+            for k in KEYS:
+                np.testing.assert_array_equal(np.asarray(m[k]), np.asarray(r[k]), err_msg=k)
+        # a configuration without polynomial correction: separate runs
+        c0 = _cfg(W, H, T, 2, (2, 0), bg, 0)
+        engine.run(c0, d)
+        engine.synchronize()
+        r0 = engine.results(c0)
+        imfcs.Engine.run_multi(engines[:2], [c0, cfgs[0]], d)
+        engines[0].synchronize()
+        engines[1].synchronize()
+        m0 = engines[0].results(c0)
+        for k in ("acf", "var", "fitres"):
+            np.testing.assert_array_equal(np.asarray(m0[k]), np.asarray(r0[k]), err_msg=k)
+    finally:
+        for e in engines:
+            e.close()
