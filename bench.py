@@ -32,6 +32,7 @@ T = 50000
 N_EMITTERS = 2000
 D_QUADRANTS = [0.5, 1.0, 2.0, 5.0]
 BINNINGS = (1, 2, 4)
+ENQUEUE_ORDER = (2, 4, 1)
 PIX = 0.24e-6
 PSF = 0.38e-6
 HBM_PEAK_GBS = 8000.0
@@ -124,10 +125,12 @@ def main():
 
     def step():
         # the coarse binnings are enqueued first: their pipelines are short
-        # and end in a few long LM fits, which then overlap the b=1 pipeline.
-        # One pass over the stack gives every binning its bleach trace
+        # and end in a few long LM fits, which then overlap the b=1 pipeline
+        # (b=2, whose CCF fit tail is the longest, first of all).  One pass
+        # over the stack gives every binning its bleach trace
         # (imfcs_run_multi)
-        imfcs.Engine.run_multi(engines[::-1], cfgs[::-1], d_stack)
+        order = [BINNINGS.index(b) for b in ENQUEUE_ORDER]
+        imfcs.Engine.run_multi([engines[i] for i in order], [cfgs[i] for i in order], d_stack)
         stage, kern = [], []
         for e in engines:
             e.synchronize()
