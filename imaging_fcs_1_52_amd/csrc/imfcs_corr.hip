@@ -335,11 +335,11 @@ __device__ void run_role(const MainArgs &A, const Tile *tl2, int lane, int p0, i
                     // tail terms sum_{k<d} b[ye+k]: the next group's values
                     float tail[NL];
                     {
-                        float t = 0.f;
+                        float run = 0.f;
 #pragma unroll
                         for (int k = 0; k < DMIN + NL; k++) {
-                            if (k >= DMIN) tail[k - DMIN] = t;
-                            if (k + 1 < DMIN + NL) t += w[16 + k];
+                            if (k >= DMIN) tail[k - DMIN] = run;
+                            if (k + 1 < DMIN + NL) run += w[16 + k];
                         }
                     }
                     const float sbb = CCF ? sb_blk : sa_blk;
