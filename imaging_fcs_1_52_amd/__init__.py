@@ -306,6 +306,11 @@ class Engine:
     def synchronize(self):
         _check(self._lib.imfcs_synchronize(self.ctx))
 
+    def refit(self, cfg):
+        """imfcs_refit: the LM stage alone on the last run's correlations,
+        with cfg's fit models (cfg must match the last run elsewhere)."""
+        _check(self._lib.imfcs_refit(self.ctx, ctypes.byref(cfg)))
+
     @staticmethod
     def run_multi(engines, cfgs, d_stack):
         """imfcs_run_multi: cfgs[i] on engines[i] over one device stack (the

@@ -217,6 +217,10 @@ struct imfcs_ctx {
     int force_v1 = 0;   // IMFCS_CORRELATOR=generic selects the one-thread-per-level kernel
     double chi2_norm = 1.0;
     std::vector<double> coef_scale;   // u-basis -> seconds conversion per power
+    // what imfcs_refit needs of the last run: its configuration and lag times
+    imfcs_config last_cfg{};
+    int has_last = 0;
+    std::vector<double> last_lagtime;
 };
 
 extern "C" {
@@ -648,6 +652,77 @@ int check_geom(const imfcs_config *c, int ow, int oh) {
 
 }  // namespace
 
+// The LM stage over the correlations of the last run held in ctx (corav,
+// bvar, or the GLS-whitened data), with cfg's fit models and settings.
+static int run_fit(imfcs_ctx *ctx, const imfcs_config *c) {
+    {
+        hipStream_t st = ctx->stream;
+        const int np = ctx->npairs, npts = ctx->chanum - 1;
+        HIP_TRY(ctx->fdata.ensure(sizeof(float) * (size_t)np * npts));
+        HIP_TRY(ctx->fw.ensure(sizeof(float) * (size_t)np * npts));
+        HIP_TRY(ctx->fout.ensure(sizeof(float) * (size_t)np * 20));
+        HIP_TRY(ctx->fstate.ensure(sizeof(int) * np));
+        HIP_TRY(ctx->fchi2.ensure(sizeof(float) * np));
+        HIP_TRY(ctx->fiter.ensure(sizeof(int) * np));
+        if (!ctx->did_gls)
+            launch_pack_fit_inputs(ctx->chanum, np, ctx->corav.as<double>(), ctx->bvar.as<double>(), ctx->fdata.as<float>(),
+                               ctx->fw.as<float>(), st);
+        // lag times of channels 1..chanum-1 (IFCS:15122-15125) and initial
+        // parameters with the fixed geometry in slots 11..19 (IFCS:15090-15110):
+        // every ACF fit starts from the ACF model's row, every CCF fit from
+        // the CCF model's, so two rows travel and the kernel picks one
+        std::vector<float> &xs = ctx->h_xs, &init = ctx->h_init;
+        std::vector<int> &pfit = ctx->h_pfit;
+        xs.assign(npts, 0.f);
+        init.assign(2 * 20, 0.f);
+        pfit.assign(20, 0);
+        for (int i = 0; i < npts; i++) xs[i] = (float)ctx->last_lagtime[i + 1];
+        const imfcs_fit_model *models[2] = {&c->acf_model, &c->ccf_model};
+        for (int k = 0; k < 11; k++) pfit[k] = c->acf_model.paramfit[k] ? 1 : 0;
+        int nfree = 0;
+        for (int k = 0; k < 11; k++) nfree += pfit[k];
+        const int init_split = c->do_acf ? ctx->npix : 0;
+        for (int r = 0; r < 2; r++) {
+            const imfcs_fit_model *m = models[r];
+            float *q = &init[(size_t)r * 20];
+            for (int k = 0; k < 11; k++) q[k] = (float)m->paraminit[k];
+            q[11] = (float)m->ax;
+            q[12] = (float)m->ay;
+            q[13] = (float)m->s;
+            q[14] = (float)m->sz;
+            q[15] = (float)m->rx;
+            q[16] = (float)m->ry;
+            q[17] = (float)m->fitobsvol;
+            q[18] = (float)m->q2;
+            q[19] = (float)m->q3;
+        }
+        HIP_TRY(ctx->up_fx.put(ctx->fx, xs.data(), sizeof(float) * npts, st));
+        HIP_TRY(ctx->up_finit.put(ctx->finit, init.data(), sizeof(float) * init.size(), st));
+        HIP_TRY(ctx->up_fpfit.put(ctx->fpfit, pfit.data(), sizeof(int) * 20, st));
+        HIP_TRY(hipEventRecord(ctx->ev_h2d, st));
+        unsigned mask = 0;
+        for (int k = 0; k < 11; k++) mask |= (pfit[k] ? 1u : 0u) << k;
+        ctx->nfree = nfree;
+        if (ctx->did_gls) {
+            if (mask != 0x13u) return fail(IMFCS_ERR_ARG, "the GLS fit supports the free set N, D, G_inf");
+            launch_lm_gls(np, npts, ctx->fx.as<float>(), ctx->finit.as<float>(), ctx->fpfit.as<int>(),
+                          c->fit_tolerance, c->fit_max_iterations, ctx->glsW.as<double>(), ctx->glsy.as<double>(),
+                          ctx->glsok.as<int32_t>(), ctx->fout.as<float>(), ctx->fstate.as<int>(),
+                          ctx->fchi2.as<float>(), ctx->fiter.as<int>(), st, init_split);
+            // chi2 of GLSFit: sum of whitened residuals^2 / (chanum - nfree - 1) (IFCS:10733-10735)
+            ctx->chi2_norm = (double)(ctx->chanum - nfree - 1);
+        } else {
+            launch_lm_acf(np, npts, ctx->fdata.as<float>(), ctx->fw.as<float>(), ctx->fx.as<float>(),
+                          ctx->finit.as<float>(), ctx->fpfit.as<int>(), mask, LMModelGeom{}, c->fit_tolerance,
+                          c->fit_max_iterations, c->fit_chi2_fp64 ? 1 : 0, ctx->fout.as<float>(),
+                          ctx->fstate.as<int>(), ctx->fchi2.as<float>(), ctx->fiter.as<int>(), st, init_split);
+            // chi2 normalization of GPU_ACF_Fit (IFCS:15234)
+            ctx->chi2_norm = (double)((npts - 1) - nfree - 1);
+        }
+    }
+    return 0;
+}
+
 static int run_config(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_config *c, const int32_t *psums) {
     if (!ctx || !d_stack || !c) return fail(IMFCS_ERR_ARG, "null argument");
     if (!c->do_acf && !c->do_ccf) return fail(IMFCS_ERR_ARG, "neither ACF nor CCF requested");
@@ -743,70 +818,13 @@ static int run_config(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_confi
         rc = run_engine(ctx, ea, ht, ct);
         if (rc) return rc;
     }
+    ctx->last_cfg = *c;
+    ctx->last_lagtime = ht.lagtime;
+    ctx->has_last = 1;
     ctx->did_fit = c->do_fit;
     if (c->do_fit) {
-        const int np = ea.pd.npairs, npts = ct.chanum - 1;
-        HIP_TRY(ctx->fdata.ensure(sizeof(float) * (size_t)np * npts));
-        HIP_TRY(ctx->fw.ensure(sizeof(float) * (size_t)np * npts));
-        HIP_TRY(ctx->fout.ensure(sizeof(float) * (size_t)np * 20));
-        HIP_TRY(ctx->fstate.ensure(sizeof(int) * np));
-        HIP_TRY(ctx->fchi2.ensure(sizeof(float) * np));
-        HIP_TRY(ctx->fiter.ensure(sizeof(int) * np));
-        if (!ctx->did_gls)
-            launch_pack_fit_inputs(ct, np, ctx->corav.as<double>(), ctx->bvar.as<double>(), ctx->fdata.as<float>(),
-                               ctx->fw.as<float>(), st);
-        // lag times of channels 1..chanum-1 (IFCS:15122-15125) and initial
-        // parameters with the fixed geometry in slots 11..19 (IFCS:15090-15110):
-        // every ACF fit starts from the ACF model's row, every CCF fit from
-        // the CCF model's, so two rows travel and the kernel picks one
-        std::vector<float> &xs = ctx->h_xs, &init = ctx->h_init;
-        std::vector<int> &pfit = ctx->h_pfit;
-        xs.assign(npts, 0.f);
-        init.assign(2 * 20, 0.f);
-        pfit.assign(20, 0);
-        for (int i = 0; i < npts; i++) xs[i] = (float)ht.lagtime[i + 1];
-        const imfcs_fit_model *models[2] = {&c->acf_model, &c->ccf_model};
-        for (int k = 0; k < 11; k++) pfit[k] = c->acf_model.paramfit[k] ? 1 : 0;
-        int nfree = 0;
-        for (int k = 0; k < 11; k++) nfree += pfit[k];
-        const int init_split = c->do_acf ? g.npix : 0;
-        for (int r = 0; r < 2; r++) {
-            const imfcs_fit_model *m = models[r];
-            float *q = &init[(size_t)r * 20];
-            for (int k = 0; k < 11; k++) q[k] = (float)m->paraminit[k];
-            q[11] = (float)m->ax;
-            q[12] = (float)m->ay;
-            q[13] = (float)m->s;
-            q[14] = (float)m->sz;
-            q[15] = (float)m->rx;
-            q[16] = (float)m->ry;
-            q[17] = (float)m->fitobsvol;
-            q[18] = (float)m->q2;
-            q[19] = (float)m->q3;
-        }
-        HIP_TRY(ctx->up_fx.put(ctx->fx, xs.data(), sizeof(float) * npts, st));
-        HIP_TRY(ctx->up_finit.put(ctx->finit, init.data(), sizeof(float) * init.size(), st));
-        HIP_TRY(ctx->up_fpfit.put(ctx->fpfit, pfit.data(), sizeof(int) * 20, st));
-        HIP_TRY(hipEventRecord(ctx->ev_h2d, st));
-        unsigned mask = 0;
-        for (int k = 0; k < 11; k++) mask |= (pfit[k] ? 1u : 0u) << k;
-        ctx->nfree = nfree;
-        if (ctx->did_gls) {
-            if (mask != 0x13u) return fail(IMFCS_ERR_ARG, "the GLS fit supports the free set N, D, G_inf");
-            launch_lm_gls(np, npts, ctx->fx.as<float>(), ctx->finit.as<float>(), ctx->fpfit.as<int>(),
-                          c->fit_tolerance, c->fit_max_iterations, ctx->glsW.as<double>(), ctx->glsy.as<double>(),
-                          ctx->glsok.as<int32_t>(), ctx->fout.as<float>(), ctx->fstate.as<int>(),
-                          ctx->fchi2.as<float>(), ctx->fiter.as<int>(), st, init_split);
-            // chi2 of GLSFit: sum of whitened residuals^2 / (chanum - nfree - 1) (IFCS:10733-10735)
-            ctx->chi2_norm = (double)(ct.chanum - nfree - 1);
-        } else {
-            launch_lm_acf(np, npts, ctx->fdata.as<float>(), ctx->fw.as<float>(), ctx->fx.as<float>(),
-                          ctx->finit.as<float>(), ctx->fpfit.as<int>(), mask, LMModelGeom{}, c->fit_tolerance,
-                          c->fit_max_iterations, c->fit_chi2_fp64 ? 1 : 0, ctx->fout.as<float>(),
-                          ctx->fstate.as<int>(), ctx->fchi2.as<float>(), ctx->fiter.as<int>(), st, init_split);
-            // chi2 normalization of GPU_ACF_Fit (IFCS:15234)
-            ctx->chi2_norm = (double)((npts - 1) - nfree - 1);
-        }
+        rc = run_fit(ctx, c);
+        if (rc) return rc;
     }
     HIP_TRY(hipEventRecord(ctx->ev[5], st));
     HIP_TRY(hipGetLastError());
@@ -815,6 +833,40 @@ static int run_config(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_confi
 
 extern "C" int imfcs_run(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_config *c) {
     return run_config(ctx, d_stack, c, nullptr);
+}
+
+// Configurations that give the same correlations (every field but the fit's).
+static bool same_correlation(const imfcs_config &a, const imfcs_config &b) {
+    return a.width == b.width && a.height == b.height && a.frames == b.frames && a.firstframe == b.firstframe &&
+           a.lastframe == b.lastframe && a.binning_x == b.binning_x && a.binning_y == b.binning_y &&
+           a.pixbin_x == b.pixbin_x && a.pixbin_y == b.pixbin_y && a.cf_x == b.cf_x && a.cf_y == b.cf_y &&
+           a.background == b.background && a.correlator_p == b.correlator_p &&
+           a.correlator_q == b.correlator_q && a.frametime == b.frametime && a.poly_order == b.poly_order &&
+           a.roi_x == b.roi_x && a.roi_y == b.roi_y && a.roi_w == b.roi_w && a.roi_h == b.roi_h &&
+           a.do_acf == b.do_acf && a.do_ccf == b.do_ccf && a.gls == b.gls && a.bleach_mode == b.bleach_mode &&
+           a.sliding_window_length == b.sliding_window_length;
+}
+
+// Fit stage only, on the correlations of the last imfcs_run of ctx: the
+// PSF calibration (determinePSF, IFCS:8118-8160) repeats the whole
+// diffusion-law pipeline for every trial PSF size although only the fit
+// depends on it.
+extern "C" int imfcs_refit(imfcs_ctx *ctx, const imfcs_config *c) {
+    if (!ctx || !c) return fail(IMFCS_ERR_ARG, "null argument");
+    if (!ctx->has_last) return fail(IMFCS_ERR_ARG, "no correlations to refit: run the pipeline first");
+    if (!same_correlation(*c, ctx->last_cfg))
+        return fail(IMFCS_ERR_ARG, "refit configuration differs from the last run outside the fit settings");
+    HIP_TRY(hipSetDevice(ctx->device));
+    // the host staging of the fit parameters is reused: wait for its last copies
+    HIP_TRY(hipEventSynchronize(ctx->ev_h2d));
+    HIP_TRY(hipEventRecord(ctx->ev[4], ctx->stream));
+    ctx->did_fit = 1;
+    const int rc = run_fit(ctx, c);
+    if (rc) return rc;
+    ctx->last_cfg = *c;
+    HIP_TRY(hipEventRecord(ctx->ev[5], ctx->stream));
+    HIP_TRY(hipGetLastError());
+    return 0;
 }
 
 // Several configurations of one stack (binnings of a diffusion law, or the

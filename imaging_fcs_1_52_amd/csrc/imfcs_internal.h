@@ -203,7 +203,7 @@ void launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st);
 void launch_accum_div(double *dst, const double *src, int64_t n, double div, int first, hipStream_t st);
 void launch_nb_extract(const CorrTables *dct, const SeriesGeom &g, const PairDesc &pd, const double *F,
                        const double *corav, double *nbmean, double *nbcov, hipStream_t st);
-void launch_pack_fit_inputs(const CorrTables &ct, int npairs, const double *corav, const double *bvar,
+void launch_pack_fit_inputs(int chanum, int npairs, const double *corav, const double *bvar,
                             float *data, float *weights, hipStream_t st);
 
 }  // namespace imfcs

@@ -2061,11 +2061,10 @@ void launch_bleach_average_f32(const float *in, float *out, int wt, int ht, int 
     hipLaunchKernelGGL(k_bleach_average_f32, grid, dim3(64), 0, st, in, out, wt, ht, nopit, ave);
 }
 
-void launch_pack_fit_inputs(const CorrTables &ct, int npairs, const double *corav, const double *bvar, float *data,
+void launch_pack_fit_inputs(int chanum, int npairs, const double *corav, const double *bvar, float *data,
                             float *weights, hipStream_t st) {
     if (npairs <= 0) return;
-    hipLaunchKernelGGL(k_pack_fit_inputs, dim3(npairs), dim3(128), 0, st, ct.chanum, npairs, corav, bvar, data,
-                       weights);
+    hipLaunchKernelGGL(k_pack_fit_inputs, dim3(npairs), dim3(128), 0, st, chanum, npairs, corav, bvar, data, weights);
 }
 
 }  // namespace imfcs

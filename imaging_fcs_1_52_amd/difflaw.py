@@ -97,3 +97,78 @@ def fit_diffusion_law(aeff, tau, err, first=1, last=5):
     slope = (sw * sxy - sx * sy) / den
     intercept = (sy - slope * sx) / sw
     return intercept, slope
+
+
+def psf_calibration(stack, psf_values_m, bin_start, bin_end, pixel_size_m, background=None, poly_order=4,
+                    frametime=1e-3, sz=None, engines=None, d_stack=None):
+    """PSF calibration (determinePSF, IFCS:8118-8160): the diffusion-law
+    statistics of binnings 1..numbin (numbin = bin_end - bin_start + 1; dlfit
+    runs binnings 1..numbin whatever bin_start is, and the plot labels them
+    u + bin_start, IFCS:8143-8146) for each trial PSF size.
+
+    The reference reruns the whole pipeline per PSF size; only the fit
+    depends on it, so every binning is correlated once and refitted
+    (imfcs_refit) for the other sizes.  psf_values_m: the trial PSF sizes
+    in metres (the plugin's (psfStart + x psfStep) * lambda / NA / 1e9,
+    IFCS:8142), fewer than 10 (IFCS:8135).
+
+    Returns a dict with 'label' (u + bin_start), 'd_mean' and 'd_var'
+    ([npsf][numbin], um^2/s and (um^2/s)^2, over the converged pixels), 'err'
+    (the reference's error bar: the variance divided by the square root of
+    Java's integer pixel count ((W / b) * H) / b, IFCS:8147) and 'n_fitted'."""
+    stack = np.asarray(stack)
+    T, H, W = stack.shape
+    psf_values_m = list(psf_values_m)
+    numbin = bin_end - bin_start + 1
+    if W < 20 or H < 20:
+        raise ValueError("at least 20x20 pixels are required (IFCS:8123)")
+    if not psf_values_m or len(psf_values_m) >= 10:
+        raise ValueError("1 to 9 PSF sizes (IFCS:8135)")
+    if numbin < 1 or numbin > min(W, H):
+        raise ValueError("bad binning range")
+    bg = int(stack.min()) if background is None else int(background)
+    bins = list(range(1, numbin + 1))
+    own = engines is None
+    if own:
+        engines = [Engine(0) for _ in bins]
+    if len(engines) < len(bins):
+        raise ValueError("one engine per binning is needed")
+
+    def cfg_for(u, psf):
+        ax = u * pixel_size_m
+        return make_pipeline_config(W, H, T, binning=u, cf=(0, 0), background=bg, poly_order=poly_order,
+                                    frametime=frametime, do_fit=True, acf_model=make_fit_model(ax, ax, psf, sz=sz))
+
+    try:
+        dev = d_stack if d_stack is not None else engines[0].upload(stack)
+        out = {"label": np.asarray(bins) + (bin_start - 1),
+               "d_mean": np.full((len(psf_values_m), numbin), np.nan), "d_var": np.full((len(psf_values_m), numbin),
+                                                                                        np.nan),
+               "err": np.full((len(psf_values_m), numbin), np.nan), "n_fitted": np.zeros((len(psf_values_m), numbin),
+                                                                                         int)}
+        for x, psf in enumerate(psf_values_m):
+            cfgs = [cfg_for(u, psf) for u in bins]
+            if x == 0:
+                Engine.run_multi(engines[:numbin], cfgs, dev)
+            else:
+                for e, c in zip(engines, cfgs):
+                    e.refit(c)
+            for i, (u, e, c) in enumerate(zip(bins, engines, cfgs)):
+                e.synchronize()
+                r = e.results(c)
+                d = np.asarray(r["fitres"])[0][..., 1] * 1e12
+                ok = (np.asarray(r["states"])[0] == 0) & np.isfinite(d)
+                n = int(ok.sum())
+                out["n_fitted"][x, i] = n
+                if n:
+                    dv = d[ok]
+                    m = float(dv.mean())
+                    var = float((dv * dv).mean() - m * m)
+                    out["d_mean"][x, i] = m
+                    out["d_var"][x, i] = var
+                    out["err"][x, i] = var / np.sqrt(((W // u) * H) // u)
+        return out
+    finally:
+        if own:
+            for e in engines:
+                e.close()

@@ -78,6 +78,7 @@ SIGNATURES = {
     "imfcs_synchronize": (ctypes.c_int, [_VP]),
     "imfcs_run": (ctypes.c_int, [_VP, _VP, _P(PipelineConfig)]),
     "imfcs_run_multi": (ctypes.c_int, [_P(_VP), ctypes.c_int, _VP, _P(PipelineConfig)]),
+    "imfcs_refit": (ctypes.c_int, [_VP, _P(PipelineConfig)]),
     "imfcs_get_results": (ctypes.c_int, [_VP, _P(_f64), _P(_f64), _P(_i32), _P(_i32), _P(_f64), _P(_f64),
                                          _P(_i32), _P(_i32), _P(_f64)]),
     "imfcs_get_stage_ms": (ctypes.c_int, [_VP, _P(_f32)]),

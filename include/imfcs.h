@@ -231,6 +231,14 @@ int imfcs_run(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_config *cfg);
  * calls.  Asynchronous; results per context as after imfcs_run. */
 int imfcs_run_multi(imfcs_ctx *const *ctxs, int n, const uint16_t *d_stack, const imfcs_config *cfgs);
 
+/* The fit stage alone on the correlations of the last imfcs_run of ctx,
+ * with cfg's fit models and settings (every other field of cfg must equal
+ * the last run's).  The PSF calibration (determinePSF, IFCS:8118-8160)
+ * reruns dlfit -- the whole pipeline of every binning -- for each trial
+ * PSF size although only the fit depends on it; no single reference call
+ * corresponds.  Asynchronous; results as after imfcs_run. */
+int imfcs_refit(imfcs_ctx *ctx, const imfcs_config *cfg);
+
 /* Regularized covariance (chanum-1)^2, FP64, row-major, of one pair of the
  * last imfcs_run with cfg.gls = 1 (covmats of IFCS:9601-9613). */
 int imfcs_get_gls_covariance(imfcs_ctx *ctx, int pair, double *cov);
