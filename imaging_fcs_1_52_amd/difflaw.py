@@ -172,3 +172,79 @@ def psf_calibration(stack, psf_values_m, bin_start, bin_end, pixel_size_m, backg
         if own:
             for e in engines:
                 e.close()
+
+
+def diffusion_law_map(stack, numbin, roi_dim, pixel_size_m, psf_m, background=None, poly_order=4, frametime=1e-3,
+                      sz=None, fit_first=1, fit_last=5, engines=None, d_stack=None):
+    """Diffusion-law map ("ROI" mode of dlfit, IFCS:11068-11080, 11084-11181,
+    and of determineDiffusionLaw, IFCS:8218-8245): the image is cut into
+    floor(W / roi_dim) x floor(H / roi_dim) square regions; for every
+    binning u = 1..numbin the pixels are taken on the overlapping grid
+    (stride 1, output pixel x = 0..W-u) and region (r, s) averages the fitted
+    D over its pixels x = roi_dim r .. roi_dim (r+1) - u (the same in y);
+    each region's diffusion law is then fitted over points fit_first..
+    fit_last (fitDiffLaw, limits 1..5 in the plugin).
+
+    One pipeline per binning over the whole image (imfcs_run_multi), the
+    regions are sums over its D map.  Returns a dict with 'aeff' (numbin,
+    um^2), 'd_mean', 'd_var', 'tau', 'err' ([mh][mw][numbin]), 'n_fitted'
+    and 'fit' ([mh][mw][2]: intercept in s, slope in s/um^2; NaN where a
+    region has no fitted pixel at some binning of the fit range)."""
+    stack = np.asarray(stack)
+    T, H, W = stack.shape
+    if roi_dim < 1 or roi_dim > min(W, H) or numbin < 1 or numbin > roi_dim:
+        raise ValueError("roi_dim must fit the image and numbin the region")
+    mw, mh = W // roi_dim, H // roi_dim
+    bg = int(stack.min()) if background is None else int(background)
+    bins = list(range(1, numbin + 1))
+    own = engines is None
+    if own:
+        engines = [Engine(0) for _ in bins]
+    if len(engines) < len(bins):
+        raise ValueError("one engine per binning is needed")
+    try:
+        dev = d_stack if d_stack is not None else engines[0].upload(stack)
+        cfgs = []
+        for u in bins:
+            ax = u * pixel_size_m
+            cfgs.append(make_pipeline_config(W, H, T, binning=u, cf=(0, 0), background=bg, poly_order=poly_order,
+                                             frametime=frametime, do_fit=True, overlap=True,
+                                             acf_model=make_fit_model(ax, ax, psf_m, sz=sz)))
+        Engine.run_multi(engines[:len(bins)], cfgs, dev)
+        shape = (mh, mw, numbin)
+        out = {"aeff": np.zeros(numbin), "d_mean": np.full(shape, np.nan), "d_var": np.full(shape, np.nan),
+               "n_fitted": np.zeros(shape, int), "d_maps": []}
+        for i, (u, eng, cfg) in enumerate(zip(bins, engines, cfgs)):
+            eng.synchronize()
+            r = eng.results(cfg)
+            d = np.asarray(r["fitres"])[0][..., 1] * 1e12
+            ok = (np.asarray(r["states"])[0] == 0) & np.isfinite(d)
+            out["aeff"][i] = obsvol(u * pixel_size_m, u * pixel_size_m, psf_m) * 1e12
+            out["d_maps"].append(np.where(ok, d, np.nan))
+            for s in range(mh):
+                for q in range(mw):
+                    ys = slice(roi_dim * s, roi_dim * (s + 1) - u + 1)
+                    xs = slice(roi_dim * q, roi_dim * (q + 1) - u + 1)
+                    sel = ok[ys, xs]
+                    n = int(sel.sum())
+                    out["n_fitted"][s, q, i] = n
+                    if n:
+                        dv = d[ys, xs][sel]
+                        m = float(dv.mean())
+                        out["d_mean"][s, q, i] = m
+                        out["d_var"][s, q, i] = float((dv * dv).mean() - m * m)
+        out["tau"] = out["aeff"] / out["d_mean"]
+        out["err"] = out["aeff"] / out["d_mean"] ** 2 * np.sqrt(np.maximum(out["d_var"], 0.0))
+        fit = np.full((mh, mw, 2), np.nan)
+        for s in range(mh):
+            for q in range(mw):
+                lo, hi = fit_first - 1, fit_last - 1
+                seg_ok = np.isfinite(out["tau"][s, q, lo:hi + 1]).all() and (out["err"][s, q, lo:hi + 1] > 0).all()
+                if hi < numbin and seg_ok:
+                    fit[s, q] = fit_diffusion_law(out["aeff"], out["tau"][s, q], out["err"][s, q], fit_first, fit_last)
+        out["fit"] = fit
+        return out
+    finally:
+        if own:
+            for e in engines:
+                e.close()

@@ -1,0 +1,61 @@
+"""Diffusion-law map ("ROI" mode of dlfit / determineDiffusionLaw): the
+overlapping-grid pipeline it runs per binning against the oracle, and the
+map's per-region statistics and line fits recomputed from its D maps with
+the region bounds of dlfit (pixels roi r .. roi (r+1) - u)."""
+import numpy as np
+import pytest
+
+import imaging_fcs_1_52_amd as imfcs
+import oracle
+from imaging_fcs_1_52_amd.difflaw import diffusion_law_map, fit_diffusion_law
+from parity import PIX, PSF, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("b", [2, 3])
+def test_overlap_grid_matches_oracle(engine, b):
+    W, H, T = 11, 9, 6000
+    stack = imfcs.simulate_2d(W, H, T, 50, D_um2s=1.0, tau_bleach_s=20.0, seed=300 + b)
+    bg = int(stack.min())
+    ax = b * PIX
+    cfg = imfcs.make_pipeline_config(W, H, T, binning=b, background=bg, poly_order=4, overlap=True,
+                                     acf_model=imfcs.make_fit_model(ax, ax, PSF))
+    engine.upload(stack)
+    engine.run(cfg)
+    g = engine.results(cfg)
+    assert g["acf"].shape[1:3] == (H - b + 1, W - b + 1)
+    o = oracle.pipeline(stack, oracle.make_config(W, H, T, binning=b, background=bg, poly_order=4, overlap=True,
+                                                  model=oracle.make_model(ax, ax, PSF), nthreads=8))
+    fin = np.isfinite(o["acf"]).all(-1)
+    assert np.array_equal(np.isfinite(g["acf"][0]).all(-1), fin)
+    amp = np.abs(o["acf"][fin][:, 1:]).max(-1, keepdims=True)
+    assert rel_err(g["acf"][0][fin], o["acf"][fin], 1e-3 * amp).max() < 1e-4
+    same = (g["ind"][0] == o["ind"]) & fin
+    assert same[fin].mean() >= 0.9
+    ok = (g["states"][0] == 0) & o["fitted"] & same
+    assert ok.any()
+    assert rel_err(g["fitres"][0][..., 1][ok], o["fitres"][..., 1][ok], 1e-30).max() < 1e-3
+
+
+def test_diffusion_law_map_regions():
+    W, H, T, roi, nb = 24, 24, 5000, 12, 5
+    stack = imfcs.simulate_2d(W, H, T, 200, D_um2s=1.0, tau_bleach_s=20.0, seed=321)
+    m = diffusion_law_map(stack, nb, roi, PIX, PSF)
+    assert m["d_mean"].shape == (2, 2, nb) and m["fit"].shape == (2, 2, 2)
+    for i, u in enumerate(range(1, nb + 1)):
+        dmap = m["d_maps"][i]
+        assert dmap.shape == (H - u + 1, W - u + 1)
+        for s in range(2):
+            for q in range(2):
+                reg = dmap[roi * s: roi * (s + 1) - u + 1, roi * q: roi * (q + 1) - u + 1]
+                v = reg[np.isfinite(reg)]
+                assert m["n_fitted"][s, q, i] == v.size
+                assert m["d_mean"][s, q, i] == pytest.approx(v.mean(), rel=1e-12)
+                assert m["d_var"][s, q, i] == pytest.approx((v * v).mean() - v.mean() ** 2, rel=1e-9, abs=1e-18)
+    for s in range(2):
+        for q in range(2):
+            ic, sl = fit_diffusion_law(m["aeff"], m["tau"][s, q], m["err"][s, q], 1, 5)
+            assert m["fit"][s, q, 0] == pytest.approx(ic, rel=1e-12)
+            assert m["fit"][s, q, 1] == pytest.approx(sl, rel=1e-12)
+            assert 0.2 < 1.0 / sl < 5.0   # D_eff (um^2/s) of the simulated D = 1 um^2/s
