@@ -57,3 +57,32 @@ def test_run_multi_matches_separate_runs(engine, T):
     finally:
         for e in engines:
             e.close()
+
+
+def test_run_multi_frame_range_and_roi(engine):
+    W, H, T = 20, 14, 5000
+    stack = imfcs.simulate_2d(W, H, T, 120, D_um2s=1.0, tau_bleach_s=10.0, seed=41)
+    d = engine.upload(stack)
+    bg = int(stack.min())
+    cfgs = []
+    for b, cf, roi in ((1, (1, 0), (3, 2, 12, 9)), (2, (0, 2), (2, 0, 14, 12)), (3, (0, 0), (1, 1, 18, 12))):
+        acf, ccf, _, _ = models(b, cf)
+        cfgs.append(imfcs.make_pipeline_config(W, H, T, binning=b, cf=cf, background=bg, poly_order=4,
+                                               acf_model=acf, ccf_model=ccf, roi=roi, firstframe=101,
+                                               lastframe=T - 50))
+    sep = []
+    for c in cfgs:
+        engine.run(c, d)
+        engine.synchronize()
+        sep.append(engine.results(c))
+    engines = [imfcs.Engine(0) for _ in cfgs]
+    try:
+        imfcs.Engine.run_multi(engines, cfgs, d)
+        for e, c, r in zip(engines, cfgs, sep):
+            e.synchronize()
+            m = e.results(c)
+            for k in KEYS:
+                np.testing.assert_array_equal(np.asarray(m[k]), np.asarray(r[k]), err_msg=k)
+    finally:
+        for e in engines:
+            e.close()
