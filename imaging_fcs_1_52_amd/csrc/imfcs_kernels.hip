@@ -1514,8 +1514,10 @@ __global__ void __launch_bounds__(128) k_lm_fcs_2w(LMArgs a) {
     // decides step acceptance and convergence, needs FP64)
     __shared__ double red_c2[2][2];
     __shared__ float red[2][2][NS];
-    const int fit = blockIdx.x;
-    if (fit >= a.nfits) return;
+    // last fit first: the CCF fits (the second half), whose few longest
+    // fits set the kernel's duration, are dispatched before the ACF fits
+    const int fit = a.nfits - 1 - (int)blockIdx.x;
+    if (fit < 0) return;
     const int tid = threadIdx.x, wv = tid >> 6;
     float par[NP];
 #pragma unroll
