@@ -117,6 +117,23 @@ __device__ __forceinline__ void raw_ints16(const SeriesGeom &g, int ox, int oy, 
         }
         return;
     }
+    if ((int64_t)pl * 32 < 0x7fffffff) {   // wave-uniform
+        // buffer loads: the 16 frames differ by a scalar offset, so a load
+        // needs no address arithmetic of its own
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc((void *)q, (short)0, (int)(pl * 32), 0x00020000);
+        for (int k = 0; k < g.by; k++)
+            for (int i = 0; i < g.bx; i++) {
+                const int d2 = (int)(lo + (uint32_t)(k * g.stride + i)) * 2;
+                uint16_t r[16];
+#pragma unroll
+                for (int j = 0; j < 16; j++)
+                    r[j] = (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(rs, d2, min(j, nv - 1) * (int)pl * 2, 0);
+#pragma unroll
+                for (int j = 0; j < 16; j++) acc[j] += r[j];
+            }
+        return;
+    }
     for (int k = 0; k < g.by; k++)
         for (int i = 0; i < g.bx; i++) {
             const uint32_t d = lo + (uint32_t)(k * g.stride + i);
@@ -154,35 +171,38 @@ __device__ __forceinline__ double raw_value(const SeriesGeom &g, int ox, int oy,
 // Per-slot bleach polynomial held in registers, normalized by its value at
 // t = 0 so that g(u) = f(u) / c0.  MAXO is a compile-time degree >= the run's
 // order; the coefficients above the order are zero, which leaves Horner's
-// result unchanged (0 * u + 0 = 0 until the leading coefficient).
+// result unchanged (0 * u + 0 = 0 until the leading coefficient).  MAXO = 0
+// is the uncorrected series; MAXO > 0 is instantiated only for order >= 1
+// (bleach_maxo), so apply() needs no per-value test of the order.
 template <int MAXO>
 struct BleachP {
     double c[MAXO + 1];
     double c0, scale, off;
-    bool on;
     __device__ void load(const BleachDesc &bd, int slot) {
         const int order = bd.order;
-        on = order > 0;
-        c0 = on ? bd.coef[(int64_t)slot * (order + 1)] : 1.0;
+        c0 = MAXO > 0 ? bd.coef[(int64_t)slot * (order + 1)] : 1.0;
 #pragma unroll
         for (int k = 0; k <= MAXO; k++)
-            c[k] = (on && k <= order) ? bd.coef[(int64_t)slot * (order + 1) + k] / c0 : 0.0;
+            c[k] = (MAXO > 0 && k <= order) ? bd.coef[(int64_t)slot * (order + 1) + k] / c0 : 0.0;
         scale = bd.frametime / bd.tscale;
         off = bd.t_off;
     }
     // I' = I / sqrt(f/c0) + c0 (1 - sqrt(f/c0))   (IFCS:9744-9750), with
-    // 1/sqrt(g) from the FP32 reciprocal square root refined by one FP64
-    // Newton step (relative error ~1e-14).  g <= 0 yields NaN as in Java.
+    // g = f/c0 and r = 1/sqrt(g) written as r (I - c0 g) + c0 (sqrt(g) =
+    // g r).  r is the hardware FP64 reciprocal square root (about 2^-23
+    // relative) refined by one Newton step (relative error ~1e-14).  g < 0
+    // yields NaN as in Java; g = 0 yields NaN where Java has +-inf (the
+    // pixel's correlation is NaN either way).
     __device__ __forceinline__ double apply(double v, int f) const {
-        if (MAXO == 0 || !on) return v;
+        if (MAXO == 0) return v;
         const double u = ((double)f + off) * scale;
         double g = 0.0;
 #pragma unroll
         for (int k = MAXO; k >= 0; k--) g = fma(g, u, c[k]);
-        double r = (double)rsqrtf((float)g);
-        r = r * fma(-0.5 * g * r, r, 1.5);
-        const double sq = g * r;
-        return fma(v, r, c0 * (1.0 - sq));
+        const double y = __builtin_amdgcn_rsq(g);
+        const double e = fma(-(g * y), y, 1.0);   // 1 - g y^2
+        const double r = fma(0.5 * y, e, y);
+        return fma(r, v - c0 * g, c0);
     }
 };
 

@@ -228,10 +228,25 @@ __global__ void __launch_bounds__(64) k_apply(SeriesGeom g, BleachDesc bd, const
     int next = (q < npos) ? ct.pos[q] : 0x7fffffff;
     double run = 0.0;
     const uint32_t ld = (uint32_t)g.nslots;
+    const bool bstore = (int64_t)ld * 64 < 0x7fffffff;
     for (int fb = f0; fb < f1; fb += 16) {
         double v[16];
         raw_values16<NB>(g, ox, oy, fb, f1 - 1, v);
         float *xw = X + (int64_t)fb * g.nslots;   // wave-uniform
+        if (bstore && fb + 16 <= f1 && next >= fb + 16) {   // wave-uniform
+            // a full group with no prefix record: straight-line code, buffer
+            // stores with the frame's row as the scalar offset
+            const __amdgpu_buffer_rsrc_t rs =
+                __builtin_amdgcn_make_buffer_rsrc((void *)xw, (short)0, (int)(ld * 64), 0x00020000);
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const double a = bl.apply(v[i], fb + i);
+                run += a;
+                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, (float)(a - mu)), rs, k * 4,
+                                                      i * (int)ld * 4, 0);
+            }
+            continue;
+        }
 #pragma unroll
         for (int i = 0; i < 16; i++) {
             const int f = fb + i;
@@ -792,8 +807,11 @@ __device__ __forceinline__ void fcs_dim(float a, float r, float v, float x, floa
 __device__ __forceinline__ void fcs_component(const LMModelGeom &m, float x, float D, float vx, float vy, bool grad,
                                               Comp &o) {
     const float isqrpi = 0.56418958354775628f;
-    float p0t = sqrtf(4.f * D * x + m.s * m.s);
-    float ip0 = frcp(p0t), ip02 = ip0 * ip0;
+    // p0 = sqrt(4 D x + s^2) and 1/p0 from one hardware reciprocal square
+    // root (1 ulp), the model's precision elsewhere
+    const float p0sq = 4.f * D * x + m.s * m.s;
+    const float ip0 = __builtin_amdgcn_rsqf(p0sq), ip02 = ip0 * ip0;
+    const float p0t = p0sq * ip0;
     DimTerms tx, ty;
     fcs_dim(m.ax, m.rx, vx, x, p0t, ip0, ip02, grad, tx);
     if (m.ay == m.ax && m.ry == m.rx && vy == vx)
@@ -804,11 +822,12 @@ __device__ __forceinline__ void fcs_component(const LMModelGeom &m, float x, flo
     o.plat = tx.l * ty.l * inorm;
     float szi2 = frcp(m.sz * m.sz);
     float q = 1.f + 4.f * D * x * szi2;
-    o.pspim = rsqrtf(q);
+    o.pspim = __builtin_amdgcn_rsqf(q);
     o.acf = o.plat * o.pspim;
     if (grad) {
         o.dDplat = (ip0 * isqrpi) * (ty.dDpexp * x * tx.l + tx.dDpexp * x * ty.l) * inorm;
-        o.dDpspim = -4.f * x * frcp(2.f * m.sz * m.sz * q * sqrtf(q));
+        // -4 x / (2 sz^2 q^(3/2)), q^(-3/2) = pspim^3
+        o.dDpspim = -2.f * x * szi2 * (o.pspim * o.pspim * o.pspim);
         o.dvx = ty.l * ((tx.e2 + 2.f * tx.e3 - tx.e1) * x);
         o.dvy = tx.l * ((ty.e2 + 2.f * ty.e3 - ty.e1) * x);
     }
@@ -1249,7 +1268,9 @@ __device__ __forceinline__ bool chol_solve_fixed(double (&A)[NF][NF], double (&b
 #pragma unroll
         for (int k = 0; k < j; k++) s -= A[j][k] * A[j][k];
         if (!(s > 0.0)) return false;
-        inv[j] = rsqrt(s);
+        // hardware FP64 reciprocal square root and one Newton step
+        const double y = __builtin_amdgcn_rsq(s);
+        inv[j] = fma(0.5 * y, fma(-(s * y), y, 1.0), y);
 #pragma unroll
         for (int i = j + 1; i < NF; i++) {
             double t = A[i][j];
@@ -1329,7 +1350,7 @@ __device__ void lm_fixed_iterate(const LMArgs &a, PT *par, Eval &eval, double &c
         // cuda_modify_step_widths (agpufitjni.cu:2044-2076)
 #pragma unroll
         for (int k = 0; k < NF; k++) {
-            if (failed) H[k][k] -= scale[k] * lambda / 10.0;
+            if (failed) H[k][k] -= scale[k] * (lambda * 0.1);   // the previous lambda
             scale[k] = fmax(scale[k], H[k][k]);
             H[k][k] += scale[k] * lambda;
         }
