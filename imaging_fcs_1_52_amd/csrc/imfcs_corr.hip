@@ -653,6 +653,7 @@ __global__ void k_corr_finalize(MainArgs A, const double *hpart, int nhigh, doub
     const double alpha = (double)(float)(am - mua), beta = (double)(float)(bm - mub);
     double cross = 0.0, a1 = 0.0, a2 = 0.0;
     if (x < kMainChan) {
+#pragma unroll 4
         for (int k = 0; k < A.nchunk; k++) {
             const float *o = A.part + (((int64_t)k * A.pd.npairs + pidx) * kMainChan + x) * 3;
             cross += (double)o[0];
@@ -660,6 +661,7 @@ __global__ void k_corr_finalize(MainArgs A, const double *hpart, int nhigh, doub
             a2 += (double)o[2];
         }
     } else {
+#pragma unroll 4
         for (int k = 0; k < A.nchunk; k++) {
             const double *o = hpart + (((int64_t)k * A.pd.npairs + pidx) * nhigh + (x - kMainChan)) * 3;
             cross += o[0];

@@ -172,6 +172,7 @@ __global__ void k_polyfit(const double *part, int nchunk, int nseries, int order
     double r[kMaxOrder + 1], z[kMaxOrder + 1];
 #pragma unroll
     for (int k = 0; k <= kMaxOrder; k++) r[k] = 0.0;
+#pragma unroll 8
     for (int c = 0; c < nchunk; c++) {
         int64_t base = ((int64_t)c * nseries + s) * n;
 #pragma unroll
@@ -270,20 +271,48 @@ __global__ void __launch_bounds__(64) k_apply(SeriesGeom g, BleachDesc bd, const
     ctot[(int64_t)chunk * g.nslots + k] = run;
 }
 
-__global__ void k_prefix_combine(const CorrTables *__restrict__ ctp, int nslots, const double *ctot,
-                                 const double *rec, int nchunk, int chunk_len, double *F) {
+// F[s][k] = sum_{c < ck} ctot[c][s] + rec[s][k], ck = the chunk of position
+// k (positions ascending).  One wave per slot: the chunk totals are scanned
+// 64 at a time across the lanes, and each lane completes one position, so
+// the kernel's latency is a few round trips instead of one per chunk and
+// position.
+__global__ void __launch_bounds__(256) k_prefix_combine(const CorrTables *__restrict__ ctp, int nslots,
+                                                        const double *ctot, const double *rec, int nchunk,
+                                                        int chunk_len, double *F) {
     const CorrTables &ct = *ctp;
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nslots) return;
-    double acc = 0.0;
-    int c = 0;
-    for (int k = 0; k < ct.npos; k++) {
-        int ck = min(ct.pos[k] / chunk_len, nchunk - 1);
-        while (c < ck) {
-            acc += ctot[(int64_t)c * nslots + s];
-            c++;
+    const int s = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (s >= nslots) return;   // wave-uniform
+    const int npos = ct.npos;
+    const double *recs = rec + (int64_t)s * npos;
+    double *Fs = F + (int64_t)s * npos;
+    double carry = 0.0;   // sum of the chunks before this segment
+    int k = 0;            // next position
+    for (int c0 = 0; c0 < nchunk && k < npos; c0 += 64) {
+        const int c = c0 + lane;
+        const double v = c < nchunk ? ctot[(int64_t)c * nslots + s] : 0.0;
+        double incl = v;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const double t = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += t;
         }
-        F[(int64_t)s * ct.npos + k] = acc + rec[(int64_t)s * ct.npos + k];
+        double excl = __shfl_up(incl, 1, 64);
+        if (lane == 0) excl = 0.0;
+        const double P = carry + excl;   // sum of the chunks before chunk c
+        // positions whose chunk lies in this segment (the last segment
+        // holds the chunk of every remaining position)
+        for (;;) {
+            const int kk = k + lane;
+            const int ck = kk < npos ? min(ct.pos[kk] / chunk_len, nchunk - 1) : 0x7fffffff;
+            const bool mine = ck < c0 + 64;
+            const double Pk = __shfl(P, (ck - c0) & 63, 64);
+            if (mine) Fs[kk] = Pk + recs[kk];
+            const int nm = __popcll(__ballot(mine));
+            k += nm;
+            if (nm < 64) break;
+        }
+        carry += __shfl(incl, 63, 64);
     }
 }
 
@@ -1961,7 +1990,7 @@ void launch_seg_means(const SeriesGeom &g, int seg_len, int nseg, double *coef, 
 
 void launch_prefix_combine(const CorrTables *dct, int nslots, const double *ctot, const double *rec, int nchunk,
                            int chunk_len, double *F, hipStream_t st) {
-    hipLaunchKernelGGL(k_prefix_combine, dim3(cdiv(nslots, 64)), dim3(64), 0, st, dct, nslots, ctot, rec, nchunk,
+    hipLaunchKernelGGL(k_prefix_combine, dim3(cdiv(nslots, 4)), dim3(256), 0, st, dct, nslots, ctot, rec, nchunk,
                        chunk_len, F);
 }
 
