@@ -333,6 +333,10 @@ __global__ void __launch_bounds__(256) k_prefix_combine(const CorrTables *__rest
 // ------------------------------------------------------------------
 constexpr int kBlkJ = 8;
 
+// In a full group every block of levels 0..3 is complete, so the level sums
+// s1[0..3] all grow by the group total h (the pairwise tree's root; the
+// levels differ only in FP64 rounding order); a partial group keeps the
+// complete blocks of each level.
 template <bool FULL>
 __device__ __forceinline__ void blk_group_stats(const double (&c)[16], int lim, double (&s1)[kBlkJ],
                                                 double (&s2)[kBlkJ], double &h) {
@@ -340,31 +344,35 @@ __device__ __forceinline__ void blk_group_stats(const double (&c)[16], int lim, 
 #pragma unroll
     for (int i = 0; i < 16; i++) {
         double v = (FULL || i < lim) ? c[i] : 0.0;
-        s1[0] += v;
+        if (!FULL) s1[0] += v;
         s2[0] = fma(v, v, s2[0]);
     }
 #pragma unroll
     for (int j = 0; j < 8; j++) {
         d[j] = c[2 * j] + c[2 * j + 1];
         double v = (FULL || 2 * j + 2 <= lim) ? d[j] : 0.0;
-        s1[1] += v;
+        if (!FULL) s1[1] += v;
         s2[1] = fma(v, v, s2[1]);
     }
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         e[j] = d[2 * j] + d[2 * j + 1];
         double v = (FULL || 4 * j + 4 <= lim) ? e[j] : 0.0;
-        s1[2] += v;
+        if (!FULL) s1[2] += v;
         s2[2] = fma(v, v, s2[2]);
     }
 #pragma unroll
     for (int j = 0; j < 2; j++) {
         f2[j] = e[2 * j] + e[2 * j + 1];
         double v = (FULL || 8 * j + 8 <= lim) ? f2[j] : 0.0;
-        s1[3] += v;
+        if (!FULL) s1[3] += v;
         s2[3] = fma(v, v, s2[3]);
     }
     h = f2[0] + f2[1];
+    if (FULL) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) s1[k] += h;
+    }
 }
 
 // Levels 4..8 of one curve over the group index gi of a full group.
@@ -410,6 +418,7 @@ __global__ void __launch_bounds__(64) k_blocking(SeriesGeom g, const CorrTables 
     const double beta_b = F[(int64_t)kb * ct.npos + ct.pos_blk_b] / n0 - mub;
     const int nchunk = (n0 + (1 << kBlkJ) - 1) >> kBlkJ;
     const int t0 = blockIdx.y * chunks_per_split, t1 = min(nchunk, t0 + chunks_per_split);
+    const bool bload = ld * 64 < 0x7fffffff;   // 16 rows addressable by one buffer resource
     double s1a[kBlkJ], s2a[kBlkJ], s1c[kBlkJ], s2c[kBlkJ];
 #pragma unroll
     for (int k = 0; k < kBlkJ; k++) s1a[k] = s2a[k] = s1c[k] = s2c[k] = 0.0;
@@ -427,11 +436,24 @@ __global__ void __launch_bounds__(64) k_blocking(SeriesGeom g, const CorrTables 
             float xa[16], xb[16];
             const float *xq = X + (int64_t)yb * ld;   // wave-uniform
             const int nv = min(16, n0 - yb);
+            if (bload) {
+                // buffer loads: one lane offset per series, the row as the
+                // scalar offset (no address arithmetic per load)
+                const __amdgpu_buffer_rsrc_t rs =
+                    __builtin_amdgcn_make_buffer_rsrc((void *)xq, (short)0, (int)(ld * 64), 0x00020000);
 #pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const uint32_t o = (uint32_t)min(i, nv - 1) * (uint32_t)ld;
-                xa[i] = xq[o + (uint32_t)ka];
-                if (CCF) xb[i] = xq[o + (uint32_t)kb];
+                for (int i = 0; i < 16; i++) {
+                    const int so = min(i, nv - 1) * (int)ld * 4;
+                    xa[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, ka * 4, so, 0));
+                    if (CCF) xb[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, kb * 4, so, 0));
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < 16; i++) {
+                    const uint32_t o = (uint32_t)min(i, nv - 1) * (uint32_t)ld;
+                    xa[i] = xq[o + (uint32_t)ka];
+                    if (CCF) xb[i] = xq[o + (uint32_t)kb];
+                }
             }
             double am[16];
 #pragma unroll
