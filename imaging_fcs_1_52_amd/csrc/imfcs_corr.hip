@@ -30,8 +30,6 @@ namespace imfcs {
 constexpr int kTileF = 64;     // frames per LDS tile
 constexpr int kMainChan = 33;  // channels of levels 0..2 (p = 16)
 constexpr int kNS = 128;       // staged series per tile (a and partner b)
-// third word of a raw (unswizzled, 32-bit) buffer descriptor on gfx9
-constexpr int kBufferWord3 = 0x00020000;
 
 struct MainArgs {
     SeriesGeom g;

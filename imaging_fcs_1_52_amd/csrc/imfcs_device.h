@@ -5,6 +5,11 @@
 
 namespace imfcs {
 
+// Third word of a raw (unswizzled, 32-bit) buffer resource on gfx9 parts:
+// the buffer loads and stores below take one lane offset and one scalar
+// (wave-uniform) offset, instead of a 64-bit address per lane.
+constexpr int kBufferWord3 = 0x00020000;
+
 // ------------------------------------------------------------------
 // Series access
 // ------------------------------------------------------------------
@@ -121,7 +126,7 @@ __device__ __forceinline__ void raw_ints16(const SeriesGeom &g, int ox, int oy, 
         // buffer loads: the 16 frames differ by a scalar offset, so a load
         // needs no address arithmetic of its own
         const __amdgpu_buffer_rsrc_t rs =
-            __builtin_amdgcn_make_buffer_rsrc((void *)q, (short)0, (int)(pl * 32), 0x00020000);
+            __builtin_amdgcn_make_buffer_rsrc((void *)q, (short)0, (int)(pl * 32), kBufferWord3);
         for (int k = 0; k < g.by; k++)
             for (int i = 0; i < g.bx; i++) {
                 const int d2 = (int)(lo + (uint32_t)(k * g.stride + i)) * 2;

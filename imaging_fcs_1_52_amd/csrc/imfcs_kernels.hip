@@ -238,7 +238,7 @@ __global__ void __launch_bounds__(64) k_apply(SeriesGeom g, BleachDesc bd, const
             // a full group with no prefix record: straight-line code, buffer
             // stores with the frame's row as the scalar offset
             const __amdgpu_buffer_rsrc_t rs =
-                __builtin_amdgcn_make_buffer_rsrc((void *)xw, (short)0, (int)(ld * 64), 0x00020000);
+                __builtin_amdgcn_make_buffer_rsrc((void *)xw, (short)0, (int)(ld * 64), kBufferWord3);
 #pragma unroll
             for (int i = 0; i < 16; i++) {
                 const double a = bl.apply(v[i], fb + i);
@@ -440,7 +440,7 @@ __global__ void __launch_bounds__(64) k_blocking(SeriesGeom g, const CorrTables 
                 // buffer loads: one lane offset per series, the row as the
                 // scalar offset (no address arithmetic per load)
                 const __amdgpu_buffer_rsrc_t rs =
-                    __builtin_amdgcn_make_buffer_rsrc((void *)xq, (short)0, (int)(ld * 64), 0x00020000);
+                    __builtin_amdgcn_make_buffer_rsrc((void *)xq, (short)0, (int)(ld * 64), kBufferWord3);
 #pragma unroll
                 for (int i = 0; i < 16; i++) {
                     const int so = min(i, nv - 1) * (int)ld * 4;
