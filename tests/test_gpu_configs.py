@@ -6,8 +6,10 @@ Each case exercises a different load and tile path of the native pipeline:
 frame counts that end inside a 64-frame tile, binning 3 and 5 (windows read
 pixel by pixel), binning 4 (rows of four pixels read as 8-byte words),
 binning 2 (two-row windows read in one batch), a vertical and a diagonal
-partner.  Tolerances: correlation 1e-4 relative, and for channels within
-1e-3 of the curve amplitude of zero an absolute 1e-7 of the amplitude (the
+partner, and a long series of few pixels (more chunk totals than one
+wave scans at once in the prefix combine).  Tolerances: correlation 1e-4
+relative, and for channels within 1e-3 of the curve amplitude of zero an
+absolute 1e-7 of the amplitude (the
 corrected series is stored in FP32 and each 16-value group of products is
 summed in FP32, which bounds the absolute precision at a few 1e-8 of the
 amplitude; the reference's own GPU path stores data and products in FP32);
@@ -29,6 +31,9 @@ CASES = [
     (11, 8, 4097, 1, (1, 1), 2, 103),
     (20, 15, 3333, 5, (0, 0), 4, 104),
     (14, 10, 7777, 2, (2, 2), 3, 105),
+    # few slots and many frames: 79 bleach-correction chunks of 256 frames,
+    # so the window-mean prefix sums combine more than 64 chunk totals
+    (8, 6, 20000, 1, (1, 0), 4, 106),
 ]
 
 
