@@ -9,10 +9,10 @@ binning 2 (two-row windows read in one batch), a vertical and a diagonal
 partner, and a long series of few pixels (more chunk totals than one
 wave scans at once in the prefix combine).  Tolerances: correlation 1e-4
 relative, and for channels within 1e-3 of the curve amplitude of zero an
-absolute 1e-7 of the amplitude (the
-corrected series is stored in FP32 and each 16-value group of products is
-summed in FP32, which bounds the absolute precision at a few 1e-8 of the
-amplitude; the reference's own GPU path stores data and products in FP32);
+absolute 1e-7 of the amplitude (the corrected series is stored in FP32
+and each 16-value group of products is summed in FP32, which bounds the
+absolute precision at a few 1e-8 of the amplitude; the reference's own GPU
+path stores data and products in FP32);
 blocked variances 1e-4 where the blocking index agrees; fitted N, D 1e-3
 relative, G_inf 1e-3 of 1/N.
 """
