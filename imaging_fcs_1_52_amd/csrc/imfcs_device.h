@@ -150,20 +150,34 @@ __device__ __forceinline__ void raw_ints16(const SeriesGeom &g, int ox, int oy, 
         }
 }
 
+// The 16 values of raw_ints16 (or of a float source) in two steps: as
+// 32-bit words (integer window sums, or FP32 values), so that a caller can
+// hold the next group's loads in flight while it works on the current
+// group, and their conversion to FP64.
 template <int NB>
-__device__ __forceinline__ void raw_values16(const SeriesGeom &g, int ox, int oy, int f0, int fmax, double (&v)[16]) {
+__device__ __forceinline__ void raw_words16(const SeriesGeom &g, int ox, int oy, int f0, int fmax, uint32_t (&r)[16]) {
     if (g.src == kSrcU16) {
         int acc[16];
         raw_ints16<NB>(g, ox, oy, f0, fmax, acc);
 #pragma unroll
-        for (int j = 0; j < 16; j++) v[j] = (double)acc[j];
+        for (int j = 0; j < 16; j++) r[j] = (uint32_t)acc[j];
     } else {
         const int nv = min(16, fmax - f0 + 1);
         const uint32_t pl = (uint32_t)g.plane;
         const uint32_t lo = (uint32_t)(oy * g.stride + ox);
         const float *q = (const float *)g.data + (int64_t)(g.f0 + f0) * g.plane;
 #pragma unroll
-        for (int j = 0; j < 16; j++) v[j] = (double)q[lo + (uint32_t)min(j, nv - 1) * pl];
+        for (int j = 0; j < 16; j++) r[j] = __builtin_bit_cast(uint32_t, q[lo + (uint32_t)min(j, nv - 1) * pl]);
+    }
+}
+
+__device__ __forceinline__ void words_to_values16(const SeriesGeom &g, const uint32_t (&r)[16], double (&v)[16]) {
+    if (g.src == kSrcU16) {
+#pragma unroll
+        for (int j = 0; j < 16; j++) v[j] = (double)(int)r[j];
+    } else {
+#pragma unroll
+        for (int j = 0; j < 16; j++) v[j] = (double)__builtin_bit_cast(float, r[j]);
     }
 }
 

@@ -230,9 +230,14 @@ __global__ void __launch_bounds__(64) k_apply(SeriesGeom g, BleachDesc bd, const
     double run = 0.0;
     const uint32_t ld = (uint32_t)g.nslots;
     const bool bstore = (int64_t)ld * 64 < 0x7fffffff;
+    // the next group's loads are issued before the current group is
+    // corrected and stored (one group of loads always in flight)
+    uint32_t cur[16];
+    if (f0 < f1) raw_words16<NB>(g, ox, oy, f0, f1 - 1, cur);
     for (int fb = f0; fb < f1; fb += 16) {
         double v[16];
-        raw_values16<NB>(g, ox, oy, fb, f1 - 1, v);
+        words_to_values16(g, cur, v);
+        if (fb + 16 < f1) raw_words16<NB>(g, ox, oy, fb + 16, f1 - 1, cur);   // wave-uniform
         float *xw = X + (int64_t)fb * g.nslots;   // wave-uniform
         if (bstore && fb + 16 <= f1 && next >= fb + 16) {   // wave-uniform
             // a full group with no prefix record: straight-line code, buffer
