@@ -22,6 +22,7 @@
 #include "imfcs_device.h"
 
 #include <math.h>
+#include <type_traits>
 
 namespace imfcs {
 
@@ -86,40 +87,50 @@ __global__ void __launch_bounds__(256) k_point_sums(const uint16_t *__restrict__
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;   // pixel group
     if (q * VEC >= plane) return;
     const int x0 = blockIdx.y * pts_per_thread, x1 = min(nopit, x0 + pts_per_thread);
-    for (int x = x0; x < x1; x++) {
-        int acc[VEC];
+    // the frames of the thread's points in batches of 16 rows (a point's
+    // last batch is partial); the next batch's loads are issued before the
+    // current one is summed
+    using Word = typename std::conditional<VEC == 4, uint2, uint16_t>::type;
+    const int nb_pt = (ave + 15) / 16, nbatch = (x1 - x0) * nb_pt;
+    auto load = [&](int b, Word (&r)[16]) {
+        const int x = x0 + b / nb_pt, f = (b % nb_pt) * 16;   // wave-uniform
+        const int nv = min(16, ave - f);
+        const uint16_t *base = data + ((int64_t)f0 + (int64_t)x * ave + f) * plane + q * VEC;
 #pragma unroll
-        for (int v = 0; v < VEC; v++) acc[v] = 0;
-        const uint16_t *base = data + ((int64_t)f0 + (int64_t)x * ave) * plane + q * VEC;   // wave-uniform frame
-        for (int f = 0; f < ave; f += 16) {
-            const int nv = min(16, ave - f);
-            if (VEC == 4) {
-                uint2 r[16];
+        for (int j = 0; j < 16; j++) r[j] = *(const Word *)(base + (int64_t)min(j, nv - 1) * plane);
+    };
+    Word cur[16];
+    if (nbatch > 0) load(0, cur);
+    int acc[VEC];
 #pragma unroll
-                for (int j = 0; j < 16; j++) r[j] = *(const uint2 *)(base + (int64_t)(f + min(j, nv - 1)) * plane);
+    for (int v = 0; v < VEC; v++) acc[v] = 0;
+    for (int b = 0; b < nbatch; b++) {
+        Word nxt[16];
+        if (b + 1 < nbatch) load(b + 1, nxt);
+        const int pb = b % nb_pt, nv = min(16, ave - pb * 16);
 #pragma unroll
-                for (int j = 0; j < 16; j++)
-                    if (j < nv) {
-                        acc[0] += (int)(r[j].x & 0xffffu);
-                        acc[VEC > 1 ? 1 : 0] += (int)(r[j].x >> 16);
-                        acc[VEC > 2 ? 2 : 0] += (int)(r[j].y & 0xffffu);
-                        acc[VEC > 3 ? 3 : 0] += (int)(r[j].y >> 16);
-                    }
-            } else {
-                uint16_t r[16];
-#pragma unroll
-                for (int j = 0; j < 16; j++) r[j] = base[(int64_t)(f + min(j, nv - 1)) * plane];
-#pragma unroll
-                for (int j = 0; j < 16; j++)
-                    if (j < nv) acc[0] += r[j];
+        for (int j = 0; j < 16; j++)
+            if (j < nv) {
+                if constexpr (VEC == 4) {
+                    acc[0] += (int)(cur[j].x & 0xffffu);
+                    acc[1] += (int)(cur[j].x >> 16);
+                    acc[2] += (int)(cur[j].y & 0xffffu);
+                    acc[3] += (int)(cur[j].y >> 16);
+                } else {
+                    acc[0] += cur[j];
+                }
             }
+        if (pb == nb_pt - 1) {
+            int32_t *out = P + (int64_t)(x0 + b / nb_pt) * plane + q * VEC;
+            if constexpr (VEC == 4)
+                *(int4 *)out = make_int4(acc[0], acc[1], acc[2], acc[3]);
+            else
+                out[0] = acc[0];
+#pragma unroll
+            for (int v = 0; v < VEC; v++) acc[v] = 0;
         }
-        int32_t *out = P + (int64_t)x * plane + q * VEC;
-        if (VEC == 4) {
-            *(int4 *)out = make_int4(acc[0], acc[VEC > 1 ? 1 : 0], acc[VEC > 2 ? 2 : 0], acc[VEC > 3 ? 3 : 0]);
-        } else {
-            out[0] = acc[0];
-        }
+#pragma unroll
+        for (int j = 0; j < 16; j++) cur[j] = nxt[j];
     }
 }
 
