@@ -9,11 +9,17 @@ variances, and the LM fit of N, D, G_inf for every ACF and CCF pixel.  The
 three binnings run as three pipelines on three HIP streams (one engine each,
 all reading the one device-resident stack, started by imfcs_run_multi: one
 pass over the stack sums every pixel over each bleach-trace point for all
-three), so the tail of one pipeline's fit overlaps the others' work.  The stack is device-resident before timing
-starts.  ``value`` counts ACF-fitted output pixels (the CCF work is inside
-the timed region but not counted).
+three), so the tail of one pipeline's fit overlaps the others' work.  The
+stack is device-resident before timing starts.  ``value`` counts ACF-fitted
+output pixels (the CCF work is inside the timed region but not counted).
 
-Prints one JSON line (see DESIGN.md for the roofline and baseline fields).
+Prints one JSON line.  ``roofline``: the main correlator kernel at binning 1,
+algorithmic bytes (one read of the FP32 corrected series, one write of the
+level-3 sums) over its HIP-event duration on its stream, against the 8 TB/s
+HBM peak, with the copy-calibrated FETCH_SIZE + WRITE_SIZE bytes per launch
+read from the committed traffic profile.  ``cpu_baseline``: the oracle (a C
+restatement of the plugin's Java CPU path) on one host core over a pixel
+sample of the binning-1 ACF workload.
 """
 import argparse
 import ctypes
@@ -108,7 +114,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (pixels shard without "
-              "communication; see DESIGN.md)", file=sys.stderr)
+              "communication)", file=sys.stderr)
         sys.exit(2)
 
     import imaging_fcs_1_52_amd as imfcs
