@@ -113,7 +113,7 @@ def main():
                     help="also run a plain device-to-device copy of the stack (known bytes) for PMC traffic scaling")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (pixels shard without "
+        print("bench.py measures one GPU: the engine has no multi-GPU path (pixels shard without "
               "communication)", file=sys.stderr)
         sys.exit(2)
 
