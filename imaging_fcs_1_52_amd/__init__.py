@@ -15,12 +15,13 @@ import os
 import numpy as np
 
 from . import native
-from .native import (ACFParameters, FitModelGeom, PipelineConfig, MODEL_ACF_1D, MODEL_LINEAR_1D, ESTIMATOR_LSE,
+from .native import (ACFParameters, FitModelGeom, PipelineConfig, SimDomain, MODEL_ACF_1D, MODEL_LINEAR_1D, ESTIMATOR_LSE,
                      STATE_CONVERGED, STATE_MAX_ITERATION, STATE_SINGULAR_HESSIAN, lib, sim_lib)
 
 __all__ = ["ACFParameters", "FitModel", "FitResult", "Engine", "PipelineConfig", "FitModelGeom", "fit",
            "get_last_error", "is_cuda_available", "get_cuda_version", "reset_gpu", "calc_data_bleach_correction",
            "is_binning_memory_sufficient", "calc_binning", "is_acf_memory_sufficient", "calc_acf", "simulate_2d",
+           "SimDomain", "sim_domain", "simulate_2d_device",
            "correlator_tables", "obsvol", "make_fit_model", "make_pipeline_config"]
 
 
@@ -284,17 +285,31 @@ class Engine:
         except Exception:
             pass
 
+    def alloc_stack(self, nbytes):
+        """Device memory of nbytes for the engine's stack (reused when large
+        enough); returns its pointer."""
+        if self._stack is None or self._stack_bytes < nbytes:
+            if self._stack:
+                self._lib.imfcs_device_free(self.ctx, self._stack)
+                self._stack = None
+            self._stack = self._lib.imfcs_device_alloc(self.ctx, ctypes.c_size_t(nbytes))
+            if not self._stack:
+                raise ImFCSError(get_last_error())
+            self._stack_bytes = nbytes
+        return self._stack
+
+    def download(self, d_ptr, shape, dtype=np.uint16):
+        """Copies device memory at d_ptr into a new host array."""
+        out = np.empty(shape, dtype)
+        _check(self._lib.imfcs_memcpy_d2h(self.ctx, out.ctypes.data_as(ctypes.c_void_p), d_ptr,
+                                          ctypes.c_size_t(out.nbytes)))
+        return out
+
     def upload(self, stack):
         """Copies a uint16 [T][H][W] stack to device memory owned by the
         engine and returns its device pointer."""
         stack = np.ascontiguousarray(stack, np.uint16)
-        if self._stack is None or self._stack_bytes < stack.nbytes:
-            if self._stack:
-                self._lib.imfcs_device_free(self.ctx, self._stack)
-            self._stack = self._lib.imfcs_device_alloc(self.ctx, ctypes.c_size_t(stack.nbytes))
-            if not self._stack:
-                raise ImFCSError(get_last_error())
-            self._stack_bytes = stack.nbytes
+        self.alloc_stack(stack.nbytes)
         _check(self._lib.imfcs_memcpy_h2d(self.ctx, self._stack, stack.ctypes.data_as(ctypes.c_void_p),
                                           ctypes.c_size_t(stack.nbytes)))
         return self._stack
@@ -387,3 +402,35 @@ def simulate_2d(W, H, T, n_emitters, D_um2s=1.0, pixel_um=0.24, psf_sigma_um=0.1
     if rc != 0:
         raise ValueError("imfcs_simulate_2d: bad arguments")
     return out
+
+
+def sim_domain(x0_um, y0_um, w_um, h_um, n_emitters, D_um2s=1.0, D2_um2s=0.0, frac2=0.0, counts=20.0,
+               counts_hi=None):
+    """One periodic box of the GPU simulator: n_emitters start uniformly in
+    [x0, x0 + w) x [y0, y0 + h) (um) and wrap at its edges; D_um2s is one value
+    or four quadrant values of component 1; a fraction frac2 of the emitters
+    diffuses with D2_um2s; brightness `counts` photons per frame, or a
+    log-ramp from `counts` to `counts_hi` along x of the box."""
+    d = SimDomain()
+    d.x0_um, d.y0_um, d.w_um, d.h_um = x0_um, y0_um, w_um, h_um
+    d.n_emitters = int(n_emitters)
+    dq = np.asarray(D_um2s if np.ndim(D_um2s) else [D_um2s] * 4, np.float64)
+    for q in range(4):
+        d.D_um2s[q] = float(dq[q])
+    d.D2_um2s, d.frac2 = float(D2_um2s), float(frac2)
+    d.counts_lo = float(counts)
+    d.counts_hi = float(counts if counts_hi is None else counts_hi)
+    return d
+
+
+def simulate_2d_device(engine, W, H, T, domains, pixel_um=0.24, psf_sigma_um=0.19, frametime=1e-3, offset=100,
+                       tau_bleach_s=0.0, seed=1):
+    """The particle simulator on the GPU into a uint16 [T][H][W] stack owned by
+    `engine` (replacing its current stack); returns the device pointer."""
+    doms = domains if isinstance(domains, (list, tuple)) else [domains]
+    arr = (SimDomain * len(doms))(*doms)
+    nbytes = 2 * W * H * T
+    d_stack = engine.alloc_stack(nbytes)
+    _check(lib().imfcs_simulate_2d_device(d_stack, W, H, T, arr, len(doms), pixel_um, psf_sigma_um, frametime,
+                                          offset, tau_bleach_s, ctypes.c_uint64(seed), None))
+    return d_stack

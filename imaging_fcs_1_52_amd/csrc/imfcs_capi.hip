@@ -177,6 +177,19 @@ int make_corr_tables(const HostTables &h, int num, int q, CorrTables &ct) {
     return 0;
 }
 
+// The correlator kernels are specialised for the plugin's default channel
+// scheme p = 16 (17 delays in the first group, 8 in every further group,
+// IFCS:7391-7424); q is the number of groups (kMaxLevel at most).  Any
+// other p is refused before work is enqueued.
+int check_correlator_layout(int p, int q) {
+    if (p != 16)
+        return fail(IMFCS_ERR_ARG, "correlator p = " + std::to_string(p) +
+                                       " is not supported: the kernels implement the p = 16 channel scheme");
+    if (q < 1 || q > kMaxLevel)
+        return fail(IMFCS_ERR_ARG, "correlator q = " + std::to_string(q) + " outside 1.." + std::to_string(kMaxLevel));
+    return 0;
+}
+
 bool device_present() {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return false;
@@ -184,6 +197,8 @@ bool device_present() {
 }
 
 }  // namespace
+
+int imfcs::set_error(int code, const std::string &msg) { return fail(code, msg); }
 
 // ------------------------------------------------------------------
 // Context
@@ -215,7 +230,8 @@ struct imfcs_ctx {
     int npix = 0, npairs = 0, nseries = 0, nslots = 0, chanum = 0, order = 0, ow = 0, oh = 0, did_fit = 0, nfree = 0;
     int did_gls = 0;
     int force_v1 = 0;   // IMFCS_CORRELATOR=generic selects the one-thread-per-level kernel
-    double chi2_norm = 1.0;
+    double chi2_norm[2] = {1.0, 1.0};   // ACF fits, CCF fits
+    int fit_split = 0;                  // fits below are ACF fits
     std::vector<double> coef_scale;   // u-basis -> seconds conversion per power
     // what imfcs_refit needs of the last run: its configuration and lag times
     imfcs_config last_cfg{};
@@ -254,6 +270,8 @@ int imfcs_plan(const imfcs_config *c, int32_t *out_w, int32_t *out_h, int32_t *c
     if (!c) return fail(IMFCS_ERR_ARG, "null config");
     if (c->pixbin_x <= 0 || c->pixbin_y <= 0 || c->binning_x <= 0 || c->binning_y <= 0)
         return fail(IMFCS_ERR_ARG, "binning and pixbin must be positive");
+    const int rc = check_correlator_layout(c->correlator_p, c->correlator_q);
+    if (rc) return rc;
     // correlateROI ROI mapping (IFCS:8458-8466)
     int sx = (int)std::ceil((double)c->roi_x / c->pixbin_x);
     int sy = (int)std::ceil((double)c->roi_y / c->pixbin_y);
@@ -669,23 +687,26 @@ static int run_fit(imfcs_ctx *ctx, const imfcs_config *c) {
                                ctx->fw.as<float>(), st);
         // lag times of channels 1..chanum-1 (IFCS:15122-15125) and initial
         // parameters with the fixed geometry in slots 11..19 (IFCS:15090-15110):
-        // every ACF fit starts from the ACF model's row, every CCF fit from
-        // the CCF model's, so two rows travel and the kernel picks one
+        // every ACF fit starts from the ACF model's row and takes its free set,
+        // every CCF fit the CCF model's, so two rows of each travel
         std::vector<float> &xs = ctx->h_xs, &init = ctx->h_init;
         std::vector<int> &pfit = ctx->h_pfit;
         xs.assign(npts, 0.f);
         init.assign(2 * 20, 0.f);
-        pfit.assign(20, 0);
+        pfit.assign(2 * 20, 0);
         for (int i = 0; i < npts; i++) xs[i] = (float)ctx->last_lagtime[i + 1];
         const imfcs_fit_model *models[2] = {&c->acf_model, &c->ccf_model};
-        for (int k = 0; k < 11; k++) pfit[k] = c->acf_model.paramfit[k] ? 1 : 0;
-        int nfree = 0;
-        for (int k = 0; k < 11; k++) nfree += pfit[k];
-        const int init_split = c->do_acf ? ctx->npix : 0;
+        unsigned mask[2] = {0u, 0u};
+        int nfree[2] = {0, 0};
         for (int r = 0; r < 2; r++) {
             const imfcs_fit_model *m = models[r];
             float *q = &init[(size_t)r * 20];
-            for (int k = 0; k < 11; k++) q[k] = (float)m->paraminit[k];
+            for (int k = 0; k < 11; k++) {
+                q[k] = (float)m->paraminit[k];
+                pfit[(size_t)r * 20 + k] = m->paramfit[k] ? 1 : 0;
+                mask[r] |= (m->paramfit[k] ? 1u : 0u) << k;
+                nfree[r] += m->paramfit[k] ? 1 : 0;
+            }
             q[11] = (float)m->ax;
             q[12] = (float)m->ay;
             q[13] = (float)m->s;
@@ -696,29 +717,59 @@ static int run_fit(imfcs_ctx *ctx, const imfcs_config *c) {
             q[18] = (float)m->q2;
             q[19] = (float)m->q3;
         }
+        // fits [0, n_acf) are ACF fits, the rest CCF fits
+        const int n_acf = c->do_acf ? ctx->npix : 0, n_ccf = np - n_acf;
+        for (int r = 0; r < 2; r++)
+            if ((r == 0 ? n_acf : n_ccf) > 0 && nfree[r] == 0) return fail(IMFCS_ERR_ARG, "no free fit parameter");
         HIP_TRY(ctx->up_fx.put(ctx->fx, xs.data(), sizeof(float) * npts, st));
         HIP_TRY(ctx->up_finit.put(ctx->finit, init.data(), sizeof(float) * init.size(), st));
-        HIP_TRY(ctx->up_fpfit.put(ctx->fpfit, pfit.data(), sizeof(int) * 20, st));
+        HIP_TRY(ctx->up_fpfit.put(ctx->fpfit, pfit.data(), sizeof(int) * pfit.size(), st));
         HIP_TRY(hipEventRecord(ctx->ev_h2d, st));
-        unsigned mask = 0;
-        for (int k = 0; k < 11; k++) mask |= (pfit[k] ? 1u : 0u) << k;
-        ctx->nfree = nfree;
-        if (ctx->did_gls) {
-            if (mask != 0x13u) return fail(IMFCS_ERR_ARG, "the GLS fit supports the free set N, D, G_inf");
-            launch_lm_gls(np, npts, ctx->fx.as<float>(), ctx->finit.as<float>(), ctx->fpfit.as<int>(),
-                          c->fit_tolerance, c->fit_max_iterations, ctx->glsW.as<double>(), ctx->glsy.as<double>(),
-                          ctx->glsok.as<int32_t>(), ctx->fout.as<float>(), ctx->fstate.as<int>(),
-                          ctx->fchi2.as<float>(), ctx->fiter.as<int>(), st, init_split);
-            // chi2 of GLSFit: sum of whitened residuals^2 / (chanum - nfree - 1) (IFCS:10733-10735)
-            ctx->chi2_norm = (double)(ctx->chanum - nfree - 1);
-        } else {
-            launch_lm_acf(np, npts, ctx->fdata.as<float>(), ctx->fw.as<float>(), ctx->fx.as<float>(),
-                          ctx->finit.as<float>(), ctx->fpfit.as<int>(), mask, LMModelGeom{}, c->fit_tolerance,
-                          c->fit_max_iterations, c->fit_chi2_fp64 ? 1 : 0, ctx->fout.as<float>(),
-                          ctx->fstate.as<int>(), ctx->fchi2.as<float>(), ctx->fiter.as<int>(), st, init_split);
-            // chi2 normalization of GPU_ACF_Fit (IFCS:15234)
-            ctx->chi2_norm = (double)((npts - 1) - nfree - 1);
+        ctx->fit_split = n_acf;
+        ctx->nfree = nfree[n_acf > 0 ? 0 : 1];
+        // one launch when both kinds share the free set (the last fit first:
+        // the CCF fits); else the ACF and the CCF fits as two launches, each
+        // with its own parameter row
+        const bool one = n_acf == 0 || n_ccf == 0 || mask[0] == mask[1];
+        struct Part { int first, count, row; };
+        Part parts[2];
+        int nparts = 0;
+        if (one) parts[nparts++] = {0, np, n_acf > 0 ? 0 : 1};
+        else {
+            parts[nparts++] = {0, n_acf, 0};
+            parts[nparts++] = {n_acf, n_ccf, 1};
         }
+        if (ctx->did_gls)
+            for (int i = 0; i < nparts; i++)
+                if (mask[parts[i].row] != 0x13u) return fail(IMFCS_ERR_ARG, "the GLS fit supports the free set N, D, G_inf");
+        for (int i = 0; i < nparts; i++) {
+            const Part &pt = parts[i];
+            // two init rows: with one launch, fits below n_acf take row 0 and
+            // the others row 1; a split launch takes its own row for every fit
+            const float *ini = ctx->finit.as<float>() + (one ? 0 : (size_t)pt.row * 20);
+            const int *pf = ctx->fpfit.as<int>() + (size_t)pt.row * 20;
+            const int split = one ? n_acf : pt.count;
+            const size_t f0 = (size_t)pt.first;
+            if (ctx->did_gls) {
+                launch_lm_gls(pt.count, npts, ctx->fx.as<float>(), ini, pf, c->fit_tolerance, c->fit_max_iterations,
+                              ctx->glsW.as<double>() + f0 * npts * npts, ctx->glsy.as<double>() + f0 * npts,
+                              ctx->glsok.as<int32_t>() + f0, ctx->fout.as<float>() + f0 * 20,
+                              ctx->fstate.as<int>() + f0, ctx->fchi2.as<float>() + f0, ctx->fiter.as<int>() + f0,
+                              st, split);
+            } else {
+                launch_lm_acf(pt.count, npts, ctx->fdata.as<float>() + f0 * npts, ctx->fw.as<float>() + f0 * npts,
+                              ctx->fx.as<float>(), ini, pf, mask[pt.row], LMModelGeom{}, c->fit_tolerance,
+                              c->fit_max_iterations, c->fit_chi2_fp64 ? 1 : 0, ctx->fout.as<float>() + f0 * 20,
+                              ctx->fstate.as<int>() + f0, ctx->fchi2.as<float>() + f0, ctx->fiter.as<int>() + f0, st,
+                              split);
+            }
+        }
+        for (int r = 0; r < 2; r++)
+            ctx->chi2_norm[r] = ctx->did_gls
+                                    // chi2 of GLSFit: sum of whitened residuals^2 / (chanum - nfree - 1) (IFCS:10733-10735)
+                                    ? (double)(ctx->chanum - nfree[r] - 1)
+                                    // chi2 normalization of GPU_ACF_Fit (IFCS:15234)
+                                    : (double)((npts - 1) - nfree[r] - 1);
     }
     return 0;
 }
@@ -973,7 +1024,7 @@ extern "C" int imfcs_get_results(imfcs_ctx *ctx, double *acf, double *var, int32
             for (size_t p = 0; p < np; p++)
                 for (int k = 0; k < 11; k++) fitres[p * 11 + k] = out[p * 20 + k];
         if (chi2)
-            for (size_t p = 0; p < np; p++) chi2[p] = c2[p] / ctx->chi2_norm;
+            for (size_t p = 0; p < np; p++) chi2[p] = c2[p] / ctx->chi2_norm[(int)p < ctx->fit_split ? 0 : 1];
         if (states) HIP_TRY(hipMemcpy(states, ctx->fstate.p, sizeof(int32_t) * np, hipMemcpyDeviceToHost));
         if (iters) HIP_TRY(hipMemcpy(iters, ctx->fiter.p, sizeof(int32_t) * np, hipMemcpyDeviceToHost));
     }
@@ -1072,10 +1123,41 @@ extern "C" int imfcs_calc_data_bleach_correction(const float *pixels, float *out
 extern "C" int imfcs_calc_acf(const float *pixels, double *pixels1, double *blockvararray, double *NBmean,
                               double *NBcov, double *blocked1D, const double *bleachcorr_params, const double *samp,
                               const int *lag, const imfcs_acf_params *p) {
-    (void)samp;
-    (void)lag;
     if (!pixels || !pixels1 || !blockvararray || !blocked1D || !p) return fail(IMFCS_ERR_ARG, "null argument");
     if (p->isNBcalculation && (!NBmean || !NBcov)) return fail(IMFCS_ERR_ARG, "null N&B output");
+    // host-side argument checks, before any device work
+    if (p->cfXDistance < 0 || p->cfYDistance < 0)
+        return fail(IMFCS_ERR_ARG, "negative CF distances are not supported (as in IFCS:15296-15299)");
+    if (p->width <= 0 || p->height <= 0 || p->pixbinX <= 0 || p->pixbinY <= 0)
+        return fail(IMFCS_ERR_ARG, "empty output grid or non-positive pixel pitch");
+    if (p->lastframe - p->firstframe + 1 < 2) return fail(IMFCS_ERR_ARG, "frame range holds fewer than 2 frames");
+    if ((int64_t)(p->width - 1) * p->pixbinX + p->cfXDistance >= p->w_temp ||
+        (int64_t)(p->height - 1) * p->pixbinY + p->cfYDistance >= p->h_temp)
+        return fail(IMFCS_ERR_ARG, "output grid exceeds the binned input");
+    if (p->bleachcorr_gpu && (!bleachcorr_params || p->bleachcorr_order < 1 || p->bleachcorr_order - 1 > kMaxOrder))
+        return fail(IMFCS_ERR_ARG, "bleach correction needs 1..9 coefficients per pixel");
+    if ((double)(int)p->correlatorp != p->correlatorp || (double)(int)p->correlatorq != p->correlatorq)
+        return fail(IMFCS_ERR_ARG, "correlator p and q must be integers");
+    {
+        int rc0 = check_correlator_layout((int)p->correlatorp, (int)p->correlatorq);
+        if (rc0) return rc0;
+        HostTables ht0;
+        rc0 = make_host_tables((int)p->correlatorp, (int)p->correlatorq, p->lastframe - p->firstframe + 1,
+                               p->frametime, ht0);
+        if (rc0) return rc0;
+        // the output arrays are sized by the caller from p->chanum
+        if (p->chanum != ht0.chanum)
+            return fail(IMFCS_ERR_ARG, "ACFParameters.chanum = " + std::to_string(p->chanum) + " but p, q give " +
+                                           std::to_string(ht0.chanum) + " channels");
+        // the tables the plugin passes (setParameters, IFCS:7391-7424) must be
+        // the ones the kernels derive from p and q
+        for (int x = 0; x < ht0.chanum; x++) {
+            if (lag && lag[x] != ht0.lag[x])
+                return fail(IMFCS_ERR_ARG, "lag table differs from the p, q correlator at channel " + std::to_string(x));
+            if (samp && samp[x] != ht0.samp[x])
+                return fail(IMFCS_ERR_ARG, "samp table differs from the p, q correlator at channel " + std::to_string(x));
+        }
+    }
     std::lock_guard<std::mutex> lk(g_dropin_mu);
     imfcs_ctx *ctx = dropin_ctx();
     if (!ctx) return IMFCS_ERR_DEVICE;
@@ -1104,8 +1186,6 @@ extern "C" int imfcs_calc_acf(const float *pixels, double *pixels1, double *bloc
     g.cfy = p->cfYDistance;
     bool ccf = (p->cfXDistance != 0 || p->cfYDistance != 0);
     g.nseries = g.npix * (ccf ? 2 : 1);
-    if ((g.ow - 1) * g.pbx + g.cfx >= p->w_temp || (g.oh - 1) * g.pby + g.cfy >= p->h_temp)
-        return fail(IMFCS_ERR_ARG, "output grid exceeds the binned input");
     int order = p->bleachcorr_gpu ? p->bleachcorr_order - 1 : 0;
     std::vector<double> hc;
     if (order > 0) {

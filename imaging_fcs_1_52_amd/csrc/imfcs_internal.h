@@ -3,7 +3,13 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 namespace imfcs {
+
+// Records msg as the calling thread's last error (imfcs_get_last_error) and
+// returns code.
+int set_error(int code, const std::string &msg);
 
 constexpr int kMaxChan = 128;   // correlator channels (73 for p=16, q=8)
 constexpr int kMaxLevel = 12;   // multi-tau levels (q)

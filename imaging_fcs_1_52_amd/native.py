@@ -47,6 +47,13 @@ class PipelineConfig(ctypes.Structure):
                 ("fit_chi2_fp64", _i32), ("gls", _i32), ("acf_model", FitModelGeom), ("ccf_model", FitModelGeom),
                 ("bleach_mode", _i32), ("sliding_window_length", _i32)]
 
+class SimDomain(ctypes.Structure):
+    """imfcs_sim_domain: one periodic box of the GPU simulator."""
+    _fields_ = [("x0_um", _f64), ("y0_um", _f64), ("w_um", _f64), ("h_um", _f64), ("n_emitters", _i32),
+                ("D_um2s", _f64 * 4), ("D2_um2s", _f64), ("frac2", _f64), ("counts_lo", _f64),
+                ("counts_hi", _f64)]
+
+
 BLEACH_POLYNOMIAL, BLEACH_SLIDING_WINDOW, BLEACH_LIN_SEGMENT, BLEACH_SINGLE_EXP = 0, 1, 2, 3
 
 
@@ -84,6 +91,9 @@ SIGNATURES = {
     "imfcs_get_stage_ms": (ctypes.c_int, [_VP, _P(_f32)]),
     "imfcs_get_correlator_kernel_ms": (ctypes.c_int, [_VP, _P(_f32)]),
     "imfcs_get_gls_covariance": (ctypes.c_int, [_VP, ctypes.c_int, _P(_f64)]),
+    "imfcs_simulate_2d_device": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P(SimDomain),
+                                                ctypes.c_int, _f64, _f64, _f64, ctypes.c_int, _f64, ctypes.c_uint64,
+                                                _VP]),
 }
 
 JNI_EXPORTS = ["fit", "getLastError", "isCudaAvailableInt", "getCudaVersionAsArray", "resetGPU",

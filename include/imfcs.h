@@ -262,6 +262,30 @@ int imfcs_get_results(imfcs_ctx *ctx, double *acf, double *var, int32_t *ind,
  * [0] trace+polyfit, [1] means, [2] blocking, [3] correlation, [4] fit. */
 int imfcs_get_stage_ms(imfcs_ctx *ctx, float *stage_ms5);
 
+/* One periodic box of the GPU simulator (object-space micrometres; detector
+ * pixel (i, j) covers [i, i+1) x [j, j+1) pixel sizes).  Its emitters start
+ * uniformly inside the box and wrap at its edges.  Component 1 diffuses with
+ * D_um2s[q] in quadrant q of the box ((x<cx,y<cy), (x>=cx,y<cy), (x<cx,y>=cy),
+ * (x>=cx,y>=cy)); an emitter is of component 2 (D2_um2s everywhere) with
+ * probability frac2.  Brightness in photons per frame: counts_lo *
+ * (counts_hi / counts_lo)^((x - x0) / w) at the emitter's position. */
+typedef struct imfcs_sim_domain {
+    double x0_um, y0_um, w_um, h_um;
+    int32_t n_emitters;
+    double D_um2s[4];
+    double D2_um2s;
+    double frac2;
+    double counts_lo, counts_hi;
+} imfcs_sim_domain;
+
+/* The simulator on the GPU, into a device uint16 stack [T][H][W] (physics of
+ * simulateACF2D, Imaging_FCS_1_52.java:13963-14393; shot noise drawn per
+ * pixel as Poisson with the PSF-integrated mean).  stream: a hipStream_t or
+ * NULL.  Synchronous; 0 or a negative code. */
+int imfcs_simulate_2d_device(uint16_t *d_out, int W, int H, int T, const imfcs_sim_domain *domains,
+                             int ndomains, double pixel_um, double psf_sigma_um, double frametime_s,
+                             int offset, double tau_bleach_s, uint64_t seed, void *stream);
+
 /* Seeded 2D particle simulator into a host uint16 [T][H][W] buffer
  * (physics of the plugin's simulateACF2D, Imaging_FCS_1_52.java:13963-14393).
  * Lives in libimfcs_sim.so. D_um2s: 4 quadrant diffusion coefficients. */

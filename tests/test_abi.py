@@ -101,9 +101,78 @@ def test_jni_calc_acf_throws_java_exception_without_device():
     outs = [env.handle(np.zeros(n)) for _ in range(3)]
     nb = [env.handle(np.zeros(16)) for _ in range(2)]
     bc = env.handle(np.zeros(1))
-    samp = env.handle(np.ones(73))
-    lag = env.handle(np.arange(73, dtype=np.int32))
+    import imaging_fcs_1_52_amd as imfcs
+    lg, sp, _, _ = imfcs.correlator_tables(16, 8, 4096, 1e-3)
+    samp = env.handle(np.asarray(sp, np.float64))
+    lag = env.handle(np.asarray(lg, np.int32))
     env.call("calcACF", pixels, outs[0], outs[1], nb[0], nb[1], outs[2], bc, samp, lag, _params(env))
     assert env.exceptions and env.exceptions[0][0] == "java/lang/Exception"
     assert "device" in env.exceptions[0][1]
     assert not env.trapped
+
+
+def test_plan_refuses_correlator_p_other_than_16():
+    """The kernels implement the plugin's default p = 16 channel scheme; a
+    panel value of P = 32 (IFCS:7204) is refused before any device work."""
+    import imaging_fcs_1_52_amd as imfcs
+    c = imfcs.make_pipeline_config(32, 32, 20000, p=32)
+    ow, oh, ch = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    rc = native.lib().imfcs_plan(ctypes.byref(c), ctypes.byref(ow), ctypes.byref(oh), ctypes.byref(ch))
+    assert rc == -2
+    assert b"p = 32 is not supported" in native.lib().imfcs_get_last_error()
+    c = imfcs.make_pipeline_config(32, 32, 20000, q=13)
+    assert native.lib().imfcs_plan(ctypes.byref(c), ctypes.byref(ow), ctypes.byref(oh), ctypes.byref(ch)) == -2
+
+
+def _acf_params(**kw):
+    a = native.ACFParameters()
+    base = dict(width=4, height=4, win_star=5, hin_star=4, w_temp=5, h_temp=4, pixbinX=1, pixbinY=1, binningX=1,
+                binningY=1, firstframe=1, lastframe=4096, framediff=4096, cfXDistance=1, cfYDistance=0,
+                correlatorp=16.0, correlatorq=8.0, frametime=1e-3, background=0, mtab1=4095.0,
+                mtabchanumminus1=16.0, sampchanumminus1=128.0, chanum=73, isNBcalculation=0, bleachcorr_gpu=0,
+                bleachcorr_order=0, nopit=1000, ave=4)
+    base.update(kw)
+    for k, v in base.items():
+        setattr(a, k, v)
+    return a
+
+
+def _calc_acf_rc(a, samp=None, lag=None):
+    """imfcs_calc_acf on zero-filled buffers; the argument checks run on the
+    host before any device work, so this needs no GPU."""
+    L = native.lib()
+    n = max(a.width * a.height * max(a.chanum, 1), 1)
+    pix = np.zeros(max(a.w_temp * a.h_temp, 1) * 4096, np.float32)
+    outs = [np.zeros(n) for _ in range(3)]
+    nb = [np.zeros(max(a.width * a.height, 1)) for _ in range(2)]
+    bc = np.zeros(max(a.w_temp * a.h_temp * 5, 1))
+    P = lambda x, t: None if x is None else x.ctypes.data_as(ctypes.POINTER(t))
+    rc = L.imfcs_calc_acf(P(pix, ctypes.c_float), P(outs[0], ctypes.c_double), P(outs[1], ctypes.c_double),
+                          P(nb[0], ctypes.c_double), P(nb[1], ctypes.c_double), P(outs[2], ctypes.c_double),
+                          P(bc, ctypes.c_double), P(samp, ctypes.c_double), P(lag, ctypes.c_int), ctypes.byref(a))
+    return rc, L.imfcs_get_last_error().decode()
+
+
+def test_calc_acf_rejects_bad_arguments_on_the_host():
+    import imaging_fcs_1_52_amd as imfcs
+    rc, msg = _calc_acf_rc(_acf_params(cfXDistance=-1))
+    assert rc == -2 and "negative CF" in msg
+    rc, msg = _calc_acf_rc(_acf_params(cfXDistance=0, cfYDistance=-2))
+    assert rc == -2 and "negative CF" in msg
+    rc, msg = _calc_acf_rc(_acf_params(chanum=72))
+    assert rc == -2 and "chanum = 72" in msg
+    rc, msg = _calc_acf_rc(_acf_params(correlatorp=32.0, chanum=145))
+    assert rc == -2 and "p = 32" in msg
+    rc, msg = _calc_acf_rc(_acf_params(cfXDistance=2))   # partner column 3 + 2 >= w_temp 5
+    assert rc == -2 and "exceeds" in msg
+    lag, samp, _, _ = imfcs.correlator_tables(16, 8, 4096, 1e-3)
+    bad_lag = lag.astype(np.int32).copy()
+    bad_lag[20] += 1
+    rc, msg = _calc_acf_rc(_acf_params(), samp=np.asarray(samp, np.float64), lag=bad_lag)
+    assert rc == -2 and "lag table" in msg and "channel 20" in msg
+    bad_samp = np.asarray(samp, np.float64).copy()
+    bad_samp[40] *= 2
+    rc, msg = _calc_acf_rc(_acf_params(), samp=bad_samp, lag=lag.astype(np.int32))
+    assert rc == -2 and "samp table" in msg
+    rc, msg = _calc_acf_rc(_acf_params(bleachcorr_gpu=1, bleachcorr_order=12))
+    assert rc == -2 and "coefficients" in msg

@@ -246,3 +246,45 @@ def test_plugin_glue_acf_all_matches_native_pipeline(engine):
     amp = np.nanmax(np.abs(ccf[..., 1:]), axis=-1, keepdims=True)
     assert rel_err(g["acf"][0], ccf, 1e-4 * amp).max() < 1e-4
     np.testing.assert_array_equal(g["ind"][0], res["blocked"][0].astype(np.int32))
+
+
+def test_fit_acf_1d_on_noisy_correlations_matches_java_fit():
+    """The drop-in `fit` (ACF_1D, LSE, weights 1/variance, tol 1e-16f, 2000
+    iterations: GPU_ACF_Fit, IFCS:15076-15256) on the correlations and
+    blocked variances of simulated pixels, against the Java CPU fit
+    (standardFit, IFCS:10288-10471) of the same curves.  N and D within 1e-3
+    relative, G_inf within 1e-3 of 1/N, for fits CONVERGED on the GPU (the
+    plugin keeps only those, IFCS:15249-15256)."""
+    import oracle
+    W = H = 8
+    T = 20000
+    stack = imfcs.simulate_2d(W, H, T, 60, tau_bleach_s=0.0, seed=303)
+    bg = int(stack.min())
+    om = oracle.make_model(PIX, PIX, PSF)
+    o = oracle.pipeline(stack, oracle.make_config(W, H, T, background=bg, poly_order=0, do_fit=False, model=om,
+                                                  nthreads=8))
+    acf, var = o["acf"].reshape(-1, 73), o["var"].reshape(-1, 73)
+    _, _, _, lagtime = imfcs.correlator_tables(16, 8, T, 1e-3)
+    nfits, npts = acf.shape[0], 72
+    geom = imfcs.make_fit_model(PIX, PIX, PSF)
+    m = imfcs.FitModel(nfits, npts, True, imfcs.MODEL_ACF_1D, tolerance=np.float32(1e-16),
+                       max_number_iterations=2000, num_valid_coefs=0,
+                       parameters_to_fit=[True, True, False, False, True] + [False] * 15, user_info_size=npts * 4)
+    m.data[:] = acf[:, 1:].reshape(-1).astype(np.float32)
+    m.weights[:] = (np.float32(1.0) / var[:, 1:].astype(np.float32)).reshape(-1)
+    row = [1.0, 1e-12, 0, 0, 0, 0, 0, 0, 0, 0, 0, geom.ax, geom.ay, geom.s, geom.sz, geom.rx, geom.ry,
+           geom.fitobsvol, geom.q2, geom.q3]
+    m.initial_parameters[:] = np.tile(np.asarray(row, np.float32), nfits)
+    m.user_info[:npts] = lagtime[1:].astype(np.float32)
+    r = imfcs.fit(m)
+    p = r.parameters.reshape(nfits, 20)
+    compared = 0
+    for k in range(nfits):
+        fr, chi2, ok, _, _ = oracle.standard_fit(om, lagtime, acf[k], var[k])
+        if not ok or r.states[k] != 0:
+            continue
+        compared += 1
+        assert abs(p[k, 0] - fr[0]) / fr[0] < 1e-3, (k, p[k, 0], fr[0])
+        assert abs(p[k, 1] - fr[1]) / fr[1] < 1e-3, (k, p[k, 1], fr[1])
+        assert abs(p[k, 4] - fr[4]) * fr[0] < 1e-3, (k, p[k, 4], fr[4])
+    assert compared >= 0.9 * nfits, compared

@@ -79,3 +79,34 @@ def test_polynomial_coefficients_match_least_squares(engine):
         ref = np.polyfit(it, tr, 4)[::-1]
         k = y * 8 + x
         np.testing.assert_allclose(gres["coef"][k], ref, rtol=1e-7, atol=1e-9 * abs(ref[0]))
+
+
+def test_ccf_fits_take_the_ccf_models_free_parameters(engine):
+    """ACF fits take acf_model.paramfit and CCF fits ccf_model.paramfit
+    (GPU_ACF_Fit fits every curve with the panel's hold flags,
+    IFCS:15131-15141): with different free sets the ACF and the CCF results
+    equal those of runs where both models carry that kind's set."""
+    stack = _stack(W=8, H=6, seed=91)
+    T, H, W = stack.shape
+    bg = int(stack.min())
+    pa = [1, 1, 0, 0, 1, 0, 0, 0, 0, 0, 0]
+    pc = [1, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0]   # D held at its initial value for the CCF
+    ini = [1.0, 0.8e-12, 0, 0, 0, 0, 0, 0, 0, 0, 0]
+
+    def run(fa, fc):
+        acf = imfcs.make_fit_model(0.24e-6, 0.24e-6, 0.38e-6, paramfit=fa, paraminit=ini)
+        ccf = imfcs.make_fit_model(0.24e-6, 0.24e-6, 0.38e-6, rx=0.24e-6, paramfit=fc, paraminit=ini)
+        cfg = imfcs.make_pipeline_config(W, H, T, cf=(1, 0), background=bg, poly_order=4, acf_model=acf,
+                                         ccf_model=ccf)
+        engine.upload(stack)
+        engine.run(cfg)
+        return engine.results(cfg)
+
+    mixed, both_a, both_c = run(pa, pc), run(pa, pa), run(pc, pc)
+    assert np.array_equal(mixed["fitres"][0], both_a["fitres"][0], equal_nan=True)
+    assert np.array_equal(mixed["fitres"][1], both_c["fitres"][1], equal_nan=True)
+    assert np.array_equal(mixed["chi2"][1], both_c["chi2"][1], equal_nan=True)
+    assert np.array_equal(mixed["states"], np.stack([both_a["states"][0], both_c["states"][1]]))
+    ok = mixed["states"][1] == 0
+    assert ok.any()
+    np.testing.assert_allclose(mixed["fitres"][1][..., 1][ok], 0.8e-12, rtol=1e-6)
