@@ -298,6 +298,27 @@ class Engine:
             self._stack_bytes = nbytes
         return self._stack
 
+    def device_buffer(self, nbytes):
+        """Device memory owned by the caller (free with free_buffer)."""
+        p = self._lib.imfcs_device_alloc(self.ctx, ctypes.c_size_t(max(int(nbytes), 1)))
+        if not p:
+            raise ImFCSError(get_last_error())
+        return p
+
+    def free_buffer(self, d_ptr):
+        _check(self._lib.imfcs_device_free(self.ctx, d_ptr))
+
+    def copy_in(self, d_ptr, arr):
+        a = np.ascontiguousarray(arr)
+        _check(self._lib.imfcs_memcpy_h2d(self.ctx, d_ptr, a.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(a.nbytes)))
+
+    def set_fit_seeder(self, acf_net=None, ccf_net=None):
+        """Start the LM fits of later runs from the networks' predictions
+        (learned.InitialGuessNet, or None for the plain start)."""
+        self._seeders = (acf_net, ccf_net)
+        _check(self._lib.imfcs_set_fit_seeder(self.ctx, acf_net.h if acf_net else None,
+                                              ccf_net.h if ccf_net else None))
+
     def download(self, d_ptr, shape, dtype=np.uint16):
         """Copies device memory at d_ptr into a new host array."""
         out = np.empty(shape, dtype)

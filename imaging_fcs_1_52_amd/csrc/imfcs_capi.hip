@@ -237,7 +237,14 @@ struct imfcs_ctx {
     imfcs_config last_cfg{};
     int has_last = 0;
     std::vector<double> last_lagtime;
+    // learned initial guesses for the ACF and the CCF fits (or null), and the
+    // per-fit starting rows they write
+    imfcs::MlpNet *seeder[2] = {nullptr, nullptr};
+    DevBuf fseed;
 };
+
+hipStream_t imfcs::ctx_stream(imfcs_ctx *ctx) { return ctx->stream; }
+int imfcs::ctx_device(imfcs_ctx *ctx) { return ctx->device; }
 
 extern "C" {
 
@@ -316,7 +323,7 @@ static void release_all(imfcs_ctx *ctx) {
                       &ctx->X, &ctx->glsblk, &ctx->glsW, &ctx->glsy, &ctx->glsok, &ctx->glscov,
                       &ctx->dropin_in, &ctx->nbm, &ctx->nbc, &ctx->swacf, &ctx->swvar, &ctx->etrace,
                       &ctx->eitime, &ctx->epfit, &ctx->einit, &ctx->eout, &ctx->estate, &ctx->echi2,
-                      &ctx->eiter, &ctx->psums};
+                      &ctx->eiter, &ctx->psums, &ctx->fseed};
     for (DevBuf *b : bufs) b->release();
 }
 
@@ -730,7 +737,8 @@ static int run_fit(imfcs_ctx *ctx, const imfcs_config *c) {
         // one launch when both kinds share the free set (the last fit first:
         // the CCF fits); else the ACF and the CCF fits as two launches, each
         // with its own parameter row
-        const bool one = n_acf == 0 || n_ccf == 0 || mask[0] == mask[1];
+        const bool seeded = !ctx->did_gls && (ctx->seeder[0] || ctx->seeder[1]);
+        const bool one = n_acf == 0 || n_ccf == 0 || (mask[0] == mask[1] && !seeded);
         struct Part { int first, count, row; };
         Part parts[2];
         int nparts = 0;
@@ -757,11 +765,31 @@ static int run_fit(imfcs_ctx *ctx, const imfcs_config *c) {
                               ctx->fstate.as<int>() + f0, ctx->fchi2.as<float>() + f0, ctx->fiter.as<int>() + f0,
                               st, split);
             } else {
-                launch_lm_acf(pt.count, npts, ctx->fdata.as<float>() + f0 * npts, ctx->fw.as<float>() + f0 * npts,
-                              ctx->fx.as<float>(), ini, pf, mask[pt.row], LMModelGeom{}, c->fit_tolerance,
-                              c->fit_max_iterations, c->fit_chi2_fp64 ? 1 : 0, ctx->fout.as<float>() + f0 * 20,
-                              ctx->fstate.as<int>() + f0, ctx->fchi2.as<float>() + f0, ctx->fiter.as<int>() + f0, st,
-                              split);
+                const float *fd = ctx->fdata.as<float>() + f0 * npts, *fw = ctx->fw.as<float>() + f0 * npts;
+                float *fo = ctx->fout.as<float>() + f0 * 20, *fc = ctx->fchi2.as<float>() + f0;
+                int *fs = ctx->fstate.as<int>() + f0, *fi = ctx->fiter.as<int>() + f0;
+                MlpNet *net = ctx->seeder[pt.row];
+                if (net && mlp_points(net) == npts) {
+                    // learned starting rows (one per fit, the model's row where
+                    // the network gives nothing usable), then the plain method
+                    // again for the fits that did not converge from them
+                    HIP_TRY(ctx->fseed.ensure(sizeof(float) * 20 * (size_t)np));
+                    float *rows = ctx->fseed.as<float>() + f0 * 20;
+                    const int rc = mlp_seed_init(net, fd, pt.count, ctx->finit.as<float>() + (size_t)pt.row * 20, rows,
+                                                 st);
+                    if (rc) return rc;
+                    launch_lm_acf(pt.count, npts, fd, fw, ctx->fx.as<float>(), rows, pf, mask[pt.row], LMModelGeom{},
+                                  c->fit_tolerance, c->fit_max_iterations, c->fit_chi2_fp64 ? 1 : 0, fo, fs, fc, fi,
+                                  st, -1);
+                    launch_lm_acf(pt.count, npts, fd, fw, ctx->fx.as<float>(),
+                                  ctx->finit.as<float>() + (size_t)pt.row * 20, pf, mask[pt.row], LMModelGeom{},
+                                  c->fit_tolerance, c->fit_max_iterations, c->fit_chi2_fp64 ? 1 : 0, fo, fs, fc, fi,
+                                  st, pt.count, 1);
+                } else {
+                    launch_lm_acf(pt.count, npts, fd, fw, ctx->fx.as<float>(), ini, pf, mask[pt.row], LMModelGeom{},
+                                  c->fit_tolerance, c->fit_max_iterations, c->fit_chi2_fp64 ? 1 : 0, fo, fs, fc, fi,
+                                  st, split);
+                }
             }
         }
         for (int r = 0; r < 2; r++)
@@ -965,6 +993,13 @@ extern "C" int imfcs_run_multi(imfcs_ctx *const *ctxs, int n, const uint16_t *d_
         const int rc = run_config(ctxs[i], d_stack, &cfgs[i], c0->psums.as<int32_t>());
         if (rc) return rc;
     }
+    return 0;
+}
+
+extern "C" int imfcs_set_fit_seeder(imfcs_ctx *ctx, imfcs_mlp *acf_net, imfcs_mlp *ccf_net) {
+    if (!ctx) return fail(IMFCS_ERR_ARG, "null context");
+    ctx->seeder[0] = mlp_of(acf_net);
+    ctx->seeder[1] = mlp_of(ccf_net);
     return 0;
 }
 

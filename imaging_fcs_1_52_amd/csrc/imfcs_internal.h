@@ -5,7 +5,13 @@
 
 #include <string>
 
+#include "../../include/imfcs.h"
+
 namespace imfcs {
+
+// The stream and device of a context (imfcs_capi.hip).
+hipStream_t ctx_stream(imfcs_ctx *ctx);
+int ctx_device(imfcs_ctx *ctx);
 
 // Records msg as the calling thread's last error (imfcs_get_last_error) and
 // returns code.
@@ -140,6 +146,39 @@ void launch_lm_gls(int nfits, int m, const float *x, const float *init, const in
                    const double *W, const double *yw, const int32_t *ok, float *out_params, int *states, float *chi2,
                    int *iters, hipStream_t st, int init_split = -1);
 
+// Training curves of the learned initial guesses (imfcs_kernels.hip): model
+// curves of FCS_3p at one fit geometry with random parameters and noise,
+// normalised by their largest magnitude s, and the standardised targets
+// (ln(N s), ln D, ln D2, F2, G_inf / s).
+struct CurveGenArgs {
+    LMModelGeom geom;          // micrometres
+    const float *x;            // [npts] lag times (s)
+    const float *nw;           // [npts] noise shape sqrt(mtab_1 / mtab_i)
+    int npts;
+    int ncomp;                 // 1: D2 = D, F2 = 0; 2: two components
+    float lnN[2], lnD[2], lnD2[2];   // uniform ranges of ln N, ln D, ln D2 (um^2/s)
+    float ginf;                // |G_inf| * N <= ginf
+    float lg_noise[2];         // log10 range of the noise relative to 1/N
+    const float *tmu, *tsd;    // [5] target standardisation
+    uint64_t seed;
+    int64_t base;              // index of the first curve (random stream)
+    uint16_t *X;               // bf16 inputs [n][ldx] (columns >= npts zero)
+    int ldx;
+    float *Tg;                 // [n][5]
+    int n;
+};
+void launch_mlp_curves(const CurveGenArgs &a, hipStream_t st);
+
+// Learned initial guesses (imfcs_mlp.hip): writes one LM parameter row per
+// fit into init ([n][20]): the template row with N, D, G_inf (and D2, F2 for
+// a two-component network) replaced by the network's prediction from the
+// fit's data curve ([n][npts], channels 1..npts); rows whose prediction is
+// not finite and positive keep the template.
+struct MlpNet;
+MlpNet *mlp_of(imfcs_mlp *m);
+int mlp_seed_init(MlpNet *net, const float *data, int n, const float *tmpl, float *init, hipStream_t st);
+int mlp_points(const MlpNet *net);
+
 // Launchers (imfcs_kernels.hip)
 void launch_trace_moments(const SeriesGeom &g, int nopit, int ave, double tscale, double frametime,
                           int order, int integer_trace, double *part, int nchunk, hipStream_t st,
@@ -196,7 +235,7 @@ void launch_correlate_fast(const SeriesGeom &g, const BleachDesc &bd, const Corr
 void launch_lm_acf(int nfits, int npts, const float *data, const float *weights, const float *x,
                    const float *init, const int *pfit, unsigned free_mask, const LMModelGeom &geom, float tol, int maxit,
                    int chi2_fp64, float *out_params, int *states, float *chi2, int *iters, hipStream_t st,
-                   int init_split = -1);
+                   int init_split = -1, int only_failed = 0);
 void launch_lm_poly(int nfits, int npts, const float *data, const float *weights, const float *x,
                     const float *init, const int *pfit, unsigned free_mask, int ncoef, float tol, int maxit,
                     int chi2_fp64, float *out_params, int *states, float *chi2, int *iters, hipStream_t st);

@@ -20,6 +20,7 @@
 // FP64.
 #include "imfcs_internal.h"
 #include "imfcs_device.h"
+#include "imfcs_rng.h"
 
 #include <math.h>
 #include <type_traits>
@@ -1052,7 +1053,20 @@ struct LMArgs {
     int *iters;
     int init_two_rows;   // 0: init holds a row per fit; 1: two rows, split at init_split
     int init_split;
+    // 1: fit again only the fits whose state is not CONVERGED (from init),
+    // adding the iterations of the earlier attempt
+    int only_failed;
 };
+
+// Whether this fit is skipped by an only_failed launch, and the iterations
+// an earlier attempt spent on it.
+__device__ __forceinline__ bool lm_skip(const LMArgs &a, int fit, int &prev_iters) {
+    prev_iters = 0;
+    if (!a.only_failed) return false;
+    if (a.states[fit] == kStateConverged) return true;
+    prev_iters = a.iters[fit];
+    return false;
+}
 
 __device__ __forceinline__ const float *init_row(const LMArgs &a, int fit) {
     if (a.init_two_rows) return a.init + (fit < a.init_split ? 0 : a.nparams);
@@ -1131,6 +1145,8 @@ __device__ void lm_eval(const LMArgs &a, int fit, const float *par, const int *f
 __global__ void __launch_bounds__(64) k_lm(LMArgs a) {
     const int fit = blockIdx.x;
     if (fit >= a.nfits) return;
+    int prev_iters;
+    if (lm_skip(a, fit, prev_iters)) return;
     const int lane = threadIdx.x & 63;
     // parameters in model units
     float par[20], prev[20];
@@ -1233,7 +1249,7 @@ __global__ void __launch_bounds__(64) k_lm(LMArgs a) {
         for (int k = 0; k < a.nparams; k++) a.out_params[(int64_t)fit * a.nparams + k] = par[k];
         a.states[fit] = state;
         a.chi2[fit] = (float)chi2;
-        a.iters[fit] = iters;
+        a.iters[fit] = iters + prev_iters;
     }
 }
 
@@ -1480,6 +1496,8 @@ __global__ void __launch_bounds__(64) k_lm_fixed(LMArgs a) {
     constexpr int NP = (KIND == kModelFCS) ? 20 : 11;
     const int fit = blockIdx.x;
     if (fit >= a.nfits) return;
+    int prev_iters;
+    if (lm_skip(a, fit, prev_iters)) return;
     const int lane = threadIdx.x & 63;
     float par[NP];
 #pragma unroll
@@ -1498,7 +1516,7 @@ __global__ void __launch_bounds__(64) k_lm_fixed(LMArgs a) {
         for (int k = 0; k < NP; k++) a.out_params[(int64_t)fit * NP + k] = par[k];
         a.states[fit] = state;
         a.chi2[fit] = (float)chi2;
-        a.iters[fit] = iters;
+        a.iters[fit] = iters + prev_iters;
     }
 }
 
@@ -1606,6 +1624,8 @@ __global__ void __launch_bounds__(128) k_lm_fcs_2w(LMArgs a) {
     // fits set the kernel's duration, are dispatched before the ACF fits
     const int fit = a.nfits - 1 - (int)blockIdx.x;
     if (fit < 0) return;
+    int prev_iters;
+    if (lm_skip(a, fit, prev_iters)) return;
     const int tid = threadIdx.x, wv = tid >> 6;
     float par[NP];
 #pragma unroll
@@ -1665,7 +1685,7 @@ __global__ void __launch_bounds__(128) k_lm_fcs_2w(LMArgs a) {
         for (int k = 0; k < NP; k++) a.out_params[(int64_t)fit * NP + k] = par[k];
         a.states[fit] = state;
         a.chi2[fit] = (float)chi2;
-        a.iters[fit] = iters;
+        a.iters[fit] = iters + prev_iters;
     }
 }
 
@@ -2115,9 +2135,9 @@ void launch_lm_gls(int nfits, int m, const float *x, const float *init, const in
 void launch_lm_acf(int nfits, int npts, const float *data, const float *weights, const float *x, const float *init,
                    const int *pfit, unsigned free_mask, const LMModelGeom &geom, float tol, int maxit,
                    int chi2_fp64, float *out_params, int *states, float *chi2, int *iters, hipStream_t st,
-                   int init_split) {
+                   int init_split, int only_failed) {
     LMArgs a{nfits, npts, 20, kModelFCS, 0, data, weights, x, init, pfit, geom, tol, maxit, chi2_fp64, out_params,
-             states, chi2, iters, init_split >= 0 ? 1 : 0, init_split};
+             states, chi2, iters, init_split >= 0 ? 1 : 0, init_split, only_failed};
     launch_lm(a, free_mask, st);
 }
 
@@ -2155,6 +2175,55 @@ void launch_pack_fit_inputs(int chanum, int npairs, const double *corav, const d
                             float *weights, hipStream_t st) {
     if (npairs <= 0) return;
     hipLaunchKernelGGL(k_pack_fit_inputs, dim3(npairs), dim3(128), 0, st, chanum, npairs, corav, bvar, data, weights);
+}
+
+// ------------------------------------------------------------------
+// Training curves for the learned initial guesses: one thread per curve
+// draws its parameters (log-uniform N and D, uniform F2 and G_inf N),
+// evaluates FCS_3p at the lag times, adds Gaussian noise whose scale relative
+// to 1/N is log-uniform and grows with the lag as sqrt(mtab_1 / mtab_i) (the
+// number of products behind a channel), normalises by the largest magnitude
+// and writes bf16 inputs and standardised targets.
+// ------------------------------------------------------------------
+__global__ void __launch_bounds__(64) k_mlp_curves(CurveGenArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const uint64_t key = rkey(a.seed, (uint64_t)(a.base + i), 0, 11);
+    const float lnN = a.lnN[0] + (a.lnN[1] - a.lnN[0]) * u01(key, 0);
+    const float lnD = a.lnD[0] + (a.lnD[1] - a.lnD[0]) * u01(key, 1);
+    const float N = expf(lnN), D = expf(lnD);
+    float lnD2 = lnD, F2 = 0.f;
+    if (a.ncomp == 2) {
+        lnD2 = a.lnD2[0] + (a.lnD2[1] - a.lnD2[0]) * u01(key, 2);
+        F2 = u01(key, 3);
+    }
+    const float G = a.ginf * (2.f * u01(key, 4) - 1.f) / N;
+    const float noise = exp10f(a.lg_noise[0] + (a.lg_noise[1] - a.lg_noise[0]) * u01(key, 5)) / N;
+    float pq[11] = {N, D, 0.f, 0.f, G, F2, expf(lnD2), 0.f, 0.f, 0.f, 0.f};
+    float g[128];
+    float s = 0.f;
+    const int np = a.npts < 128 ? a.npts : 128;
+    for (int k = 0; k < np; k++) {
+        const float v = fcs3p(a.geom, a.x[k], pq, false, a.ncomp == 2, false, nullptr) +
+                        noise * a.nw[k] * nrm(key, 8 + 2 * k);
+        g[k] = v;
+        s = fmaxf(s, fabsf(v));
+    }
+    const float is = 1.f / s;
+    uint16_t *xr = a.X + (int64_t)i * a.ldx;
+    for (int k = 0; k < a.ldx; k++) {
+        const float v = k < np ? g[k] * is : 0.f;
+        uint32_t u = __float_as_uint(v);
+        u += 0x7fffu + ((u >> 16) & 1u);   // round to nearest even
+        xr[k] = (uint16_t)(u >> 16);
+    }
+    const float t[5] = {logf(N * s), lnD, lnD2, F2, G * is};
+    for (int k = 0; k < 5; k++) a.Tg[(int64_t)i * 5 + k] = (t[k] - a.tmu[k]) / a.tsd[k];
+}
+
+void launch_mlp_curves(const CurveGenArgs &a, hipStream_t st) {
+    if (a.n <= 0) return;
+    hipLaunchKernelGGL(k_mlp_curves, dim3(cdiv(a.n, 64)), dim3(64), 0, st, a);
 }
 
 }  // namespace imfcs

@@ -15,12 +15,13 @@
 // One wavefront per emitter: every lane repeats the emitter's walk (the
 // same counter-based random numbers, so the values are wave-uniform) and
 // lane l < 49 draws the counts of pixel (l % 7 - 3, l / 7 - 3) around it.
-// Random numbers are a hash of (seed, emitter, frame, purpose, lane), so a
-// stack depends only on its parameters and seed.  Frames are simulated in
+// Random numbers are a hash of (seed, emitter, frame, purpose, lane)
+// (imfcs_rng.h), so a stack depends only on its parameters and seed.  Frames are simulated in
 // chunks into an int32 accumulation buffer, then written as uint16 with the
 // offset (saturating at 65535); the emitter state carries across chunks.
 #include "../../include/imfcs.h"
 #include "imfcs_internal.h"
+#include "imfcs_rng.h"
 
 #include <algorithm>
 #include <cmath>
@@ -30,29 +31,6 @@
 namespace imfcs {
 
 namespace {
-
-__device__ __forceinline__ uint64_t fmix64(uint64_t z) {
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
-
-// key of the random numbers of (emitter, frame, purpose)
-__device__ __forceinline__ uint64_t rkey(uint64_t seed, uint64_t e, uint64_t t, uint64_t purpose) {
-    const uint64_t h = fmix64(seed ^ (e * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull));
-    return fmix64(h ^ (t * 0xD1B54A32D192ED03ull + purpose * 0x8CB92BA72F3D8DD7ull));
-}
-
-// k-th uniform of a key in (0, 1]
-__device__ __forceinline__ float u01(uint64_t key, uint32_t k) {
-    const uint64_t z = fmix64(key + (uint64_t)(k + 1) * 0x9E3779B97F4A7C15ull);
-    return ((float)(z >> 40) + 1.0f) * (1.0f / 16777216.0f);
-}
-
-__device__ __forceinline__ double u01d(uint64_t key, uint32_t k) {
-    const uint64_t z = fmix64(key + (uint64_t)(k + 1) * 0x9E3779B97F4A7C15ull);
-    return ((double)(z >> 11) + 1.0) * (1.0 / 9007199254740992.0);
-}
 
 // Poisson(mu) draw from the uniforms of key.  mu < 12: multiplication of
 // uniforms until the product falls below e^-mu (Knuth, The Art of Computer

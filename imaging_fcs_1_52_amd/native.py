@@ -91,6 +91,16 @@ SIGNATURES = {
     "imfcs_get_stage_ms": (ctypes.c_int, [_VP, _P(_f32)]),
     "imfcs_get_correlator_kernel_ms": (ctypes.c_int, [_VP, _P(_f32)]),
     "imfcs_get_gls_covariance": (ctypes.c_int, [_VP, ctypes.c_int, _P(_f64)]),
+    "imfcs_mlp_create": (_VP, [_VP, ctypes.c_int, _P(FitModelGeom), _P(_f64), ctypes.c_int, _P(_f64),
+                               ctypes.c_uint64]),
+    "imfcs_mlp_destroy": (None, [_VP]),
+    "imfcs_mlp_train": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _f32, _P(_f32)]),
+    "imfcs_mlp_param_count": (ctypes.c_int, []),
+    "imfcs_mlp_get_params": (ctypes.c_int, [_VP, _P(_f32)]),
+    "imfcs_mlp_gradient": (ctypes.c_int, [_VP, _P(_f32), _P(_f32), ctypes.c_int, _P(_f32), _P(_f32), _P(_f32)]),
+    "imfcs_mlp_curves": (ctypes.c_int, [_VP, ctypes.c_int, _P(_f32), _P(_f32)]),
+    "imfcs_mlp_predict": (ctypes.c_int, [_VP, _VP, ctypes.c_int, _VP]),
+    "imfcs_set_fit_seeder": (ctypes.c_int, [_VP, _VP, _VP]),
     "imfcs_simulate_2d_device": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P(SimDomain),
                                                 ctypes.c_int, _f64, _f64, _f64, ctypes.c_int, _f64, ctypes.c_uint64,
                                                 _VP]),

@@ -262,6 +262,62 @@ int imfcs_get_results(imfcs_ctx *ctx, double *acf, double *var, int32_t *ind,
  * [0] trace+polyfit, [1] means, [2] blocking, [3] correlation, [4] fit. */
 int imfcs_get_stage_ms(imfcs_ctx *ctx, float *stage_ms5);
 
+/* ------------------------------------------------------------------ */
+/* 3. Learned initial guesses for the LM fit                           */
+/* ------------------------------------------------------------------ */
+
+/* No reference call corresponds: the plugin starts every LM fit from the
+ * fit panel's values (Imaging_FCS_1_52.java:1959-1968, packed for every
+ * pixel by GPU_ACF_Fit, 15090-15121).  A network maps a fit's data curve
+ * (channels 1..npts divided by their largest magnitude) to starting values
+ * N, D, G_inf (and D2, F2 with two components), for one fit geometry. */
+typedef struct imfcs_mlp imfcs_mlp;
+
+/* A BF16 perceptron npts -> 256 -> 256 -> 5 (GELU) on ctx's device and
+ * stream for the fit geometry `geom` (ax, ay, s, sz, rx, ry, fitobsvol, q2,
+ * q3 used; SI units), lag times lagtime[npts] (s) and product counts
+ * mtab[npts] of channels 1..npts; ncomp 1 or 2 components.  Weights are
+ * He-initialised from `seed`.  NULL on failure. */
+imfcs_mlp *imfcs_mlp_create(imfcs_ctx *ctx, int ncomp, const imfcs_fit_model *geom, const double *lagtime,
+                            int npts, const double *mtab, uint64_t seed);
+void imfcs_mlp_destroy(imfcs_mlp *net);
+
+/* `steps` AdamW steps on batches of `batch` fresh model curves of the
+ * network's geometry (FCS_3p with log-uniform N and D, noise relative to 1/N
+ * log-uniform in 1e-3..0.5 and growing with the lag as sqrt(mtab_1 /
+ * mtab_i)), learning rate lr with cosine decay.  loss_out[steps] (may be
+ * NULL) receives the loss of every step.  Synchronous. */
+int imfcs_mlp_train(imfcs_mlp *net, int steps, int batch, float lr, float *loss_out);
+
+/* Number of FP32 parameters, and a copy of them (W1 [80][256], b1 [256],
+ * W2 [256][256], b2 [256], W3 [256][16], b3 [16]; input rows >= npts and
+ * output columns >= 5 are padding). */
+int imfcs_mlp_param_count(void);
+int imfcs_mlp_get_params(imfcs_mlp *net, float *params);
+
+/* One forward and backward pass on host data (the gradient check): inputs
+ * [rows][80] (rounded to BF16), standardised targets [rows][5]; grad
+ * receives the parameter gradient of the weighted mean squared error,
+ * outputs [rows][5] the network outputs, loss the loss. */
+int imfcs_mlp_gradient(imfcs_mlp *net, const float *inputs, const float *targets, int rows, float *grad,
+                       float *outputs, float *loss);
+
+/* `rows` training curves as the trainer draws them next: inputs [rows][80]
+ * (BF16 values) and standardised targets [rows][5]. */
+int imfcs_mlp_curves(imfcs_mlp *net, int rows, float *inputs, float *targets);
+
+/* Predictions for device curves [rows][npts] into device out [rows][5]:
+ * N, D (um^2/s), D2 (um^2/s), F2, G_inf; NaN rows for curves that are not
+ * finite.  Synchronous. */
+int imfcs_mlp_predict(imfcs_mlp *net, const float *d_curves, int rows, float *d_out);
+
+/* Seeds the LM fits of ctx's next runs: ACF fits start from acf_net's
+ * prediction, CCF fits from ccf_net's (either may be NULL: those fits start
+ * from the model's paraminit).  Fits that do not converge from the seed are
+ * fitted again from paraminit (the plain method), their iteration counts
+ * added.  The networks must outlive their use; pass NULLs to stop. */
+int imfcs_set_fit_seeder(imfcs_ctx *ctx, imfcs_mlp *acf_net, imfcs_mlp *ccf_net);
+
 /* One periodic box of the GPU simulator (object-space micrometres; detector
  * pixel (i, j) covers [i, i+1) x [j, j+1) pixel sizes).  Its emitters start
  * uniformly inside the box and wrap at its edges.  Component 1 diffuses with
