@@ -1,30 +1,40 @@
 #!/usr/bin/env python3
 """Headline benchmark: fitted pixels/s of the whole imaging-FCS hot path
 (bin + polynomial bleach correction + multi-tau ACF/CCF with blocked variance
-+ LM fit) on a 128x128x50,000-frame uint16 stack, on one MI355X.
++ LM fit) on a 128x128x50,000-frame uint16 stack, on one MI355X, with the LM
+started from the fit panel's values (plain) and from the learned initial
+guesses (seeded).
 
 One step = for each binning b in (1, 2, 4): fused b x b binning, order-4
 polynomial bleach correction, ACF and CCF to the x+b neighbour with blocked
-variances, and the LM fit of N, D, G_inf for every ACF and CCF pixel.  The
-three binnings run as three pipelines on three HIP streams (one engine each,
-all reading the one device-resident stack, started by imfcs_run_multi: one
-pass over the stack sums every pixel over each bleach-trace point for all
-three), so the tail of one pipeline's fit overlaps the others' work.  The
-stack is device-resident before timing starts.  ``value`` counts ACF-fitted
-output pixels (the CCF work is inside the timed region but not counted).
+variances, and the LM fit of N, D, G_inf of every ACF and CCF pixel.  The
+three binnings run as three pipelines on three HIP streams over the one
+device-resident stack (imfcs_run_multi).  The stack is in HBM before timing
+starts.
 
-Prints one JSON line.  ``roofline``: the main correlator kernel at binning 1,
-algorithmic bytes (one read of the FP32 corrected series, one write of the
-level-3 sums) over its HIP-event duration on its stream, against the 8 TB/s
-HBM peak, with the copy-calibrated FETCH_SIZE + WRITE_SIZE bytes per launch
-read from the committed traffic profile.  ``cpu_baseline``: the oracle (a C
-restatement of the plugin's Java CPU path) on one host core over a pixel
-sample of the binning-1 ACF workload.
+``value``: ACF pixels fitted to state CONVERGED (finite correlation; the
+plugin keeps only those, Imaging_FCS_1_52.java:15249-15256) per second of the
+seeded pipeline, the layered method; ``value_plain`` the same for the plain
+start, ``all_pixels_per_s`` every ACF pixel processed.  The seeded pipeline
+runs the network's inference inside the timed step; its training (six
+networks: three binnings x ACF/CCF geometry) happens once before timing and
+is reported as ``seeder_training_s``.  ``lm_stage``: the fit stage alone
+(imfcs_refit on the same correlations, HIP events), plain vs seeded.
+
+``roofline``: the main correlator kernel at binning 1 against HBM, with the
+algorithmic bytes of one pass over the stack, 2 B x W x H x T (one uint16
+read), over its HIP-event duration on its stream; ``traffic`` is the
+copy-calibrated FETCH_SIZE + WRITE_SIZE per launch from the committed
+profile.  ``cpu_baseline``: the oracle (a C restatement of the plugin's Java
+CPU path) on one host core over a pixel sample of the binning-1 ACF
+workload; ``cpu_baseline_all_cores`` the same sample with OpenMP over pixels
+on every host core.
 """
 import argparse
 import ctypes
 import json
 import os
+import platform
 import sys
 import time
 
@@ -63,6 +73,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
 def make_stack(seed=2024):
     import imaging_fcs_1_52_amd as imfcs
     t0 = time.time()
@@ -73,17 +93,20 @@ def make_stack(seed=2024):
     return st
 
 
+def models(b):
+    import imaging_fcs_1_52_amd as imfcs
+    ax = PIX * b
+    return imfcs.make_fit_model(ax, ax, PSF), imfcs.make_fit_model(ax, ax, PSF, rx=PIX * b)
+
+
 def configs(stack_min):
     import imaging_fcs_1_52_amd as imfcs
     out = []
     for b in BINNINGS:
-        ax = PIX * b
-        acf = imfcs.make_fit_model(ax, ax, PSF)
-        ccf = imfcs.make_fit_model(ax, ax, PSF, rx=PIX * b)
-        cfg = imfcs.make_pipeline_config(W, H, T, binning=b, cf=(b, 0), background=stack_min, poly_order=4,
-                                         do_acf=True, do_ccf=True, do_fit=True, acf_model=acf, ccf_model=ccf,
-                                         fit_tolerance=1e-16, fit_max_iterations=2000)
-        out.append(cfg)
+        acf, ccf = models(b)
+        out.append(imfcs.make_pipeline_config(W, H, T, binning=b, cf=(b, 0), background=stack_min, poly_order=4,
+                                              do_acf=True, do_ccf=True, do_fit=True, acf_model=acf,
+                                              ccf_model=ccf, fit_tolerance=1e-16, fit_max_iterations=2000))
     return out
 
 
@@ -98,8 +121,16 @@ def cpu_baseline(stack, stack_min, sample_side, nthreads):
     t0 = time.time()
     r = oracle.pipeline(stack, cfg)
     dt = time.time() - t0
-    npx = sample_side * sample_side
-    return npx / dt, dt, r
+    nfit = int(r["fitted"].sum())
+    return nfit / dt, sample_side * sample_side / dt, dt
+
+
+def fit_stats(states, iters):
+    ok = states == 0
+    it = iters[ok] if ok.any() else iters
+    return {"converged": int(ok.sum()), "failed": int((~ok).sum()),
+            "iter_mean": round(float(iters.mean()), 2), "iter_p99": float(np.percentile(iters, 99)),
+            "iter_max": int(iters.max()), "iter_mean_converged": round(float(it.mean()), 2)}
 
 
 def main():
@@ -108,7 +139,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=48, help="side of the oracle's pixel sample (0 = skip)")
-    ap.add_argument("--cpu-threads", type=int, default=1)
+    ap.add_argument("--train-steps", type=int, default=400, help="AdamW steps of each seeding network")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain device-to-device copy of the stack (known bytes) for PMC traffic scaling")
     args = ap.parse_args()
@@ -118,69 +149,115 @@ def main():
         sys.exit(2)
 
     import imaging_fcs_1_52_amd as imfcs
+    from imaging_fcs_1_52_amd.learned import InitialGuessNet
     stack = make_stack()
     smin = int(stack.min())
     engines = [imfcs.Engine(0) for _ in BINNINGS]
+    t0 = time.perf_counter()
     d_stack = engines[0].upload(stack)
+    h2d_s = time.perf_counter() - t0
     cfgs = configs(smin)
-    n_fitted_pixels = 0
+    n_acf_pixels = 0
     for c in cfgs:
         ow, oh, _ = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         imfcs.lib().imfcs_plan(ctypes.byref(c), ctypes.byref(ow), ctypes.byref(oh), ctypes.byref(_))
-        n_fitted_pixels += ow.value * oh.value
+        n_acf_pixels += ow.value * oh.value
+    order = [BINNINGS.index(b) for b in ENQUEUE_ORDER]
 
     def step():
-        # the coarse binnings are enqueued first: their pipelines are short
-        # and end in a few long LM fits, which then overlap the b=1 pipeline
-        # (b=2, whose CCF fit tail is the longest, first of all).  One pass
-        # over the stack gives every binning its bleach trace
-        # (imfcs_run_multi)
-        order = [BINNINGS.index(b) for b in ENQUEUE_ORDER]
         imfcs.Engine.run_multi([engines[i] for i in order], [cfgs[i] for i in order], d_stack)
-        stage, kern = [], []
+        kern = []
         for e in engines:
             e.synchronize()
-            stage.append(e.stage_ms())
             kern.append(e.correlator_kernel_ms())
-        return stage, kern
+        return kern
 
-    for i in range(args.warmup):
-        step()
-        log("warmup step %d done" % i)
-    per_bin = np.zeros((len(BINNINGS), 5))
-    kern_ms = np.zeros(len(BINNINGS))
+    def converged_acf():
+        n, res = 0, []
+        for c, e in zip(cfgs, engines):
+            r = e.results(c)
+            ok = (np.asarray(r["states"])[0] == 0) & np.isfinite(np.asarray(r["acf"])[0]).all(-1)
+            n += int(ok.sum())
+            res.append(r)
+        return n, res
+
+    def timed(label):
+        for i in range(args.warmup):
+            step()
+        kern = np.zeros(len(BINNINGS))
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            kern += np.asarray(step())
+        dt = (time.perf_counter() - t0) / args.steps
+        log("%s: %d steps, %.3f ms/step" % (label, args.steps, dt * 1e3))
+        return dt, kern / args.steps
+
+    # plain LM start
+    dt_plain, kern_plain = timed("plain")
+    n_conv_plain, res_plain = converged_acf()
+
+    # seeding networks: one per binning and correlation kind (their fit geometry)
     t0 = time.perf_counter()
-    for i in range(args.steps):
-        st, km = step()
-        per_bin += np.asarray(st)
-        kern_ms += np.asarray(km)
-        log("step %d done" % i)
-    dt = time.perf_counter() - t0
-    ms_step = 1e3 * dt / args.steps
-    per_bin /= args.steps
-    kern_ms /= args.steps
-    stage_sum = per_bin.sum(0)
-    value = n_fitted_pixels / (dt / args.steps)
+    nets = []
+    for b, e in zip(BINNINGS, engines):
+        _, _, mtab, lagtime = imfcs.correlator_tables(16, 8, T, 1e-3)
+        pair = []
+        for k, m in enumerate(models(b)):
+            net = InitialGuessNet(e, m, lagtime[1:], mtab[1:], ncomp=1, seed=100 + 10 * b + k)
+            net.train(steps=args.train_steps, batch=4096, lr=2e-3)
+            pair.append(net)
+        nets.append(pair)
+        e.set_fit_seeder(*pair)
+    train_s = time.perf_counter() - t0
+    log("trained %d seeding networks in %.1f s" % (2 * len(BINNINGS), train_s))
+    dt_seed, kern_seed = timed("seeded")
+    n_conv_seed, res_seed = converged_acf()
+
+    # fit stage alone on the last correlations (imfcs_refit): plain vs seeded
+    lm = {}
+    agree_all = []
+    for b, c, e, pair, rs in zip(BINNINGS, cfgs, engines, nets, res_seed):
+        row = {}
+        for mode in ("plain", "seeded"):
+            e.set_fit_seeder(*(pair if mode == "seeded" else (None, None)))
+            ms = []
+            for _ in range(max(1, args.steps)):
+                e.refit(c)
+                e.synchronize()
+                ms.append(e.stage_ms()[4])
+            r = e.results(c)
+            st, it = np.asarray(r["states"]), np.asarray(r["iters"])
+            row[mode] = {"fit_ms": round(float(np.median(ms)), 3),
+                         "fitted_px_per_s": round(float((st == 0).sum() / (np.median(ms) * 1e-3)), 1),
+                         "acf": fit_stats(st[0], it[0]), "ccf": fit_stats(st[1], it[1]), "_r": r}
+        rp, rs2 = row["plain"].pop("_r"), row["seeded"].pop("_r")
+        both = (np.asarray(rp["states"]) == 0) & (np.asarray(rs2["states"]) == 0)
+        dD = np.abs(np.asarray(rs2["fitres"])[..., 1] / np.asarray(rp["fitres"])[..., 1] - 1)
+        agree = dD[both] < 1e-3
+        agree_all.append(agree)
+        row["D_within_1e-3"] = round(float(agree.mean()), 5) if agree.size else None
+        row["speedup"] = round(row["plain"]["fit_ms"] / row["seeded"]["fit_ms"], 3)
+        lm["b%d" % b] = row
+    agree_frac = float(np.concatenate(agree_all).mean())
+    fit_ms_plain = sum(lm[k]["plain"]["fit_ms"] for k in lm)
+    fit_ms_seed = sum(lm[k]["seeded"]["fit_ms"] for k in lm)
+    for e in engines:
+        e.set_fit_seeder(*([None, None]))
+
+    value_plain = n_conv_plain / dt_plain
+    value_seed = n_conv_seed / dt_seed
 
     # roofline of the main correlator kernel at b = 1 (HIP events on its
-    # stream, averaged over the timed steps): it reads the corrected FP32
-    # series of every slot once (4 B per slot-frame) and writes the level-3
-    # sums of every slot once (4 B per 8 frames)
-    nslots_b1 = W * H   # 128 x 128 slot grid (127 output columns + the x+1 partner column)
-    alg_bytes = 4.0 * nslots_b1 * T + 4.0 * nslots_b1 * (T // 8)
-    corr_ms = kern_ms[0]
+    # stream, averaged over the seeded timed steps), on the algorithmic bytes
+    # of one pass over the stack: one uint16 read of W x H x T
+    alg_bytes = 2.0 * W * H * T
+    corr_ms = kern_seed[0]
     achieved = alg_bytes / (corr_ms * 1e-3) / 1e9
     roof = {"kernel": "k_corr_main (b=1, ACF+CCF)", "bound": "hbm", "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(),
             "avg_ms": round(float(corr_ms), 3), "alg_bytes": alg_bytes}
 
-    traffic = read_traffic()
-    if traffic is not None:
-        roof["traffic"] = traffic
-
     if args.copy_calibration:
-        # a plain copy of COPY_BYTES: its FETCH_SIZE / WRITE_SIZE under
-        # rocprofv3 --pmc give the counters' bytes-per-byte scale
         lib = imfcs.lib()
         dst = lib.imfcs_device_alloc(engines[0].ctx, ctypes.c_size_t(COPY_BYTES))
         if dst:
@@ -188,40 +265,47 @@ def main():
             lib.imfcs_device_free(engines[0].ctx, dst)
             log("copy calibration: %d bytes" % COPY_BYTES)
 
-    # LM iteration statistics of every binning (untimed re-run)
     iters = {}
-    for b, c, e in zip(BINNINGS, cfgs, engines):
-        e.run(c, d_stack)
-        r = e.results(c)
-        # ACF fits (pair 0).  Pixels whose bleach polynomial crosses zero get
-        # NaN correlations (the reference's correction takes sqrt(f/c0) of a
-        # negative ratio) and their fits stop at once; iteration counts are
-        # over the converged fits
+    for b, r in zip(BINNINGS, res_plain):
         acf_ok = np.isfinite(np.asarray(r["acf"])[0]).all(-1)
         stt = np.asarray(r["states"])[0]
-        it = np.asarray(r["iters"])[0][stt == 0]
-        iters["b%d" % b] = {"median": float(np.median(it)), "p99": float(np.percentile(it, 99)),
-                            "max": int(it.max()), "finite_acf_frac": round(float(acf_ok.mean()), 4),
+        iters["b%d" % b] = {"finite_acf_frac": round(float(acf_ok.mean()), 4),
                             "converged_frac": round(float((stt == 0).mean()), 4)}
 
-    cpu = None
+    cpu = cpu_all = None
     if args.cpu_sample > 0:
-        rate, cdt, _ = cpu_baseline(stack, smin, args.cpu_sample, args.cpu_threads)
-        cpu = {"value": round(rate, 3), "unit": "fitted pixels/s", "cores": args.cpu_threads, "kind": "port",
-               "sample": "%dx%d px ROI of the b=1 ACF workload, 50,000 frames, poly4, LM fit (%.1f s)"
-                         % (args.cpu_sample, args.cpu_sample, cdt)}
+        ncores = os.cpu_count() or 1
+        rate, rate_px, cdt = cpu_baseline(stack, smin, args.cpu_sample, 1)
+        smp = "%dx%d px ROI of the b=1 ACF workload, 50,000 frames, poly4, LM fit" % (args.cpu_sample,
+                                                                                     args.cpu_sample)
+        cpu = {"value": round(rate, 3), "unit": "fitted pixels/s", "cores": 1, "kind": "port",
+               "sample": smp + " (%.1f s; fitted = optimizer without exception)" % cdt,
+               "all_pixels_per_s": round(rate_px, 3), "cpu_model": cpu_model()}
+        rate_a, rate_pa, cdt_a = cpu_baseline(stack, smin, args.cpu_sample, ncores)
+        cpu_all = {"value": round(rate_a, 3), "unit": "fitted pixels/s", "cores": ncores, "kind": "port",
+                   "sample": smp + " (%.1f s)" % cdt_a, "all_pixels_per_s": round(rate_pa, 3),
+                   "nproc": ncores, "cpu_model": cpu_model()}
 
-    line = {"metric": "fitted pixels/s (bin+bleach+ACF+LM), 128^2x50k", "value": round(value, 1),
-            "unit": "fitted pixels/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-            "dtype": "f32/f64", "data": "synthetic (seeded 2D particle simulation)",
+    line = {"metric": "fitted pixels/s (bin+bleach+ACF+LM), 128^2x50k; seeded vs plain LM at 1e-3 D",
+            "value": round(value_seed, 1), "unit": "fitted pixels/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(dt_seed * 1e3, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "u16/f32/f64 (bf16 seeding network)",
+            "data": "synthetic (seeded 2D particle simulation)",
             "config": {"workload": "128x128x50000 uint16; binning 1,2,4; ACF+CCF(x+b); poly4 bleach; LM N,D,G",
-                       "fitted_pixels_per_step": n_fitted_pixels},
-            "stages_ms_sum_of_streams": {"trace_polyfit": round(stage_sum[0], 3), "means": round(stage_sum[1], 3),
-                          "blocking": round(stage_sum[2], 3), "correlate": round(stage_sum[3], 3),
-                          "fit": round(stage_sum[4], 3)},
-            "fit_iterations": iters, "roofline": roof, "cpu_baseline": cpu}
+                       "acf_pixels_per_step": n_acf_pixels, "converged_acf_fits_per_step": n_conv_seed},
+            "value_plain": round(value_plain, 1), "ms_per_step_plain": round(dt_plain * 1e3, 3),
+            "converged_acf_fits_per_step_plain": n_conv_plain,
+            "all_pixels_per_s": round(n_acf_pixels / dt_seed, 1),
+            "host_to_host_per_s": round(n_conv_seed / (dt_seed + h2d_s), 1), "h2d_s_pageable": round(h2d_s, 3),
+            "seeded_vs_plain": {"D_within_1e-3_frac": round(agree_frac, 5),
+                                "lm_stage_speedup": round(fit_ms_plain / fit_ms_seed, 3),
+                                "pipeline_speedup": round(dt_plain / dt_seed, 3)},
+            "lm_stage": lm, "seeder_training_s": round(train_s, 2), "fit_iterations": iters,
+            "roofline": roof, "cpu_baseline": cpu, "cpu_baseline_all_cores": cpu_all}
     print(json.dumps(line), flush=True)
+    for pair in nets:
+        for n in pair:
+            n.close()
     for e in engines[::-1]:
         e.close()
 

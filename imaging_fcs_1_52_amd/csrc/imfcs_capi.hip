@@ -750,6 +750,36 @@ static int run_fit(imfcs_ctx *ctx, const imfcs_config *c) {
         if (ctx->did_gls)
             for (int i = 0; i < nparts; i++)
                 if (mask[parts[i].row] != 0x13u) return fail(IMFCS_ERR_ARG, "the GLS fit supports the free set N, D, G_inf");
+        // learned starting rows: a seeded attempt of at most 15 % of the
+        // iteration limit (a fit converging from a good start takes a few
+        // tens; one still running by then is wandering along a flat valley),
+        // then the plain start (the model's row) for the fits it did not
+        // converge (their iterations added)
+        const int seed_maxit = std::max(1, std::min(c->fit_max_iterations, (c->fit_max_iterations * 3 + 19) / 20));
+        const int chi2_64 = c->fit_chi2_fp64 ? 1 : 0;
+        float *fo = ctx->fout.as<float>(), *fc = ctx->fchi2.as<float>();
+        int *fs = ctx->fstate.as<int>(), *fi = ctx->fiter.as<int>();
+        const float *fd = ctx->fdata.as<float>(), *fw = ctx->fw.as<float>(), *fx = ctx->fx.as<float>();
+        const bool all_seeded = seeded && (n_acf == 0 || (ctx->seeder[0] && mlp_points(ctx->seeder[0]) == npts)) &&
+                                (n_ccf == 0 || (ctx->seeder[1] && mlp_points(ctx->seeder[1]) == npts));
+        if (all_seeded && (n_acf == 0 || n_ccf == 0 || mask[0] == mask[1])) {
+            // every fit seeded: one LM launch over ACF and CCF fits, one fallback launch
+            HIP_TRY(ctx->fseed.ensure(sizeof(float) * 20 * (size_t)np));
+            float *rows = ctx->fseed.as<float>();
+            int rc = 0;
+            if (n_acf) rc = mlp_seed_init(ctx->seeder[0], fd, n_acf, ctx->finit.as<float>(), rows, st);
+            if (!rc && n_ccf)
+                rc = mlp_seed_init(ctx->seeder[1], fd + (size_t)n_acf * npts, n_ccf, ctx->finit.as<float>() + 20,
+                                   rows + (size_t)n_acf * 20, st);
+            if (rc) return rc;
+            const int row = n_acf ? 0 : 1;
+            const int *pf = ctx->fpfit.as<int>() + (size_t)row * 20;
+            launch_lm_acf(np, npts, fd, fw, fx, rows, pf, mask[row], LMModelGeom{}, c->fit_tolerance, seed_maxit,
+                          chi2_64, fo, fs, fc, fi, st, -1);
+            launch_lm_acf(np, npts, fd, fw, fx, ctx->finit.as<float>(), pf, mask[row], LMModelGeom{},
+                          c->fit_tolerance, c->fit_max_iterations, chi2_64, fo, fs, fc, fi, st, n_acf, 1);
+            nparts = 0;
+        }
         for (int i = 0; i < nparts; i++) {
             const Part &pt = parts[i];
             // two init rows: with one launch, fits below n_acf take row 0 and
@@ -759,37 +789,27 @@ static int run_fit(imfcs_ctx *ctx, const imfcs_config *c) {
             const int split = one ? n_acf : pt.count;
             const size_t f0 = (size_t)pt.first;
             if (ctx->did_gls) {
-                launch_lm_gls(pt.count, npts, ctx->fx.as<float>(), ini, pf, c->fit_tolerance, c->fit_max_iterations,
+                launch_lm_gls(pt.count, npts, fx, ini, pf, c->fit_tolerance, c->fit_max_iterations,
                               ctx->glsW.as<double>() + f0 * npts * npts, ctx->glsy.as<double>() + f0 * npts,
-                              ctx->glsok.as<int32_t>() + f0, ctx->fout.as<float>() + f0 * 20,
-                              ctx->fstate.as<int>() + f0, ctx->fchi2.as<float>() + f0, ctx->fiter.as<int>() + f0,
-                              st, split);
+                              ctx->glsok.as<int32_t>() + f0, fo + f0 * 20, fs + f0, fc + f0, fi + f0, st, split);
+                continue;
+            }
+            MlpNet *net = ctx->seeder[pt.row];
+            if (net && mlp_points(net) == npts) {
+                HIP_TRY(ctx->fseed.ensure(sizeof(float) * 20 * (size_t)np));
+                float *rows = ctx->fseed.as<float>() + f0 * 20;
+                const int rc = mlp_seed_init(net, fd + f0 * npts, pt.count, ini, rows, st);
+                if (rc) return rc;
+                launch_lm_acf(pt.count, npts, fd + f0 * npts, fw + f0 * npts, fx, rows, pf, mask[pt.row],
+                              LMModelGeom{}, c->fit_tolerance, seed_maxit, chi2_64, fo + f0 * 20, fs + f0, fc + f0,
+                              fi + f0, st, -1);
+                launch_lm_acf(pt.count, npts, fd + f0 * npts, fw + f0 * npts, fx, ini, pf, mask[pt.row],
+                              LMModelGeom{}, c->fit_tolerance, c->fit_max_iterations, chi2_64, fo + f0 * 20, fs + f0,
+                              fc + f0, fi + f0, st, pt.count, 1);
             } else {
-                const float *fd = ctx->fdata.as<float>() + f0 * npts, *fw = ctx->fw.as<float>() + f0 * npts;
-                float *fo = ctx->fout.as<float>() + f0 * 20, *fc = ctx->fchi2.as<float>() + f0;
-                int *fs = ctx->fstate.as<int>() + f0, *fi = ctx->fiter.as<int>() + f0;
-                MlpNet *net = ctx->seeder[pt.row];
-                if (net && mlp_points(net) == npts) {
-                    // learned starting rows (one per fit, the model's row where
-                    // the network gives nothing usable), then the plain method
-                    // again for the fits that did not converge from them
-                    HIP_TRY(ctx->fseed.ensure(sizeof(float) * 20 * (size_t)np));
-                    float *rows = ctx->fseed.as<float>() + f0 * 20;
-                    const int rc = mlp_seed_init(net, fd, pt.count, ctx->finit.as<float>() + (size_t)pt.row * 20, rows,
-                                                 st);
-                    if (rc) return rc;
-                    launch_lm_acf(pt.count, npts, fd, fw, ctx->fx.as<float>(), rows, pf, mask[pt.row], LMModelGeom{},
-                                  c->fit_tolerance, c->fit_max_iterations, c->fit_chi2_fp64 ? 1 : 0, fo, fs, fc, fi,
-                                  st, -1);
-                    launch_lm_acf(pt.count, npts, fd, fw, ctx->fx.as<float>(),
-                                  ctx->finit.as<float>() + (size_t)pt.row * 20, pf, mask[pt.row], LMModelGeom{},
-                                  c->fit_tolerance, c->fit_max_iterations, c->fit_chi2_fp64 ? 1 : 0, fo, fs, fc, fi,
-                                  st, pt.count, 1);
-                } else {
-                    launch_lm_acf(pt.count, npts, fd, fw, ctx->fx.as<float>(), ini, pf, mask[pt.row], LMModelGeom{},
-                                  c->fit_tolerance, c->fit_max_iterations, c->fit_chi2_fp64 ? 1 : 0, fo, fs, fc, fi,
-                                  st, split);
-                }
+                launch_lm_acf(pt.count, npts, fd + f0 * npts, fw + f0 * npts, fx, ini, pf, mask[pt.row],
+                              LMModelGeom{}, c->fit_tolerance, c->fit_max_iterations, chi2_64, fo + f0 * 20, fs + f0,
+                              fc + f0, fi + f0, st, split);
             }
         }
         for (int r = 0; r < 2; r++)

@@ -1063,7 +1063,10 @@ struct LMArgs {
 __device__ __forceinline__ bool lm_skip(const LMArgs &a, int fit, int &prev_iters) {
     prev_iters = 0;
     if (!a.only_failed) return false;
-    if (a.states[fit] == kStateConverged) return true;
+    // converged, or data without a finite chi-square (NaN correlations: the
+    // fit fails from any start)
+    const float c2 = a.chi2[fit];
+    if (a.states[fit] == kStateConverged || !(c2 == c2)) return true;
     prev_iters = a.iters[fit];
     return false;
 }

@@ -173,10 +173,13 @@ __global__ void k_fill_f32(float *p, int64_t n, float v) {
 }
 
 // AdamW (Loshchilov & Hutter, "Decoupled weight decay regularization",
-// ICLR 2019) on the FP32 master parameters; BF16 copies of the weight
-// matrices (wmask: 1 where the parameter is a weight, with its copy index).
+// ICLR 2019) on the FP32 master parameters, and the BF16 copies of the
+// weight matrices: [K][N] for the training products (copy_of: index of a
+// parameter's copy, -1 for biases) and [N][K] for the inference kernel
+// (copyT_of).
 __global__ void k_adamw(float *p, const float *gr, float *m, float *v, int64_t n, float lr, float b1, float b2,
-                        float eps, float wd, float c1, float c2, const int32_t *copy_of, uint16_t *pb) {
+                        float eps, float wd, float c1, float c2, const int32_t *copy_of, uint16_t *pb,
+                        const int32_t *copyT_of, uint16_t *pbT) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const float gi = gr[i];
         const float mi = b1 * m[i] + (1.f - b1) * gi;
@@ -187,28 +190,13 @@ __global__ void k_adamw(float *p, const float *gr, float *m, float *v, int64_t n
         float pi = p[i];
         pi -= lr * ((mi * c1) / (sqrtf(vi * c2) + eps) + (c >= 0 ? wd * pi : 0.f));
         p[i] = pi;
-        if (c >= 0) pb[c] = f2bf(pi);
+        if (c >= 0) {
+            pb[c] = f2bf(pi);
+            pbT[copyT_of[i]] = pb[c];
+        }
     }
 }
 
-// Inputs of real curves: channels divided by their largest magnitude, BF16;
-// scale[i] = s (0 when the curve is not finite or all zero).
-__global__ void k_mlp_inputs(const float *data, int n, int npts, uint16_t *X, float *scale) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float *d = data + (int64_t)i * npts;
-    float s = 0.f;
-    bool fin = true;
-    for (int k = 0; k < npts; k++) {
-        const float v = d[k];
-        fin = fin && isfinite(v);
-        s = fmaxf(s, fabsf(v));
-    }
-    const bool ok = fin && s > 0.f;
-    const float is = ok ? 1.f / s : 0.f;
-    for (int k = 0; k < kIn; k++) X[(int64_t)i * kIn + k] = f2bf(ok && k < npts ? d[k] * is : 0.f);
-    scale[i] = ok ? s : 0.f;
-}
 
 // Standardised network outputs -> physical values: N, D (um^2/s), D2, F2, G_inf.
 __device__ __forceinline__ void decode(const float *y, const float *mu, const float *sd, float s, float *o) {
@@ -221,42 +209,131 @@ __device__ __forceinline__ void decode(const float *y, const float *mu, const fl
     o[4] = t[4] * s;
 }
 
-__global__ void k_mlp_decode(const float *Y, const float *scale, int n, const float *mu, const float *sd,
-                             float *out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float o[kNOut];
-    const float s = scale[i];
-    if (s > 0.f) decode(Y + (int64_t)i * kOut, mu, sd, s, o);
-    for (int k = 0; k < kNOut; k++) out[(int64_t)i * kNOut + k] = s > 0.f ? o[k] : __builtin_nanf("");
-}
 
-// LM starting rows: the template with N, D, G_inf (and D2, F2 for two
-// components) from the network, SI units (D in m^2/s) as the LM kernels take
-// them; the template where the prediction is not usable.
-__global__ void k_mlp_seed(const float *Y, const float *scale, int n, const float *mu, const float *sd, int ncomp,
-                           const float *tmpl, float *init) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float row[20];
-    for (int k = 0; k < 20; k++) row[k] = tmpl[k];
-    const float s = scale[i];
-    if (s > 0.f) {
+
+// The whole forward pass for 64 curves per workgroup (4 wavefronts), for
+// seeding fits and predictions: inputs normalised in registers, the three
+// layers on v_mfma_f32_16x16x16_bf16 with the activations kept in LDS
+// (BF16, as the training path stores them) and the weights read as [N][K]
+// BF16 copies straight into the MFMA B operand (one 8-byte load per lane
+// and step); the outputs decoded into LM starting rows (init, from the
+// template tmpl) and/or physical values (out5).
+struct FusedArgs {
+    const float *data;
+    int n, npts;
+    const uint16_t *W1T, *W2T, *W3T;
+    const float *b1, *b2, *b3, *mu, *sd;
+    int ncomp;
+    const float *tmpl;
+    float *init;   // [n][20] or null
+    float *out5;   // [n][5] or null
+};
+
+__global__ void __launch_bounds__(256) k_mlp_fused(FusedArgs a) {
+    __shared__ __attribute__((aligned(16))) uint16_t Xs[64][kIn + 8];
+    __shared__ __attribute__((aligned(16))) uint16_t H1s[64][kHid + 8];
+    __shared__ __attribute__((aligned(16))) uint16_t H2s[64][kHid + 8];
+    __shared__ float Ys[64][kOut + 1];
+    __shared__ float Ss[64];
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int r0 = blockIdx.x * 64;
+    // inputs: wave w normalises rows w, w + 4, ...; lane l holds channels l and l + 64
+    for (int rr = w; rr < 64; rr += 4) {
+        const int row = r0 + rr;
+        float v0 = 0.f, v1 = 0.f;
+        if (row < a.n) {
+            const float *d = a.data + (int64_t)row * a.npts;
+            if (lane < a.npts) v0 = d[lane];
+            if (lane + 64 < a.npts) v1 = d[lane + 64];
+        }
+        float mx = fmaxf(fabsf(v0), fabsf(v1));
+        int bad = !(isfinite(v0) && isfinite(v1));
+        for (int off = 32; off > 0; off >>= 1) {
+            mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+            bad |= __shfl_xor(bad, off, 64);
+        }
+        const bool ok = row < a.n && !bad && mx > 0.f;
+        const float is = ok ? 1.f / mx : 0.f;
+        Xs[rr][lane] = f2bf(v0 * is);
+        if (lane < kIn - 64) Xs[rr][64 + lane] = f2bf(v1 * is);
+        if (lane == 0) Ss[rr] = ok ? mx : 0.f;
+    }
+    __syncthreads();
+    const int lr = lane & 15, lq = 4 * (lane >> 4);
+    // hidden layers: wave w computes columns [64 w, 64 w + 64) of all 64 rows
+    auto hidden = [&](const uint16_t(*In)[kHid + 8], const uint16_t *WT, int K, const float *bias,
+                      uint16_t(*Out)[kHid + 8], bool from_x) {
+        f32x4 acc[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int k0 = 0; k0 < K; k0 += 16) {
+            bf16x4 av[4], bv[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                av[i] = from_x ? *(const bf16x4 *)&Xs[i * 16 + lr][k0 + lq] : *(const bf16x4 *)&In[i * 16 + lr][k0 + lq];
+#pragma unroll
+            for (int j = 0; j < 4; j++) bv[j] = *(const bf16x4 *)&WT[(int64_t)(64 * w + j * 16 + lr) * K + k0 + lq];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[i], bv[j], acc[i][j], 0, 0, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int col = 64 * w + j * 16 + lr;
+                    Out[i * 16 + lq + r][col] = f2bf(gelu(acc[i][j][r] + bias[col]));
+                }
+    };
+    hidden(H2s, a.W1T, kIn, a.b1, H1s, true);
+    __syncthreads();
+    hidden(H1s, a.W2T, kHid, a.b2, H2s, false);
+    __syncthreads();
+    // output layer: wave w computes rows [16 w, 16 w + 16) x 16 outputs
+    {
+        f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int k0 = 0; k0 < kHid; k0 += 16) {
+            const bf16x4 av = *(const bf16x4 *)&H2s[16 * w + lr][k0 + lq];
+            const bf16x4 bv = *(const bf16x4 *)&a.W3T[(int64_t)lr * kHid + k0 + lq];
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) Ys[16 * w + lq + r][lr] = acc[r] + a.b3[lr];
+    }
+    __syncthreads();
+    if (tid < 64) {
+        const int row = r0 + tid;
+        if (row >= a.n) return;
+        const float s = Ss[tid];
         float o[kNOut];
-        decode(Y + (int64_t)i * kOut, mu, sd, s, o);
-        const bool ok = isfinite(o[0]) && o[0] > 0.f && isfinite(o[1]) && o[1] > 0.f && isfinite(o[4]) &&
-                        (ncomp == 1 || (isfinite(o[2]) && o[2] > 0.f));
+        bool ok = s > 0.f;
         if (ok) {
-            row[0] = o[0];
-            row[1] = o[1] * 1e-12f;
-            row[4] = o[4];
-            if (ncomp == 2) {
-                row[6] = o[2] * 1e-12f;
-                row[5] = o[3];
+            decode(&Ys[tid][0], a.mu, a.sd, s, o);
+            ok = isfinite(o[0]) && o[0] > 0.f && isfinite(o[1]) && o[1] > 0.f && isfinite(o[4]) &&
+                 (a.ncomp == 1 || (isfinite(o[2]) && o[2] > 0.f));
+        }
+        if (a.out5)
+            for (int k = 0; k < kNOut; k++) a.out5[(int64_t)row * kNOut + k] = s > 0.f ? o[k] : __builtin_nanf("");
+        if (a.init) {
+            float rowv[20];
+            for (int k = 0; k < 20; k++) rowv[k] = a.tmpl[k];
+            if (ok) {
+                rowv[0] = o[0];
+                rowv[1] = o[1] * 1e-12f;
+                rowv[4] = o[4];
+                if (a.ncomp == 2) {
+                    rowv[6] = o[2] * 1e-12f;
+                    rowv[5] = o[3];
+                }
             }
+            for (int k = 0; k < 20; k++) a.init[(int64_t)row * 20 + k] = rowv[k];
         }
     }
-    for (int k = 0; k < 20; k++) init[(int64_t)i * 20 + k] = row[k];
 }
 
 inline unsigned cdivu(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
@@ -279,8 +356,8 @@ struct MlpNet {
                              oW3 = ob2 + kHid, ob3 = oW3 + kHid * kOut, nP = ob3 + kOut;
     static constexpr int64_t bW1 = 0, bW2 = bW1 + kIn * kHid, bW3 = bW2 + kHid * kHid, nB = bW3 + kHid * kOut;
     float *P = nullptr, *G = nullptr, *Mo = nullptr, *Vo = nullptr;
-    uint16_t *Pb = nullptr;
-    int32_t *copy_of = nullptr;
+    uint16_t *Pb = nullptr, *PbT = nullptr;
+    int32_t *copy_of = nullptr, *copyT_of = nullptr;
     float *consts = nullptr;   // lag[128], nw[128], tmu[5], tsd[5], lossw[5]
     float *loss = nullptr;     // [max steps]
     int loss_cap = 0;
@@ -326,7 +403,7 @@ struct MlpNet {
     }
     ~MlpNet() {
         free_rows();
-        void *ps[] = {P, G, Mo, Vo, Pb, copy_of, consts, loss};
+        void *ps[] = {P, G, Mo, Vo, Pb, PbT, copy_of, copyT_of, consts, loss};
         for (void *p : ps)
             if (p) (void)hipFree(p);
     }
@@ -405,33 +482,30 @@ struct MlpNet {
         drawn += rows;
     }
 
+    void fused(const float *data, int n, const float *tmpl, float *init, float *out5, hipStream_t s) const;
+
     void adamw(float lr) {
         adam_t++;
         const float b1 = 0.9f, b2 = 0.999f;
         const float c1 = 1.f / (1.f - std::pow(b1, (float)adam_t)), c2 = 1.f / (1.f - std::pow(b2, (float)adam_t));
         hipLaunchKernelGGL(k_adamw, dim3(cdivu(nP, 256)), dim3(256), 0, st, P, G, Mo, Vo, (int64_t)nP, lr, b1, b2,
-                           1e-8f, 1e-4f, c1, c2, copy_of, Pb);
+                           1e-8f, 1e-4f, c1, c2, copy_of, Pb, copyT_of, PbT);
     }
 };
 
 int mlp_points(const MlpNet *net) { return net->npts; }
 
+void MlpNet::fused(const float *data, int n, const float *tmpl, float *init, float *out5, hipStream_t s) const {
+    if (n <= 0) return;
+    FusedArgs f{data, n, npts, PbT + bW1, PbT + bW2, PbT + bW3, P + ob1, P + ob2, P + ob3, d_mu(), d_sd(), ncomp,
+                tmpl, init, out5};
+    hipLaunchKernelGGL(k_mlp_fused, dim3(cdivu(n, 64)), dim3(256), 0, s, f);
+}
+
 int mlp_seed_init(MlpNet *net, const float *data, int n, const float *tmpl, float *init, hipStream_t st) {
     if (!net || n <= 0) return 0;
-    hipStream_t keep = net->st;
-    net->st = st;
-    const int chunk = 16384;
-    hipError_t e = net->ensure_rows(std::min(n, chunk));
-    for (int r0 = 0; r0 < n && e == hipSuccess; r0 += chunk) {
-        const int rows = std::min(chunk, n - r0);
-        hipLaunchKernelGGL(k_mlp_inputs, dim3(cdivu(rows, 64)), dim3(64), 0, st, data + (int64_t)r0 * net->npts, rows,
-                           net->npts, net->X, net->scale);
-        net->forward(rows);
-        hipLaunchKernelGGL(k_mlp_seed, dim3(cdivu(rows, 64)), dim3(64), 0, st, net->Y, net->scale, rows, net->d_mu(),
-                           net->d_sd(), net->ncomp, tmpl, init + (int64_t)r0 * 20);
-        e = hipGetLastError();
-    }
-    net->st = keep;
+    net->fused(data, n, tmpl, init, nullptr, st);
+    const hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : set_error(IMFCS_ERR_DEVICE, std::string("mlp seed: ") + hipGetErrorString(e));
 }
 
@@ -484,7 +558,7 @@ imfcs_mlp *imfcs_mlp_create(imfcs_ctx *ctx, int ncomp, const imfcs_fit_model *ge
     }
     // He-normal weights, zero biases (host RNG, seeded)
     std::vector<float> P(MlpNet::nP, 0.f);
-    std::vector<int32_t> copy_of(MlpNet::nP, -1);
+    std::vector<int32_t> copy_of(MlpNet::nP, -1), copyT_of(MlpNet::nP, -1);
     std::mt19937_64 rng(seed * 0x9E3779B97F4A7C15ull + 7);
     auto fill = [&](int64_t off, int64_t boff, int rows, int cols, int used_rows, int used_cols) {
         std::normal_distribution<float> nd(0.f, std::sqrt(2.f / (float)used_rows));
@@ -493,19 +567,20 @@ imfcs_mlp *imfcs_mlp_create(imfcs_ctx *ctx, int ncomp, const imfcs_fit_model *ge
                 const int64_t i = off + (int64_t)r * cols + c;
                 P[i] = (r < used_rows && c < used_cols) ? nd(rng) : 0.f;
                 copy_of[i] = (int32_t)(boff + (int64_t)r * cols + c);
+                copyT_of[i] = (int32_t)(boff + (int64_t)c * rows + r);
             }
     };
     fill(MlpNet::oW1, MlpNet::bW1, kIn, kHid, npts, kHid);
     fill(MlpNet::oW2, MlpNet::bW2, kHid, kHid, kHid, kHid);
     fill(MlpNet::oW3, MlpNet::bW3, kHid, kOut, kHid, kNOut);
     for (int64_t i = MlpNet::oW3; i < MlpNet::ob3; i++) P[i] *= 0.5f;
-    std::vector<uint16_t> Pb(MlpNet::nB, 0);
+    std::vector<uint16_t> Pb(MlpNet::nB, 0), PbT(MlpNet::nB, 0);
     for (int64_t i = 0; i < MlpNet::nP; i++)
         if (copy_of[i] >= 0) {
             uint32_t u;
             std::memcpy(&u, &P[i], 4);
             u += 0x7fffu + ((u >> 16) & 1u);
-            Pb[copy_of[i]] = (uint16_t)(u >> 16);
+            Pb[copy_of[i]] = PbT[copyT_of[i]] = (uint16_t)(u >> 16);
         }
     hipError_t e = hipSuccess;
     auto al = [&](void **p, size_t b) {
@@ -517,10 +592,15 @@ imfcs_mlp *imfcs_mlp_create(imfcs_ctx *ctx, int ncomp, const imfcs_fit_model *ge
     al((void **)&n.Vo, sizeof(float) * MlpNet::nP);
     al((void **)&n.Pb, sizeof(uint16_t) * MlpNet::nB);
     al((void **)&n.copy_of, sizeof(int32_t) * MlpNet::nP);
+    al((void **)&n.PbT, sizeof(uint16_t) * MlpNet::nB);
+    al((void **)&n.copyT_of, sizeof(int32_t) * MlpNet::nP);
     al((void **)&n.consts, sizeof(float) * consts.size());
     if (e == hipSuccess) e = hipMemcpy(n.P, P.data(), sizeof(float) * P.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(n.Pb, Pb.data(), sizeof(uint16_t) * Pb.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(n.copy_of, copy_of.data(), sizeof(int32_t) * copy_of.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(n.PbT, PbT.data(), sizeof(uint16_t) * PbT.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(n.copyT_of, copyT_of.data(), sizeof(int32_t) * copyT_of.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(n.consts, consts.data(), sizeof(float) * consts.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(n.Mo, 0, sizeof(float) * MlpNet::nP);
     if (e == hipSuccess) e = hipMemset(n.Vo, 0, sizeof(float) * MlpNet::nP);
@@ -567,6 +647,15 @@ int imfcs_mlp_train(imfcs_mlp *m, int steps, int batch, float lr, float *loss_ou
 }
 
 int imfcs_mlp_param_count(void) { return (int)MlpNet::nP; }
+
+int imfcs_mlp_target_scale(imfcs_mlp *m, float *mu, float *sd) {
+    if (!m || !mu || !sd) return set_error(IMFCS_ERR_ARG, "mlp: null argument");
+    for (int k = 0; k < kNOut; k++) {
+        mu[k] = m->net.tmu[k];
+        sd[k] = m->net.tsd[k];
+    }
+    return 0;
+}
 
 int imfcs_mlp_get_params(imfcs_mlp *m, float *params) {
     if (!m || !params) return set_error(IMFCS_ERR_ARG, "mlp: null argument");
@@ -636,17 +725,8 @@ int imfcs_mlp_predict(imfcs_mlp *m, const float *d_curves, int rows, float *d_ou
     if (!m || !d_curves || !d_out || rows < 1) return set_error(IMFCS_ERR_ARG, "mlp predict: bad arguments");
     MlpNet &n = m->net;
     if (hipSetDevice(n.device) != hipSuccess) return set_error(IMFCS_ERR_DEVICE, "mlp: hipSetDevice failed");
-    const int chunk = 16384;
-    hipError_t e = n.ensure_rows(std::min(rows, chunk));
-    for (int r0 = 0; r0 < rows && e == hipSuccess; r0 += chunk) {
-        const int r = std::min(chunk, rows - r0);
-        hipLaunchKernelGGL(k_mlp_inputs, dim3(cdivu(r, 64)), dim3(64), 0, n.st, d_curves + (int64_t)r0 * n.npts, r,
-                           n.npts, n.X, n.scale);
-        n.forward(r);
-        hipLaunchKernelGGL(k_mlp_decode, dim3(cdivu(r, 64)), dim3(64), 0, n.st, n.Y, n.scale, r, n.d_mu(), n.d_sd(),
-                           d_out + (int64_t)r0 * kNOut);
-        e = hipGetLastError();
-    }
+    n.fused(d_curves, rows, nullptr, nullptr, d_out, n.st);
+    hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(n.st);
     return e == hipSuccess ? 0 : set_error(IMFCS_ERR_DEVICE, std::string("mlp predict: ") + hipGetErrorString(e));
 }

@@ -69,6 +69,12 @@ class InitialGuessNet:
         self._check(lib().imfcs_mlp_get_params(self.h, _fp(p)))
         return p
 
+    def target_scale(self):
+        """(mu, sd) of the standardised outputs."""
+        mu, sd = np.zeros(N_OUT, np.float32), np.zeros(N_OUT, np.float32)
+        self._check(lib().imfcs_mlp_target_scale(self.h, _fp(mu), _fp(sd)))
+        return mu, sd
+
     def gradient(self, inputs, targets):
         """One forward + backward pass: (parameter gradient, outputs, loss)."""
         x = np.ascontiguousarray(inputs, np.float32)

@@ -97,6 +97,7 @@ SIGNATURES = {
     "imfcs_mlp_train": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _f32, _P(_f32)]),
     "imfcs_mlp_param_count": (ctypes.c_int, []),
     "imfcs_mlp_get_params": (ctypes.c_int, [_VP, _P(_f32)]),
+    "imfcs_mlp_target_scale": (ctypes.c_int, [_VP, _P(_f32), _P(_f32)]),
     "imfcs_mlp_gradient": (ctypes.c_int, [_VP, _P(_f32), _P(_f32), ctypes.c_int, _P(_f32), _P(_f32), _P(_f32)]),
     "imfcs_mlp_curves": (ctypes.c_int, [_VP, ctypes.c_int, _P(_f32), _P(_f32)]),
     "imfcs_mlp_predict": (ctypes.c_int, [_VP, _VP, ctypes.c_int, _VP]),

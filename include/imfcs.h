@@ -295,6 +295,11 @@ int imfcs_mlp_train(imfcs_mlp *net, int steps, int batch, float lr, float *loss_
 int imfcs_mlp_param_count(void);
 int imfcs_mlp_get_params(imfcs_mlp *net, float *params);
 
+/* The standardisation of the five outputs: the network predicts (t - mu) /
+ * sd of t = (ln(N s), ln D, ln D2, F2, G_inf / s), s the curve's largest
+ * magnitude. */
+int imfcs_mlp_target_scale(imfcs_mlp *net, float *mu, float *sd);
+
 /* One forward and backward pass on host data (the gradient check): inputs
  * [rows][80] (rounded to BF16), standardised targets [rows][5]; grad
  * receives the parameter gradient of the weighted mean squared error,

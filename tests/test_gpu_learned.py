@@ -51,6 +51,31 @@ def test_gradients_match_cpu_oracle(engine):
         assert exact_err < 5e-2, (name, exact_err)
 
 
+def test_fused_inference_matches_cpu_oracle(engine):
+    """The one-kernel inference (seeding and predictions) against the
+    oracle's forward pass on the same BF16-normalised inputs: N, D within
+    2e-3 relative, G_inf within 2e-3 of 1/N; non-finite curves give NaN."""
+    net, geom, lagtime = _net(engine, seed=11)
+    net.train(steps=50, batch=2048)
+    cur, N, D, G = _model_curves(geom, lagtime, 333, 4)     # 333 rows: a partial 64-row tile
+    cur[5, 7] = np.nan
+    got = net.predict(cur)
+    p = net.params()
+    mu, sd = net.target_scale()
+    s = np.abs(cur).max(1)
+    x = np.zeros((cur.shape[0], 80))
+    x[:, :72] = cur / s[:, None]
+    _, _, y = mlp.forward_backward(p, x, np.zeros((cur.shape[0], 5)), mode="mirror")
+    t = y * sd + mu
+    ref = np.stack([np.exp(t[:, 0]) / s, np.exp(t[:, 1]), np.exp(t[:, 2]), np.clip(t[:, 3], 0, 1), t[:, 4] * s], 1)
+    ok = np.ones(cur.shape[0], bool)
+    ok[5] = False
+    assert np.isnan(got[5]).all()
+    np.testing.assert_allclose(got[ok, 0], ref[ok, 0], rtol=2e-3)
+    np.testing.assert_allclose(got[ok, 1], ref[ok, 1], rtol=2e-3)
+    assert (np.abs(got[ok, 4] - ref[ok, 4]) * ref[ok, 0]).max() < 2e-3
+
+
 def _model_curves(geom, lagtime, n, seed):
     import oracle
     om = oracle.make_model(geom.ax, geom.ay, geom.s, rx=geom.rx)
