@@ -35,12 +35,34 @@ int fail(int code, const std::string &msg) {
             return fail(IMFCS_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-// A growable device buffer.
+// Carving of device buffers from the scratch array during an
+// imfcs_run_noalloc call on this thread (null otherwise).
+struct Arena {
+    unsigned char *base;
+    size_t size, off, need;
+    bool overflow;
+};
+thread_local Arena *g_arena = nullptr;
+
+// A growable device buffer: hipMalloc'd, or carved from the scratch arena
+// while one is active (then nothing is freed or allocated).
 struct DevBuf {
     void *p = nullptr;
     size_t n = 0;
     hipError_t ensure(size_t bytes) {
         if (bytes <= n) return hipSuccess;
+        if (g_arena) {
+            const size_t a = (g_arena->off + 255) & ~(size_t)255;
+            if (a + bytes > g_arena->size) {
+                g_arena->overflow = true;
+                g_arena->need = a + bytes;
+                return hipErrorOutOfMemory;
+            }
+            p = g_arena->base + a;
+            n = bytes;
+            g_arena->off = a + bytes;
+            return hipSuccess;
+        }
         if (p) (void)hipFree(p);
         p = nullptr;
         n = 0;
@@ -200,6 +222,37 @@ bool device_present() {
 
 int imfcs::set_error(int code, const std::string &msg) { return fail(code, msg); }
 
+// Scratch of the allocation-free entry point (imfcs_run_noalloc): every
+// device buffer of one pipeline run is carved from it.  Each GPU holds its
+// own copy until the process ends; its address on each device is looked up
+// once, with that device current (imfcs_create), and kept in g_scratch_ptr:
+// host code uses that address, never the array's name.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<void *> g_scratch_ptr;   // per device
+
+// With `device` current: records the device address of scratch.
+int scratch_setup(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+    if (g_scratch_ptr[device]) return 0;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(IMFCS_ERR_DEVICE, std::string("scratch address: ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = ptr;
+    return 0;
+}
+
+void *scratch_ptr(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return device < (int)g_scratch_ptr.size() ? g_scratch_ptr[device] : nullptr;
+}
+}  // namespace
+
 // ------------------------------------------------------------------
 // Context
 // ------------------------------------------------------------------
@@ -241,6 +294,9 @@ struct imfcs_ctx {
     // per-fit starting rows they write
     imfcs::MlpNet *seeder[2] = {nullptr, nullptr};
     DevBuf fseed;
+    // 1: the last run was imfcs_run_noalloc (its results went to the
+    // caller's buffers; nothing of it stays in the context)
+    int last_noalloc = 0;
 };
 
 hipStream_t imfcs::ctx_stream(imfcs_ctx *ctx) { return ctx->stream; }
@@ -299,6 +355,9 @@ imfcs_ctx *imfcs_create(int device) {
         fail(IMFCS_ERR_DEVICE, "hipSetDevice failed");
         return nullptr;
     }
+    // the scratch address of this device (imfcs_run_noalloc); a failure
+    // leaves the other entry points usable and is reported by that call
+    (void)scratch_setup(device);
     imfcs_ctx *ctx = new imfcs_ctx();
     ctx->device = device;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -315,8 +374,8 @@ imfcs_ctx *imfcs_create(int device) {
     return ctx;
 }
 
-static void release_all(imfcs_ctx *ctx) {
-    DevBuf *bufs[] = {&ctx->part, &ctx->coef, &ctx->ctot, &ctx->rec, &ctx->F, &ctx->bpart, &ctx->ind,
+static std::vector<DevBuf *> all_bufs(imfcs_ctx *ctx) {
+    return {&ctx->part, &ctx->coef, &ctx->ctot, &ctx->rec, &ctx->F, &ctx->bpart, &ctx->ind,
                       &ctx->bis, &ctx->corav, &ctx->bvar, &ctx->fdata, &ctx->fw, &ctx->fx, &ctx->finit,
                       &ctx->fpfit, &ctx->fout, &ctx->fstate, &ctx->fchi2, &ctx->fiter, &ctx->cpart,
                       &ctx->hpart, &ctx->l3, &ctx->lvd, &ctx->ctab, &ctx->csum, &ctx->slots,
@@ -324,7 +383,16 @@ static void release_all(imfcs_ctx *ctx) {
                       &ctx->dropin_in, &ctx->nbm, &ctx->nbc, &ctx->swacf, &ctx->swvar, &ctx->etrace,
                       &ctx->eitime, &ctx->epfit, &ctx->einit, &ctx->eout, &ctx->estate, &ctx->echi2,
                       &ctx->eiter, &ctx->psums, &ctx->fseed};
-    for (DevBuf *b : bufs) b->release();
+}
+
+static void invalidate_uploads(imfcs_ctx *ctx) {
+    CachedUpload *ups[] = {&ctx->up_ctab, &ctx->up_slots, &ctx->up_fx, &ctx->up_finit, &ctx->up_fpfit,
+                           &ctx->up_eitime, &ctx->up_epfit};
+    for (CachedUpload *u : ups) u->invalidate();
+}
+
+static void release_all(imfcs_ctx *ctx) {
+    for (DevBuf *b : all_bufs(ctx)) b->release();
 }
 
 void imfcs_destroy(imfcs_ctx *ctx) {
@@ -931,7 +999,14 @@ static int run_config(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_confi
 }
 
 extern "C" int imfcs_run(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_config *c) {
+    if (ctx) ctx->last_noalloc = 0;
     return run_config(ctx, d_stack, c, nullptr);
+}
+
+static int noalloc_guard(imfcs_ctx *ctx) {
+    if (ctx->last_noalloc)
+        return fail(IMFCS_ERR_ARG, "the last run was imfcs_run_noalloc: its results are in the caller's buffers");
+    return 0;
 }
 
 // Configurations that give the same correlations (every field but the fit's).
@@ -953,6 +1028,7 @@ static bool same_correlation(const imfcs_config &a, const imfcs_config &b) {
 extern "C" int imfcs_refit(imfcs_ctx *ctx, const imfcs_config *c) {
     if (!ctx || !c) return fail(IMFCS_ERR_ARG, "null argument");
     if (!ctx->has_last) return fail(IMFCS_ERR_ARG, "no correlations to refit: run the pipeline first");
+    if (noalloc_guard(ctx)) return IMFCS_ERR_ARG;
     if (!same_correlation(*c, ctx->last_cfg))
         return fail(IMFCS_ERR_ARG, "refit configuration differs from the last run outside the fit settings");
     HIP_TRY(hipSetDevice(ctx->device));
@@ -978,6 +1054,7 @@ extern "C" int imfcs_run_multi(imfcs_ctx *const *ctxs, int n, const uint16_t *d_
     if (!ctxs || !cfgs || !d_stack || n < 1) return fail(IMFCS_ERR_ARG, "null argument");
     for (int i = 0; i < n; i++)
         if (!ctxs[i]) return fail(IMFCS_ERR_ARG, "null context");
+    for (int i = 0; i < n; i++) ctxs[i]->last_noalloc = 0;
     bool share = true;
     for (int i = 0; i < n && share; i++) {
         const imfcs_config &c = cfgs[i];
@@ -1025,6 +1102,7 @@ extern "C" int imfcs_set_fit_seeder(imfcs_ctx *ctx, imfcs_mlp *acf_net, imfcs_ml
 
 extern "C" int imfcs_get_stage_ms(imfcs_ctx *ctx, float *ms) {
     if (!ctx || !ms) return fail(IMFCS_ERR_ARG, "null argument");
+    if (noalloc_guard(ctx)) return IMFCS_ERR_ARG;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     for (int i = 0; i < 5; i++) {
         float t = 0;
@@ -1036,6 +1114,7 @@ extern "C" int imfcs_get_stage_ms(imfcs_ctx *ctx, float *ms) {
 
 extern "C" int imfcs_get_gls_covariance(imfcs_ctx *ctx, int pair, double *cov) {
     if (!ctx || !cov) return fail(IMFCS_ERR_ARG, "null argument");
+    if (noalloc_guard(ctx)) return IMFCS_ERR_ARG;
     if (!ctx->did_gls) return fail(IMFCS_ERR_ARG, "the last run computed no GLS covariance");
     if (pair < 0 || pair >= ctx->npairs) return fail(IMFCS_ERR_ARG, "pair index out of range");
     const size_t m = ctx->chanum - 1;
@@ -1047,6 +1126,7 @@ extern "C" int imfcs_get_gls_covariance(imfcs_ctx *ctx, int pair, double *cov) {
 
 extern "C" int imfcs_get_correlator_kernel_ms(imfcs_ctx *ctx, float *ms) {
     if (!ctx || !ms) return fail(IMFCS_ERR_ARG, "null argument");
+    if (noalloc_guard(ctx)) return IMFCS_ERR_ARG;
     if (!ctx->evk_valid) return fail(IMFCS_ERR_ARG, "no correlation run yet");
     HIP_TRY(hipEventSynchronize(ctx->evk[1]));
     HIP_TRY(hipEventElapsedTime(ms, ctx->evk[0], ctx->evk[1]));
@@ -1056,6 +1136,7 @@ extern "C" int imfcs_get_correlator_kernel_ms(imfcs_ctx *ctx, float *ms) {
 extern "C" int imfcs_get_results(imfcs_ctx *ctx, double *acf, double *var, int32_t *ind, int32_t *blocked,
                                  double *fitres, double *chi2, int32_t *states, int32_t *iters, double *coeffs) {
     if (!ctx) return fail(IMFCS_ERR_ARG, "null context");
+    if (noalloc_guard(ctx)) return IMFCS_ERR_ARG;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     const size_t np = ctx->npairs, nc = ctx->chanum;
     if (acf) HIP_TRY(hipMemcpy(acf, ctx->corav.p, sizeof(double) * np * nc, hipMemcpyDeviceToHost));
@@ -1084,6 +1165,76 @@ extern "C" int imfcs_get_results(imfcs_ctx *ctx, double *acf, double *var, int32
         if (iters) HIP_TRY(hipMemcpy(iters, ctx->fiter.p, sizeof(int32_t) * np, hipMemcpyDeviceToHost));
     }
     return 0;
+}
+
+// Fit results of a run into the caller's device arrays: parameters 0..10
+// (SI units) as double, chi-square normalised per correlation kind.
+__global__ void k_export_fit(int np, const float *fout, const float *fchi2, int split, double norm0, double norm1,
+                             double *fitres, double *chi2) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    for (int k = 0; k < 11; k++) fitres[(size_t)p * 11 + k] = (double)fout[(size_t)p * 20 + k];
+    chi2[p] = (double)fchi2[p] / (p < split ? norm0 : norm1);
+}
+
+// imfcs_run without device allocation: every buffer of the run is carved
+// from the file-scope scratch array of the device (kScratchBytes), the
+// results go to the caller's device arrays.  The context's own buffers are
+// set aside for the call and restored after it (its enqueued work keeps the
+// scratch addresses it was given).
+extern "C" int imfcs_run_noalloc(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_config *cfg, double *acf,
+                                 double *var, int32_t *ind, int32_t *blocked, double *fitres, double *chi2,
+                                 int32_t *states, int32_t *iters) {
+    if (!ctx || !d_stack || !cfg || !acf || !var || !ind || !blocked) return fail(IMFCS_ERR_ARG, "null argument");
+    if (cfg->do_fit && (!fitres || !chi2 || !states || !iters)) return fail(IMFCS_ERR_ARG, "null fit output");
+    if (ctx->seeder[0] || ctx->seeder[1])
+        return fail(IMFCS_ERR_ARG, "imfcs_run_noalloc does not seed fits (the networks hold their own buffers)");
+    HIP_TRY(hipSetDevice(ctx->device));
+    void *sp = scratch_ptr(ctx->device);
+    if (!sp) return fail(IMFCS_ERR_DEVICE, "scratch address of this device unknown (imfcs_create could not look it up)");
+    std::vector<DevBuf *> bufs = all_bufs(ctx);
+    std::vector<DevBuf> saved;
+    saved.reserve(bufs.size());
+    for (DevBuf *b : bufs) {
+        saved.push_back(*b);
+        *b = DevBuf{};
+    }
+    invalidate_uploads(ctx);
+    Arena ar{(unsigned char *)sp, kScratchBytes, 0, 0, false};
+    g_arena = &ar;
+    int rc = run_config(ctx, d_stack, cfg, nullptr);
+    if (rc == 0) {
+        hipStream_t st = ctx->stream;
+        const size_t np = ctx->npairs, nc = ctx->chanum;
+        hipError_t e = hipMemcpyAsync(acf, ctx->corav.p, sizeof(double) * np * nc, hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(var, ctx->bvar.p, sizeof(double) * np * nc, hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(ind, ctx->ind.p, sizeof(int32_t) * np, hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(blocked, ctx->bis.p, sizeof(int32_t) * np, hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess && cfg->do_fit) {
+            hipLaunchKernelGGL(k_export_fit, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, (int)np,
+                               ctx->fout.as<float>(), ctx->fchi2.as<float>(), ctx->fit_split, ctx->chi2_norm[0],
+                               ctx->chi2_norm[1], fitres, chi2);
+            e = hipGetLastError();
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(states, ctx->fstate.p, sizeof(int32_t) * np, hipMemcpyDeviceToDevice, st);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(iters, ctx->fiter.p, sizeof(int32_t) * np, hipMemcpyDeviceToDevice, st);
+        }
+        if (e != hipSuccess) rc = fail(IMFCS_ERR_DEVICE, std::string("imfcs_run_noalloc: ") + hipGetErrorString(e));
+    }
+    g_arena = nullptr;
+    if (rc != 0 && ar.overflow)
+        rc = fail(IMFCS_ERR_MEMORY, "imfcs_run_noalloc: this configuration needs more than " +
+                                        std::to_string(kScratchBytes >> 20) + " MiB of scratch (at least " +
+                                        std::to_string(ar.need) + " bytes)");
+    for (size_t i = 0; i < bufs.size(); i++) *bufs[i] = saved[i];
+    invalidate_uploads(ctx);
+    ctx->has_last = 0;
+    ctx->evk_valid = 0;
+    ctx->last_noalloc = 1;
+    return rc;
 }
 
 // ------------------------------------------------------------------

@@ -84,6 +84,7 @@ SIGNATURES = {
     "imfcs_memcpy_d2d": (ctypes.c_int, [_VP, _VP, _VP, _SZ]),
     "imfcs_synchronize": (ctypes.c_int, [_VP]),
     "imfcs_run": (ctypes.c_int, [_VP, _VP, _P(PipelineConfig)]),
+    "imfcs_run_noalloc": (ctypes.c_int, [_VP, _VP, _P(PipelineConfig), _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "imfcs_run_multi": (ctypes.c_int, [_P(_VP), ctypes.c_int, _VP, _P(PipelineConfig)]),
     "imfcs_refit": (ctypes.c_int, [_VP, _P(PipelineConfig)]),
     "imfcs_get_results": (ctypes.c_int, [_VP, _P(_f64), _P(_f64), _P(_i32), _P(_i32), _P(_f64), _P(_f64),

@@ -218,6 +218,23 @@ int imfcs_synchronize(imfcs_ctx *ctx);
  * on the context's stream. */
 int imfcs_run(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_config *cfg);
 
+/* No counterpart in the reference: imfcs_run for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Every scratch buffer of the run is
+ * carved from a file-scope device array of 64 MiB per GPU (held until the
+ * process ends); configurations that need more are refused with
+ * IMFCS_ERR_MEMORY.  No hipMalloc or hipFree.  Results go to the caller's
+ * device arrays, laid out as imfcs_get_results' host arrays (acf, var:
+ * [pair][chanum]; ind, blocked, chi2, states, iters: [pair]; fitres:
+ * [pair][11]; the fit arrays may be NULL when cfg->do_fit is 0).
+ * Asynchronous on ctx's stream.  Every call on one GPU shares its scratch
+ * array: calls that overlap on one GPU (other contexts, other threads)
+ * race.  No fit seeding; imfcs_get_results and imfcs_refit do not apply to
+ * such a run. */
+int imfcs_run_noalloc(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_config *cfg, double *acf, double *var,
+                      int32_t *ind, int32_t *blocked, double *fitres, double *chi2, int32_t *states,
+                      int32_t *iters);
+
 /* n configurations of one device stack, cfgs[i] on ctxs[i] (distinct
  * contexts, one stream each) -- the binnings of a diffusion law
  * (determineDiffusionLaw, IFCS:8190-8215, runs one pipeline per binning).
