@@ -385,29 +385,35 @@ class Engine:
         return ms.value
 
     def results(self, cfg):
-        ow, oh, chanum = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        _check(self._lib.imfcs_plan(ctypes.byref(cfg), ctypes.byref(ow), ctypes.byref(oh), ctypes.byref(chanum)))
-        ow, oh, nc = ow.value, oh.value, chanum.value
-        npairs = ow * oh * (int(cfg.do_acf) + int(cfg.do_ccf))
-        nseries = ow * oh * (2 if cfg.do_ccf else 1)
-        acf, var = np.zeros((npairs, nc)), np.zeros((npairs, nc))
-        ind, bis = np.zeros(npairs, np.int32), np.zeros(npairs, np.int32)
-        fitres, chi2 = np.full((npairs, 11), np.nan), np.full(npairs, np.nan)
-        states, iters = np.full(npairs, -1, np.int32), np.zeros(npairs, np.int32)
-        if cfg.bleach_mode == 2:   # bcInt0 and the segment means
-            ncoef = (cfg.lastframe - cfg.firstframe + 1) // cfg.sliding_window_length + 1
-        elif cfg.bleach_mode == 3:   # f(0), A, t1, C
-            ncoef = 4
-        else:
-            ncoef = max(cfg.poly_order, 0) + 1
-        coef = np.zeros((nseries, ncoef))
-        _check(self._lib.imfcs_get_results(self.ctx, _dp(acf), _dp(var), _ip(ind), _ip(bis), _dp(fitres),
-                                           _dp(chi2), _ip(states), _ip(iters), _dp(coef)))
-        npair_types = int(cfg.do_acf) + int(cfg.do_ccf)
-        sh = (npair_types, oh, ow)
-        return dict(acf=acf.reshape(sh + (nc,)), var=var.reshape(sh + (nc,)), ind=ind.reshape(sh),
-                    blockIndS=bis.reshape(sh), fitres=fitres.reshape(sh + (11,)), chi2=chi2.reshape(sh),
-                    states=states.reshape(sh), iters=iters.reshape(sh), coef=coef)
+        return _results(self.ctx, cfg)
+
+
+def _results(ctx, cfg):
+    """imfcs_get_results of context ctx's last run of cfg, as arrays shaped
+    (pair kind, out_h, out_w, ...)."""
+    ow, oh, chanum = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _check(lib().imfcs_plan(ctypes.byref(cfg), ctypes.byref(ow), ctypes.byref(oh), ctypes.byref(chanum)))
+    ow, oh, nc = ow.value, oh.value, chanum.value
+    npairs = ow * oh * (int(cfg.do_acf) + int(cfg.do_ccf))
+    nseries = ow * oh * (2 if cfg.do_ccf else 1)
+    acf, var = np.zeros((npairs, nc)), np.zeros((npairs, nc))
+    ind, bis = np.zeros(npairs, np.int32), np.zeros(npairs, np.int32)
+    fitres, chi2 = np.full((npairs, 11), np.nan), np.full(npairs, np.nan)
+    states, iters = np.full(npairs, -1, np.int32), np.zeros(npairs, np.int32)
+    if cfg.bleach_mode == 2:   # bcInt0 and the segment means
+        ncoef = (cfg.lastframe - cfg.firstframe + 1) // cfg.sliding_window_length + 1
+    elif cfg.bleach_mode == 3:   # f(0), A, t1, C
+        ncoef = 4
+    else:
+        ncoef = max(cfg.poly_order, 0) + 1
+    coef = np.zeros((nseries, ncoef))
+    _check(lib().imfcs_get_results(ctx, _dp(acf), _dp(var), _ip(ind), _ip(bis), _dp(fitres),
+                                       _dp(chi2), _ip(states), _ip(iters), _dp(coef)))
+    npair_types = int(cfg.do_acf) + int(cfg.do_ccf)
+    sh = (npair_types, oh, ow)
+    return dict(acf=acf.reshape(sh + (nc,)), var=var.reshape(sh + (nc,)), ind=ind.reshape(sh),
+                blockIndS=bis.reshape(sh), fitres=fitres.reshape(sh + (11,)), chi2=chi2.reshape(sh),
+                states=states.reshape(sh), iters=iters.reshape(sh), coef=coef)
 
 
 def simulate_2d(W, H, T, n_emitters, D_um2s=1.0, pixel_um=0.24, psf_sigma_um=0.19, frametime=1e-3,
@@ -445,13 +451,72 @@ def sim_domain(x0_um, y0_um, w_um, h_um, n_emitters, D_um2s=1.0, D2_um2s=0.0, fr
 
 
 def simulate_2d_device(engine, W, H, T, domains, pixel_um=0.24, psf_sigma_um=0.19, frametime=1e-3, offset=100,
-                       tau_bleach_s=0.0, seed=1):
-    """The particle simulator on the GPU into a uint16 [T][H][W] stack owned by
-    `engine` (replacing its current stack); returns the device pointer."""
+                       tau_bleach_s=0.0, seed=1, d_out=None):
+    """The particle simulator on the GPU into a uint16 [T][H][W] stack: the
+    device buffer d_out, or one owned by `engine` (replacing its current
+    stack); returns the device pointer."""
     doms = domains if isinstance(domains, (list, tuple)) else [domains]
     arr = (SimDomain * len(doms))(*doms)
     nbytes = 2 * W * H * T
-    d_stack = engine.alloc_stack(nbytes)
+    d_stack = d_out if d_out else engine.alloc_stack(nbytes)
     _check(lib().imfcs_simulate_2d_device(d_stack, W, H, T, arr, len(doms), pixel_um, psf_sigma_um, frametime,
                                           offset, tau_bleach_s, ctypes.c_uint64(seed), None))
     return d_stack
+
+
+class Batch:
+    """A batch of independent stacks (imfcs_batch_*): one context per stack,
+    every stack's pipeline on its own stream and all fits in one persistent
+    launch; with graph=True later runs of the same stacks and configurations
+    replay a captured hipGraph; autotune=True times the frame chunking per
+    stack shape first."""
+
+    def __init__(self, n, device=0, graph=True, autotune=True):
+        self.n = n
+        self.h = lib().imfcs_batch_create(device, n, (1 if graph else 0) | (2 if autotune else 0))
+        if not self.h:
+            raise ImFCSError(get_last_error())
+
+    def run(self, d_stacks, cfgs):
+        if len(d_stacks) != self.n or len(cfgs) != self.n:
+            raise ValueError("one stack and one configuration per batch entry")
+        st = (ctypes.c_void_p * self.n)(*d_stacks)
+        arr = (PipelineConfig * self.n)(*cfgs)
+        _check(lib().imfcs_batch_run(self.h, st, arr))
+
+    def synchronize(self):
+        _check(lib().imfcs_batch_synchronize(self.h))
+
+    def results(self, i, cfg):
+        ctx = lib().imfcs_batch_context(self.h, i)
+        if not ctx:
+            raise ImFCSError(get_last_error())
+        return _results(ctx, cfg)
+
+    def slot_count(self, i):
+        n = ctypes.c_int()
+        _check(lib().imfcs_get_slot_count(lib().imfcs_batch_context(self.h, i), ctypes.byref(n)))
+        return n.value
+
+    def info(self):
+        g, f, t, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(lib().imfcs_batch_info(self.h, ctypes.byref(g), ctypes.byref(f), ctypes.byref(t), ctypes.byref(ms)))
+        return {"graph_nodes": g.value, "fit_groups": f.value, "tuned_shapes": t.value, "autotune_ms": ms.value}
+
+    def close(self):
+        if self.h:
+            lib().imfcs_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def get_tuning(nslots, frames):
+    """(apply_log2, block_log2) frame chunking recorded for a series shape."""
+    a, b = ctypes.c_int(), ctypes.c_int()
+    _check(lib().imfcs_get_tuning(nslots, frames, ctypes.byref(a), ctypes.byref(b)))
+    return a.value, b.value

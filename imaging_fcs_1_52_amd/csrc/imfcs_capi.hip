@@ -9,10 +9,12 @@
 #include "imfcs_internal.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -247,6 +249,28 @@ int scratch_setup(int device) {
     return 0;
 }
 
+// Frames-per-chunk choices of the bleach-correction / prefix pass and of the
+// blocking pass, as the exponent of the threads kept in flight (2^e), per
+// series shape (slots, frames).  The batch autotuner times candidates and
+// records the fastest; every later run of that shape (batched or not) uses
+// them, so batched and single runs compute identically.
+struct Tune {
+    int apply_log2 = 19, block_log2 = 19;
+};
+std::mutex g_tune_mu;
+std::map<std::pair<int, int>, Tune> g_tune;
+
+Tune tune_lookup(int nslots, int T) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find({nslots, T});
+    return it == g_tune.end() ? Tune{} : it->second;
+}
+
+void tune_store(int nslots, int T, const Tune &t) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[{nslots, T}] = t;
+}
+
 void *scratch_ptr(int device) {
     std::lock_guard<std::mutex> lk(g_scratch_mu);
     return device < (int)g_scratch_ptr.size() ? g_scratch_ptr[device] : nullptr;
@@ -297,6 +321,10 @@ struct imfcs_ctx {
     // 1: the last run was imfcs_run_noalloc (its results went to the
     // caller's buffers; nothing of it stays in the context)
     int last_noalloc = 0;
+    // autotuner trial: chunking forced for the next runs (apply_log2 > 0)
+    Tune force_tune{0, 0};
+    // the chunking the last run used
+    Tune used_tune{};
 };
 
 hipStream_t imfcs::ctx_stream(imfcs_ctx *ctx) { return ctx->stream; }
@@ -533,8 +561,12 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTabl
     if (rc) return rc;
     rc = make_corr_tables(ht, g.T, ea.q, ct);
     if (rc) return rc;
-    // the host sources of the previous run's copies are reused below
-    HIP_TRY(hipEventSynchronize(ctx->ev_h2d));
+    // the host sources of the previous run's copies are reused below (a
+    // stream being captured into a graph has drained its earlier work: the
+    // batch synchronizes before capturing)
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    HIP_TRY(hipStreamIsCapturing(st, &cap));
+    if (cap == hipStreamCaptureStatusNone) HIP_TRY(hipEventSynchronize(ctx->ev_h2d));
     HIP_TRY(ctx->up_ctab.put(ctx->ctab, &ct, sizeof(CorrTables), st));
     const CorrTables *dct = ctx->ctab.as<CorrTables>();
     g.nslots = build_slots(g, ctx->host_slots);
@@ -653,10 +685,13 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTabl
     }
     HIP_TRY(hipEventRecord(ctx->ev_h2d, st));
     HIP_TRY(hipEventRecord(ctx->ev[1], st));
+    // chunking of the frames (autotuned per shape, default 2^19 threads in flight)
+    const Tune tn = ctx->force_tune.apply_log2 > 0 ? ctx->force_tune : tune_lookup(ns, g.T);
+    ctx->used_tune = tn;
     // window means
-    // chunk length: about 2^19 threads in flight, 256..4096 frames each
+    // chunk length: about 2^apply_log2 threads in flight, 256..4096 frames each
     int plen = 4096;
-    while (plen > 256 && (int64_t)ns * ((g.T + plen - 1) / plen) < (1 << 19)) plen >>= 1;
+    while (plen > 256 && (int64_t)ns * ((g.T + plen - 1) / plen) < ((int64_t)1 << tn.apply_log2)) plen >>= 1;
     const int npc = (g.T + plen - 1) / plen;
     HIP_TRY(ctx->ctot.ensure(sizeof(double) * (size_t)npc * ns));
     HIP_TRY(ctx->rec.ensure(sizeof(double) * (size_t)ns * ct.npos));
@@ -666,7 +701,7 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTabl
     launch_prefix_combine(dct, ns, ctx->ctot.as<double>(), ctx->rec.as<double>(), npc, plen, ctx->F.as<double>(), st);
     HIP_TRY(hipEventRecord(ctx->ev[2], st));
     // blocking
-    const BlockingPlan bp = plan_blocking(ct, g.npix);
+    const BlockingPlan bp = plan_blocking(ct, g.npix, tn.block_log2);
     HIP_TRY(ctx->bpart.ensure(sizeof(double) * (size_t)bp.nsplit * np * 8 * 2));
     HIP_TRY(ctx->csum.ensure(sizeof(double) * (size_t)std::max(1, bp.nchunk_prod) * np));
     HIP_TRY(ctx->ind.ensure(sizeof(int32_t) * (size_t)np));
@@ -747,7 +782,7 @@ int check_geom(const imfcs_config *c, int ow, int oh) {
 
 // The LM stage over the correlations of the last run held in ctx (corav,
 // bvar, or the GLS-whitened data), with cfg's fit models and settings.
-static int run_fit(imfcs_ctx *ctx, const imfcs_config *c) {
+static int run_fit(imfcs_ctx *ctx, const imfcs_config *c, std::vector<LMGroup> *collect = nullptr) {
     {
         hipStream_t st = ctx->stream;
         const int np = ctx->npairs, npts = ctx->chanum - 1;
@@ -818,6 +853,17 @@ static int run_fit(imfcs_ctx *ctx, const imfcs_config *c) {
         if (ctx->did_gls)
             for (int i = 0; i < nparts; i++)
                 if (mask[parts[i].row] != 0x13u) return fail(IMFCS_ERR_ARG, "the GLS fit supports the free set N, D, G_inf");
+        // a batch collects the fits of its pipelines for one persistent launch
+        if (collect && !ctx->did_gls && !seeded && (n_acf == 0 || n_ccf == 0 || mask[0] == mask[1]) &&
+            lm_queue_eligible(npts, mask[n_acf ? 0 : 1], c->fit_chi2_fp64 ? 1 : 0)) {
+            const int row = n_acf ? 0 : 1;
+            collect->push_back(LMGroup{np, npts, ctx->fdata.as<float>(), ctx->fw.as<float>(), ctx->fx.as<float>(),
+                                       ctx->finit.as<float>(), ctx->fpfit.as<int>() + (size_t)row * 20,
+                                       c->fit_tolerance, c->fit_max_iterations, ctx->fout.as<float>(),
+                                       ctx->fstate.as<int>(), ctx->fchi2.as<float>(), ctx->fiter.as<int>(), 1,
+                                       n_acf});
+            nparts = 0;
+        }
         // learned starting rows: a seeded attempt of at most 15 % of the
         // iteration limit (a fit converging from a good start takes a few
         // tens; one still running by then is wandering along a flat valley),
@@ -830,7 +876,7 @@ static int run_fit(imfcs_ctx *ctx, const imfcs_config *c) {
         const float *fd = ctx->fdata.as<float>(), *fw = ctx->fw.as<float>(), *fx = ctx->fx.as<float>();
         const bool all_seeded = seeded && (n_acf == 0 || (ctx->seeder[0] && mlp_points(ctx->seeder[0]) == npts)) &&
                                 (n_ccf == 0 || (ctx->seeder[1] && mlp_points(ctx->seeder[1]) == npts));
-        if (all_seeded && (n_acf == 0 || n_ccf == 0 || mask[0] == mask[1])) {
+        if (nparts > 0 && all_seeded && (n_acf == 0 || n_ccf == 0 || mask[0] == mask[1])) {
             // every fit seeded: one LM launch over ACF and CCF fits, one fallback launch
             HIP_TRY(ctx->fseed.ensure(sizeof(float) * 20 * (size_t)np));
             float *rows = ctx->fseed.as<float>();
@@ -890,7 +936,8 @@ static int run_fit(imfcs_ctx *ctx, const imfcs_config *c) {
     return 0;
 }
 
-static int run_config(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_config *c, const int32_t *psums) {
+static int run_config(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_config *c, const int32_t *psums,
+                      std::vector<LMGroup> *collect = nullptr) {
     if (!ctx || !d_stack || !c) return fail(IMFCS_ERR_ARG, "null argument");
     if (!c->do_acf && !c->do_ccf) return fail(IMFCS_ERR_ARG, "neither ACF nor CCF requested");
     HIP_TRY(hipSetDevice(ctx->device));
@@ -990,7 +1037,7 @@ static int run_config(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_confi
     ctx->has_last = 1;
     ctx->did_fit = c->do_fit;
     if (c->do_fit) {
-        rc = run_fit(ctx, c);
+        rc = run_fit(ctx, c, collect);
         if (rc) return rc;
     }
     HIP_TRY(hipEventRecord(ctx->ev[5], st));
@@ -1235,6 +1282,269 @@ extern "C" int imfcs_run_noalloc(imfcs_ctx *ctx, const uint16_t *d_stack, const 
     ctx->evk_valid = 0;
     ctx->last_noalloc = 1;
     return rc;
+}
+
+
+// ------------------------------------------------------------------
+// Batches of stacks: one hipGraph of every stack's pipeline, one persistent
+// fit launch over all of them, autotuned frame chunking
+// ------------------------------------------------------------------
+struct imfcs_batch {
+    int device = 0, flags = 0;
+    hipStream_t st = nullptr;
+    std::vector<imfcs_ctx *> ctx;
+    std::vector<hipEvent_t> ev_join;
+    hipEvent_t ev_fork = nullptr, ev_fit = nullptr;
+    DevBuf groups, prefix, counter;
+    CachedUpload up_groups, up_prefix;
+    std::vector<LMGroup> host_groups;
+    std::vector<int64_t> host_prefix;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    size_t graph_nodes = 0;
+    std::vector<imfcs_config> last_cfgs;
+    std::vector<const uint16_t *> last_stacks;
+    int lm_blocks = 0;
+    double autotune_ms = 0.0;
+    int tuned_shapes = 0;
+};
+
+namespace {
+
+void batch_drop_graph(imfcs_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->graph_nodes = 0;
+}
+
+// Every stack's pipeline on its own context stream, forked from and joined
+// back into the batch stream, then the fits of all of them in one launch.
+int batch_enqueue(imfcs_batch *b, const uint16_t *const *stacks, const imfcs_config *cfgs) {
+    const int n = (int)b->ctx.size();
+    HIP_TRY(hipEventRecord(b->ev_fork, b->st));
+    b->host_groups.clear();
+    std::vector<int> fitted_alone;
+    for (int i = 0; i < n; i++) {
+        imfcs_ctx *c = b->ctx[i];
+        HIP_TRY(hipStreamWaitEvent(c->stream, b->ev_fork, 0));
+        c->last_noalloc = 0;
+        const int rc = run_config(c, stacks[i], &cfgs[i], nullptr, &b->host_groups);
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(b->ev_join[i], c->stream));
+        HIP_TRY(hipStreamWaitEvent(b->st, b->ev_join[i], 0));
+    }
+    const int ng = (int)b->host_groups.size();
+    if (ng > 0) {
+        b->host_prefix.assign(ng + 1, 0);
+        for (int g = 0; g < ng; g++) b->host_prefix[g + 1] = b->host_prefix[g] + b->host_groups[g].nfits;
+        HIP_TRY(b->up_groups.put(b->groups, b->host_groups.data(), sizeof(LMGroup) * ng, b->st));
+        HIP_TRY(b->up_prefix.put(b->prefix, b->host_prefix.data(), sizeof(int64_t) * (ng + 1), b->st));
+        HIP_TRY(b->counter.ensure(sizeof(unsigned long long)));
+        launch_lm_queue(b->groups.as<LMGroup>(), b->prefix.as<int64_t>(), ng, b->host_prefix[ng],
+                        b->counter.as<unsigned long long>(), b->lm_blocks, b->st);
+        HIP_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
+// Later work on each context (results, further runs) follows the batch.
+int batch_release_contexts(imfcs_batch *b) {
+    HIP_TRY(hipEventRecord(b->ev_fit, b->st));
+    for (imfcs_ctx *c : b->ctx) HIP_TRY(hipStreamWaitEvent(c->stream, b->ev_fit, 0));
+    return 0;
+}
+
+// Autotuner: for every stack whose series shape (slots, frames) has no
+// entry yet, the bleach-correction/prefix pass and the blocking pass are
+// timed (HIP events around each stage) with 2^17 .. 2^21 threads in flight,
+// one pass after the other, and the fastest choices are recorded.
+int batch_autotune(imfcs_batch *b, const uint16_t *const *stacks, const imfcs_config *cfgs) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (size_t i = 0; i < b->ctx.size(); i++) {
+        imfcs_ctx *c = b->ctx[i];
+        imfcs_config cf = cfgs[i];
+        cf.do_fit = 0;
+        c->force_tune = Tune{0, 0};
+        int rc = run_config(c, stacks[i], &cf, nullptr);   // warm-up: allocations, and the slot count
+        if (rc) return rc;
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        const int T = cf.lastframe - cf.firstframe + 1;
+        const std::pair<int, int> key{c->nslots, T};
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            if (g_tune.count(key)) continue;
+        }
+        Tune best{19, 19};
+        float best_ms[2] = {1e30f, 1e30f};
+        for (int pass = 0; pass < 2; pass++)
+            for (int e = 17; e <= 21; e++) {
+                Tune t = best;
+                (pass == 0 ? t.apply_log2 : t.block_log2) = e;
+                c->force_tune = t;
+                rc = run_config(c, stacks[i], &cf, nullptr);
+                if (rc) {
+                    c->force_tune = Tune{0, 0};
+                    return rc;
+                }
+                float ms[5];
+                HIP_TRY(hipStreamSynchronize(c->stream));
+                for (int k = 0; k < 5; k++) HIP_TRY(hipEventElapsedTime(&ms[k], c->ev[k], c->ev[k + 1]));
+                const float m = pass == 0 ? ms[1] : ms[2];
+                if (m < best_ms[pass]) {
+                    best_ms[pass] = m;
+                    best = t;
+                }
+            }
+        c->force_tune = Tune{0, 0};
+        tune_store(key.first, key.second, best);
+        b->tuned_shapes++;
+    }
+    b->autotune_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+bool same_batch(const imfcs_batch *b, const uint16_t *const *stacks, const imfcs_config *cfgs) {
+    if (!b->exec || b->last_cfgs.size() != b->ctx.size()) return false;
+    for (size_t i = 0; i < b->ctx.size(); i++)
+        if (stacks[i] != b->last_stacks[i] || std::memcmp(&cfgs[i], &b->last_cfgs[i], sizeof(imfcs_config)) != 0)
+            return false;
+    return true;
+}
+
+}  // namespace
+
+extern "C" imfcs_batch *imfcs_batch_create(int device, int n, int flags) {
+    if (n < 1) {
+        fail(IMFCS_ERR_ARG, "a batch needs at least one stack");
+        return nullptr;
+    }
+    imfcs_batch *b = new imfcs_batch();
+    b->device = device;
+    b->flags = flags;
+    for (int i = 0; i < n; i++) {
+        imfcs_ctx *c = imfcs_create(device);
+        if (!c) {
+            imfcs_batch_destroy(b);
+            return nullptr;
+        }
+        b->ctx.push_back(c);
+        hipEvent_t e = nullptr;
+        (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
+        b->ev_join.push_back(e);
+    }
+    if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) {
+        fail(IMFCS_ERR_DEVICE, "hipStreamCreate failed");
+        imfcs_batch_destroy(b);
+        return nullptr;
+    }
+    (void)hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming);
+    (void)hipEventCreateWithFlags(&b->ev_fit, hipEventDisableTiming);
+    b->lm_blocks = lm_queue_blocks();
+    return b;
+}
+
+extern "C" void imfcs_batch_destroy(imfcs_batch *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->st) (void)hipStreamSynchronize(b->st);
+    batch_drop_graph(b);
+    for (imfcs_ctx *c : b->ctx) imfcs_destroy(c);
+    for (hipEvent_t e : b->ev_join)
+        if (e) (void)hipEventDestroy(e);
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+    if (b->ev_fit) (void)hipEventDestroy(b->ev_fit);
+    b->groups.release();
+    b->prefix.release();
+    b->counter.release();
+    if (b->st) (void)hipStreamDestroy(b->st);
+    delete b;
+}
+
+extern "C" int imfcs_batch_run(imfcs_batch *b, const uint16_t *const *stacks, const imfcs_config *cfgs) {
+    if (!b || !stacks || !cfgs) return fail(IMFCS_ERR_ARG, "null argument");
+    HIP_TRY(hipSetDevice(b->device));
+    for (size_t i = 0; i < b->ctx.size(); i++) {
+        if (!stacks[i]) return fail(IMFCS_ERR_ARG, "null stack " + std::to_string(i));
+        if (b->ctx[i]->seeder[0] || b->ctx[i]->seeder[1])
+            return fail(IMFCS_ERR_ARG, "batched pipelines are not seeded");
+    }
+    if (same_batch(b, stacks, cfgs)) {
+        HIP_TRY(hipGraphLaunch(b->exec, b->st));
+        return batch_release_contexts(b);
+    }
+    batch_drop_graph(b);
+    if (b->flags & IMFCS_BATCH_AUTOTUNE) {
+        const int rc = batch_autotune(b, stacks, cfgs);
+        if (rc) return rc;
+    }
+    // first run of these stacks: eagerly (allocations, parameter uploads)
+    int rc = batch_enqueue(b, stacks, cfgs);
+    if (rc) return rc;
+    rc = batch_release_contexts(b);
+    if (rc) return rc;
+    if (b->flags & IMFCS_BATCH_GRAPH) {
+        // the same enqueue captured into a graph for the next runs: buffers
+        // and uploads are in place, so only kernels, event records and
+        // stream waits are recorded
+        HIP_TRY(hipStreamSynchronize(b->st));
+        for (imfcs_ctx *c : b->ctx) HIP_TRY(hipStreamSynchronize(c->stream));
+        HIP_TRY(hipStreamBeginCapture(b->st, hipStreamCaptureModeRelaxed));
+        rc = batch_enqueue(b, stacks, cfgs);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(b->st, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (ec != hipSuccess) return fail(IMFCS_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(ec));
+        b->graph = g;
+        HIP_TRY(hipGraphGetNodes(g, nullptr, &b->graph_nodes));
+        HIP_TRY(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    }
+    b->last_cfgs.assign(cfgs, cfgs + b->ctx.size());
+    b->last_stacks.assign(stacks, stacks + b->ctx.size());
+    return 0;
+}
+
+extern "C" int imfcs_batch_synchronize(imfcs_batch *b) {
+    if (!b) return fail(IMFCS_ERR_ARG, "null batch");
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipStreamSynchronize(b->st));
+    for (imfcs_ctx *c : b->ctx) HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" imfcs_ctx *imfcs_batch_context(imfcs_batch *b, int i) {
+    if (!b || i < 0 || i >= (int)b->ctx.size()) {
+        fail(IMFCS_ERR_ARG, "batch context index out of range");
+        return nullptr;
+    }
+    return b->ctx[i];
+}
+
+extern "C" int imfcs_batch_info(imfcs_batch *b, int *graph_nodes, int *fit_groups, int *tuned_shapes,
+                                double *autotune_ms) {
+    if (!b) return fail(IMFCS_ERR_ARG, "null batch");
+    if (graph_nodes) *graph_nodes = (int)b->graph_nodes;
+    if (fit_groups) *fit_groups = (int)b->host_groups.size();
+    if (tuned_shapes) *tuned_shapes = b->tuned_shapes;
+    if (autotune_ms) *autotune_ms = b->autotune_ms;
+    return 0;
+}
+
+extern "C" int imfcs_get_tuning(int nslots, int frames, int *apply_log2, int *block_log2) {
+    const Tune t = tune_lookup(nslots, frames);
+    if (apply_log2) *apply_log2 = t.apply_log2;
+    if (block_log2) *block_log2 = t.block_log2;
+    return 0;
+}
+
+extern "C" int imfcs_get_slot_count(imfcs_ctx *ctx, int *nslots) {
+    if (!ctx || !nslots) return fail(IMFCS_ERR_ARG, "null argument");
+    *nslots = ctx->nslots;
+    return 0;
 }
 
 // ------------------------------------------------------------------

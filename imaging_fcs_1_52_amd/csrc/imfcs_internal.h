@@ -179,6 +179,27 @@ MlpNet *mlp_of(imfcs_mlp *m);
 int mlp_seed_init(MlpNet *net, const float *data, int n, const float *tmpl, float *init, hipStream_t st);
 int mlp_points(const MlpNet *net);
 
+// One group of FCS fits (N, D, G_inf free, FP64 chi-square, 65..128
+// points) of one pipeline run, for the persistent fit launch over a batch.
+struct LMGroup {
+    int nfits, npts;
+    const float *data, *weights, *x, *init;
+    const int *pfit;
+    float tol;
+    int maxit;
+    float *out_params;
+    int *states;
+    float *chi2;
+    int *iters;
+    int init_two_rows, init_split;
+};
+bool lm_queue_eligible(int npts, unsigned free_mask, int chi2_fp64);
+// workgroups of the persistent fit kernel that are resident at once
+int lm_queue_blocks();
+// groups, prefix (fit offsets, ngroups + 1 entries) and counter on the device
+void launch_lm_queue(const LMGroup *groups, const int64_t *prefix, int ngroups, int64_t total,
+                     unsigned long long *counter, int blocks, hipStream_t st);
+
 // Launchers (imfcs_kernels.hip)
 void launch_trace_moments(const SeriesGeom &g, int nopit, int ave, double tscale, double frametime,
                           int order, int integer_trace, double *part, int nchunk, hipStream_t st,
@@ -215,7 +236,9 @@ struct BlockingPlan {
     int chunks_per_split;  // 256-product chunks per thread
     int nchunk_prod;       // chunks of the whole series (csum rows)
 };
-BlockingPlan plan_blocking(const CorrTables &ct, int npix);
+// threads_log2: about 2^threads_log2 threads in flight (the split of the
+// frames into per-thread chunks; tuned per shape by the batch autotuner)
+BlockingPlan plan_blocking(const CorrTables &ct, int npix, int threads_log2 = 19);
 void launch_blocking(const SeriesGeom &g, const BleachDesc &bd, const CorrTables *dct, const PairDesc &pd,
                      const double *F, const float *X, const BlockingPlan &bp, double *bpart, double *csum,
                      hipStream_t st);

@@ -23,6 +23,7 @@
 #include "imfcs_rng.h"
 
 #include <math.h>
+#include <algorithm>
 #include <type_traits>
 
 namespace imfcs {
@@ -1616,17 +1617,13 @@ __global__ void __launch_bounds__(64) k_lm_exp64(LMArgs a, double *out64) {
 // per-wave sums meet in LDS (double-buffered, one barrier per evaluation);
 // both waves then take identical LM steps from identical sums.
 template <int NF>
-__global__ void __launch_bounds__(128) k_lm_fcs_2w(LMArgs a) {
+__device__ void lm_fcs_2w_fit(const LMArgs &a, int fit) {
     constexpr int NP = 20, NH = NF * (NF + 1) / 2, NS = NF + NH;
     // per buffer and wave: the FP64 chi-square and the FP32 gradient and
     // normal-matrix sums (FP32 as in Gpufit; only the chi-square, which
     // decides step acceptance and convergence, needs FP64)
     __shared__ double red_c2[2][2];
     __shared__ float red[2][2][NS];
-    // last fit first: the CCF fits (the second half), whose few longest
-    // fits set the kernel's duration, are dispatched before the ACF fits
-    const int fit = a.nfits - 1 - (int)blockIdx.x;
-    if (fit < 0) return;
     int prev_iters;
     if (lm_skip(a, fit, prev_iters)) return;
     const int tid = threadIdx.x, wv = tid >> 6;
@@ -1690,6 +1687,65 @@ __global__ void __launch_bounds__(128) k_lm_fcs_2w(LMArgs a) {
         a.chi2[fit] = (float)chi2;
         a.iters[fit] = iters + prev_iters;
     }
+}
+
+template <int NF>
+__global__ void __launch_bounds__(128) k_lm_fcs_2w(LMArgs a) {
+    // last fit first: the CCF fits (the second half), whose few longest
+    // fits set the kernel's duration, are dispatched before the ACF fits
+    const int fit = a.nfits - 1 - (int)blockIdx.x;
+    if (fit < 0) return;
+    lm_fcs_2w_fit<NF>(a, fit);
+}
+
+// One persistent launch over the fits of many pipelines (a batch of
+// stacks): workgroups take fits from a global counter until every fit of
+// every group is taken, so the long fits of one stack overlap the others'
+// work instead of each stack's launch waiting for its slowest fit.  Within a
+// group the last fit is dispatched first, as in k_lm_fcs_2w.
+__global__ void __launch_bounds__(128) k_lm_queue(const LMGroup *groups, const int64_t *prefix, int ngroups,
+                                                  int64_t total, unsigned long long *counter) {
+    __shared__ long long s_idx;
+    for (;;) {
+        __syncthreads();   // the previous fit's reads of shared memory are done
+        if (threadIdx.x == 0) s_idx = (long long)atomicAdd(counter, 1ull);
+        __syncthreads();
+        const int64_t idx = s_idx;
+        if (idx >= total) break;
+        int lo = 0, hi = ngroups - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (prefix[mid] <= idx) lo = mid;
+            else hi = mid - 1;
+        }
+        const LMGroup &G = groups[lo];
+        LMArgs a{G.nfits, G.npts, 20, kModelFCS, 0, G.data, G.weights, G.x, G.init, G.pfit, LMModelGeom{}, G.tol,
+                 G.maxit, 1, G.out_params, G.states, G.chi2, G.iters, G.init_two_rows, G.init_split, 0};
+        lm_fcs_2w_fit<3>(a, G.nfits - 1 - (int)(idx - prefix[lo]));
+    }
+}
+
+__global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
+
+bool lm_queue_eligible(int npts, unsigned free_mask, int chi2_fp64) {
+    return free_mask == 0x13u && chi2_fp64 && npts > 64 && npts <= 128;
+}
+
+int lm_queue_blocks() {
+    int dev = 0, ncu = 0, per = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)k_lm_queue, 128, 0);
+    return std::max(1, ncu) * std::max(1, per);
+}
+
+void launch_lm_queue(const LMGroup *groups, const int64_t *prefix, int ngroups, int64_t total,
+                     unsigned long long *counter, int blocks, hipStream_t st) {
+    if (total <= 0 || ngroups <= 0) return;
+    // the counter is cleared by a kernel (no memset node in captured graphs)
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, counter);
+    const int64_t nb = std::min<int64_t>(total, std::max(1, blocks));
+    hipLaunchKernelGGL(k_lm_queue, dim3((unsigned)nb), dim3(128), 0, st, groups, prefix, ngroups, total, counter);
 }
 
 // GLS fit of the FCS model with N, D, G_inf free (GLSFit, IFCS:10622-10789):
@@ -2055,12 +2111,12 @@ void launch_prefix_combine(const CorrTables *dct, int nslots, const double *ctot
                        chunk_len, F);
 }
 
-BlockingPlan plan_blocking(const CorrTables &ct, int npix) {
+BlockingPlan plan_blocking(const CorrTables &ct, int npix, int threads_log2) {
     BlockingPlan bp;
     const int n0 = ct.T - 1;
     bp.nchunk_prod = (n0 + (1 << kBlkJ) - 1) >> kBlkJ;
-    // about 2^19 threads in flight, whole chunks per split
-    int want = std::max(1, (1 << 19) / std::max(1, npix));
+    // about 2^threads_log2 threads in flight, whole chunks per split
+    int want = std::max(1, (1 << threads_log2) / std::max(1, npix));
     bp.nsplit = std::min(bp.nchunk_prod, want);
     bp.chunks_per_split = (bp.nchunk_prod + bp.nsplit - 1) / bp.nsplit;
     bp.nsplit = (bp.nchunk_prod + bp.chunks_per_split - 1) / bp.chunks_per_split;

@@ -85,6 +85,14 @@ SIGNATURES = {
     "imfcs_synchronize": (ctypes.c_int, [_VP]),
     "imfcs_run": (ctypes.c_int, [_VP, _VP, _P(PipelineConfig)]),
     "imfcs_run_noalloc": (ctypes.c_int, [_VP, _VP, _P(PipelineConfig), _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "imfcs_batch_create": (_VP, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "imfcs_batch_destroy": (None, [_VP]),
+    "imfcs_batch_run": (ctypes.c_int, [_VP, _P(_VP), _P(PipelineConfig)]),
+    "imfcs_batch_synchronize": (ctypes.c_int, [_VP]),
+    "imfcs_batch_context": (_VP, [_VP, ctypes.c_int]),
+    "imfcs_batch_info": (ctypes.c_int, [_VP, _P(ctypes.c_int), _P(ctypes.c_int), _P(ctypes.c_int), _P(_f64)]),
+    "imfcs_get_tuning": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _P(ctypes.c_int), _P(ctypes.c_int)]),
+    "imfcs_get_slot_count": (ctypes.c_int, [_VP, _P(ctypes.c_int)]),
     "imfcs_run_multi": (ctypes.c_int, [_P(_VP), ctypes.c_int, _VP, _P(PipelineConfig)]),
     "imfcs_refit": (ctypes.c_int, [_VP, _P(PipelineConfig)]),
     "imfcs_get_results": (ctypes.c_int, [_VP, _P(_f64), _P(_f64), _P(_i32), _P(_i32), _P(_f64), _P(_f64),

@@ -279,6 +279,39 @@ int imfcs_get_results(imfcs_ctx *ctx, double *acf, double *var, int32_t *ind,
  * [0] trace+polyfit, [1] means, [2] blocking, [3] correlation, [4] fit. */
 int imfcs_get_stage_ms(imfcs_ctx *ctx, float *stage_ms5);
 
+/* Batches of independent stacks (no single reference call corresponds: the
+ * plugin runs the whole pipeline per stack and binning,
+ * Imaging_FCS_1_52.java:11146-11166, each stage a host loop of launches,
+ * agpufitjni.cu:5714-5741).  A batch holds one context per stack.  Its
+ * first run of a set of stacks and configurations enqueues every stack's
+ * pipeline on its own stream and fits all their pixels in ONE persistent
+ * launch (a work queue over the fits of every stack; used when every
+ * configuration fits N, D, G_inf with the FP64 chi-square, otherwise each
+ * stack fits on its own); with IMFCS_BATCH_GRAPH the same work is captured
+ * into a hipGraph that later runs of the same stacks and configurations
+ * replay; with IMFCS_BATCH_AUTOTUNE the frame chunking of the bleach/prefix
+ * and blocking passes is first timed per stack shape and the fastest choice
+ * kept for every later run of that shape (batched or not).  Results per
+ * stack: imfcs_get_results(imfcs_batch_context(b, i), ...) after
+ * imfcs_batch_synchronize; identical to imfcs_run of each stack with the
+ * same tuning. */
+typedef struct imfcs_batch imfcs_batch;
+#define IMFCS_BATCH_GRAPH 1
+#define IMFCS_BATCH_AUTOTUNE 2
+imfcs_batch *imfcs_batch_create(int device, int n, int flags);
+void imfcs_batch_destroy(imfcs_batch *b);
+int imfcs_batch_run(imfcs_batch *b, const uint16_t *const *d_stacks, const imfcs_config *cfgs);
+int imfcs_batch_synchronize(imfcs_batch *b);
+imfcs_ctx *imfcs_batch_context(imfcs_batch *b, int i);
+/* graph nodes of the captured run, fit groups of the persistent launch,
+ * shapes tuned and time spent tuning. */
+int imfcs_batch_info(imfcs_batch *b, int *graph_nodes, int *fit_groups, int *tuned_shapes, double *autotune_ms);
+/* The frame chunking recorded for a series shape (2^apply_log2 and
+ * 2^block_log2 threads in flight; 19 / 19 untuned), and a context's slot
+ * count (the shape key of its last run). */
+int imfcs_get_tuning(int nslots, int frames, int *apply_log2, int *block_log2);
+int imfcs_get_slot_count(imfcs_ctx *ctx, int *nslots);
+
 /* ------------------------------------------------------------------ */
 /* 3. Learned initial guesses for the LM fit                           */
 /* ------------------------------------------------------------------ */
