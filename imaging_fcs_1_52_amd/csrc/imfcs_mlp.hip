@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstring>
 #include <random>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -158,13 +159,15 @@ __global__ void k_mlp_loss(const float *Y, const float *T, int B, const float *w
     if ((threadIdx.x & 63) == 0) atomicAdd(loss, l / (float)B);
 }
 
-// db[n] += sum_b dZ[b][n] (BF16 rows of length N)
+// db[n] += sum_b dZ[b][n] (BF16 rows of length N): blockIdx.y takes rows
+// [128 y, 128 y + 128), one thread per column, one atomic add per thread
 __global__ void k_colsum(const uint16_t *dZ, int B, int N, float *db) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
+    const int b0 = blockIdx.y * 128, b1 = min(B, b0 + 128);
     float s = 0.f;
-    for (int b = 0; b < B; b++) s += bf2f(dZ[(int64_t)b * N + n]);
-    db[n] += s;
+    for (int b = b0; b < b1; b++) s += bf2f(dZ[(int64_t)b * N + n]);
+    atomicAdd(&db[n], s);
 }
 
 __global__ void k_fill_f32(float *p, int64_t n, float v) {
@@ -211,13 +214,15 @@ __device__ __forceinline__ void decode(const float *y, const float *mu, const fl
 
 
 
-// The whole forward pass for 64 curves per workgroup (4 wavefronts), for
+// The whole forward pass for kFusedRows curves per workgroup (4 wavefronts), for
 // seeding fits and predictions: inputs normalised in registers, the three
 // layers on v_mfma_f32_16x16x16_bf16 with the activations kept in LDS
 // (BF16, as the training path stores them) and the weights read as [N][K]
 // BF16 copies straight into the MFMA B operand (one 8-byte load per lane
 // and step); the outputs decoded into LM starting rows (init, from the
 // template tmpl) and/or physical values (out5).
+constexpr int kFusedRows = 32;   // curves per workgroup (LDS ~42 KB: 3 workgroups per CU)
+
 struct FusedArgs {
     const float *data;
     int n, npts;
@@ -230,15 +235,15 @@ struct FusedArgs {
 };
 
 __global__ void __launch_bounds__(256) k_mlp_fused(FusedArgs a) {
-    __shared__ __attribute__((aligned(16))) uint16_t Xs[64][kIn + 8];
-    __shared__ __attribute__((aligned(16))) uint16_t H1s[64][kHid + 8];
-    __shared__ __attribute__((aligned(16))) uint16_t H2s[64][kHid + 8];
-    __shared__ float Ys[64][kOut + 1];
-    __shared__ float Ss[64];
+    __shared__ __attribute__((aligned(16))) uint16_t Xs[kFusedRows][kIn + 8];
+    __shared__ __attribute__((aligned(16))) uint16_t H1s[kFusedRows][kHid + 8];
+    __shared__ __attribute__((aligned(16))) uint16_t H2s[kFusedRows][kHid + 8];
+    __shared__ float Ys[kFusedRows][kOut + 1];
+    __shared__ float Ss[kFusedRows];
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-    const int r0 = blockIdx.x * 64;
+    const int r0 = blockIdx.x * kFusedRows;
     // inputs: wave w normalises rows w, w + 4, ...; lane l holds channels l and l + 64
-    for (int rr = w; rr < 64; rr += 4) {
+    for (int rr = w; rr < kFusedRows; rr += 4) {
         const int row = r0 + rr;
         float v0 = 0.f, v1 = 0.f;
         if (row < a.n) {
@@ -260,28 +265,33 @@ __global__ void __launch_bounds__(256) k_mlp_fused(FusedArgs a) {
     }
     __syncthreads();
     const int lr = lane & 15, lq = 4 * (lane >> 4);
-    // hidden layers: wave w computes columns [64 w, 64 w + 64) of all 64 rows
-    auto hidden = [&](const uint16_t(*In)[kHid + 8], const uint16_t *WT, int K, const float *bias,
+    // hidden layers: wave w computes columns [64 w, 64 w + 64) of all rows;
+    // K is a compile-time constant so the unrolled steps issue their weight
+    // loads ahead of the MFMAs instead of one L2 round trip per step
+    auto hidden = [&](auto kconst, const uint16_t(*In)[kHid + 8], const uint16_t *WT, const float *bias,
                       uint16_t(*Out)[kHid + 8], bool from_x) {
-        f32x4 acc[4][4];
+        constexpr int K = decltype(kconst)::value;
+        constexpr int RT = kFusedRows / 16;   // row tiles
+        f32x4 acc[RT][4];
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < RT; i++)
 #pragma unroll
             for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-        for (int k0 = 0; k0 < K; k0 += 16) {
-            bf16x4 av[4], bv[4];
 #pragma unroll
-            for (int i = 0; i < 4; i++)
-                av[i] = from_x ? *(const bf16x4 *)&Xs[i * 16 + lr][k0 + lq] : *(const bf16x4 *)&In[i * 16 + lr][k0 + lq];
+        for (int k0 = 0; k0 < K; k0 += 16) {
+            bf16x4 av[RT], bv[4];
 #pragma unroll
             for (int j = 0; j < 4; j++) bv[j] = *(const bf16x4 *)&WT[(int64_t)(64 * w + j * 16 + lr) * K + k0 + lq];
 #pragma unroll
-            for (int i = 0; i < 4; i++)
+            for (int i = 0; i < RT; i++)
+                av[i] = from_x ? *(const bf16x4 *)&Xs[i * 16 + lr][k0 + lq] : *(const bf16x4 *)&In[i * 16 + lr][k0 + lq];
+#pragma unroll
+            for (int i = 0; i < RT; i++)
 #pragma unroll
                 for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[i], bv[j], acc[i][j], 0, 0, 0);
         }
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < RT; i++)
 #pragma unroll
             for (int j = 0; j < 4; j++)
 #pragma unroll
@@ -290,13 +300,14 @@ __global__ void __launch_bounds__(256) k_mlp_fused(FusedArgs a) {
                     Out[i * 16 + lq + r][col] = f2bf(gelu(acc[i][j][r] + bias[col]));
                 }
     };
-    hidden(H2s, a.W1T, kIn, a.b1, H1s, true);
+    hidden(std::integral_constant<int, kIn>{}, H2s, a.W1T, a.b1, H1s, true);
     __syncthreads();
-    hidden(H1s, a.W2T, kHid, a.b2, H2s, false);
+    hidden(std::integral_constant<int, kHid>{}, H1s, a.W2T, a.b2, H2s, false);
     __syncthreads();
-    // output layer: wave w computes rows [16 w, 16 w + 16) x 16 outputs
-    {
+    // output layer: wave w < kFusedRows / 16 computes rows [16 w, 16 w + 16) x 16 outputs
+    if (w < kFusedRows / 16) {
         f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
         for (int k0 = 0; k0 < kHid; k0 += 16) {
             const bf16x4 av = *(const bf16x4 *)&H2s[16 * w + lr][k0 + lq];
             const bf16x4 bv = *(const bf16x4 *)&a.W3T[(int64_t)lr * kHid + k0 + lq];
@@ -306,7 +317,7 @@ __global__ void __launch_bounds__(256) k_mlp_fused(FusedArgs a) {
         for (int r = 0; r < 4; r++) Ys[16 * w + lq + r][lr] = acc[r] + a.b3[lr];
     }
     __syncthreads();
-    if (tid < 64) {
+    if (tid < kFusedRows) {
         const int row = r0 + tid;
         if (row >= a.n) return;
         const float s = Ss[tid];
@@ -432,19 +443,19 @@ struct MlpNet {
         // layer 3: dW3 = H2^T dY, db3, dH2 = dY W3^T -> dZ2 = dH2 * GELU'(Z2)
         gemm(H2, 1, kHid, dY, kOut, 1, kHid, kOut, rows, kEpiAtomic, nullptr, G + oW3, kOut, nullptr, 0, nullptr,
              0, ks);
-        hipLaunchKernelGGL(k_colsum, dim3(1), dim3(64), 0, st, dY, rows, kOut, G + ob3);
+        hipLaunchKernelGGL(k_colsum, dim3(1, cdivu(rows, 128)), dim3(64), 0, st, dY, rows, kOut, G + ob3);
         gemm(dY, kOut, 1, Pb + bW3, 1, kOut, rows, kHid, kOut, kEpiGeluGrad, nullptr, nullptr, 0, dZ2, kHid, Z2,
              kHid);
         // layer 2
         gemm(H1, 1, kHid, dZ2, kHid, 1, kHid, kHid, rows, kEpiAtomic, nullptr, G + oW2, kHid, nullptr, 0, nullptr,
              0, ks);
-        hipLaunchKernelGGL(k_colsum, dim3(kHid / 64), dim3(64), 0, st, dZ2, rows, kHid, G + ob2);
+        hipLaunchKernelGGL(k_colsum, dim3(kHid / 64, cdivu(rows, 128)), dim3(64), 0, st, dZ2, rows, kHid, G + ob2);
         gemm(dZ2, kHid, 1, Pb + bW2, 1, kHid, rows, kHid, kHid, kEpiGeluGrad, nullptr, nullptr, 0, dZ1, kHid, Z1,
              kHid);
         // layer 1
         gemm(X, 1, kIn, dZ1, kHid, 1, kIn, kHid, rows, kEpiAtomic, nullptr, G + oW1, kHid, nullptr, 0, nullptr, 0,
              ks);
-        hipLaunchKernelGGL(k_colsum, dim3(kHid / 64), dim3(64), 0, st, dZ1, rows, kHid, G + ob1);
+        hipLaunchKernelGGL(k_colsum, dim3(kHid / 64, cdivu(rows, 128)), dim3(64), 0, st, dZ1, rows, kHid, G + ob1);
     }
 
     void curves(int rows) {
@@ -499,7 +510,7 @@ void MlpNet::fused(const float *data, int n, const float *tmpl, float *init, flo
     if (n <= 0) return;
     FusedArgs f{data, n, npts, PbT + bW1, PbT + bW2, PbT + bW3, P + ob1, P + ob2, P + ob3, d_mu(), d_sd(), ncomp,
                 tmpl, init, out5};
-    hipLaunchKernelGGL(k_mlp_fused, dim3(cdivu(n, 64)), dim3(256), 0, s, f);
+    hipLaunchKernelGGL(k_mlp_fused, dim3(cdivu(n, kFusedRows)), dim3(256), 0, s, f);
 }
 
 int mlp_seed_init(MlpNet *net, const float *data, int n, const float *tmpl, float *init, hipStream_t st) {
