@@ -227,7 +227,8 @@ def make_pipeline_config(W, H, T, binning=1, cf=(0, 0), background=0, poly_order
     sliding_window > 0 selects the sliding-window bleach mode with windows of
     that many frames, lin_segment > 0 the linear-segment correction with
     T // lin_segment segments, exp_bleach=True the single-exponential
-    correction (poly_order must then be 0)."""
+    correction (exp_bleach=2: the double exponential; poly_order must then
+    be 0)."""
     c = PipelineConfig()
     c.width, c.height, c.frames = W, H, T
     c.firstframe = firstframe
@@ -248,7 +249,8 @@ def make_pipeline_config(W, H, T, binning=1, cf=(0, 0), background=0, poly_order
     c.fit_tolerance = fit_tolerance
     c.fit_chi2_fp64 = int(bool(fit_chi2_fp64))
     c.gls = int(bool(gls))
-    c.bleach_mode = 1 if sliding_window else (2 if lin_segment else (3 if exp_bleach else 0))
+    c.bleach_mode = 1 if sliding_window else (2 if lin_segment else ((4 if exp_bleach == 2 else 3) if exp_bleach
+                                                                     else 0))
     c.sliding_window_length = int(sliding_window or lin_segment)
     if acf_model is not None:
         c.acf_model = acf_model
@@ -404,6 +406,8 @@ def _results(ctx, cfg):
         ncoef = (cfg.lastframe - cfg.firstframe + 1) // cfg.sliding_window_length + 1
     elif cfg.bleach_mode == 3:   # f(0), A, t1, C
         ncoef = 4
+    elif cfg.bleach_mode == 4:   # f(0), A, t1, B, t2, C
+        ncoef = 6
     else:
         ncoef = max(cfg.poly_order, 0) + 1
     coef = np.zeros((nseries, ncoef))

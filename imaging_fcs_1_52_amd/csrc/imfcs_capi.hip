@@ -609,9 +609,9 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTabl
         ctx->order = nseg;
         ctx->coef_scale.assign(nseg + 1, 1.0);
     } else if (ea.exp_n > 0) {
-        // "Single Exp" (IFCS:9682-9709): LM fit of the nopit-point intensity
-        // trace (SingleExpFit, IFCS:10845-10895), then the correction with
-        // the fitted curve
+        // "Single Exp" / "Double Exp" (IFCS:9682-9738): LM fit of the
+        // nopit-point intensity trace (SingleExpFit / DoubleExpFit,
+        // IFCS:10845-10950), then the correction with the fitted curve
         const int nexp = ea.exp_n, np = nexp == 1 ? 3 : 5;
         const int nopit = ea.nopit;
         const int nchunk = std::min(std::min(nopit, 128), std::max(40, (1 << 19) / std::max(1, ns)));
@@ -990,10 +990,10 @@ static int run_config(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_confi
     if (c->bleach_mode == IMFCS_BLEACH_LIN_SEGMENT) {
         if (c->poly_order != 0) return fail(IMFCS_ERR_ARG, "linear segments exclude the polynomial correction");
         ea.seg_swl = c->sliding_window_length;
-    } else if (c->bleach_mode == IMFCS_BLEACH_SINGLE_EXP) {
+    } else if (c->bleach_mode == IMFCS_BLEACH_SINGLE_EXP || c->bleach_mode == IMFCS_BLEACH_DOUBLE_EXP) {
         if (c->poly_order != 0) return fail(IMFCS_ERR_ARG, "exponential correction excludes the polynomial one");
         if (ea.nopit < 10) return fail(IMFCS_ERR_ARG, "too few frames for an exponential bleach fit");
-        ea.exp_n = 1;
+        ea.exp_n = c->bleach_mode == IMFCS_BLEACH_SINGLE_EXP ? 1 : 2;
     } else if (c->bleach_mode != IMFCS_BLEACH_POLYNOMIAL && c->bleach_mode != IMFCS_BLEACH_SLIDING_WINDOW) {
         return fail(IMFCS_ERR_ARG, "unknown bleach correction mode");
     }

@@ -2047,6 +2047,11 @@ static void launch_apply_t(const SeriesGeom &g, const BleachDesc &bd, const Corr
                            X);
         return;
     }
+    if (bd.mode == 3) {
+        hipLaunchKernelGGL((k_apply<BleachExp<2>, NB>), grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec,
+                           X);
+        return;
+    }
     switch (bleach_maxo(bd.order)) {
         case 0:
             hipLaunchKernelGGL((k_apply<BleachP<0>, NB>), grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec,
@@ -2206,8 +2211,11 @@ void launch_lm_exp(int nfits, int npts, const float *data, const float *x, const
     LMModelGeom g{};
     LMArgs a{nfits, npts, 11, nexp == 1 ? kModelExp1 : kModelExp2, 0, data, nullptr, x, init, pfit, g, tol, maxit, 1,
              nullptr, states, chi2, iters, 0, 0};
-    if (nfits <= 0 || nexp != 1) return;
-    hipLaunchKernelGGL(k_lm_exp64<1>, dim3(nfits), dim3(64), 0, st, a, out64);
+    if (nfits <= 0) return;
+    if (nexp == 1)
+        hipLaunchKernelGGL(k_lm_exp64<1>, dim3(nfits), dim3(64), 0, st, a, out64);
+    else
+        hipLaunchKernelGGL(k_lm_exp64<2>, dim3(nfits), dim3(64), 0, st, a, out64);
 }
 
 void launch_lm_poly(int nfits, int npts, const float *data, const float *weights, const float *x, const float *init,

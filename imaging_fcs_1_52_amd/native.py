@@ -54,7 +54,7 @@ class SimDomain(ctypes.Structure):
                 ("counts_hi", _f64)]
 
 
-BLEACH_POLYNOMIAL, BLEACH_SLIDING_WINDOW, BLEACH_LIN_SEGMENT, BLEACH_SINGLE_EXP = 0, 1, 2, 3
+BLEACH_POLYNOMIAL, BLEACH_SLIDING_WINDOW, BLEACH_LIN_SEGMENT, BLEACH_SINGLE_EXP, BLEACH_DOUBLE_EXP = 0, 1, 2, 3, 4
 
 
 # every C-ABI entry point of include/imfcs.h with its ctypes signature

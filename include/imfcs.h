@@ -174,8 +174,8 @@ typedef struct imfcs_config {
     imfcs_fit_model acf_model, ccf_model;
     int32_t bleach_mode;             /* IMFCS_BLEACH_*: POLYNOMIAL (of poly_order; 0 = no
                                         correction), SLIDING_WINDOW (IFCS:9146-9172),
-                                        LIN_SEGMENT (IFCS:9767-9805) or SINGLE_EXP
-                                        (IFCS:9682-9709) */
+                                        LIN_SEGMENT (IFCS:9767-9805), SINGLE_EXP
+                                        (IFCS:9682-9709) or DOUBLE_EXP (IFCS:9710-9738) */
     int32_t sliding_window_length;   /* frames per window (SLIDING_WINDOW), or the length
                                         that sets the segment count (LIN_SEGMENT) */
 } imfcs_config;
@@ -192,9 +192,14 @@ enum {
     IMFCS_BLEACH_LIN_SEGMENT = 2,
     /* LM fit of A e^{-t/t1} + C to the intensity trace and the correction
        with that curve (IFCS:9682-9709); coefficients returned per series:
-       f(0), A, t1, C.  ("Double Exp" is not offered: its fit ends in
-       different local optima depending on the LM's path.) */
-    IMFCS_BLEACH_SINGLE_EXP = 3
+       f(0), A, t1, C */
+    IMFCS_BLEACH_SINGLE_EXP = 3,
+    /* A e^{-t/t1} + B e^{-t/t2} + C (DoubleExpFit, IFCS:9710-9738,
+       10898-10950) from the reference's initial guesses; coefficients per
+       series: f(0), A, t1, B, t2, C.  Where the two-exponential least-squares
+       problem has several minima, the LM's path decides which one is found
+       (as in the reference, whose CPU and GPU fits differ the same way). */
+    IMFCS_BLEACH_DOUBLE_EXP = 4
 };
 
 /* Output geometry for a config: out_w x out_h pixels, chanum channels. */

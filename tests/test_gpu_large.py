@@ -58,7 +58,8 @@ def test_full_scale_crops_match_oracle(engine):
                 continue
             compared += int(fin.sum())
             amp = np.abs(o["acf"][fin][:, 1:]).max(-1, keepdims=True)
-            assert rel_err(ga[fin], o["acf"][fin], 1e-3 * amp).max() < 1e-4
+            e = rel_err(ga[fin], o["acf"][fin], 1e-4 * amp).max()
+            assert e < 1e-4, e
             same = (gi == o["ind"]) & fin
             assert same[fin].mean() >= 0.8
             assert rel_err(gv[same][:, 1:], o["var"][same][:, 1:], 1e-300).max() < 1e-4
