@@ -9,8 +9,10 @@ One step = for each binning b in (1, 2, 4): fused b x b binning, order-4
 polynomial bleach correction, ACF and CCF to the x+b neighbour with blocked
 variances, and the LM fit of N, D, G_inf of every ACF and CCF pixel.  The
 three binnings run as three pipelines on three HIP streams over the one
-device-resident stack (imfcs_run_multi).  The stack is in HBM before timing
-starts.
+device-resident stack, captured once into a hipGraph (imfcs_batch with one
+shared stack; every kernel runs in every replay) and replayed each step
+(--runner multi: imfcs_run_multi enqueued each step).  The stack is in HBM
+before timing starts.
 
 ``value``: ACF pixels fitted to state CONVERGED (finite correlation; the
 plugin keeps only those, Imaging_FCS_1_52.java:15249-15256) per second of the
@@ -140,6 +142,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=48, help="side of the oracle's pixel sample (0 = skip)")
     ap.add_argument("--train-steps", type=int, default=400, help="AdamW steps of each seeding network")
+    ap.add_argument("--runner", choices=("graph", "multi"), default="graph",
+                    help="graph: the three pipelines captured once into a hipGraph (imfcs_batch with one shared "
+                         "stack) and replayed every step; multi: imfcs_run_multi enqueued every step")
+    ap.add_argument("--autotune", action="store_true", help="graph runner: time the frame chunking first")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain device-to-device copy of the stack (known bytes) for PMC traffic scaling")
     args = ap.parse_args()
@@ -152,7 +158,15 @@ def main():
     from imaging_fcs_1_52_amd.learned import InitialGuessNet
     stack = make_stack()
     smin = int(stack.min())
-    engines = [imfcs.Engine(0) for _ in BINNINGS]
+    order = [BINNINGS.index(b) for b in ENQUEUE_ORDER]
+    batch = None
+    if args.runner == "graph":
+        # batch entry j runs binning ENQUEUE_ORDER[j]; engines[] are views of
+        # its contexts in BINNINGS order
+        batch = imfcs.Batch(len(BINNINGS), graph=True, autotune=args.autotune, own_fits=True)
+        engines = [batch.engine(ENQUEUE_ORDER.index(b)) for b in BINNINGS]
+    else:
+        engines = [imfcs.Engine(0) for _ in BINNINGS]
     t0 = time.perf_counter()
     d_stack = engines[0].upload(stack)
     h2d_s = time.perf_counter() - t0
@@ -162,10 +176,12 @@ def main():
         ow, oh, _ = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         imfcs.lib().imfcs_plan(ctypes.byref(c), ctypes.byref(ow), ctypes.byref(oh), ctypes.byref(_))
         n_acf_pixels += ow.value * oh.value
-    order = [BINNINGS.index(b) for b in ENQUEUE_ORDER]
 
     def step():
-        imfcs.Engine.run_multi([engines[i] for i in order], [cfgs[i] for i in order], d_stack)
+        if batch is not None:
+            batch.run([d_stack] * len(order), [cfgs[i] for i in order])
+        else:
+            imfcs.Engine.run_multi([engines[i] for i in order], [cfgs[i] for i in order], d_stack)
         kern = []
         for e in engines:
             e.synchronize()
@@ -292,7 +308,9 @@ def main():
             "scaling": "weak", "vs_baseline": None, "dtype": "u16/f32/f64 (bf16 seeding network)",
             "data": "synthetic (seeded 2D particle simulation)",
             "config": {"workload": "128x128x50000 uint16; binning 1,2,4; ACF+CCF(x+b); poly4 bleach; LM N,D,G",
-                       "acf_pixels_per_step": n_acf_pixels, "converged_acf_fits_per_step": n_conv_seed},
+                       "acf_pixels_per_step": n_acf_pixels, "converged_acf_fits_per_step": n_conv_seed,
+                       "runner": ("hipGraph replay of the three pipelines (imfcs_batch)" if batch is not None
+                                  else "imfcs_run_multi")},
             "value_plain": round(value_plain, 1), "ms_per_step_plain": round(dt_plain * 1e3, 3),
             "converged_acf_fits_per_step_plain": n_conv_plain,
             "all_pixels_per_s": round(n_acf_pixels / dt_seed, 1),
@@ -308,6 +326,8 @@ def main():
             n.close()
     for e in engines[::-1]:
         e.close()
+    if batch is not None:
+        batch.close()
 
 
 if __name__ == "__main__":

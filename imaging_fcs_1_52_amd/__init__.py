@@ -264,9 +264,11 @@ def make_pipeline_config(W, H, T, binning=1, cf=(0, 0), background=0, poly_order
 class Engine:
     """A device context for the native pipeline (imfcs_create / imfcs_run)."""
 
-    def __init__(self, device=0):
+    def __init__(self, device=0, _ctx=None):
         self._lib = lib()
-        self.ctx = self._lib.imfcs_create(device)
+        # _ctx: a context owned elsewhere (a batch entry), not destroyed here
+        self._owned = _ctx is None
+        self.ctx = self._lib.imfcs_create(device) if _ctx is None else _ctx
         if not self.ctx:
             raise ImFCSError(get_last_error())
         self._stack = None
@@ -278,7 +280,8 @@ class Engine:
             if self._stack:
                 self._lib.imfcs_device_free(self.ctx, self._stack)
                 self._stack = None
-            self._lib.imfcs_destroy(self.ctx)
+            if self._owned:
+                self._lib.imfcs_destroy(self.ctx)
             self.ctx = None
 
     def __del__(self):
@@ -471,15 +474,30 @@ def simulate_2d_device(engine, W, H, T, domains, pixel_um=0.24, psf_sigma_um=0.1
 class Batch:
     """A batch of independent stacks (imfcs_batch_*): one context per stack,
     every stack's pipeline on its own stream and all fits in one persistent
-    launch; with graph=True later runs of the same stacks and configurations
-    replay a captured hipGraph; autotune=True times the frame chunking per
-    stack shape first."""
+    launch (own_fits=True: each pipeline fits its own pixels at the end of
+    its stream, seeded fits allowed); with graph=True later runs of the same
+    stacks, configurations and seeders replay a captured hipGraph;
+    autotune=True times the frame chunking per stack shape first.  The same
+    device stack under several configurations (binnings) shares one pass
+    for the bleach traces, as Engine.run_multi."""
 
-    def __init__(self, n, device=0, graph=True, autotune=True):
+    def __init__(self, n, device=0, graph=True, autotune=True, own_fits=False):
         self.n = n
-        self.h = lib().imfcs_batch_create(device, n, (1 if graph else 0) | (2 if autotune else 0))
+        self.h = lib().imfcs_batch_create(device, n, (1 if graph else 0) | (2 if autotune else 0) |
+                                          (4 if own_fits else 0))
         if not self.h:
             raise ImFCSError(get_last_error())
+        self._views = {}
+
+    def engine(self, i):
+        """Engine view of entry i's context (owned by the batch): results,
+        refit, set_fit_seeder."""
+        if i not in self._views:
+            ctx = lib().imfcs_batch_context(self.h, i)
+            if not ctx:
+                raise ImFCSError(get_last_error())
+            self._views[i] = Engine(_ctx=ctx)
+        return self._views[i]
 
     def run(self, d_stacks, cfgs):
         if len(d_stacks) != self.n or len(cfgs) != self.n:
@@ -509,6 +527,9 @@ class Batch:
 
     def close(self):
         if self.h:
+            for v in self._views.values():
+                v.close()
+            self._views = {}
             lib().imfcs_batch_destroy(self.h)
             self.h = None
 

@@ -1097,11 +1097,10 @@ extern "C" int imfcs_refit(imfcs_ctx *ctx, const imfcs_config *c) {
 // and full frame size, one pass over the stack writes the per-pixel sums of
 // each bleach-trace point (on the first context's stream) and every
 // pipeline takes its trace from them; the other streams wait for that pass.
-extern "C" int imfcs_run_multi(imfcs_ctx *const *ctxs, int n, const uint16_t *d_stack, const imfcs_config *cfgs) {
-    if (!ctxs || !cfgs || !d_stack || n < 1) return fail(IMFCS_ERR_ARG, "null argument");
-    for (int i = 0; i < n; i++)
-        if (!ctxs[i]) return fail(IMFCS_ERR_ARG, "null context");
-    for (int i = 0; i < n; i++) ctxs[i]->last_noalloc = 0;
+// Configurations whose polynomial bleach traces can all come from one pass
+// over the same stack: the polynomial correction, one image size and frame
+// range, one device, distinct contexts.
+static bool can_share_point_sums(imfcs_ctx *const *ctxs, int n, const imfcs_config *cfgs) {
     bool share = true;
     for (int i = 0; i < n && share; i++) {
         const imfcs_config &c = cfgs[i];
@@ -1110,6 +1109,28 @@ extern "C" int imfcs_run_multi(imfcs_ctx *const *ctxs, int n, const uint16_t *d_
                 c.lastframe == cfgs[0].lastframe && ctxs[i]->device == ctxs[0]->device;
         for (int j = 0; j < i && share; j++) share = ctxs[j] != ctxs[i];
     }
+    return share;
+}
+
+// The per-pixel integer sums of every bleach point of a stack (nopit points
+// of ave frames, as imfcs_run: IFCS:7323-7327), the input of every shared
+// configuration's trace.
+static int enqueue_point_sums(DevBuf &psums, const uint16_t *d_stack, const imfcs_config &k, hipStream_t st) {
+    const int T = k.lastframe - k.firstframe + 1;
+    if (T < 1 || k.firstframe < 1 || k.width < 1 || k.height < 1) return fail(IMFCS_ERR_ARG, "bad frame range");
+    const int nopit = std::min(T, 1000), ave = T / nopit;
+    const int64_t plane = (int64_t)k.width * k.height;
+    HIP_TRY(psums.ensure(sizeof(int32_t) * (size_t)nopit * plane));
+    launch_point_sums(d_stack, plane, k.firstframe - 1, nopit, ave, psums.as<int32_t>(), st);
+    return 0;
+}
+
+extern "C" int imfcs_run_multi(imfcs_ctx *const *ctxs, int n, const uint16_t *d_stack, const imfcs_config *cfgs) {
+    if (!ctxs || !cfgs || !d_stack || n < 1) return fail(IMFCS_ERR_ARG, "null argument");
+    for (int i = 0; i < n; i++)
+        if (!ctxs[i]) return fail(IMFCS_ERR_ARG, "null context");
+    for (int i = 0; i < n; i++) ctxs[i]->last_noalloc = 0;
+    const bool share = can_share_point_sums(ctxs, n, cfgs);
     if (!share) {
         for (int i = 0; i < n; i++) {
             const int rc = run_config(ctxs[i], d_stack, &cfgs[i], nullptr);
@@ -1118,16 +1139,11 @@ extern "C" int imfcs_run_multi(imfcs_ctx *const *ctxs, int n, const uint16_t *d_
         return 0;
     }
     imfcs_ctx *c0 = ctxs[0];
-    const imfcs_config &k = cfgs[0];
-    const int T = k.lastframe - k.firstframe + 1;
-    if (T < 1 || k.firstframe < 1 || k.width < 1 || k.height < 1) return fail(IMFCS_ERR_ARG, "bad frame range");
-    const int nopit = std::min(T, 1000), ave = T / nopit;   // as imfcs_run (IFCS:7323-7327)
-    const int64_t plane = (int64_t)k.width * k.height;
     HIP_TRY(hipSetDevice(c0->device));
     // the previous runs of every context may still read the sums: wait for them
     for (int i = 1; i < n; i++) HIP_TRY(hipStreamWaitEvent(c0->stream, ctxs[i]->ev[5], 0));
-    HIP_TRY(c0->psums.ensure(sizeof(int32_t) * (size_t)nopit * plane));
-    launch_point_sums(d_stack, plane, k.firstframe - 1, nopit, ave, c0->psums.as<int32_t>(), c0->stream);
+    const int prc = enqueue_point_sums(c0->psums, d_stack, cfgs[0], c0->stream);
+    if (prc) return prc;
     HIP_TRY(hipEventRecord(c0->ev_psums, c0->stream));
     for (int i = 1; i < n; i++) HIP_TRY(hipStreamWaitEvent(ctxs[i]->stream, c0->ev_psums, 0));
     // enqueued in the given order: a pipeline is ~25 launches (≈0.4 ms of
@@ -1296,6 +1312,7 @@ struct imfcs_batch {
     std::vector<hipEvent_t> ev_join;
     hipEvent_t ev_fork = nullptr, ev_fit = nullptr;
     DevBuf groups, prefix, counter;
+    DevBuf psums;   // bleach-point sums of a stack every configuration shares
     CachedUpload up_groups, up_prefix;
     std::vector<LMGroup> host_groups;
     std::vector<int64_t> host_prefix;
@@ -1304,6 +1321,7 @@ struct imfcs_batch {
     size_t graph_nodes = 0;
     std::vector<imfcs_config> last_cfgs;
     std::vector<const uint16_t *> last_stacks;
+    std::vector<const void *> last_seeders;   // the seeding networks of each context at capture
     int lm_blocks = 0;
     double autotune_ms = 0.0;
     int tuned_shapes = 0;
@@ -1323,14 +1341,24 @@ void batch_drop_graph(imfcs_batch *b) {
 // back into the batch stream, then the fits of all of them in one launch.
 int batch_enqueue(imfcs_batch *b, const uint16_t *const *stacks, const imfcs_config *cfgs) {
     const int n = (int)b->ctx.size();
+    // one stack under every configuration (the binnings of a diffusion law):
+    // the bleach traces of all of them from one pass over it, as imfcs_run_multi
+    bool one_stack = true;
+    for (int i = 1; i < n && one_stack; i++) one_stack = stacks[i] == stacks[0];
+    const bool shared = one_stack && can_share_point_sums(b->ctx.data(), n, cfgs);
+    if (shared) {
+        const int rc = enqueue_point_sums(b->psums, stacks[0], cfgs[0], b->st);
+        if (rc) return rc;
+    }
     HIP_TRY(hipEventRecord(b->ev_fork, b->st));
     b->host_groups.clear();
-    std::vector<int> fitted_alone;
+    const bool own_fits = (b->flags & IMFCS_BATCH_OWN_FITS) != 0;
     for (int i = 0; i < n; i++) {
         imfcs_ctx *c = b->ctx[i];
         HIP_TRY(hipStreamWaitEvent(c->stream, b->ev_fork, 0));
         c->last_noalloc = 0;
-        const int rc = run_config(c, stacks[i], &cfgs[i], nullptr, &b->host_groups);
+        const int rc = run_config(c, stacks[i], &cfgs[i], shared ? b->psums.as<int32_t>() : nullptr,
+                                  own_fits ? nullptr : &b->host_groups);
         if (rc) return rc;
         HIP_TRY(hipEventRecord(b->ev_join[i], c->stream));
         HIP_TRY(hipStreamWaitEvent(b->st, b->ev_join[i], 0));
@@ -1405,12 +1433,21 @@ int batch_autotune(imfcs_batch *b, const uint16_t *const *stacks, const imfcs_co
     return 0;
 }
 
+std::vector<const void *> batch_seeders(const imfcs_batch *b) {
+    std::vector<const void *> v;
+    for (const imfcs_ctx *c : b->ctx) {
+        v.push_back(c->seeder[0]);
+        v.push_back(c->seeder[1]);
+    }
+    return v;
+}
+
 bool same_batch(const imfcs_batch *b, const uint16_t *const *stacks, const imfcs_config *cfgs) {
     if (!b->exec || b->last_cfgs.size() != b->ctx.size()) return false;
     for (size_t i = 0; i < b->ctx.size(); i++)
         if (stacks[i] != b->last_stacks[i] || std::memcmp(&cfgs[i], &b->last_cfgs[i], sizeof(imfcs_config)) != 0)
             return false;
-    return true;
+    return batch_seeders(b) == b->last_seeders;
 }
 
 }  // namespace
@@ -1467,8 +1504,8 @@ extern "C" int imfcs_batch_run(imfcs_batch *b, const uint16_t *const *stacks, co
     HIP_TRY(hipSetDevice(b->device));
     for (size_t i = 0; i < b->ctx.size(); i++) {
         if (!stacks[i]) return fail(IMFCS_ERR_ARG, "null stack " + std::to_string(i));
-        if (b->ctx[i]->seeder[0] || b->ctx[i]->seeder[1])
-            return fail(IMFCS_ERR_ARG, "batched pipelines are not seeded");
+        if ((b->ctx[i]->seeder[0] || b->ctx[i]->seeder[1]) && !(b->flags & IMFCS_BATCH_OWN_FITS))
+            return fail(IMFCS_ERR_ARG, "seeded pipelines need IMFCS_BATCH_OWN_FITS");
     }
     if (same_batch(b, stacks, cfgs)) {
         HIP_TRY(hipGraphLaunch(b->exec, b->st));
@@ -1505,6 +1542,7 @@ extern "C" int imfcs_batch_run(imfcs_batch *b, const uint16_t *const *stacks, co
     }
     b->last_cfgs.assign(cfgs, cfgs + b->ctx.size());
     b->last_stacks.assign(stacks, stacks + b->ctx.size());
+    b->last_seeders = batch_seeders(b);
     return 0;
 }
 

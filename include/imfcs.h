@@ -299,10 +299,17 @@ int imfcs_get_stage_ms(imfcs_ctx *ctx, float *stage_ms5);
  * kept for every later run of that shape (batched or not).  Results per
  * stack: imfcs_get_results(imfcs_batch_context(b, i), ...) after
  * imfcs_batch_synchronize; identical to imfcs_run of each stack with the
- * same tuning. */
+ * same tuning.  Where every entry is the same device stack under the
+ * polynomial correction with one frame range (the binnings of a diffusion
+ * law), the bleach traces of all come from one pass over it, as in
+ * imfcs_run_multi, whose results the batch then reproduces. */
 typedef struct imfcs_batch imfcs_batch;
 #define IMFCS_BATCH_GRAPH 1
 #define IMFCS_BATCH_AUTOTUNE 2
+/* every pipeline fits its own pixels at the end of its own stream (no
+ * shared fit launch), as imfcs_run does; seeded contexts
+ * (imfcs_set_fit_seeder) need this flag */
+#define IMFCS_BATCH_OWN_FITS 4
 imfcs_batch *imfcs_batch_create(int device, int n, int flags);
 void imfcs_batch_destroy(imfcs_batch *b);
 int imfcs_batch_run(imfcs_batch *b, const uint16_t *const *d_stacks, const imfcs_config *cfgs);

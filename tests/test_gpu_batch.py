@@ -129,3 +129,56 @@ def test_48_uneven_stacks_batch_graph_equals_per_stack_runs(engine):
         for d in stacks:
             eng.free_buffer(d)
         eng.close()
+
+
+def test_shared_stack_graph_with_own_and_seeded_fits_equals_run_multi(engine):
+    """The binnings of a diffusion law as one batch over one device stack
+    (own_fits: each pipeline fits at the end of its stream): the captured
+    graph's replays give, bit for bit, what imfcs_run_multi gives, with the
+    plain and with the seeded LM start (the seeders are part of what a
+    replay is keyed on)."""
+    from imaging_fcs_1_52_amd.learned import InitialGuessNet
+    W = H = 64
+    T = 20000
+    stack = imfcs.simulate_2d(W, H, T, 400, D_um2s=[0.5, 1.0, 2.0, 5.0], tau_bleach_s=20.0, seed=4242)
+    bins = (1, 2)
+    cfgs = [_cfg(W, T, b) for b in bins]
+    batch = imfcs.Batch(len(bins), graph=True, autotune=False, own_fits=True)
+    ref = [imfcs.Engine(0) for _ in bins]
+    try:
+        views = [batch.engine(i) for i in range(len(bins))]
+        d = views[0].upload(stack)
+        nets = []
+        for mode in ("plain", "seeded", "plain"):
+            if mode == "seeded" and not nets:
+                for b, cfg in zip(bins, cfgs):
+                    _, _, mtab, lagtime = imfcs.correlator_tables(16, 8, T, 1e-3)
+                    pix = 0.24e-6 * b
+                    pair = [InitialGuessNet(views[0], imfcs.make_fit_model(pix, pix, PSF, rx=rx), lagtime[1:],
+                                            mtab[1:], ncomp=1, seed=60 + 2 * b + k) for k, rx in enumerate((0, pix))]
+                    for n in pair:
+                        n.train(steps=60, batch=2048)
+                    nets.append(pair)
+            for i in range(len(bins)):
+                seed = nets[i] if mode == "seeded" else (None, None)
+                views[i].set_fit_seeder(*seed)
+                ref[i].set_fit_seeder(*seed)
+            for rep in range(3):           # eager + capture, then two replays
+                batch.run([d] * len(bins), cfgs)
+                batch.synchronize()
+            info = batch.info()
+            imfcs.Engine.run_multi(ref, cfgs, d)
+            for i, cfg in enumerate(cfgs):
+                a, r = batch.results(i, cfg), ref[i].results(cfg)
+                for k in ("acf", "var", "ind", "blockIndS", "fitres", "chi2", "states", "iters", "coef"):
+                    assert np.array_equal(a[k], r[k], equal_nan=True), (mode, bins[i], k)
+            print("%s: graph of %d nodes, results equal imfcs_run_multi" % (mode, info["graph_nodes"]), flush=True)
+        for pair in nets:
+            for n in pair:
+                n.close()
+    finally:
+        for e in ref:
+            e.set_fit_seeder(None, None)
+        batch.close()
+        for e in ref:
+            e.close()
