@@ -854,6 +854,7 @@ static int run_fit(imfcs_ctx *ctx, const imfcs_config *c, std::vector<LMGroup> *
             for (int i = 0; i < nparts; i++)
                 if (mask[parts[i].row] != 0x13u) return fail(IMFCS_ERR_ARG, "the GLS fit supports the free set N, D, G_inf");
         // a batch collects the fits of its pipelines for one persistent launch
+        bool collected = false;
         if (collect && !ctx->did_gls && !seeded && (n_acf == 0 || n_ccf == 0 || mask[0] == mask[1]) &&
             lm_queue_eligible(npts, mask[n_acf ? 0 : 1], c->fit_chi2_fp64 ? 1 : 0)) {
             const int row = n_acf ? 0 : 1;
@@ -863,6 +864,7 @@ static int run_fit(imfcs_ctx *ctx, const imfcs_config *c, std::vector<LMGroup> *
                                        ctx->fstate.as<int>(), ctx->fchi2.as<float>(), ctx->fiter.as<int>(), 1,
                                        n_acf});
             nparts = 0;
+            collected = true;
         }
         // learned starting rows: a seeded attempt of at most 15 % of the
         // iteration limit (a fit converging from a good start takes a few
@@ -925,6 +927,16 @@ static int run_fit(imfcs_ctx *ctx, const imfcs_config *c, std::vector<LMGroup> *
                               LMModelGeom{}, c->fit_tolerance, c->fit_max_iterations, chi2_64, fo + f0 * 20, fs + f0,
                               fc + f0, fi + f0, st, split);
             }
+        }
+        // FP64 polish of the converged N, D, G_inf fits (the pipeline's FP64
+        // chi-square mode; fits the batch collected are polished after its
+        // shared launch)
+        if (!ctx->did_gls && chi2_64 && !collected) {
+            if (n_acf > 0 && mask[0] == 0x13u)
+                launch_lm_polish(n_acf, npts, fd, fw, fx, fo, fs, fc, st);
+            if (n_ccf > 0 && mask[1] == 0x13u)
+                launch_lm_polish(n_ccf, npts, fd + (size_t)n_acf * npts, fw + (size_t)n_acf * npts, fx,
+                                 fo + (size_t)n_acf * 20, fs + n_acf, fc + n_acf, st);
         }
         for (int r = 0; r < 2; r++)
             ctx->chi2_norm[r] = ctx->did_gls
@@ -1372,6 +1384,7 @@ int batch_enqueue(imfcs_batch *b, const uint16_t *const *stacks, const imfcs_con
         HIP_TRY(b->counter.ensure(sizeof(unsigned long long)));
         launch_lm_queue(b->groups.as<LMGroup>(), b->prefix.as<int64_t>(), ng, b->host_prefix[ng],
                         b->counter.as<unsigned long long>(), b->lm_blocks, b->st);
+        launch_lm_polish_groups(b->groups.as<LMGroup>(), b->prefix.as<int64_t>(), ng, b->host_prefix[ng], b->st);
         HIP_TRY(hipGetLastError());
     }
     return 0;

@@ -255,6 +255,12 @@ void launch_correlate_fast(const SeriesGeom &g, const BleachDesc &bd, const Corr
                            double *hpart, float *l3, float *lvd, int chunk_len, double *corav, double *bvar,
                            hipStream_t st, hipEvent_t ev_main_begin = nullptr, hipEvent_t ev_main_end = nullptr,
                            float *gls_blk = nullptr, int gls_nblk = 0);
+// FP64 polish of the converged N, D, G_inf fits of the one-component model
+// (up to 128 points): rewrites their parameters and chi-square in place.
+void launch_lm_polish(int nfits, int npts, const float *data, const float *weights, const float *x,
+                      float *out_params, int *states, float *chi2, hipStream_t st);
+void launch_lm_polish_groups(const LMGroup *groups, const int64_t *prefix, int ngroups, int64_t total,
+                             hipStream_t st);
 void launch_lm_acf(int nfits, int npts, const float *data, const float *weights, const float *x,
                    const float *init, const int *pfit, unsigned free_mask, const LMModelGeom &geom, float tol, int maxit,
                    int chi2_fp64, float *out_params, int *states, float *chi2, int *iters, hipStream_t st,

@@ -1610,6 +1610,60 @@ __global__ void __launch_bounds__(64) k_lm_exp64(LMArgs a, double *out64) {
     }
 }
 
+// One dimension of the one-component model in FP64 (the device library's
+// exp and erf): the lateral factor l and the D derivative of its
+// exponential terms, the same expressions as fcs_dim (agpufitjni.cu:440-535
+// evaluates the model in FP64 as well).
+__device__ void fcs_dim64(double a, double r, double v, double x, double p0, double &l, double &dDpexp) {
+    const double isqrpi = 0.56418958354775628;
+    if (r == 0.0 && v == 0.0) {
+        const double u = a / p0, E = exp(-u * u);
+        l = p0 * isqrpi * (2.0 * E - 2.0) + 2.0 * a * erf(u);
+        dDpexp = 2.0 * (2.0 * E - 2.0);
+        return;
+    }
+    const double p1 = a + r - v * x, p2 = a - r + v * x, p3 = r - v * x;
+    const double u1 = p1 / p0, u2 = p2 / p0, u3 = p3 / p0;
+    const double pexp = exp(-u1 * u1) + exp(-u2 * u2) - 2.0 * exp(-u3 * u3);
+    l = p0 * isqrpi * pexp + (p1 * erf(u1) + p2 * erf(u2) - 2.0 * p3 * erf(u3));
+    const double ip02 = 1.0 / (p0 * p0);
+    const double p4 = 2.0 * a * a + 3.0 * r * r - 6.0 * x * r * v + 3.0 * (x * v) * (x * v);
+    const double p5 = p3 * p3 + p1 * p1, p6 = p3 * p3 + p2 * p2;
+    const double p7 = 2.0 * (a * a + r * r - 2.0 * x * r * v + (x * v) * (x * v));
+    dDpexp = 2.0 * (exp((p5 - p4) * ip02) + exp((p6 - p4) * ip02) - 2.0 * exp((p7 - p4) * ip02));
+}
+
+// The one-component model (no second or third component, no triplet; flow
+// fixed) in FP64: value and derivatives in N, D, G_inf (fcs3p's slots 0, 1,
+// 4), parameters in the fit's internal units.
+__device__ double fcs1_f64(const LMModelGeom &m, double x, double N, double D, double vx, double vy, double G,
+                           double (&j)[3], double &acf_out) {
+    const double isqrpi = 0.56418958354775628;
+    const double ax = m.ax, ay = m.ay, s = m.s, sz = m.sz;
+    const double p0 = sqrt(4.0 * D * x + s * s);
+    double lx, dx, ly, dy;
+    fcs_dim64(ax, m.rx, vx, x, p0, lx, dx);
+    if (m.ay == m.ax && m.ry == m.rx && vy == vx) {
+        ly = lx;
+        dy = dx;
+    } else {
+        fcs_dim64(ay, m.ry, vy, x, p0, ly, dy);
+    }
+    const double inorm = (double)m.obsvol / (4.0 * (ax * ay) * (ax * ay));
+    const double plat = lx * ly * inorm;
+    const double szi2 = 1.0 / (sz * sz);
+    const double pspim = 1.0 / sqrt(1.0 + 4.0 * D * x * szi2);
+    const double acf = plat * pspim;
+    const double dDplat = (isqrpi / p0) * (dy * x * lx + dx * x * ly) * inorm;
+    const double dDpspim = -2.0 * x * szi2 * (pspim * pspim * pspim);
+    const double iN = 1.0 / N;
+    j[0] = -acf * iN * iN;
+    j[1] = iN * (plat * dDpspim + pspim * dDplat);
+    j[2] = 1.0;
+    acf_out = acf;
+    return acf * iN + G;
+}
+
 // The same LM fit of the FCS model with NF fixed free slots for curves of
 // 65..128 points and an FP64 chi-square: two wavefronts per fit, one point
 // per lane, so one model evaluation per iteration instead of two passes of
@@ -1687,6 +1741,167 @@ __device__ void lm_fcs_2w_fit(const LMArgs &a, int fit) {
         a.chi2[fit] = (float)chi2;
         a.iters[fit] = iters + prev_iters;
     }
+}
+
+// FP64 polish of a converged N, D, G_inf fit of the one-component model
+// (no second or third component, no triplet): Newton steps on chi^2 with
+// the FP64 model and FP64 sums from the LM's end point, kept while chi^2
+// falls.  The FP32 model's rounding (about 1e-6 relative) moves the LM's
+// end point along a shallow chi^2 valley by up to ~1e-3 in D on short
+// series; the steps end at the FP64 optimum, where the reference's FP64 CPU
+// fit ends.  Along that valley the residuals are not small, so Gauss-Newton
+// steps (J^T W J alone) overshoot and zig-zag; the Hessian here adds the
+// second-order term -sum w r d2f, whose N-N and N-D entries are analytic
+// and whose D-D entry comes from a forward difference of the analytic D
+// derivative (step 1e-5 D), taken once at the start point.  Two waves, one
+// point per lane (up to 128 points); a separate launch, so that the FP64
+// model's registers do not lower the LM kernel's occupancy.  Reads and
+// rewrites the fit's output row and chi-square.
+__device__ void lm_polish_fit(const LMArgs &a, int fit) {
+    __shared__ double red64[2][2][13];
+    const int tid = threadIdx.x, wv = tid >> 6;
+    if (a.states[fit] != kStateConverged) return;   // uniform per workgroup
+    float *row = a.out_params + (int64_t)fit * 20;
+    float par[20];
+#pragma unroll
+    for (int k = 0; k < 20; k++) par[k] = row[k];
+    if (par[5] != 0.f || par[7] != 0.f || par[9] != 0.f || !(par[0] != 0.f) || !(par[1] > 0.f)) return;
+    LMModelGeom geom;
+    fcs_params_in(par, geom);
+    const bool act = tid < a.npts;
+    const double x64 = act ? (double)a.x[tid] : 0.0;
+    const double y64 = act ? (double)a.data[(int64_t)fit * a.npts + tid] : 0.0;
+    const double wi = !act ? 0.0 : (a.weights ? (double)a.weights[(int64_t)fit * a.npts + tid] : 1.0);
+    const double vx = (double)par[2], vy = (double)par[3];
+    int buf = 0;
+    // chi^2, gradient g = J^T W r, packed J^T W J and (second) the
+    // second-order sums sum w r d2f/dN2, d2f/dNdD, d2f/dD2
+    auto eval64 = [&](const double (&p)[3], bool second, double &c2, double (&g)[3], double (&h)[6],
+                      double (&sm)[3]) {
+        double v[13];
+#pragma unroll
+        for (int k = 0; k < 13; k++) v[k] = 0.0;
+        if (act) {
+            double jq[3], acf;
+            const double r = y64 - fcs1_f64(geom, x64, p[0], p[1], vx, vy, p[2], jq, acf);
+            v[0] = wi * r * r;
+            int hh = 4;
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                v[1 + q] = wi * r * jq[q];
+#pragma unroll
+                for (int l = 0; l <= q; l++, hh++) v[hh] = wi * jq[q] * jq[l];
+            }
+            if (second) {
+                const double hD = 1e-5 * p[1], iN = 1.0 / p[0];
+                double jh[3], acfh;
+                (void)fcs1_f64(geom, x64, 1.0, p[1] + hD, vx, vy, 0.0, jh, acfh);
+                const double aD = jq[1] * p[0];   // d acf / dD
+                v[10] = wi * r * (2.0 * acf * iN * iN * iN);
+                v[11] = wi * r * (-aD * iN * iN);
+                v[12] = wi * r * ((jh[1] - aD) / hD * iN);
+            }
+        }
+        const int nv = second ? 13 : 10;   // uniform
+        for (int k = 0; k < nv; k++) v[k] = wave_sum(v[k]);
+        if ((tid & 63) == 0)
+            for (int k = 0; k < nv; k++) red64[buf][wv][k] = v[k];
+        __syncthreads();
+        c2 = red64[buf][0][0] + red64[buf][1][0];
+#pragma unroll
+        for (int q = 0; q < 3; q++) g[q] = red64[buf][0][1 + q] + red64[buf][1][1 + q];
+#pragma unroll
+        for (int q = 0; q < 6; q++) h[q] = red64[buf][0][4 + q] + red64[buf][1][4 + q];
+        if (second)
+#pragma unroll
+            for (int q = 0; q < 3; q++) sm[q] = red64[buf][0][10 + q] + red64[buf][1][10 + q];
+        buf ^= 1;
+    };
+    double p[3] = {(double)par[0], (double)par[1], (double)par[4]};
+    double c2, g[3], h[6], sm[3];
+    eval64(p, true, c2, g, h, sm);
+    for (int it = 0; it < 3; it++) {
+        double d[3];
+        bool solved = false;
+        for (int full = 1; full >= 0 && !solved; full--) {
+            // Jacobi-scaled Hessian (the columns differ by orders of
+            // magnitude); without the second-order term if that one is not
+            // positive definite
+            double sc[3], H[3][3];
+            bool ok = true;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const double hk = h[k * (k + 1) / 2 + k];
+                ok = ok && hk > 0.0;
+                sc[k] = hk > 0.0 ? 1.0 / sqrt(hk) : 0.0;
+            }
+            if (!ok) break;
+            int hh = 0;
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+#pragma unroll
+                for (int l = 0; l <= k; l++, hh++) {
+                    double e = h[hh];
+                    if (full) e -= (k == 0 && l == 0) ? sm[0] : (k == 1 && l == 0) ? sm[1] : (k == 1 && l == 1) ? sm[2] : 0.0;
+                    H[k][l] = H[l][k] = e * sc[k] * sc[l];
+                }
+#pragma unroll
+            for (int k = 0; k < 3; k++) d[k] = g[k] * sc[k];
+            solved = chol_solve_fixed<3>(H, d);
+#pragma unroll
+            for (int k = 0; k < 3; k++) d[k] *= sc[k];
+        }
+        if (!solved) break;
+        double t[3];
+        bool moves = false;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            t[k] = p[k] + d[k];
+            moves = moves || fabs(d[k]) > 1e-12 * fabs(p[k]);
+        }
+        if (!moves) break;
+        double c2t, gt[3], ht[6], unused[3];
+        eval64(t, false, c2t, gt, ht, unused);
+        if (!(c2t < c2)) break;
+        c2 = c2t;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            p[k] = t[k];
+            g[k] = gt[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 6; k++) h[k] = ht[k];
+    }
+    if (tid == 0) {
+        // back to SI units, as fcs_params_out
+        row[0] = (float)p[0];
+        row[1] = (float)p[1] * 1e-12f;
+        row[4] = (float)p[2];
+        a.chi2[fit] = (float)c2;
+    }
+}
+
+__global__ void __launch_bounds__(128) k_lm_polish(LMArgs a) {
+    const int fit = blockIdx.x;
+    if (fit < a.nfits) lm_polish_fit(a, fit);
+}
+
+// The polish over the fits of many groups (a batch's persistent fit
+// launch): one workgroup per fit, its group found in the prefix table.
+__global__ void __launch_bounds__(128) k_lm_polish_groups(const LMGroup *groups, const int64_t *prefix, int ngroups,
+                                                          int64_t total) {
+    const int64_t idx = blockIdx.x;
+    if (idx >= total) return;
+    int lo = 0, hi = ngroups - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= idx) lo = mid;
+        else hi = mid - 1;
+    }
+    const LMGroup &G = groups[lo];
+    LMArgs a{G.nfits, G.npts, 20, kModelFCS, 0, G.data, G.weights, G.x, G.init, G.pfit, LMModelGeom{}, G.tol,
+             G.maxit, 1, G.out_params, G.states, G.chi2, G.iters, G.init_two_rows, G.init_split, 0};
+    lm_polish_fit(a, (int)(idx - prefix[lo]));
 }
 
 template <int NF>
@@ -2203,6 +2418,20 @@ void launch_lm_acf(int nfits, int npts, const float *data, const float *weights,
     LMArgs a{nfits, npts, 20, kModelFCS, 0, data, weights, x, init, pfit, geom, tol, maxit, chi2_fp64, out_params,
              states, chi2, iters, init_split >= 0 ? 1 : 0, init_split, only_failed};
     launch_lm(a, free_mask, st);
+}
+
+void launch_lm_polish(int nfits, int npts, const float *data, const float *weights, const float *x,
+                      float *out_params, int *states, float *chi2, hipStream_t st) {
+    if (nfits <= 0 || npts <= 0 || npts > 128) return;
+    LMArgs a{nfits, npts, 20, kModelFCS, 0, data, weights, x, nullptr, nullptr, LMModelGeom{}, 0.f, 0, 1, out_params,
+             states, chi2, nullptr, 0, 0, 0};
+    hipLaunchKernelGGL(k_lm_polish, dim3(nfits), dim3(128), 0, st, a);
+}
+
+void launch_lm_polish_groups(const LMGroup *groups, const int64_t *prefix, int ngroups, int64_t total,
+                             hipStream_t st) {
+    if (total <= 0 || ngroups <= 0) return;
+    hipLaunchKernelGGL(k_lm_polish_groups, dim3((unsigned)total), dim3(128), 0, st, groups, prefix, ngroups, total);
 }
 
 void launch_lm_exp(int nfits, int npts, const float *data, const float *x, const float *init, const int *pfit,
