@@ -242,15 +242,24 @@ __global__ void __launch_bounds__(256) k_mlp_fused(FusedArgs a) {
     __shared__ float Ss[kFusedRows];
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const int r0 = blockIdx.x * kFusedRows;
-    // inputs: wave w normalises rows w, w + 4, ...; lane l holds channels l and l + 64
-    for (int rr = w; rr < kFusedRows; rr += 4) {
-        const int row = r0 + rr;
-        float v0 = 0.f, v1 = 0.f;
+    // inputs: wave w normalises rows w, w + 4, ...; lane l holds channels l
+    // and l + 64; the loads of all its rows are issued first
+    constexpr int RPW = kFusedRows / 4;
+    float in0[RPW], in1[RPW];
+#pragma unroll
+    for (int q = 0; q < RPW; q++) {
+        const int row = r0 + w + 4 * q;
+        in0[q] = in1[q] = 0.f;
         if (row < a.n) {
             const float *d = a.data + (int64_t)row * a.npts;
-            if (lane < a.npts) v0 = d[lane];
-            if (lane + 64 < a.npts) v1 = d[lane + 64];
+            if (lane < a.npts) in0[q] = d[lane];
+            if (lane + 64 < a.npts) in1[q] = d[lane + 64];
         }
+    }
+#pragma unroll
+    for (int q = 0; q < RPW; q++) {
+        const int rr = w + 4 * q, row = r0 + rr;
+        const float v0 = in0[q], v1 = in1[q];
         float mx = fmaxf(fabsf(v0), fabsf(v1));
         int bad = !(isfinite(v0) && isfinite(v1));
         for (int off = 32; off > 0; off >>= 1) {
@@ -277,18 +286,28 @@ __global__ void __launch_bounds__(256) k_mlp_fused(FusedArgs a) {
         for (int i = 0; i < RT; i++)
 #pragma unroll
             for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        // every weight operand of the wave's columns is loaded first (K / 16
+        // steps x 4 column tiles, 8 bytes each per lane): one wait for L2
+        // instead of one per step
+        bf16x4 bv[K / 16][4];
 #pragma unroll
-        for (int k0 = 0; k0 < K; k0 += 16) {
-            bf16x4 av[RT], bv[4];
+        for (int ks = 0; ks < K / 16; ks++)
 #pragma unroll
-            for (int j = 0; j < 4; j++) bv[j] = *(const bf16x4 *)&WT[(int64_t)(64 * w + j * 16 + lr) * K + k0 + lq];
+            for (int j = 0; j < 4; j++)
+                bv[ks][j] = *(const bf16x4 *)&WT[(int64_t)(64 * w + j * 16 + lr) * K + 16 * ks + lq];
+        __builtin_amdgcn_sched_barrier(0);   // keep the loads ahead of the products
+#pragma unroll
+        for (int ks = 0; ks < K / 16; ks++) {
+            const int k0 = 16 * ks;
+            bf16x4 av[RT];
 #pragma unroll
             for (int i = 0; i < RT; i++)
                 av[i] = from_x ? *(const bf16x4 *)&Xs[i * 16 + lr][k0 + lq] : *(const bf16x4 *)&In[i * 16 + lr][k0 + lq];
 #pragma unroll
             for (int i = 0; i < RT; i++)
 #pragma unroll
-                for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[i], bv[j], acc[i][j], 0, 0, 0);
+                for (int j = 0; j < 4; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[i], bv[ks][j], acc[i][j], 0, 0, 0);
         }
 #pragma unroll
         for (int i = 0; i < RT; i++)
