@@ -732,8 +732,8 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTabl
         HIP_TRY(ctx->cpart.ensure(sizeof(float) * (size_t)nch * np * 33 * 3));
         HIP_TRY(ctx->hpart.ensure(sizeof(double) * (size_t)nch * np * (ct.chanum - 33) * 3));
         size_t nlv = 0;
-        for (int c = 4; c < ct.nlevels; c++) nlv += (size_t)ct.numbin[c];
-        HIP_TRY(ctx->l3.ensure(sizeof(float) * (size_t)ct.numbin[3] * g.nslots));
+        for (int c = 4; c < ct.nlevels; c++) nlv += (size_t)ct.numbin[c] + kLevelPad;
+        HIP_TRY(ctx->l3.ensure(sizeof(float) * ((size_t)ct.numbin[3] + kLevelPad) * g.nslots));
         HIP_TRY(ctx->lvd.ensure(sizeof(float) * std::max<size_t>(nlv, 1) * g.nslots));
         launch_correlate_fast(g, bd, ct, dct, ea.pd, ctx->F.as<double>(), ctx->X.as<float>(), ctx->ind.as<int32_t>(),
                               ctx->cpart.as<float>(), ctx->hpart.as<double>(), ctx->l3.as<float>(),

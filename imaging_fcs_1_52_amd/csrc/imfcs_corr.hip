@@ -31,6 +31,13 @@ constexpr int kTileF = 64;     // frames per LDS tile
 constexpr int kMainChan = 33;  // channels of levels 0..2 (p = 16)
 constexpr int kNS = 128;       // staged series per tile (a and partner b)
 
+// The level >= 3 arrays: level 3 + k at (k == 0 ? l3 : lvd + off[k]), numbin[k] rows
+struct LevelPads {
+    int nlev, nslots;
+    int numbin[kMaxLevel];
+    int64_t off[kMaxLevel];
+};
+
 struct MainArgs {
     SeriesGeom g;
     BleachDesc bd;
@@ -417,14 +424,22 @@ __global__ void __launch_bounds__(768) k_corr_main(MainArgs A) {
 // from there on in FP64).  Branch-free (clamped index, zero past the end)
 // so that a group's loads issue back to back.
 struct LevelCtx {
-    const float *lv;   // this level's values [numbin][nslots]
+    const float *lv;   // this level's values [numbin + kLevelPad][nslots], zero past numbin
     int nslots, sa, sb, c, numbin;   // sa, sb: slots of the pair's series
-    __device__ __forceinline__ float val(int s, int y) const {
-        const int64_t i = (int64_t)min(y, numbin - 1) * nslots + s;
-        const float v = lv[i];
-        return (y < numbin) ? v : 0.f;
-    }
+    // y < numbin + kLevelPad: no clamp and no select, so that nothing uses a
+    // loaded value before the group that needs it (the loads stay in flight)
+    __device__ __forceinline__ float val(int s, int y) const { return lv[(int64_t)y * nslots + s]; }
 };
+
+// Zero rows [numbin_c, numbin_c + kLevelPad) of every level >= 3 array.
+__global__ void k_zero_level_pads(float *l3, float *lvd, LevelPads P) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t per = (int64_t)kLevelPad * P.nslots;
+    const int lvl = (int)blockIdx.y;
+    if (i >= per || lvl >= P.nlev) return;
+    float *base = lvl == 0 ? l3 : lvd + P.off[lvl];
+    base[(int64_t)P.numbin[lvl] * P.nslots + i] = 0.f;
+}
 
 // Levels c0+1 .. c0+4 from level c0 by the pairwise-sum rebinning
 // (IFCS:9477-9478): v_c[y] = v_{c-1}[2y] + v_{c-1}[2y+1], summed in FP64
@@ -595,7 +610,16 @@ __global__ void __launch_bounds__(64, 2) k_corr_high(MainArgs A, double *hpart /
     const int c = 3 + blockIdx.y;
     const int chunk = blockIdx.z;
     const CorrTables &ct = *A.ctp;
-    if (p >= A.pd.npix || pidx >= A.pd.npairs || c >= ct.nlevels) return;
+    if (c >= ct.nlevels) return;   // uniform per workgroup
+    const bool live = p < A.pd.npix && pidx < A.pd.npairs;
+    const int indp = live ? A.ind[pidx] : 0x7fff;
+    // group size: the largest power of two up to 16 that divides the block
+    // length 2^(ind - c) of every pair of the wave (blocks hold whole
+    // groups); fewer, longer groups carry less per-group overhead
+    int imin = indp;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) imin = min(imin, __shfl_xor(imin, off, 64));
+    if (!live) return;
     int sa, sb;
     pair_series(A.pd, pidx, sa, sb);
     const int ka = A.g.slot_of[sa], kb = A.g.slot_of[sb];   // coarse levels are stored per slot
@@ -610,7 +634,6 @@ __global__ void __launch_bounds__(64, 2) k_corr_high(MainArgs A, double *hpart /
         alpha[i] = Fa[ct.pos_a_end[x]] / n - mua;
         beta[i] = (Fb[ct.pos_b_end[x]] - Fb[ct.pos_b_beg[x]]) / n - mub;
     }
-    const int indp = A.ind[pidx];
     const int B = 1 << (indp - c);
     const int pnum = (int)floor(ct.mtab_last / pow(2.0, fmax(indp - ct.log2_samp_last, 0.0)));
     double cross[8], a1[8], a2[8];
@@ -618,7 +641,7 @@ __global__ void __launch_bounds__(64, 2) k_corr_high(MainArgs A, double *hpart /
     const int ybeg = chunk * len;
     const int yend = (chunk == A.nchunk - 1) ? ct.numbin[c] : ybeg + len;
     float *grow = A.gls ? A.gls + (int64_t)pidx * A.gls_nblk * (ct.chanum - 1) + (x0 - 1) : nullptr;
-    const int G = min(16, 1 << max(ct.q - 1 - c, 0));
+    const int G = min(16, 1 << max(imin - c, 0));
     const int ld = ct.chanum - 1;
     // (wave-uniform: a workgroup holds only ACF or only CCF pairs)
     if (acf_block && A.pd.acf_first)
@@ -719,7 +742,7 @@ void launch_correlate_fast(const SeriesGeom &g, const BleachDesc &bd, const Corr
     int64_t off = 0;
     for (int c = 0; c < kMaxLevel; c++) {
         A.loff[c] = off;
-        if (c >= 4 && c < ct.nlevels) off += (int64_t)ct.numbin[c] * g.nslots;
+        if (c >= 4 && c < ct.nlevels) off += ((int64_t)ct.numbin[c] + kLevelPad) * g.nslots;
     }
     const int tiles = (g.npix + 63) / 64;
     const int threads = pd.has_ccf ? 768 : 384;
@@ -739,6 +762,16 @@ void launch_correlate_fast(const SeriesGeom &g, const BleachDesc &bd, const Corr
         const int64_t n = (int64_t)((ct.numbin[c0] + 15) / 16) * g.nslots;
         hipLaunchKernelGGL(k_corr_rebin4, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, R);
     }
+    LevelPads P{};
+    P.nlev = ct.nlevels - 3;
+    P.nslots = g.nslots;
+    for (int k = 0; k < P.nlev; k++) {
+        P.numbin[k] = ct.numbin[3 + k];
+        P.off[k] = A.loff[3 + k];
+    }
+    if (P.nlev > 0)
+        hipLaunchKernelGGL(k_zero_level_pads, dim3((unsigned)(((int64_t)kLevelPad * g.nslots + 255) / 256), P.nlev),
+                           dim3(256), 0, st, l3, lvd, P);
     const int nhigh = ct.chanum - kMainChan;
     const int nbp = (pd.npix + 63) / 64;   // workgroups per pair type
     const int nb_acf = pd.acf_first ? nbp : 0;

@@ -19,6 +19,9 @@ int set_error(int code, const std::string &msg);
 
 constexpr int kMaxChan = 128;   // correlator channels (73 for p=16, q=8)
 constexpr int kMaxLevel = 12;   // multi-tau levels (q)
+// zero rows stored after each level >= 3 series array of the correlator, so
+// that its readers load past the last value without a clamp
+constexpr int kLevelPad = 64;
 constexpr int kMaxOrder = 8;    // polynomial bleach order (BCmaxorder, IFCS:201)
 constexpr int kMaxBlock = 20;   // blocking-curve levels
 constexpr int kMaxPos = 2 * kMaxChan + kMaxLevel + 8;  // prefix positions
