@@ -178,20 +178,38 @@ struct PolyL {
 };
 
 // Solves M c = r with the shared Cholesky factor of the moment matrix.
-__global__ void k_polyfit(const double *part, int nchunk, int nseries, int order, PolyL Lf, double *coef) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nseries) return;
-    int n = order + 1;
+// Workgroups of 64 series x kPolyQ chunk quarters: thread (s, j) sums the
+// chunks j, j + kPolyQ, ... (the chunk totals are independent loads, so
+// four short chains instead of one long one), the quarters meet in LDS and
+// are added in the order j = 0, 1, 2, 3.
+constexpr int kPolyQ = 4;
+__global__ void __launch_bounds__(64 * kPolyQ) k_polyfit(const double *part, int nchunk, int nseries, int order,
+                                                        PolyL Lf, double *coef) {
+    __shared__ double red[kPolyQ - 1][kMaxOrder + 1][64];
+    const int sl = threadIdx.x & 63, j = threadIdx.x >> 6;
+    const int s = blockIdx.x * 64 + sl;
+    const int n = order + 1;
     double r[kMaxOrder + 1], z[kMaxOrder + 1];
 #pragma unroll
     for (int k = 0; k <= kMaxOrder; k++) r[k] = 0.0;
+    if (s < nseries) {
 #pragma unroll 8
-    for (int c = 0; c < nchunk; c++) {
-        int64_t base = ((int64_t)c * nseries + s) * n;
+        for (int c = j; c < nchunk; c += kPolyQ) {
+            int64_t base = ((int64_t)c * nseries + s) * n;
 #pragma unroll
-        for (int k = 0; k <= kMaxOrder; k++)
-            if (k < n) r[k] += part[base + k];
+            for (int k = 0; k <= kMaxOrder; k++)
+                if (k < n) r[k] += part[base + k];
+        }
     }
+    if (j > 0)
+#pragma unroll
+        for (int k = 0; k <= kMaxOrder; k++) red[j - 1][k][sl] = r[k];
+    __syncthreads();
+    if (j > 0 || s >= nseries) return;
+#pragma unroll
+    for (int q = 0; q < kPolyQ - 1; q++)
+#pragma unroll
+        for (int k = 0; k <= kMaxOrder; k++) r[k] += red[q][k][sl];
 #pragma unroll
     for (int i = 0; i <= kMaxOrder; i++) {
         if (i < n) {
@@ -523,30 +541,31 @@ __global__ void __launch_bounds__(64) k_blocking(SeriesGeom g, const CorrTables 
 // bpart holds [nsplit][pair][kBlkJ][2] partial sums of the levels < kBlkJ;
 // the levels >= kBlkJ are cascaded from the 256-product chunk totals
 // csum[chunk][pair].
-__global__ void k_blocking_finalize(const CorrTables *__restrict__ ctp, SeriesGeom g, PairDesc pd,
-                                    const double *F, const double *bpart, int nsplit, const double *csum,
-                                    int32_t *ind_out, int32_t *bis_out) {
+// Four lanes per pair (a wave holds 16 pairs): quarter j = lane & 3 sums
+// the split partials j, j + 4, ... and cascades its own segment of the chunk
+// totals (segments of a whole number of the longest blocks, so that every
+// block lies in one segment); the quarters' sums meet by two lane swaps
+// ((q0 + q1) + (q2 + q3) on every lane of the quad) and every lane applies
+// the criteria, quarter 0 writing the result.
+constexpr int kBfinQ = 4;
+__global__ void __launch_bounds__(64) k_blocking_finalize(const CorrTables *__restrict__ ctp, SeriesGeom g,
+                                                          PairDesc pd, const double *F, const double *bpart,
+                                                          int nsplit, const double *csum, int32_t *ind_out,
+                                                          int32_t *bis_out) {
     const CorrTables &ct = *ctp;
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= pd.npairs) return;
-    int sa, sb;
-    pair_series(pd, p, sa, sb);
-    const int ka = g.slot_of[sa], kb = g.slot_of[sb];
+    const int j = threadIdx.x & (kBfinQ - 1);
+    const int p0 = blockIdx.x * (64 / kBfinQ) + (threadIdx.x / kBfinQ);
+    const bool live = p0 < pd.npairs;
+    const int p = live ? p0 : pd.npairs - 1;   // (dead quads compute a copy, write nothing)
     const int n0 = g.T - 1;
-    const double am = F[(int64_t)ka * ct.npos + ct.pos_blk_a] / n0;
-    const double bm = F[(int64_t)kb * ct.npos + ct.pos_blk_b] / n0;
     const int K = ct.blocknum;
-    double sd[kMaxBlock], upper[kMaxBlock], lower[kMaxBlock];
-    int crt[kMaxBlock], cr12[kMaxBlock], cr3[kMaxBlock], diffpos[kMaxBlock];
-    double dd = am * bm;
-    double nb = (double)n0;
     double S1[kMaxBlock], S2[kMaxBlock], pend[kMaxBlock];
 #pragma unroll
     for (int k = 0; k < kMaxBlock; k++) S1[k] = S2[k] = pend[k] = 0.0;
     // split partials (split-major, so a split's 2 kBlkJ values are contiguous);
     // unrolled so that several splits' loads are in flight together
 #pragma unroll 4
-    for (int c = 0; c < nsplit; c++) {
+    for (int c = j; c < nsplit; c += kBfinQ) {
         const double *bp = bpart + ((int64_t)c * pd.npairs + p) * kBlkJ * 2;
 #pragma unroll
         for (int k = 0; k < kBlkJ; k++) {
@@ -559,8 +578,12 @@ __global__ void k_blocking_finalize(const CorrTables *__restrict__ ctp, SeriesGe
         if (k >= K) S1[k] = S2[k] = 0.0;
     if (K > kBlkJ) {
         const int nfull = n0 >> kBlkJ;
+        // segment length: a multiple of the longest block, 2^(K - 1 - kBlkJ) totals
+        const int blk = 1 << (K - 1 - kBlkJ);
+        const int seg = ((nfull + kBfinQ - 1) / kBfinQ + blk - 1) / blk * blk;
+        const int tb = min(nfull, j * seg), te = min(nfull, tb + seg);
 #pragma unroll 4
-        for (int t = 0; t < nfull; t++) {
+        for (int t = tb; t < te; t++) {
             double carry = csum[(int64_t)t * pd.npairs + p];
             bool open = true;
 #pragma unroll
@@ -578,6 +601,22 @@ __global__ void k_blocking_finalize(const CorrTables *__restrict__ ctp, SeriesGe
             }
         }
     }
+#pragma unroll
+    for (int k = 0; k < kMaxBlock; k++) {
+        S1[k] += __shfl_xor(S1[k], 1, 64);
+        S2[k] += __shfl_xor(S2[k], 1, 64);
+        S1[k] += __shfl_xor(S1[k], 2, 64);
+        S2[k] += __shfl_xor(S2[k], 2, 64);
+    }
+    int sa, sb;
+    pair_series(pd, p, sa, sb);
+    const int ka = g.slot_of[sa], kb = g.slot_of[sb];
+    const double am = F[(int64_t)ka * ct.npos + ct.pos_blk_a] / n0;
+    const double bm = F[(int64_t)kb * ct.npos + ct.pos_blk_b] / n0;
+    double sd[kMaxBlock], upper[kMaxBlock], lower[kMaxBlock];
+    int crt[kMaxBlock], cr12[kMaxBlock], cr3[kMaxBlock], diffpos[kMaxBlock];
+    double dd = am * bm;
+    double nb = (double)n0;
 #pragma unroll
     for (int k = 0; k < kMaxBlock; k++) {
         sd[k] = upper[k] = lower[k] = 0.0;
@@ -624,8 +663,10 @@ __global__ void k_blocking_finalize(const CorrTables *__restrict__ ctp, SeriesGe
         bis = 1;
     }
     if (ind < ct.q - 1) ind = ct.q - 1;
-    ind_out[p] = ind;
-    bis_out[p] = bis;
+    if (live && j == 0) {
+        ind_out[p] = ind;
+        bis_out[p] = bis;
+    }
 }
 
 // ------------------------------------------------------------------
@@ -2246,7 +2287,8 @@ void launch_polyfit(const double *part, int nchunk, int nseries, int nopit, int 
             Lf.L[i * n + j] = t / Lf.L[j * n + j];
         }
     }
-    hipLaunchKernelGGL(k_polyfit, dim3(cdiv(nseries, 64)), dim3(64), 0, st, part, nchunk, nseries, order, Lf, coef);
+    hipLaunchKernelGGL(k_polyfit, dim3(cdiv(nseries, 64)), dim3(64 * kPolyQ), 0, st, part, nchunk, nseries, order,
+                       Lf, coef);
 }
 
 template <int NB>
@@ -2362,8 +2404,8 @@ void launch_blocking_finalize(const CorrTables &ct, const CorrTables *dct, const
                               const double *F, const BlockingPlan &bp, const double *bpart, const double *csum,
                               int32_t *ind, int32_t *bis, hipStream_t st) {
     (void)ct;
-    hipLaunchKernelGGL(k_blocking_finalize, dim3(cdiv(pd.npairs, 64)), dim3(64), 0, st, dct, g, pd, F, bpart,
-                       bp.nsplit, csum, ind, bis);
+    hipLaunchKernelGGL(k_blocking_finalize, dim3(cdiv(pd.npairs, 64 / kBfinQ)), dim3(64), 0, st, dct, g, pd, F,
+                       bpart, bp.nsplit, csum, ind, bis);
 }
 
 void launch_correlate(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const CorrTables *dct,
