@@ -127,6 +127,29 @@ def cpu_baseline(stack, stack_min, sample_side, nthreads):
     return nfit / dt, sample_side * sample_side / dt, dt
 
 
+def solo(imfcs, stack, smin, args):
+    """One binning's plain pipeline alone on the GPU (the standalone kernel
+    times of a profile): stage times per run, HIP events."""
+    b = args.solo_binning
+    if b not in BINNINGS:
+        raise SystemExit("--solo-binning must be one of %s" % (BINNINGS,))
+    cfg = configs(smin)[BINNINGS.index(b)]
+    e = imfcs.Engine(0)
+    d = e.upload(stack)
+    rows = []
+    for i in range(args.warmup + args.steps):
+        e.run(cfg, d)
+        e.synchronize()
+        if i >= args.warmup:
+            rows.append(e.stage_ms())
+    med = np.median(np.asarray(rows), 0)
+    print(json.dumps({"solo_binning": b, "runs": args.steps,
+                      "stage_ms_median": {k: round(float(v), 3) for k, v in
+                                          zip(("trace+fit", "correction+means", "blocking", "correlation", "lm"),
+                                              med)}}), flush=True)
+    e.close()
+
+
 def fit_stats(states, iters):
     ok = states == 0
     it = iters[ok] if ok.any() else iters
@@ -146,6 +169,9 @@ def main():
                     help="graph: the three pipelines captured once into a hipGraph (imfcs_batch with one shared "
                          "stack) and replayed every step; multi: imfcs_run_multi enqueued every step")
     ap.add_argument("--autotune", action="store_true", help="graph runner: time the frame chunking first")
+    ap.add_argument("--solo-binning", type=int, default=0,
+                    help="profiling aid: run only this binning's plain pipeline --steps times (after --warmup), alone "
+                         "on the GPU, and print its per-run stage times instead of the benchmark line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain device-to-device copy of the stack (known bytes) for PMC traffic scaling")
     args = ap.parse_args()
@@ -158,6 +184,9 @@ def main():
     from imaging_fcs_1_52_amd.learned import InitialGuessNet
     stack = make_stack()
     smin = int(stack.min())
+    if args.solo_binning:
+        solo(imfcs, stack, smin, args)
+        return
     order = [BINNINGS.index(b) for b in ENQUEUE_ORDER]
     batch = None
     if args.runner == "graph":
