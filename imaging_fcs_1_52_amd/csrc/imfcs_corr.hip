@@ -58,10 +58,15 @@ struct MainArgs {
     int gls_nblk;
 };
 
+// Level rows stored per series ([series][value], padded to 4 mod 32 words
+// so that 8 lanes' 16-byte accesses cover 32 distinct banks): a lane reads
+// a group's 16 values with four 16-byte loads and the loader writes them
+// the same way.
+constexpr int kL0Ld = kTileF + 4, kL1Ld = kTileF / 2 + 4, kL2Ld = kTileF / 4 + 4;
 struct Tile {
-    float l0[kTileF][kNS];
-    float l1[kTileF / 2][kNS];
-    float l2[kTileF / 4][kNS];
+    float l0[kNS][kL0Ld];
+    float l1[kNS][kL1Ld];
+    float l2[kNS][kL2Ld];
     // sums of each 16-frame block's staged values on levels 0, 1 and 2
     // (16, 8 and 4 values): the group sums of the role waves
     float gs0[kTileF / 16][kNS];
@@ -142,17 +147,17 @@ struct TileLoader {
 #pragma unroll
             for (int k = 0; k < 16; k++) {
                 x[k] = (!TAIL || fbase + k < T) ? xr[16 * fb + k] : 0.f;
-                tl.l0[16 * fb + k][sl] = x[k];
+                tl.l0[sl][16 * fb + k] = x[k];
             }
 #pragma unroll
             for (int k = 0; k < 8; k++) {
                 u1[k] = x[2 * k] + x[2 * k + 1];
-                tl.l1[8 * fb + k][sl] = (!TAIL || fbase + 2 * k + 1 < T) ? u1[k] : 0.f;
+                tl.l1[sl][8 * fb + k] = (!TAIL || fbase + 2 * k + 1 < T) ? u1[k] : 0.f;
             }
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 u2[k] = u1[2 * k] + u1[2 * k + 1];
-                tl.l2[4 * fb + k][sl] = (!TAIL || fbase + 4 * k + 3 < T) ? u2[k] : 0.f;
+                tl.l2[sl][4 * fb + k] = (!TAIL || fbase + 4 * k + 3 < T) ? u2[k] : 0.f;
             }
             const float s8a = u2[0] + u2[1], s8b = u2[2] + u2[3];
             if (own && fbase < chunk_end) {
@@ -299,11 +304,22 @@ __device__ void run_role(const MainArgs &A, const Tile *tl2, int lane, int p0, i
             }
             gsb_cur = gsb_nxt;
             if (CCF) gsa_cur = gsa_nxt;
+            const float *rb = (LEVEL == 0) ? &tl.l0[bsl][16 * q] : (LEVEL == 1) ? &tl.l1[bsl][16 * q] : &tl.l2[bsl][16 * q];
+            const float *ra = (LEVEL == 0) ? &tl.l0[asl][16 * q] : (LEVEL == 1) ? &tl.l1[asl][16 * q] : &tl.l2[asl][16 * q];
 #pragma unroll
-            for (int k = 0; k < 16; k++) {
-                const float *row = (LEVEL == 0) ? tl.l0[16 * q + k] : (LEVEL == 1) ? tl.l1[16 * q + k] : tl.l2[16 * q + k];
-                w[16 + k] = row[bsl];
-                if (CCF) ga_nxt[k] = row[asl];
+            for (int k = 0; k < 16; k += 4) {
+                const float4 vb = *(const float4 *)(rb + k);
+                w[16 + k] = vb.x;
+                w[17 + k] = vb.y;
+                w[18 + k] = vb.z;
+                w[19 + k] = vb.w;
+                if (CCF) {
+                    const float4 va = *(const float4 *)(ra + k);
+                    ga_nxt[k] = va.x;
+                    ga_nxt[k + 1] = va.y;
+                    ga_nxt[k + 2] = va.z;
+                    ga_nxt[k + 3] = va.w;
+                }
             }
             gsb_nxt = group_sum<LEVEL>(tl, q, bsl);
             if (CCF) gsa_nxt = group_sum<LEVEL>(tl, q, asl);
@@ -389,7 +405,7 @@ __device__ void run_role(const MainArgs &A, const Tile *tl2, int lane, int p0, i
 // instructions) on the other two.  Each role has one call site, so that
 // every role is inlined.
 __global__ void __launch_bounds__(768) k_corr_main(MainArgs A) {
-    __shared__ Tile tl2[2];
+    __shared__ __attribute__((aligned(16))) Tile tl2[2];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int p0 = blockIdx.x * 64;
