@@ -173,7 +173,8 @@ def main():
                     help="profiling aid: run only this binning's plain pipeline --steps times (after --warmup), alone "
                          "on the GPU, and print its per-run stage times instead of the benchmark line")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="also run a plain device-to-device copy of the stack (known bytes) for PMC traffic scaling")
+                    help="also copy the stack device to device twice (16-byte vector copy, 4-byte buffer-word copy; known "
+                         "bytes) for PMC traffic scaling")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine has no multi-GPU path (pixels shard without "
@@ -307,8 +308,9 @@ def main():
         dst = lib.imfcs_device_alloc(engines[0].ctx, ctypes.c_size_t(COPY_BYTES))
         if dst:
             lib.imfcs_memcpy_d2d(engines[0].ctx, dst, d_stack, ctypes.c_size_t(COPY_BYTES))
+            lib.imfcs_memcpy_d2d_words(engines[0].ctx, dst, d_stack, ctypes.c_size_t(COPY_BYTES))
             lib.imfcs_device_free(engines[0].ctx, dst)
-            log("copy calibration: %d bytes" % COPY_BYTES)
+            log("copy calibration: %d bytes (16-byte vector copy, then 4-byte buffer-word copy)" % COPY_BYTES)
 
     iters = {}
     for b, r in zip(BINNINGS, res_plain):

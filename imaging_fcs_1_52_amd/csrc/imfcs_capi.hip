@@ -475,6 +475,15 @@ int imfcs_memcpy_d2d(imfcs_ctx *ctx, void *dst, const void *src, size_t bytes) {
     return 0;
 }
 
+int imfcs_memcpy_d2d_words(imfcs_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx) return fail(IMFCS_ERR_ARG, "null context");
+    if (bytes % 4) return fail(IMFCS_ERR_ARG, "the word copy takes a multiple of 4 bytes");
+    launch_copy_words(dst, src, bytes, ctx->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
 int imfcs_synchronize(imfcs_ctx *ctx) {
     if (!ctx) return fail(IMFCS_ERR_ARG, "null context");
     HIP_TRY(hipStreamSynchronize(ctx->stream));

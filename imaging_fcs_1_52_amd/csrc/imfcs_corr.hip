@@ -615,13 +615,19 @@ __device__ __forceinline__ void high_dispatch(const MainArgs &A, int c, int G, i
 }
 
 // One thread per (pair, level >= 3, time chunk of the main kernel).  The
-// ACF pairs and the CCF pairs fill separate workgroups (blocks [0, nb_acf)
-// and the rest), so whether a and b are one series is uniform per wave and
-// an ACF loads its series once.
+// ACF pairs and the CCF pairs fill separate workgroups, so whether a and b
+// are one series is uniform per wave and an ACF loads its series once.
+// With both kinds, workgroup x = 16 g + r takes the ACF of pixel block
+// 8 g + r (r < 8) or the CCF of pixel block 8 g + r - 8: the two workgroups
+// of a pixel block are 8 apart, so they run on the same XCD at about the
+// same time and the CCF's series (the ACF's own, and the neighbours' one
+// slot further) come from that XCD's L2.  With one kind, x is the pixel block.
 __global__ void __launch_bounds__(64, 2) k_corr_high(MainArgs A, double *hpart /* [chunk][pair][nhigh][3] */,
-                                                  int nhigh, int nb_acf) {
-    const bool acf_block = (int)blockIdx.x < nb_acf;
-    const int p = (acf_block ? (int)blockIdx.x : (int)blockIdx.x - nb_acf) * 64 + threadIdx.x;
+                                                  int nhigh, int both) {
+    const int bx = (int)blockIdx.x;
+    const bool acf_block = both ? (bx & 15) < 8 : A.pd.acf_first != 0;
+    const int pblk = both ? 8 * (bx >> 4) + (bx & 7) : bx;
+    const int p = pblk * 64 + threadIdx.x;
     const int pidx = acf_block ? p : (A.pd.acf_first ? A.pd.npix : 0) + p;
     const int c = 3 + blockIdx.y;
     const int chunk = blockIdx.z;
@@ -790,9 +796,9 @@ void launch_correlate_fast(const SeriesGeom &g, const BleachDesc &bd, const Corr
                            dim3(256), 0, st, l3, lvd, P);
     const int nhigh = ct.chanum - kMainChan;
     const int nbp = (pd.npix + 63) / 64;   // workgroups per pair type
-    const int nb_acf = pd.acf_first ? nbp : 0;
-    const int nb_all = nb_acf + (pd.has_ccf ? nbp : 0);
-    hipLaunchKernelGGL(k_corr_high, dim3(nb_all, ct.nlevels - 3, A.nchunk), dim3(64), 0, st, A, hpart, nhigh, nb_acf);
+    const int both = pd.acf_first && pd.has_ccf;
+    const int nb_all = both ? 16 * ((nbp + 7) / 8) : nbp;
+    hipLaunchKernelGGL(k_corr_high, dim3(nb_all, ct.nlevels - 3, A.nchunk), dim3(64), 0, st, A, hpart, nhigh, both);
     hipLaunchKernelGGL(k_corr_finalize, dim3(pd.npairs), dim3(128), 0, st, A, (const double *)hpart, nhigh, corav,
                        bvar);
 }

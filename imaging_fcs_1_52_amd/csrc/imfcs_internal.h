@@ -204,6 +204,7 @@ void launch_lm_queue(const LMGroup *groups, const int64_t *prefix, int ngroups, 
                      unsigned long long *counter, int blocks, hipStream_t st);
 
 // Launchers (imfcs_kernels.hip)
+void launch_copy_words(void *dst, const void *src, size_t bytes, hipStream_t st);
 void launch_trace_moments(const SeriesGeom &g, int nopit, int ave, double tscale, double frametime,
                           int order, int integer_trace, double *part, int nchunk, hipStream_t st,
                           float *trace_out = nullptr);

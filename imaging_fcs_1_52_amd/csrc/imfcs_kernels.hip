@@ -441,6 +441,7 @@ __global__ void __launch_bounds__(64) k_blocking(SeriesGeom g, const CorrTables 
                                                  const double *F, const float *__restrict__ X, const double *coef,
                                                  int order, double *bpart, double *csum, int chunks_per_split) {
     const CorrTables &ct = *ctp;
+    const int split = blockIdx.y;
     const int p = blockIdx.x * 64 + threadIdx.x;
     if (p >= pd.npix) return;
     const int pa = p, pc = ACF ? pd.npix + p : p;   // pair indices of the ACF and the CCF
@@ -453,7 +454,7 @@ __global__ void __launch_bounds__(64) k_blocking(SeriesGeom g, const CorrTables 
     const double beta_a = F[(int64_t)ka * ct.npos + ct.pos_blk_b] / n0 - mua;
     const double beta_b = F[(int64_t)kb * ct.npos + ct.pos_blk_b] / n0 - mub;
     const int nchunk = (n0 + (1 << kBlkJ) - 1) >> kBlkJ;
-    const int t0 = blockIdx.y * chunks_per_split, t1 = min(nchunk, t0 + chunks_per_split);
+    const int t0 = split * chunks_per_split, t1 = min(nchunk, t0 + chunks_per_split);
     const bool bload = ld * 64 < 0x7fffffff;   // 16 rows addressable by one buffer resource
     double s1a[kBlkJ], s2a[kBlkJ], s1c[kBlkJ], s2c[kBlkJ];
 #pragma unroll
@@ -521,7 +522,7 @@ __global__ void __launch_bounds__(64) k_blocking(SeriesGeom g, const CorrTables 
         }
     }
     if (ACF) {
-        const int64_t base = ((int64_t)blockIdx.y * pd.npairs + pa) * kBlkJ * 2;
+        const int64_t base = ((int64_t)split * pd.npairs + pa) * kBlkJ * 2;
 #pragma unroll
         for (int k = 0; k < kBlkJ; k++) {
             bpart[base + 2 * k] = s1a[k];
@@ -529,7 +530,7 @@ __global__ void __launch_bounds__(64) k_blocking(SeriesGeom g, const CorrTables 
         }
     }
     if (CCF) {
-        const int64_t base = ((int64_t)blockIdx.y * pd.npairs + pc) * kBlkJ * 2;
+        const int64_t base = ((int64_t)split * pd.npairs + pc) * kBlkJ * 2;
 #pragma unroll
         for (int k = 0; k < kBlkJ; k++) {
             bpart[base + 2 * k] = s1c[k];
@@ -2154,6 +2155,30 @@ __global__ void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, s
     if (blockIdx.x == 0 && (int)threadIdx.x < ntail) dst_tail[threadIdx.x] = src_tail[threadIdx.x];
 }
 
+// The same copy in 4-byte words by buffer loads and stores, one word per
+// lane and 64 rows of 64 lanes per step, with the row as the scalar offset:
+// the access pattern of the pipeline's FP32 series passes (the known-size
+// reference of the traffic counters for such kernels).  Words past n32 are
+// not touched.
+__global__ void k_copy_words(const uint32_t *src, uint32_t *dst, int64_t n32) {
+    const int64_t base = (int64_t)blockIdx.x * 4096;   // 64 rows of 64 words
+    const int lane = threadIdx.x;
+    const int64_t left = n32 - base;
+    if (left <= 0) return;
+    const int nbytes = (int)(std::min<int64_t>(left, 4096) * 4);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void *)(src + base), (short)0, nbytes,
+                                                                        kBufferWord3);
+    const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc((void *)(dst + base), (short)0, nbytes,
+                                                                        kBufferWord3);
+    uint32_t v[64];
+#pragma unroll
+    for (int r = 0; r < 64; r++)
+        v[r] = (64 * r + lane) * 4 < nbytes ? __builtin_amdgcn_raw_buffer_load_b32(rs, lane * 4, r * 256, 0) : 0u;
+#pragma unroll
+    for (int r = 0; r < 64; r++)
+        if ((64 * r + lane) * 4 < nbytes) __builtin_amdgcn_raw_buffer_store_b32(v[r], rd, lane * 4, r * 256, 0);
+}
+
 // ------------------------------------------------------------------
 // Launchers
 // ------------------------------------------------------------------
@@ -2180,6 +2205,13 @@ void launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st) {
     const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>((n16 + 255) / 256, 256 * 64));
     hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, st, (const uint4 *)src, (uint4 *)dst, n16,
                        (const uint8_t *)src + n16 * 16, (uint8_t *)dst + n16 * 16, ntail);
+}
+
+void launch_copy_words(void *dst, const void *src, size_t bytes, hipStream_t st) {
+    const int64_t n32 = (int64_t)(bytes / 4);
+    if (n32 <= 0) return;
+    hipLaunchKernelGGL(k_copy_words, dim3((unsigned)((n32 + 4095) / 4096)), dim3(64), 0, st, (const uint32_t *)src,
+                       (uint32_t *)dst, n32);
 }
 
 void launch_trace_moments(const SeriesGeom &g, int nopit, int ave, double tscale, double frametime, int order,

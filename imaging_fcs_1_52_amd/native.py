@@ -82,6 +82,7 @@ SIGNATURES = {
     "imfcs_memcpy_h2d": (ctypes.c_int, [_VP, _VP, _VP, _SZ]),
     "imfcs_memcpy_d2h": (ctypes.c_int, [_VP, _VP, _VP, _SZ]),
     "imfcs_memcpy_d2d": (ctypes.c_int, [_VP, _VP, _VP, _SZ]),
+    "imfcs_memcpy_d2d_words": (ctypes.c_int, [_VP, _VP, _VP, _SZ]),
     "imfcs_synchronize": (ctypes.c_int, [_VP]),
     "imfcs_run": (ctypes.c_int, [_VP, _VP, _P(PipelineConfig)]),
     "imfcs_run_noalloc": (ctypes.c_int, [_VP, _VP, _P(PipelineConfig), _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),

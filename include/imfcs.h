@@ -216,6 +216,12 @@ int imfcs_memcpy_d2h(imfcs_ctx *ctx, void *dst, const void *src, size_t bytes);
 /* Device-to-device copy by a plain vector-copy kernel (also the known-size
  * reference for HBM traffic counters). */
 int imfcs_memcpy_d2d(imfcs_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* The same copy in 4-byte words with buffer loads and stores (one word per
+ * lane, the row as the scalar offset), the access pattern of the pipeline's
+ * FP32 series passes: the known-size reference of the traffic counters for
+ * such kernels.  No counterpart in the reference (a measurement aid);
+ * bytes must be a multiple of 4. */
+int imfcs_memcpy_d2d_words(imfcs_ctx *ctx, void *dst, const void *src, size_t bytes);
 int imfcs_synchronize(imfcs_ctx *ctx);
 
 /* Runs the pipeline on a device-resident uint16 stack [T][H][W].  Results
