@@ -507,14 +507,22 @@ namespace {
 // load throughput.  Padding slots repeat the last window of their row.
 // buf = slot_of[nseries], the window origins [nslots][2], then the owner
 // flags [nseries] (first series of each slot).
+static int slot_pitch(const SeriesGeom &g) {
+    const bool ccf = g.nseries > g.npix;
+    const bool on_grid = ccf && (g.cfx % g.pbx == 0) && (g.cfy % g.pby == 0);
+    const int sw = g.ow + (on_grid ? g.cfx / g.pbx : 0);
+    int swp = 1;
+    while (swp < sw && swp < 64) swp <<= 1;
+    if (sw > 64) swp = (sw + 63) / 64 * 64;
+    return swp;
+}
+
 int build_slots(const SeriesGeom &g, std::vector<int32_t> &buf) {
     const bool ccf = g.nseries > g.npix;
     const bool on_grid = ccf && (g.cfx % g.pbx == 0) && (g.cfy % g.pby == 0);
     const int dx = on_grid ? g.cfx / g.pbx : 0, dy = on_grid ? g.cfy / g.pby : 0;
     const int sw = g.ow + dx, sh = g.oh + dy;
-    int swp = 1;
-    while (swp < sw && swp < 64) swp <<= 1;
-    if (sw > 64) swp = (sw + 63) / 64 * 64;
+    const int swp = slot_pitch(g);
     const int ngrid = (ccf && !on_grid) ? 2 : 1;
     const int nslots = ngrid * sh * swp;
     buf.assign(2 * (size_t)g.nseries + 2 * (size_t)nslots, 0);
@@ -579,6 +587,7 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTabl
     HIP_TRY(ctx->up_ctab.put(ctx->ctab, &ct, sizeof(CorrTables), st));
     const CorrTables *dct = ctx->ctab.as<CorrTables>();
     g.nslots = build_slots(g, ctx->host_slots);
+    g.swp = slot_pitch(g);
     // vectorized window rows: every slot origin, the row stride and the
     // frame size are multiples of the window width (uint16 source)
     g.vec = 0;

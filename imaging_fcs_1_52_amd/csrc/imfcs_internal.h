@@ -49,6 +49,9 @@ struct SeriesGeom {
     int32_t cfx, cfy;     // partner offset
     int32_t nseries;
     int32_t nslots;
+    // slot row pitch of the pixel grid: pixel (x, y) is slot y * swp + x,
+    // swp a power of two up to 64 or a multiple of 64
+    int32_t swp;
     const int32_t *slot_of;    // [nseries] device
     const int32_t *slot_org;   // [nslots][2] device
     // [nseries] device: 1 for the first series of each slot, which writes

@@ -442,8 +442,11 @@ __global__ void __launch_bounds__(64) k_blocking(SeriesGeom g, const CorrTables 
                                                  int order, double *bpart, double *csum, int chunks_per_split) {
     const CorrTables &ct = *ctp;
     const int split = blockIdx.y;
-    const int p = blockIdx.x * 64 + threadIdx.x;
-    if (p >= pd.npix) return;
+    // a wave covers 64 consecutive slots of the pixel grid (a 256-byte
+    // aligned row segment of X); lanes on padding slots have no pixel
+    const int s = blockIdx.x * 64 + threadIdx.x, sy = s / g.swp, sx = s - sy * g.swp;
+    if (sy >= g.oh || sx >= g.ow) return;
+    const int p = sy * g.ow + sx;
     const int pa = p, pc = ACF ? pd.npix + p : p;   // pair indices of the ACF and the CCF
     const int ka = g.slot_of[p], kb = CCF ? g.slot_of[pd.npix + p] : ka;
     const int64_t ld = g.nslots;
@@ -2420,7 +2423,7 @@ BlockingPlan plan_blocking(const CorrTables &ct, int npix, int threads_log2) {
 void launch_blocking(const SeriesGeom &g, const BleachDesc &bd, const CorrTables *dct, const PairDesc &pd,
                      const double *F, const float *X, const BlockingPlan &bp, double *bpart, double *csum,
                      hipStream_t st) {
-    dim3 grid(cdiv(pd.npix, 64), bp.nsplit);
+    dim3 grid(cdiv((int64_t)g.oh * g.swp, 64), bp.nsplit);
     if (pd.acf_first && pd.has_ccf)
         hipLaunchKernelGGL((k_blocking<true, true>), grid, dim3(64), 0, st, g, dct, pd, F, X, bd.coef, bd.order,
                            bpart, csum, bp.chunks_per_split);
