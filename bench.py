@@ -299,9 +299,19 @@ def main():
     alg_bytes = 2.0 * W * H * T
     corr_ms = kern_seed[0]
     achieved = alg_bytes / (corr_ms * 1e-3) / 1e9
+    # its arithmetic floor: 23 FP32 FMAs per pair-frame (17 level-0 delays on
+    # every frame, 8 level-1 on every second, 8 level-2 on every fourth) for
+    # the ACF and the CCF of every output pixel, at the non-packed FP32 VALU
+    # peak (78.6 TFLOP/s = 39.3e12 FMA/s)
+    ow1, oh1 = W - 1, H   # b = 1 output grid with the x+1 partner
+    fmas = 23.0 * 2 * ow1 * oh1 * T
+    floor_ms = fmas / 39.3e12 * 1e3
     roof = {"kernel": "k_corr_main (b=1, ACF+CCF)", "bound": "hbm", "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(),
-            "avg_ms": round(float(corr_ms), 3), "alg_bytes": alg_bytes}
+            "avg_ms": round(float(corr_ms), 3), "alg_bytes": alg_bytes,
+            "valu_floor": {"fma": fmas, "floor_ms": round(floor_ms, 3), "frac": round(floor_ms / corr_ms, 4),
+                           "note": "FP32 VALU FMA floor of the kernel's products; its HBM frac cannot exceed "
+                                   "%.3f" % ((alg_bytes / (floor_ms * 1e-3) / 1e9) / HBM_PEAK_GBS)}}
 
     if args.copy_calibration:
         lib = imfcs.lib()
