@@ -29,7 +29,7 @@ def _run(cmd):
 
 def build_hip(verbose=True):
     objs = []
-    for src in ("imfcs_kernels.hip", "imfcs_corr.hip", "imfcs_gls.hip", "imfcs_sim.hip", "imfcs_mlp.hip", "imfcs_capi.hip"):
+    for src in ("imfcs_kernels.hip", "imfcs_corr.hip", "imfcs_corr_mfma.hip", "imfcs_gls.hip", "imfcs_sim.hip", "imfcs_mlp.hip", "imfcs_capi.hip"):
         obj = os.path.join(CSRC, "_build", src + ".o")
         os.makedirs(os.path.dirname(obj), exist_ok=True)
         _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])

@@ -299,7 +299,8 @@ struct imfcs_ctx {
     hipEvent_t ev_psums = nullptr;
     DevBuf etrace, eitime, epfit, einit, eout, estate, echi2, eiter;   // exponential bleach fits
     DevBuf part, coef, ctot, rec, F, bpart, ind, bis, corav, bvar, fdata, fw, fx, finit, fpfit, fout, fstate, fchi2,
-        fiter, cpart, hpart, l3, lvd, ctab, csum, slots, X, glsblk, glsW, glsy, glsok, glscov, dropin_in, nbm, nbc;
+        fiter, cpart, hpart, l3, lvd, ctab, csum, slots, X, glsblk, glsW, glsy, glsok, glscov, dropin_in, nbm, nbc,
+        cmax, xexp;
     SeriesGeom last_g{};   // geometry (with device slot tables) of the last engine run
     std::vector<int32_t> host_slots;   // slot_of then slot origins
     std::vector<double> host_coef;     // drop-in bleach coefficients per slot (source of the async copy)   // source of the async copy into ctab
@@ -307,6 +308,7 @@ struct imfcs_ctx {
     int npix = 0, npairs = 0, nseries = 0, nslots = 0, chanum = 0, order = 0, ow = 0, oh = 0, did_fit = 0, nfree = 0;
     int did_gls = 0;
     int force_v1 = 0;   // IMFCS_CORRELATOR=generic selects the one-thread-per-level kernel
+    int want_mfma = 0;   // IMFCS_CORRELATOR=mfma: levels 0..2 on the matrix cores (k_corr_mfma)
     double chi2_norm[2] = {1.0, 1.0};   // ACF fits, CCF fits
     int fit_split = 0;                  // fits below are ACF fits
     std::vector<double> coef_scale;   // u-basis -> seconds conversion per power
@@ -399,6 +401,7 @@ imfcs_ctx *imfcs_create(int device) {
     for (auto &e : ctx->evk) (void)hipEventCreate(&e);
     const char *sel = std::getenv("IMFCS_CORRELATOR");
     ctx->force_v1 = (sel && std::strcmp(sel, "generic") == 0) ? 1 : 0;
+    ctx->want_mfma = (sel && std::strcmp(sel, "mfma") == 0) ? 1 : 0;
     return ctx;
 }
 
@@ -410,7 +413,7 @@ static std::vector<DevBuf *> all_bufs(imfcs_ctx *ctx) {
                       &ctx->X, &ctx->glsblk, &ctx->glsW, &ctx->glsy, &ctx->glsok, &ctx->glscov,
                       &ctx->dropin_in, &ctx->nbm, &ctx->nbc, &ctx->swacf, &ctx->swvar, &ctx->etrace,
                       &ctx->eitime, &ctx->epfit, &ctx->einit, &ctx->eout, &ctx->estate, &ctx->echi2,
-                      &ctx->eiter, &ctx->psums, &ctx->fseed};
+                      &ctx->eiter, &ctx->psums, &ctx->fseed, &ctx->cmax, &ctx->xexp};
 }
 
 static void invalidate_uploads(imfcs_ctx *ctx) {
@@ -715,8 +718,21 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTabl
     HIP_TRY(ctx->rec.ensure(sizeof(double) * (size_t)ns * ct.npos));
     HIP_TRY(ctx->F.ensure(sizeof(double) * (size_t)ns * ct.npos));
     HIP_TRY(ctx->X.ensure(sizeof(float) * (size_t)ns * g.T));
-    launch_apply(g, bd, dct, ctx->ctot.as<double>(), ctx->rec.as<double>(), ctx->X.as<float>(), npc, plen, st);
-    launch_prefix_combine(dct, ns, ctx->ctot.as<double>(), ctx->rec.as<double>(), npc, plen, ctx->F.as<double>(), st);
+    // levels 0..2 on the matrix cores (selected with IMFCS_CORRELATOR=mfma;
+    // the VALU kernel is faster on every measured shape): the bleach pass
+    // also writes the level-3 values and the per-slot staging exponents
+    const bool fast = correlate_fast_supported(ct) && !ctx->force_v1;
+    const bool mfma = fast && ctx->want_mfma && corr_mfma_eligible(g, ea.pd) &&
+                      (int64_t)g.nslots * 4 * (16 * 36) < 0x7fffffff;
+    if (mfma) {
+        HIP_TRY(ctx->l3.ensure(sizeof(float) * ((size_t)ct.numbin[3] + kLevelPad) * g.nslots));
+        HIP_TRY(ctx->cmax.ensure(sizeof(float) * (size_t)npc * ns));
+        HIP_TRY(ctx->xexp.ensure(sizeof(int32_t) * (size_t)ns));
+    }
+    launch_apply(g, bd, dct, ctx->ctot.as<double>(), ctx->rec.as<double>(), ctx->X.as<float>(),
+                 mfma ? ctx->l3.as<float>() : nullptr, mfma ? ctx->cmax.as<float>() : nullptr, npc, plen, st);
+    launch_prefix_combine(dct, ns, ctx->ctot.as<double>(), ctx->rec.as<double>(), npc, plen, ctx->F.as<double>(),
+                          mfma ? ctx->cmax.as<float>() : nullptr, mfma ? ctx->xexp.as<int32_t>() : nullptr, st);
     HIP_TRY(hipEventRecord(ctx->ev[2], st));
     // blocking
     const BlockingPlan bp = plan_blocking(ct, g.npix, tn.block_log2);
@@ -757,7 +773,8 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTabl
                               ctx->cpart.as<float>(), ctx->hpart.as<double>(), ctx->l3.as<float>(),
                               ctx->lvd.as<float>(), clen,
                               ctx->corav.as<double>(), ctx->bvar.as<double>(), st, ctx->evk[0], ctx->evk[1],
-                              ea.gls ? ctx->glsblk.as<float>() : nullptr, nblk);
+                              ea.gls ? ctx->glsblk.as<float>() : nullptr, nblk,
+                              mfma ? ctx->xexp.as<int32_t>() : nullptr);
         ctx->evk_valid = 1;
         if (ea.gls) {
             // mcov replaces the correlation as Java's GLS output (IFCS:9638-9639)

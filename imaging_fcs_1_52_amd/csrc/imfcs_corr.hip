@@ -743,7 +743,7 @@ void launch_correlate_fast(const SeriesGeom &g, const BleachDesc &bd, const Corr
                            const PairDesc &pd, const double *F, const float *X, const int32_t *ind, float *part,
                            double *hpart, float *l3, float *lvd, int chunk_len, double *corav, double *bvar,
                            hipStream_t st, hipEvent_t ev_main_begin, hipEvent_t ev_main_end, float *gls_blk,
-                           int gls_nblk) {
+                           int gls_nblk, const int32_t *xexp) {
     MainArgs A;
     A.g = g;
     A.bd = bd;
@@ -769,7 +769,10 @@ void launch_correlate_fast(const SeriesGeom &g, const BleachDesc &bd, const Corr
     const int tiles = (g.npix + 63) / 64;
     const int threads = pd.has_ccf ? 768 : 384;
     if (ev_main_begin) (void)hipEventRecord(ev_main_begin, st);
-    hipLaunchKernelGGL(k_corr_main, dim3(tiles, A.nchunk), dim3(threads), 0, st, A);
+    if (xexp)
+        launch_corr_mfma(g, bd, dct, pd, F, X, ind, xexp, chunk_len, A.nchunk, part, gls_blk, gls_nblk, st);
+    else
+        hipLaunchKernelGGL(k_corr_main, dim3(tiles, A.nchunk), dim3(threads), 0, st, A);
     if (ev_main_end) (void)hipEventRecord(ev_main_end, st);
     for (int c0 = 3; c0 + 1 < ct.nlevels; c0 += 4) {
         RebinArgs R{};

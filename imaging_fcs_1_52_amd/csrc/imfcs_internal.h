@@ -229,13 +229,15 @@ void launch_lm_exp(int nfits, int npts, const float *data, const float *x, const
 void launch_exp_pack(const double *out64, int nslots, int nexp, double *coef, hipStream_t st);
 void launch_polyfit(const double *part, int nchunk, int nseries, int nopit, int ave, double tscale,
                     double frametime, int order, double *coef, hipStream_t st);
+// l3 (level-3 values of every slot) and cmax (per-chunk max |X|) may be null.
 void launch_apply(const SeriesGeom &g, const BleachDesc &bd, const CorrTables *dct, double *ctot, double *rec,
-                  float *X, int nchunk, int chunk_len, hipStream_t st);
+                  float *X, float *l3, float *cmax, int nchunk, int chunk_len, hipStream_t st);
 // Linear-segment bleach correction: per slot the segment means bcTrace and
 // the extrapolated start bcInt0, into coef[slot][0..nseg].
 void launch_seg_means(const SeriesGeom &g, int seg_len, int nseg, double *coef, hipStream_t st);
+// cmax / xexp (null: none): per-slot staging exponents of the matrix-core correlator.
 void launch_prefix_combine(const CorrTables *dct, int nslots, const double *ctot, const double *rec,
-                           int nchunk, int chunk_len, double *F, hipStream_t st);
+                           int nchunk, int chunk_len, double *F, const float *cmax, int32_t *xexp, hipStream_t st);
 // Work split of the blocking pass: products in chunks of 256 (levels 0..7
 // inside a chunk, levels >= 8 cascaded from the chunk totals).
 struct BlockingPlan {
@@ -257,11 +259,19 @@ void launch_correlate(const SeriesGeom &g, const BleachDesc &bd, const CorrTable
                       double *bvar, hipStream_t st);
 bool correlate_fast_supported(const CorrTables &ct);
 int corr_fast_chunk_len(const CorrTables &ct, int npairs);
+// xexp non-null: levels 0..2 on the matrix cores (k_corr_mfma; the level-3
+// values must already be in l3), else the VALU kernel k_corr_main.
 void launch_correlate_fast(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const CorrTables *dct,
                            const PairDesc &pd, const double *F, const float *X, const int32_t *ind, float *part,
                            double *hpart, float *l3, float *lvd, int chunk_len, double *corav, double *bvar,
                            hipStream_t st, hipEvent_t ev_main_begin = nullptr, hipEvent_t ev_main_end = nullptr,
-                           float *gls_blk = nullptr, int gls_nblk = 0);
+                           float *gls_blk = nullptr, int gls_nblk = 0, const int32_t *xexp = nullptr);
+// Matrix-core correlator of levels 0..2 (imfcs_corr_mfma.hip): ACF, and
+// CCF partners on the slot grid at x offset 0..4 in the same row.
+bool corr_mfma_eligible(const SeriesGeom &g, const PairDesc &pd);
+void launch_corr_mfma(const SeriesGeom &g, const BleachDesc &bd, const CorrTables *dct, const PairDesc &pd,
+                      const double *F, const float *X, const int32_t *ind, const int32_t *xexp, int chunk_len,
+                      int nchunk, float *part, float *gls, int gls_nblk, hipStream_t st);
 // FP64 polish of the converged N, D, G_inf fits of the one-component model
 // (up to 128 points): rewrites their parameters and chi-square in place.
 void launch_lm_polish(int nfits, int npts, const float *data, const float *weights, const float *x,

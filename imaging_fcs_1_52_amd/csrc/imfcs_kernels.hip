@@ -241,9 +241,15 @@ __global__ void __launch_bounds__(64 * kPolyQ) k_polyfit(const double *part, int
 // IFCS:9487-9494, 9262-9270).  One thread per (slot, frame chunk); frames
 // are read in groups of 16 so the loads of a group are in flight together.
 // ------------------------------------------------------------------
+//
+// With l3 set, the pass also writes every slot's level-3 values (sums of 8
+// stored frames, summed pairwise in FP32: ((x0+x1)+(x2+x3))+((x4+x5)+(x6+x7))),
+// l3[y][slot] for y < numbin_3, and with cmax set the largest |X| of each
+// (chunk, slot), cmax[chunk][slot] (NaN ignored).
 template <class BL, int NB>
 __global__ void __launch_bounds__(64) k_apply(SeriesGeom g, BleachDesc bd, const CorrTables *__restrict__ ctp,
-                                              int chunk_len, double *ctot, double *rec, float *X) {
+                                              int chunk_len, double *ctot, double *rec, float *X, float *l3,
+                                              float *cmax) {
     const CorrTables &ct = *ctp;
     const int k = blockIdx.x * 64 + threadIdx.x;
     const int chunk = blockIdx.y, nchunk = gridDim.y;
@@ -263,10 +269,13 @@ __global__ void __launch_bounds__(64) k_apply(SeriesGeom g, BleachDesc bd, const
     const bool bstore = (int64_t)ld * 64 < 0x7fffffff;
     // the next group's loads are issued before the current group is
     // corrected and stored (one group of loads always in flight)
+    const int nb3 = ct.numbin[3];
+    float mx = 0.f;
     uint32_t cur[16];
     if (f0 < f1) raw_words16<NB>(g, ox, oy, f0, f1 - 1, cur);
     for (int fb = f0; fb < f1; fb += 16) {
         double v[16];
+        float xv[16];
         words_to_values16(g, cur, v);
         if (fb + 16 < f1) raw_words16<NB>(g, ox, oy, fb + 16, f1 - 1, cur);   // wave-uniform
         float *xw = X + (int64_t)fb * g.nslots;   // wave-uniform
@@ -279,26 +288,44 @@ __global__ void __launch_bounds__(64) k_apply(SeriesGeom g, BleachDesc bd, const
             for (int i = 0; i < 16; i++) {
                 const double a = bl.apply(v[i], fb + i);
                 run += a;
-                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, (float)(a - mu)), rs, k * 4,
+                xv[i] = (float)(a - mu);
+                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, xv[i]), rs, k * 4,
                                                       i * (int)ld * 4, 0);
             }
-            continue;
-        }
+        } else {
 #pragma unroll
-        for (int i = 0; i < 16; i++) {
-            const int f = fb + i;
-            if (f < f1) {
-                while (next == f) {
-                    rec[(int64_t)k * npos + q] = run;
-                    q++;
-                    next = (q < npos) ? ct.pos[q] : 0x7fffffff;
+            for (int i = 0; i < 16; i++) {
+                const int f = fb + i;
+                xv[i] = 0.f;
+                if (f < f1) {
+                    while (next == f) {
+                        rec[(int64_t)k * npos + q] = run;
+                        q++;
+                        next = (q < npos) ? ct.pos[q] : 0x7fffffff;
+                    }
+                    const double a = bl.apply(v[i], f);
+                    run += a;
+                    xv[i] = (float)(a - mu);
+                    xw[(uint32_t)i * ld + (uint32_t)k] = xv[i];
                 }
-                const double a = bl.apply(v[i], f);
-                run += a;
-                xw[(uint32_t)i * ld + (uint32_t)k] = (float)(a - mu);
+            }
+        }
+        if (cmax) {
+#pragma unroll
+            for (int i = 0; i < 16; i++) mx = fmaxf(mx, fabsf(xv[i]));
+        }
+        if (l3) {
+            // frames past the series end are 0 and their level-3 values unused
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const float *x = xv + 8 * h;
+                const float s8 = ((x[0] + x[1]) + (x[2] + x[3])) + ((x[4] + x[5]) + (x[6] + x[7]));
+                const int y3 = fb / 8 + h;
+                if (y3 < nb3) l3[(int64_t)y3 * g.nslots + k] = s8;
             }
         }
     }
+    if (cmax) cmax[(int64_t)chunk * g.nslots + k] = mx;
     if (chunk == nchunk - 1)
         while (q < npos && ct.pos[q] >= f1) {
             rec[(int64_t)k * npos + q] = run;
@@ -312,13 +339,31 @@ __global__ void __launch_bounds__(64) k_apply(SeriesGeom g, BleachDesc bd, const
 // 64 at a time across the lanes, and each lane completes one position, so
 // the kernel's latency is a few round trips instead of one per chunk and
 // position.
+//
+// With xexp set, also the staging exponent of every slot for the matrix-core
+// correlator: the smallest e >= 0 with max |X| * 2^-e < 2^14 (from the
+// per-chunk maxima cmax), at most 60.
 __global__ void __launch_bounds__(256) k_prefix_combine(const CorrTables *__restrict__ ctp, int nslots,
                                                         const double *ctot, const double *rec, int nchunk,
-                                                        int chunk_len, double *F) {
+                                                        int chunk_len, double *F, const float *cmax,
+                                                        int32_t *xexp) {
     const CorrTables &ct = *ctp;
     const int s = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (s >= nslots) return;   // wave-uniform
+    if (xexp) {
+        float mx = 0.f;
+        for (int c = lane; c < nchunk; c += 64) mx = fmaxf(mx, cmax[(int64_t)c * nslots + s]);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+        int e = 0;
+        if (mx > 0.f && mx <= 3.0e38f) {
+            int ex;
+            (void)frexpf(mx, &ex);   // mx < 2^ex
+            e = min(max(ex - 14, 0), 60);
+        }
+        if (lane == 0) xexp[s] = e;
+    }
     const int npos = ct.npos;
     const double *recs = rec + (int64_t)s * npos;
     double *Fs = F + (int64_t)s * npos;
@@ -2328,43 +2373,43 @@ void launch_polyfit(const double *part, int nchunk, int nseries, int nopit, int 
 
 template <int NB>
 static void launch_apply_t(const SeriesGeom &g, const BleachDesc &bd, const CorrTables *dct, double *ctot,
-                           double *rec, float *X, int nchunk, int chunk_len, hipStream_t st) {
+                           double *rec, float *X, float *l3, float *cmax, int nchunk, int chunk_len, hipStream_t st) {
     dim3 grid(cdiv(g.nslots, 64), nchunk);
     if (bd.mode == 1) {
-        hipLaunchKernelGGL((k_apply<BleachSeg, NB>), grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec, X);
+        hipLaunchKernelGGL((k_apply<BleachSeg, NB>), grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec, X, l3, cmax);
         return;
     }
     if (bd.mode == 2) {
         hipLaunchKernelGGL((k_apply<BleachExp<1>, NB>), grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec,
-                           X);
+                           X, l3, cmax);
         return;
     }
     if (bd.mode == 3) {
         hipLaunchKernelGGL((k_apply<BleachExp<2>, NB>), grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec,
-                           X);
+                           X, l3, cmax);
         return;
     }
     switch (bleach_maxo(bd.order)) {
         case 0:
             hipLaunchKernelGGL((k_apply<BleachP<0>, NB>), grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec,
-                               X);
+                               X, l3, cmax);
             break;
         case 4:
             hipLaunchKernelGGL((k_apply<BleachP<4>, NB>), grid, dim3(64), 0, st, g, bd, dct, chunk_len, ctot, rec,
-                               X);
+                               X, l3, cmax);
             break;
         default:
             hipLaunchKernelGGL((k_apply<BleachP<kMaxOrder>, NB>), grid, dim3(64), 0, st, g, bd, dct, chunk_len,
-                               ctot, rec, X);
+                               ctot, rec, X, l3, cmax);
             break;
     }
 }
 
 void launch_apply(const SeriesGeom &g, const BleachDesc &bd, const CorrTables *dct, double *ctot, double *rec,
-                  float *X, int nchunk, int chunk_len, hipStream_t st) {
+                  float *X, float *l3, float *cmax, int nchunk, int chunk_len, hipStream_t st) {
     switch (raw_batch_rows(g)) {
-        case 2: launch_apply_t<2>(g, bd, dct, ctot, rec, X, nchunk, chunk_len, st); break;
-        default: launch_apply_t<0>(g, bd, dct, ctot, rec, X, nchunk, chunk_len, st); break;
+        case 2: launch_apply_t<2>(g, bd, dct, ctot, rec, X, l3, cmax, nchunk, chunk_len, st); break;
+        default: launch_apply_t<0>(g, bd, dct, ctot, rec, X, l3, cmax, nchunk, chunk_len, st); break;
     }
 }
 
@@ -2403,9 +2448,9 @@ void launch_seg_means(const SeriesGeom &g, int seg_len, int nseg, double *coef, 
 }
 
 void launch_prefix_combine(const CorrTables *dct, int nslots, const double *ctot, const double *rec, int nchunk,
-                           int chunk_len, double *F, hipStream_t st) {
+                           int chunk_len, double *F, const float *cmax, int32_t *xexp, hipStream_t st) {
     hipLaunchKernelGGL(k_prefix_combine, dim3(cdiv(nslots, 4)), dim3(256), 0, st, dct, nslots, ctot, rec, nchunk,
-                       chunk_len, F);
+                       chunk_len, F, cmax, xexp);
 }
 
 BlockingPlan plan_blocking(const CorrTables &ct, int npix, int threads_log2) {
