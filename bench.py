@@ -30,7 +30,9 @@ copy-calibrated FETCH_SIZE + WRITE_SIZE per launch from the committed
 profile.  ``cpu_baseline``: the oracle (a C restatement of the plugin's Java
 CPU path) on one host core over a pixel sample of the binning-1 ACF
 workload; ``cpu_baseline_all_cores`` the same sample with OpenMP over pixels
-on every host core.
+on every host core.  ``host_to_host_per_s``: the converged fits of a step
+per second with the uint16 stack's copy from pinned host memory added
+(``host_to_host_pageable_per_s``: from pageable memory).
 """
 import argparse
 import ctypes
@@ -172,6 +174,8 @@ def main():
     ap.add_argument("--solo-binning", type=int, default=0,
                     help="profiling aid: run only this binning's plain pipeline --steps times (after --warmup), alone "
                          "on the GPU, and print its per-run stage times instead of the benchmark line")
+    ap.add_argument("--enqueue-order", default=",".join(str(b) for b in ENQUEUE_ORDER),
+                    help="order in which the binnings' pipelines are enqueued (captured), e.g. 1,2,4")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also copy the stack device to device twice (16-byte vector copy, 4-byte buffer-word copy; known "
                          "bytes) for PMC traffic scaling")
@@ -188,18 +192,30 @@ def main():
     if args.solo_binning:
         solo(imfcs, stack, smin, args)
         return
-    order = [BINNINGS.index(b) for b in ENQUEUE_ORDER]
+    enq = tuple(int(x) for x in args.enqueue_order.split(","))
+    if sorted(enq) != sorted(BINNINGS):
+        raise SystemExit("--enqueue-order must list the binnings %s" % (BINNINGS,))
+    order = [BINNINGS.index(b) for b in enq]
     batch = None
     if args.runner == "graph":
-        # batch entry j runs binning ENQUEUE_ORDER[j]; engines[] are views of
-        # its contexts in BINNINGS order
+        # batch entry j runs binning enq[j]; engines[] are views of its
+        # contexts in BINNINGS order
         batch = imfcs.Batch(len(BINNINGS), graph=True, autotune=args.autotune, own_fits=True)
-        engines = [batch.engine(ENQUEUE_ORDER.index(b)) for b in BINNINGS]
+        engines = [batch.engine(enq.index(b)) for b in BINNINGS]
     else:
         engines = [imfcs.Engine(0) for _ in BINNINGS]
+    # host-to-host figures: the stack copied in from pageable memory, and
+    # from pinned memory (staged there before the timer starts)
     t0 = time.perf_counter()
     d_stack = engines[0].upload(stack)
     h2d_s = time.perf_counter() - t0
+    pinned = engines[0].pinned_array(stack.shape)
+    pinned[...] = stack
+    t0 = time.perf_counter()
+    engines[0].copy_in(d_stack, pinned)
+    h2d_pinned_s = time.perf_counter() - t0
+    engines[0].free_pinned(pinned)
+    del pinned
     cfgs = configs(smin)
     n_acf_pixels = 0
     for c in cfgs:
@@ -355,7 +371,9 @@ def main():
             "value_plain": round(value_plain, 1), "ms_per_step_plain": round(dt_plain * 1e3, 3),
             "converged_acf_fits_per_step_plain": n_conv_plain,
             "all_pixels_per_s": round(n_acf_pixels / dt_seed, 1),
-            "host_to_host_per_s": round(n_conv_seed / (dt_seed + h2d_s), 1), "h2d_s_pageable": round(h2d_s, 3),
+            "host_to_host_per_s": round(n_conv_seed / (dt_seed + h2d_pinned_s), 1),
+            "h2d_s_pinned": round(h2d_pinned_s, 4),
+            "host_to_host_pageable_per_s": round(n_conv_seed / (dt_seed + h2d_s), 1), "h2d_s_pageable": round(h2d_s, 3),
             "seeded_vs_plain": {"D_within_1e-3_frac": round(agree_frac, 5),
                                 "lm_stage_speedup": round(fit_ms_plain / fit_ms_seed, 3),
                                 "pipeline_speedup": round(dt_plain / dt_seed, 3)},

@@ -274,12 +274,16 @@ class Engine:
         self._stack = None
         self._stack_bytes = 0
         self._chanum_last = 0
+        self._pinned = []
 
     def close(self):
         if self.ctx:
             if self._stack:
                 self._lib.imfcs_device_free(self.ctx, self._stack)
                 self._stack = None
+            for p in self._pinned:
+                self._lib.imfcs_host_free(self.ctx, ctypes.c_void_p(p))
+            self._pinned = []
             if self._owned:
                 self._lib.imfcs_destroy(self.ctx)
             self.ctx = None
@@ -312,6 +316,23 @@ class Engine:
 
     def free_buffer(self, d_ptr):
         _check(self._lib.imfcs_device_free(self.ctx, d_ptr))
+
+    def pinned_array(self, shape, dtype=np.uint16):
+        """A numpy array in pinned host memory (imfcs_host_alloc); the memory
+        is freed with free_pinned(array) or when the engine closes."""
+        nbytes = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        p = self._lib.imfcs_host_alloc(self.ctx, ctypes.c_size_t(nbytes))
+        if not p:
+            raise MemoryError(get_last_error())
+        self._pinned.append(p)
+        buf = (ctypes.c_char * nbytes).from_address(p)
+        return np.frombuffer(buf, dtype=dtype).reshape(shape)
+
+    def free_pinned(self, arr):
+        p = arr.ctypes.data
+        if p in self._pinned:
+            self._pinned.remove(p)
+            _check(self._lib.imfcs_host_free(self.ctx, ctypes.c_void_p(p)))
 
     def copy_in(self, d_ptr, arr):
         a = np.ascontiguousarray(arr)

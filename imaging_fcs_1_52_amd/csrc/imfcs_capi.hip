@@ -456,6 +456,23 @@ int imfcs_device_free(imfcs_ctx *ctx, void *ptr) {
     return 0;
 }
 
+void *imfcs_host_alloc(imfcs_ctx *ctx, size_t bytes) {
+    void *p = nullptr;
+    if (!ctx) return nullptr;
+    (void)hipSetDevice(ctx->device);
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+        fail(IMFCS_ERR_MEMORY, "hipHostMalloc failed");
+        return nullptr;
+    }
+    return p;
+}
+
+int imfcs_host_free(imfcs_ctx *ctx, void *ptr) {
+    if (!ctx) return fail(IMFCS_ERR_ARG, "null context");
+    HIP_TRY(hipHostFree(ptr));
+    return 0;
+}
+
 int imfcs_memcpy_h2d(imfcs_ctx *ctx, void *dst, const void *src, size_t bytes) {
     if (!ctx) return fail(IMFCS_ERR_ARG, "null context");
     HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));

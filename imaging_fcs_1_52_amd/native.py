@@ -79,6 +79,8 @@ SIGNATURES = {
     "imfcs_destroy": (None, [_VP]),
     "imfcs_device_alloc": (_VP, [_VP, _SZ]),
     "imfcs_device_free": (ctypes.c_int, [_VP, _VP]),
+    "imfcs_host_alloc": (_VP, [_VP, _SZ]),
+    "imfcs_host_free": (ctypes.c_int, [_VP, _VP]),
     "imfcs_memcpy_h2d": (ctypes.c_int, [_VP, _VP, _VP, _SZ]),
     "imfcs_memcpy_d2h": (ctypes.c_int, [_VP, _VP, _VP, _SZ]),
     "imfcs_memcpy_d2d": (ctypes.c_int, [_VP, _VP, _VP, _SZ]),

@@ -211,6 +211,11 @@ void imfcs_destroy(imfcs_ctx *ctx);
 /* Device buffer helpers (so hosts need no other GPU runtime). */
 void *imfcs_device_alloc(imfcs_ctx *ctx, size_t bytes);
 int imfcs_device_free(imfcs_ctx *ctx, void *ptr);
+/* Pinned (page-locked) host memory, so that imfcs_memcpy_h2d of a stack
+ * staged there runs at the link's full rate.  No counterpart in the
+ * reference (its JNI calls copy Java arrays through pageable memory). */
+void *imfcs_host_alloc(imfcs_ctx *ctx, size_t bytes);
+int imfcs_host_free(imfcs_ctx *ctx, void *ptr);
 int imfcs_memcpy_h2d(imfcs_ctx *ctx, void *dst, const void *src, size_t bytes);
 int imfcs_memcpy_d2h(imfcs_ctx *ctx, void *dst, const void *src, size_t bytes);
 /* Device-to-device copy by a plain vector-copy kernel (also the known-size

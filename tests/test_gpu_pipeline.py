@@ -110,3 +110,23 @@ def test_ccf_fits_take_the_ccf_models_free_parameters(engine):
     ok = mixed["states"][1] == 0
     assert ok.any()
     np.testing.assert_allclose(mixed["fitres"][1][..., 1][ok], 0.8e-12, rtol=1e-6)
+
+
+def test_pinned_host_buffer_round_trip(engine):
+    """imfcs_host_alloc: a stack staged in pinned host memory reaches the
+    device unchanged, and a run on it equals a run on the pageable upload."""
+    stack = imfcs.simulate_2d(10, 8, 4096, 30, seed=7)
+    T, H, W = stack.shape
+    pin = engine.pinned_array(stack.shape)
+    pin[...] = stack
+    d = engine.upload(stack)
+    cfg = imfcs.make_pipeline_config(W, H, T, background=int(stack.min()), poly_order=4, do_fit=False)
+    engine.run(cfg, d)
+    ref = engine.results(cfg)["acf"].copy()
+    d_pin = engine.device_buffer(stack.nbytes)
+    engine.copy_in(d_pin, pin)
+    assert np.array_equal(engine.download(d_pin, stack.shape), stack)
+    engine.run(cfg, d_pin)
+    assert np.array_equal(engine.results(cfg)["acf"], ref, equal_nan=True)
+    engine.free_buffer(d_pin)
+    engine.free_pinned(pin)
