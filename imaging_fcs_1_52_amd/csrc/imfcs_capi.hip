@@ -727,9 +727,10 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTabl
     const Tune tn = ctx->force_tune.apply_log2 > 0 ? ctx->force_tune : tune_lookup(ns, g.T);
     ctx->used_tune = tn;
     // window means
-    // chunk length: about 2^apply_log2 threads in flight, 256..4096 frames each
+    // chunk length: about 2^apply_log2 threads in flight, 64..4096 frames
+    // each (few slots, e.g. binning 4 of a 128 x 128 image: short chunks)
     int plen = 4096;
-    while (plen > 256 && (int64_t)ns * ((g.T + plen - 1) / plen) < ((int64_t)1 << tn.apply_log2)) plen >>= 1;
+    while (plen > 64 && (int64_t)ns * ((g.T + plen - 1) / plen) < ((int64_t)1 << tn.apply_log2)) plen >>= 1;
     const int npc = (g.T + plen - 1) / plen;
     HIP_TRY(ctx->ctot.ensure(sizeof(double) * (size_t)npc * ns));
     HIP_TRY(ctx->rec.ensure(sizeof(double) * (size_t)ns * ct.npos));
