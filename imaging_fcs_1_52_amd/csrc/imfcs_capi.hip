@@ -109,7 +109,7 @@ struct HostTables {
 };
 
 int make_host_tables(int p, int q, int num, double frametime, HostTables &t) {
-    if (p < 4 || p % 4 != 0 || q < 1) return fail(IMFCS_ERR_ARG, "correlator p must be a multiple of 4, q >= 1");
+    if (p < 4 || p % 2 != 0 || q < 1) return fail(IMFCS_ERR_ARG, "correlator p must be even and >= 4, q >= 1");
     int base = p, hbase = p / 2, lagnum = q;
     t.chanum = p + (q - 1) * p / 2 + 1;
     if (t.chanum > kMaxChan) return fail(IMFCS_ERR_ARG, "too many correlator channels");
@@ -159,14 +159,21 @@ int make_corr_tables(const HostTables &h, int num, int q, CorrTables &ct) {
         if (ct.first_chan[c] < 0) ct.first_chan[c] = x;
         ct.nchan[c]++;
     }
-    // the kernels are specialised for p = 16: level 0 holds delays 0..16 and
-    // every other level delays 9..16
+    // every level holds a contiguous channel range with ascending delays
+    // (p = 2 mod 4 repeats one level-0 delay, IFCS:7398-7408 with the
+    // integer base / 4); the fast kernels implement the p = 16 layout, level
+    // 0 delays 0..16 and every other level delays 9..16
+    ct.p16 = 1;
     for (int c = 0; c < nlev; c++) {
-        int nexp = (c == 0) ? 17 : 8, dmin = (c == 0) ? 0 : 9;
-        if (ct.nchan[c] != nexp) return fail(IMFCS_ERR_ARG, "only correlator p = 16 is supported");
-        for (int i = 0; i < nexp; i++)
-            if (ct.del[ct.first_chan[c] + i] != dmin + i) return fail(IMFCS_ERR_ARG, "unexpected lag table");
-        if (ct.numbin[c] - 16 < 2) return fail(IMFCS_ERR_ARG, "not enough frames for the correlator");
+        const int c0 = ct.first_chan[c], n = ct.nchan[c];
+        if (n <= 0) return fail(IMFCS_ERR_ARG, "correlator level without channels");
+        for (int i = 0; i < n; i++) {
+            if (ct.level[c0 + i] != c) return fail(IMFCS_ERR_ARG, "correlator channels out of level order");
+            if (i > 0 && ct.del[c0 + i] < ct.del[c0 + i - 1]) return fail(IMFCS_ERR_ARG, "unexpected lag table");
+        }
+        if (ct.numbin[c] - ct.del[c0 + n - 1] < 2) return fail(IMFCS_ERR_ARG, "not enough frames for the correlator");
+        const int nexp = (c == 0) ? 17 : 8, dmin = (c == 0) ? 0 : 9;
+        if (n != nexp || ct.del[c0] != dmin || ct.del[c0 + n - 1] != dmin + n - 1) ct.p16 = 0;
     }
     ct.mtab_last = h.mtab[h.chanum - 1];
     ct.log2_samp_last = std::log(h.samp[h.chanum - 1]) / std::log(2.0);
@@ -201,16 +208,20 @@ int make_corr_tables(const HostTables &h, int num, int q, CorrTables &ct) {
     return 0;
 }
 
-// The correlator kernels are specialised for the plugin's default channel
-// scheme p = 16 (17 delays in the first group, 8 in every further group,
-// IFCS:7391-7424); q is the number of groups (kMaxLevel at most).  Any
-// other p is refused before work is enqueued.
+// The channel scheme of setParameters (IFCS:7391-7424): p + 1 delays in the
+// first group, p / 2 in each of the q - 1 further groups.  The fast kernels
+// implement the plugin's default p = 16; every other even p runs the
+// table-driven correlator k_correlate.  Odd p leaves channels of Java's
+// tables unset (chanum counts p / 2 as a fraction) and is refused, as is a
+// layout of more than kMaxChan channels, before work is enqueued.
 int check_correlator_layout(int p, int q) {
-    if (p != 16)
-        return fail(IMFCS_ERR_ARG, "correlator p = " + std::to_string(p) +
-                                       " is not supported: the kernels implement the p = 16 channel scheme");
+    if (p < 4 || p % 2 != 0)
+        return fail(IMFCS_ERR_ARG, "correlator p = " + std::to_string(p) + " is not supported: p must be even and >= 4");
     if (q < 1 || q > kMaxLevel)
         return fail(IMFCS_ERR_ARG, "correlator q = " + std::to_string(q) + " outside 1.." + std::to_string(kMaxLevel));
+    if (p + (int64_t)(q - 1) * p / 2 + 1 > kMaxChan)
+        return fail(IMFCS_ERR_ARG, "correlator p = " + std::to_string(p) + ", q = " + std::to_string(q) + " gives more than " +
+                                       std::to_string(kMaxChan) + " channels");
     return 0;
 }
 

@@ -729,7 +729,8 @@ bool correlate_fast_supported(const CorrTables &ct) {
     // level layout of p = 16 with at least levels 0..3 and blocks >= 16
     // values on level 2 (ind >= q - 1 >= 6); blocks never longer than the
     // 2^(blocknum-1)-frame chunks (ind <= blocknum - 1 needs blocknum >= q)
-    return ct.nlevels >= 7 && ct.nlevels <= 12 && ct.q == ct.nlevels && ct.numbin[3] >= 32 && ct.blocknum >= ct.q;
+    return ct.p16 && ct.nlevels >= 7 && ct.nlevels <= 12 && ct.q == ct.nlevels && ct.numbin[3] >= 32 &&
+           ct.blocknum >= ct.q;
 }
 
 int corr_fast_chunk_len(const CorrTables &ct, int npairs) {

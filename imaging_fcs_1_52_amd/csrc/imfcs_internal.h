@@ -17,7 +17,7 @@ int ctx_device(imfcs_ctx *ctx);
 // returns code.
 int set_error(int code, const std::string &msg);
 
-constexpr int kMaxChan = 128;   // correlator channels (73 for p=16, q=8)
+constexpr int kMaxChan = 320;   // correlator channels (73 for p=16, q=8; 289 for p=64, q=8)
 constexpr int kMaxLevel = 12;   // multi-tau levels (q)
 // zero rows stored after each level >= 3 series array of the correlator, so
 // that its readers load past the last value without a clamp
@@ -103,6 +103,9 @@ struct CorrTables {
     int16_t pos_b_beg[kMaxChan];  // index of lag
     int16_t pos_b_end[kMaxChan];  // index of 2^c numbin_c
     int16_t pos_blk_a, pos_blk_b, pos_total;
+    // 1: the p = 16 layout (level 0 delays 0..16, every other level 9..16)
+    // that the fast correlator kernels implement
+    int32_t p16;
 };
 
 // Pair p (< npairs) correlates series sa(p) with series sb(p).

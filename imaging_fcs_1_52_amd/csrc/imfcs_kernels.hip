@@ -747,11 +747,17 @@ struct LevelCtx {
     __device__ __forceinline__ double val_b(int y) const { return val(kb, y); }
 };
 
-template <int G, int NL, int DMIN>
-__device__ void correlate_level(const LevelCtx &L, const CorrTables &ct, int chan0, const double *alpha,
-                                const double *beta, int B, int pnum, double *cross, double *a1, double *a2) {
-    constexpr int WN = G + 16;
-    double w[WN];   // b~ values [y0, y0 + WN)
+// One run of nl <= NL channels of a level with the consecutive delays
+// d0 .. d0 + nl - 1: a register window of b~ values [y0 + d0, y0 + d0 + WN)
+// slides by G.  Channels i >= nl are computed and dropped.  Products past
+// the series end (y + d >= numbin) do not exist; the block sums Sa, Sb run
+// over the existing products, which for every block j < pnum are all of its
+// B products.
+template <int G, int NL>
+__device__ void correlate_level(const LevelCtx &L, int d0, const double *alpha, const double *beta, int B, int pnum,
+                                double *cross, double *a1, double *a2) {
+    constexpr int WN = G + NL - 1;
+    double w[WN];   // b~ values [y0 + d0, y0 + d0 + WN)
     double ga[G];   // a~ values [y0, y0 + G)
     double sab[NL], sb[NL], sa = 0.0;
 #pragma unroll
@@ -760,12 +766,13 @@ __device__ void correlate_level(const LevelCtx &L, const CorrTables &ct, int cha
         cross[i] = a1[i] = a2[i] = 0.0;
     }
 #pragma unroll
-    for (int k = 0; k < WN; k++) w[k] = L.val_b(k);
+    for (int k = 0; k < WN; k++) w[k] = L.val_b(d0 + k);
     const int numbin = L.numbin;
+    const bool reuse = L.same && d0 == 0;   // the a window is the b window
     for (int y0 = 0; y0 < numbin; y0 += G) {
 #pragma unroll
-        for (int k = 0; k < G; k++) ga[k] = L.same ? w[k] : L.val_a(y0 + k);
-        if (y0 + WN <= numbin) {
+        for (int k = 0; k < G; k++) ga[k] = reuse ? w[k] : L.val_a(y0 + k);
+        if (y0 + d0 + WN <= numbin) {
             double sg = 0.0;
 #pragma unroll
             for (int k = 0; k < G; k++) sg += ga[k];
@@ -775,8 +782,8 @@ __device__ void correlate_level(const LevelCtx &L, const CorrTables &ct, int cha
                 double s = 0.0, t = 0.0;
 #pragma unroll
                 for (int k = 0; k < G; k++) {
-                    s = fma(ga[k], w[k + DMIN + i], s);
-                    t += w[k + DMIN + i];
+                    s = fma(ga[k], w[k + i], s);
+                    t += w[k + i];
                 }
                 sab[i] += s;
                 sb[i] += t;
@@ -784,12 +791,19 @@ __device__ void correlate_level(const LevelCtx &L, const CorrTables &ct, int cha
         } else {
             // series tail: only products with y + d < numbin exist
 #pragma unroll
+            for (int k = 0; k < G; k++)
+                if (y0 + k < numbin) sa += ga[k];
+#pragma unroll
             for (int i = 0; i < NL; i++) {
-                double s = 0.0;
+                double s = 0.0, t = 0.0;
 #pragma unroll
                 for (int k = 0; k < G; k++)
-                    if (y0 + k + DMIN + i < numbin) s = fma(ga[k], w[k + DMIN + i], s);
+                    if (y0 + k + d0 + i < numbin) {
+                        s = fma(ga[k], w[k + i], s);
+                        t += w[k + i];
+                    }
                 sab[i] += s;
+                sb[i] += t;
             }
         }
         int yend = y0 + G;
@@ -811,14 +825,30 @@ __device__ void correlate_level(const LevelCtx &L, const CorrTables &ct, int cha
         }
         // slide the window by G
 #pragma unroll
-        for (int k = 0; k < 16; k++) w[k] = w[k + G];
+        for (int k = 0; k < NL - 1; k++) w[k] = w[k + G];
 #pragma unroll
-        for (int k = 0; k < G; k++) w[16 + k] = L.val_b(y0 + WN + k);
+        for (int k = 0; k < G; k++) w[NL - 1 + k] = L.val_b(y0 + d0 + WN + k);
     }
 #pragma unroll
     for (int i = 0; i < NL; i++) cross[i] += sab[i];
 }
 
+template <int NL>
+__device__ void correlate_run(const LevelCtx &L, int G, int d0, const double *alpha, const double *beta, int B,
+                              int pnum, double *cross, double *a1, double *a2) {
+    switch (G) {
+        case 16: correlate_level<16, NL>(L, d0, alpha, beta, B, pnum, cross, a1, a2); break;
+        case 8: correlate_level<8, NL>(L, d0, alpha, beta, B, pnum, cross, a1, a2); break;
+        case 4: correlate_level<4, NL>(L, d0, alpha, beta, B, pnum, cross, a1, a2); break;
+        case 2: correlate_level<2, NL>(L, d0, alpha, beta, B, pnum, cross, a1, a2); break;
+        default: correlate_level<1, NL>(L, d0, alpha, beta, B, pnum, cross, a1, a2); break;
+    }
+}
+
+// One thread per (pair, level) for every channel layout of setParameters:
+// the level's channels are taken in runs of at most 17 consecutive delays
+// (p = 16: delays 0..16 on level 0, 9..16 on the others; a repeated delay
+// starts a new run), each run one pass over the level's values.
 __global__ void __launch_bounds__(64) k_correlate(SeriesGeom g, BleachDesc bd, const CorrTables *__restrict__ ctp,
                                                   PairDesc pd, const double *F, const float *X, const int32_t *ind,
                                                   double *corav, double *bvar) {
@@ -840,42 +870,48 @@ __global__ void __launch_bounds__(64) k_correlate(SeriesGeom g, BleachDesc bd, c
     const double shift_a = slot_mu(bd.coef, bd.order, L.ka) * (double)(1 << c);
     const double shift_b = slot_mu(bd.coef, bd.order, L.kb) * (double)(1 << c);
     const int chan0 = ct.first_chan[c], nch = ct.nchan[c];
-    double am[17], bmn[17], alpha[17], beta[17];
-    for (int i = 0; i < nch; i++) {
-        int x = chan0 + i;
-        double n = (double)(ct.numbin[c] - ct.del[x]);
-        am[i] = Fa[ct.pos_a_end[x]] / n;
-        bmn[i] = (Fb[ct.pos_b_end[x]] - Fb[ct.pos_b_beg[x]]) / n;
-        alpha[i] = am[i] - shift_a;
-        beta[i] = bmn[i] - shift_b;
-    }
-    int indp = ind[p];
-    int B = 1 << (indp - c);
-    int pnum = (int)floor(ct.mtab_last / pow(2.0, fmax(indp - ct.log2_samp_last, 0.0)));
-    double cross[17], a1[17], a2[17];
-    if (c == 0) {
-        correlate_level<16, 17, 0>(L, ct, chan0, alpha, beta, B, pnum, cross, a1, a2);
-    } else {
-        int G = 16;
-        int bmin = 1 << max(ct.q - 1 - c, 0);
-        if (bmin < G) G = bmin;
-        switch (G) {
-            case 16: correlate_level<16, 8, 9>(L, ct, chan0, alpha, beta, B, pnum, cross, a1, a2); break;
-            case 8: correlate_level<8, 8, 9>(L, ct, chan0, alpha, beta, B, pnum, cross, a1, a2); break;
-            case 4: correlate_level<4, 8, 9>(L, ct, chan0, alpha, beta, B, pnum, cross, a1, a2); break;
-            case 2: correlate_level<2, 8, 9>(L, ct, chan0, alpha, beta, B, pnum, cross, a1, a2); break;
-            default: correlate_level<1, 8, 9>(L, ct, chan0, alpha, beta, B, pnum, cross, a1, a2); break;
+    const int indp = ind[p];
+    const int B = 1 << (indp - c);
+    const int pnum = (int)floor(ct.mtab_last / pow(2.0, fmax(indp - ct.log2_samp_last, 0.0)));
+    // products per group: 16, or fewer on levels whose blocks are shorter
+    // (B >= 2^(q-1-c) since ind >= q - 1)
+    int G = 16;
+    const int bmin = 1 << max(ct.q - 1 - c, 0);
+    if (bmin < G) G = bmin;
+    for (int r0 = 0; r0 < nch;) {
+        int nl = 1;
+        while (r0 + nl < nch && nl < 17 && ct.del[chan0 + r0 + nl] == ct.del[chan0 + r0] + nl) nl++;
+        const int d0 = ct.del[chan0 + r0];
+        double am[17], bmn[17], alpha[17], beta[17];
+#pragma unroll
+        for (int i = 0; i < 17; i++) {
+            am[i] = bmn[i] = 1.0;
+            alpha[i] = beta[i] = 0.0;
         }
-    }
-    for (int i = 0; i < nch; i++) {
-        int x = chan0 + i;
-        double n = (double)(ct.numbin[c] - ct.del[x]);
-        double tot = cross[i] - n * alpha[i] * beta[i];
-        double ab = am[i] * bmn[i];
-        corav[(int64_t)p * ct.chanum + x] = tot / (n * am[i] * bmn[i]);
-        double sp = a1[i] / B, sp2 = a2[i] / ((double)B * B);
-        double pn = (double)pnum;
-        bvar[(int64_t)p * ct.chanum + x] = (sp2 / pn - (sp / pn) * (sp / pn)) / ((pn - 1) * (ab * ab));
+        for (int i = 0; i < nl; i++) {
+            const int x = chan0 + r0 + i;
+            const double n = (double)(ct.numbin[c] - ct.del[x]);
+            am[i] = Fa[ct.pos_a_end[x]] / n;
+            bmn[i] = (Fb[ct.pos_b_end[x]] - Fb[ct.pos_b_beg[x]]) / n;
+            alpha[i] = am[i] - shift_a;
+            beta[i] = bmn[i] - shift_b;
+        }
+        double cross[17], a1[17], a2[17];
+        if (nl <= 8)
+            correlate_run<8>(L, G, d0, alpha, beta, B, pnum, cross, a1, a2);
+        else
+            correlate_run<17>(L, G, d0, alpha, beta, B, pnum, cross, a1, a2);
+        for (int i = 0; i < nl; i++) {
+            const int x = chan0 + r0 + i;
+            const double n = (double)(ct.numbin[c] - ct.del[x]);
+            const double tot = cross[i] - n * alpha[i] * beta[i];
+            const double ab = am[i] * bmn[i];
+            corav[(int64_t)p * ct.chanum + x] = tot / (n * am[i] * bmn[i]);
+            const double sp = a1[i] / B, sp2 = a2[i] / ((double)B * B);
+            const double pn = (double)pnum;
+            bvar[(int64_t)p * ct.chanum + x] = (sp2 / pn - (sp / pn) * (sp / pn)) / ((pn - 1) * (ab * ab));
+        }
+        r0 += nl;
     }
 }
 
@@ -1755,12 +1791,14 @@ __device__ double fcs1_f64(const LMModelGeom &m, double x, double N, double D, d
 }
 
 // The same LM fit of the FCS model with NF fixed free slots for curves of
-// 65..128 points and an FP64 chi-square: two wavefronts per fit, one point
-// per lane, so one model evaluation per iteration instead of two passes of
-// one wave (an iteration of a lone long-running fit is latency-bound).  The
+// more than 64 points and an FP64 chi-square: two wavefronts per fit, one
+// point per lane (up to 128 points, the p = 16 layouts up to q = 15; the
+// lanes loop over longer curves), so one model evaluation per iteration
+// instead of two passes of one wave (an iteration of a lone long-running
+// fit is latency-bound).  The
 // per-wave sums meet in LDS (double-buffered, one barrier per evaluation);
 // both waves then take identical LM steps from identical sums.
-template <int NF>
+template <int NF, bool LONG = false>
 __device__ void lm_fcs_2w_fit(const LMArgs &a, int fit) {
     constexpr int NP = 20, NH = NF * (NF + 1) / 2, NS = NF + NH;
     // per buffer and wave: the FP64 chi-square and the FP32 gradient and
@@ -1789,21 +1827,36 @@ __device__ void lm_fcs_2w_fit(const LMArgs &a, int fit) {
         float sv[NS];
 #pragma unroll
         for (int k = 0; k < NS; k++) sv[k] = 0.f;
-        if (act) {
+        // a point's terms: set (the lane's first point) or added (the
+        // points past 128 of a LONG curve)
+        auto point = [&](float xx, float yy, double ww, float wq, auto add) {
+            constexpr bool ADD = decltype(add)::value;
             float gfull[11];
-            const float f = fcs3p(ac.geom, xi, p, true, c2on, c3on, gfull);
-            const double r = (double)yi - (double)f;   // exact
-            cx = wi * r * r;
-            const float wr = wf * (float)r;
+            const float f = fcs3p(ac.geom, xx, p, true, c2on, c3on, gfull);
+            const double r = (double)yy - (double)f;   // exact
+            if constexpr (ADD) cx += ww * r * r;
+            else cx = ww * r * r;
+            const float wr = wq * (float)r;
             int hh = NF;
 #pragma unroll
             for (int q = 0; q < NF; q++) {
                 const float jq = gfull[FixedSlots<kModelFCS, NF>::slot(q)];
-                sv[q] = wr * jq;
+                if constexpr (ADD) sv[q] += wr * jq;
+                else sv[q] = wr * jq;
 #pragma unroll
-                for (int l = 0; l <= q; l++, hh++) sv[hh] = wf * jq * gfull[FixedSlots<kModelFCS, NF>::slot(l)];
+                for (int l = 0; l <= q; l++, hh++) {
+                    const float e = wq * jq * gfull[FixedSlots<kModelFCS, NF>::slot(l)];
+                    if constexpr (ADD) sv[hh] += e;
+                    else sv[hh] = e;
+                }
             }
-        }
+        };
+        if (act) point(xi, yi, wi, wf, std::false_type{});
+        if constexpr (LONG)
+            for (int i = tid + 128; i < a.npts; i += 128) {
+                const double ww = a.weights ? (double)a.weights[(int64_t)fit * a.npts + i] : 1.0;
+                point(a.x[i], a.data[(int64_t)fit * a.npts + i], ww, (float)ww, std::true_type{});
+            }
         cx = wave_sum(cx);
 #pragma unroll
         for (int k = 0; k < NS; k++) sv[k] = wave_sum_f32(sv[k]);
@@ -1844,9 +1897,11 @@ __device__ void lm_fcs_2w_fit(const LMArgs &a, int fit) {
 // second-order term -sum w r d2f, whose N-N and N-D entries are analytic
 // and whose D-D entry comes from a forward difference of the analytic D
 // derivative (step 1e-5 D), taken once at the start point.  Two waves, one
-// point per lane (up to 128 points); a separate launch, so that the FP64
-// model's registers do not lower the LM kernel's occupancy.  Reads and
-// rewrites the fit's output row and chi-square.
+// point per lane (points past 128, layouts other than p = 16, q = 8, loop
+// over the lanes again); a separate launch, so that the FP64 model's
+// registers do not lower the LM kernel's occupancy.  Reads and rewrites the
+// fit's output row and chi-square.
+template <bool LONG>
 __device__ void lm_polish_fit(const LMArgs &a, int fit) {
     __shared__ double red64[2][2][13];
     const int tid = threadIdx.x, wv = tid >> 6;
@@ -1871,27 +1926,39 @@ __device__ void lm_polish_fit(const LMArgs &a, int fit) {
         double v[13];
 #pragma unroll
         for (int k = 0; k < 13; k++) v[k] = 0.0;
-        if (act) {
+        // a point's terms: set (the lane's first point) or added (the
+        // points past 128 of a LONG curve)
+        auto point = [&](double xx, double yy, double ww, auto add) {
+            constexpr bool ADD = decltype(add)::value;
+            auto put = [&](int k, double e) {
+                if constexpr (ADD) v[k] += e;
+                else v[k] = e;
+            };
             double jq[3], acf;
-            const double r = y64 - fcs1_f64(geom, x64, p[0], p[1], vx, vy, p[2], jq, acf);
-            v[0] = wi * r * r;
+            const double r = yy - fcs1_f64(geom, xx, p[0], p[1], vx, vy, p[2], jq, acf);
+            put(0, ww * r * r);
             int hh = 4;
 #pragma unroll
             for (int q = 0; q < 3; q++) {
-                v[1 + q] = wi * r * jq[q];
+                put(1 + q, ww * r * jq[q]);
 #pragma unroll
-                for (int l = 0; l <= q; l++, hh++) v[hh] = wi * jq[q] * jq[l];
+                for (int l = 0; l <= q; l++, hh++) put(hh, ww * jq[q] * jq[l]);
             }
             if (second) {
                 const double hD = 1e-5 * p[1], iN = 1.0 / p[0];
                 double jh[3], acfh;
-                (void)fcs1_f64(geom, x64, 1.0, p[1] + hD, vx, vy, 0.0, jh, acfh);
+                (void)fcs1_f64(geom, xx, 1.0, p[1] + hD, vx, vy, 0.0, jh, acfh);
                 const double aD = jq[1] * p[0];   // d acf / dD
-                v[10] = wi * r * (2.0 * acf * iN * iN * iN);
-                v[11] = wi * r * (-aD * iN * iN);
-                v[12] = wi * r * ((jh[1] - aD) / hD * iN);
+                put(10, ww * r * (2.0 * acf * iN * iN * iN));
+                put(11, ww * r * (-aD * iN * iN));
+                put(12, ww * r * ((jh[1] - aD) / hD * iN));
             }
-        }
+        };
+        if (act) point(x64, y64, wi, std::false_type{});
+        if constexpr (LONG)
+            for (int i = tid + 128; i < a.npts; i += 128)
+                point((double)a.x[i], (double)a.data[(int64_t)fit * a.npts + i],
+                      a.weights ? (double)a.weights[(int64_t)fit * a.npts + i] : 1.0, std::true_type{});
         const int nv = second ? 13 : 10;   // uniform
         for (int k = 0; k < nv; k++) v[k] = wave_sum(v[k]);
         if ((tid & 63) == 0)
@@ -1971,9 +2038,10 @@ __device__ void lm_polish_fit(const LMArgs &a, int fit) {
     }
 }
 
+template <bool LONG>
 __global__ void __launch_bounds__(128) k_lm_polish(LMArgs a) {
     const int fit = blockIdx.x;
-    if (fit < a.nfits) lm_polish_fit(a, fit);
+    if (fit < a.nfits) lm_polish_fit<LONG>(a, fit);
 }
 
 // The polish over the fits of many groups (a batch's persistent fit
@@ -1991,16 +2059,19 @@ __global__ void __launch_bounds__(128) k_lm_polish_groups(const LMGroup *groups,
     const LMGroup &G = groups[lo];
     LMArgs a{G.nfits, G.npts, 20, kModelFCS, 0, G.data, G.weights, G.x, G.init, G.pfit, LMModelGeom{}, G.tol,
              G.maxit, 1, G.out_params, G.states, G.chi2, G.iters, G.init_two_rows, G.init_split, 0};
-    lm_polish_fit(a, (int)(idx - prefix[lo]));
+    if (a.npts > 128)   // uniform per workgroup
+        lm_polish_fit<true>(a, (int)(idx - prefix[lo]));
+    else
+        lm_polish_fit<false>(a, (int)(idx - prefix[lo]));
 }
 
-template <int NF>
+template <int NF, bool LONG>
 __global__ void __launch_bounds__(128) k_lm_fcs_2w(LMArgs a) {
     // last fit first: the CCF fits (the second half), whose few longest
     // fits set the kernel's duration, are dispatched before the ACF fits
     const int fit = a.nfits - 1 - (int)blockIdx.x;
     if (fit < 0) return;
-    lm_fcs_2w_fit<NF>(a, fit);
+    lm_fcs_2w_fit<NF, LONG>(a, fit);
 }
 
 // One persistent launch over the fits of many pipelines (a batch of
@@ -2168,12 +2239,13 @@ __global__ void k_bleach_average_f32(const float *in, float *out, int wt, int ht
 
 __global__ void k_pack_fit_inputs(int chanum, int npairs, const double *corav, const double *bvar, float *data,
                                   float *weights) {
-    int p = blockIdx.x;
-    int z = threadIdx.x;
-    if (p >= npairs || z >= chanum - 1) return;
-    int64_t src = (int64_t)p * chanum + z + 1, dst = (int64_t)p * (chanum - 1) + z;
-    data[dst] = (float)corav[src];
-    weights[dst] = 1.0f / (float)bvar[src];
+    const int p = blockIdx.x;
+    if (p >= npairs) return;
+    for (int z = threadIdx.x; z < chanum - 1; z += blockDim.x) {
+        const int64_t src = (int64_t)p * chanum + z + 1, dst = (int64_t)p * (chanum - 1) + z;
+        data[dst] = (float)corav[src];
+        weights[dst] = 1.0f / (float)bvar[src];
+    }
 }
 
 // N&B of the drop-in calcACF (isNBcalculation, agpufitjni.cu:5167-5179):
@@ -2498,8 +2570,10 @@ void launch_correlate(const SeriesGeom &g, const BleachDesc &bd, const CorrTable
 static void launch_lm(const LMArgs &a, unsigned free_mask, hipStream_t st) {
     if (a.nfits <= 0) return;
     if (a.kind == kModelFCS && free_mask == 0x13u) {   // N, D, G_inf
-        if (a.chi2_fp64 && a.npts > 64 && a.npts <= 128)
-            hipLaunchKernelGGL((k_lm_fcs_2w<3>), dim3(a.nfits), dim3(128), 0, st, a);
+        if (a.chi2_fp64 && a.npts > 128)
+            hipLaunchKernelGGL((k_lm_fcs_2w<3, true>), dim3(a.nfits), dim3(128), 0, st, a);
+        else if (a.chi2_fp64 && a.npts > 64)
+            hipLaunchKernelGGL((k_lm_fcs_2w<3, false>), dim3(a.nfits), dim3(128), 0, st, a);
         else
             hipLaunchKernelGGL((k_lm_fixed<kModelFCS, 3>), dim3(a.nfits), dim3(64), 0, st, a);
         return;
@@ -2544,10 +2618,13 @@ void launch_lm_acf(int nfits, int npts, const float *data, const float *weights,
 
 void launch_lm_polish(int nfits, int npts, const float *data, const float *weights, const float *x,
                       float *out_params, int *states, float *chi2, hipStream_t st) {
-    if (nfits <= 0 || npts <= 0 || npts > 128) return;
+    if (nfits <= 0 || npts <= 0) return;
     LMArgs a{nfits, npts, 20, kModelFCS, 0, data, weights, x, nullptr, nullptr, LMModelGeom{}, 0.f, 0, 1, out_params,
              states, chi2, nullptr, 0, 0, 0};
-    hipLaunchKernelGGL(k_lm_polish, dim3(nfits), dim3(128), 0, st, a);
+    if (npts > 128)
+        hipLaunchKernelGGL(k_lm_polish<true>, dim3(nfits), dim3(128), 0, st, a);
+    else
+        hipLaunchKernelGGL(k_lm_polish<false>, dim3(nfits), dim3(128), 0, st, a);
 }
 
 void launch_lm_polish_groups(const LMGroup *groups, const int64_t *prefix, int ngroups, int64_t total,

@@ -19,20 +19,20 @@ def models(binning, cf):
 
 
 def run_gpu(engine, stack, binning=1, cf=(0, 0), poly_order=4, do_fit=True, tol=1e-16, maxit=2000,
-            background=None, do_acf=True):
+            background=None, do_acf=True, p=16, q=8):
     T, H, W = stack.shape
     bg = int(stack.min()) if background is None else background
     acf, ccf, _, _ = models(binning, cf)
     cfg = imfcs.make_pipeline_config(W, H, T, binning=binning, cf=cf, background=bg, poly_order=poly_order,
                                      do_fit=do_fit, acf_model=acf, ccf_model=ccf, fit_tolerance=tol,
-                                     fit_max_iterations=maxit, do_acf=do_acf)
+                                     fit_max_iterations=maxit, do_acf=do_acf, p=p, q=q)
     engine.upload(stack)
     engine.run(cfg)
     return engine.results(cfg), cfg
 
 
 def run_oracle(stack, binning=1, cf=(0, 0), poly_order=4, do_fit=True, roi_cf=None, background=None,
-               nthreads=8):
+               nthreads=8, p=16, q=8):
     """kcf=0 CPU path; roi_cf fixes the ROI to the one of the GPU run."""
     T, H, W = stack.shape
     bg = int(stack.min()) if background is None else background
@@ -40,7 +40,7 @@ def run_oracle(stack, binning=1, cf=(0, 0), poly_order=4, do_fit=True, roi_cf=No
     rcf = cf if roi_cf is None else roi_cf
     cfg = oracle.make_config(W, H, T, binning=binning, cf=cf, background=bg, poly_order=poly_order,
                              roi=(0, 0, W - rcf[0], H - rcf[1]), do_fit=do_fit,
-                             model=occf if cf != (0, 0) else oacf, nthreads=nthreads)
+                             model=occf if cf != (0, 0) else oacf, nthreads=nthreads, p=p, q=q)
     return oracle.pipeline(stack, cfg)
 
 

@@ -111,17 +111,22 @@ def test_jni_calc_acf_throws_java_exception_without_device():
     assert not env.trapped
 
 
-def test_plan_refuses_correlator_p_other_than_16():
-    """The kernels implement the plugin's default p = 16 channel scheme; a
-    panel value of P = 32 (IFCS:7204) is refused before any device work."""
+def test_plan_takes_every_even_correlator_p():
+    """Any even panel P >= 4 (IFCS:7204) plans with setParameters' channel
+    count p + (q - 1) p / 2 + 1 (IFCS:6896); odd P (whose Java tables leave
+    channels unset), a layout beyond the channel limit and q > 12 are
+    refused before any device work."""
     import imaging_fcs_1_52_amd as imfcs
-    c = imfcs.make_pipeline_config(32, 32, 20000, p=32)
     ow, oh, ch = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-    rc = native.lib().imfcs_plan(ctypes.byref(c), ctypes.byref(ow), ctypes.byref(oh), ctypes.byref(ch))
-    assert rc == -2
-    assert b"p = 32 is not supported" in native.lib().imfcs_get_last_error()
-    c = imfcs.make_pipeline_config(32, 32, 20000, q=13)
-    assert native.lib().imfcs_plan(ctypes.byref(c), ctypes.byref(ow), ctypes.byref(oh), ctypes.byref(ch)) == -2
+    plan = lambda c: native.lib().imfcs_plan(ctypes.byref(c), ctypes.byref(ow), ctypes.byref(oh), ctypes.byref(ch))
+    for p, q, n in ((16, 8, 73), (32, 8, 145), (18, 8, 82), (4, 8, 19), (64, 8, 289)):
+        assert plan(imfcs.make_pipeline_config(32, 32, 20000, p=p, q=q)) == 0
+        assert ch.value == n
+    assert plan(imfcs.make_pipeline_config(32, 32, 20000, p=15)) == -2
+    assert b"p = 15 is not supported" in native.lib().imfcs_get_last_error()
+    assert plan(imfcs.make_pipeline_config(32, 32, 20000, p=64, q=9)) == -2
+    assert b"channels" in native.lib().imfcs_get_last_error()
+    assert plan(imfcs.make_pipeline_config(32, 32, 20000, q=13)) == -2
 
 
 def _acf_params(**kw):
@@ -161,8 +166,10 @@ def test_calc_acf_rejects_bad_arguments_on_the_host():
     assert rc == -2 and "negative CF" in msg
     rc, msg = _calc_acf_rc(_acf_params(chanum=72))
     assert rc == -2 and "chanum = 72" in msg
-    rc, msg = _calc_acf_rc(_acf_params(correlatorp=32.0, chanum=145))
-    assert rc == -2 and "p = 32" in msg
+    rc, msg = _calc_acf_rc(_acf_params(correlatorp=15.0, chanum=68))
+    assert rc == -2 and "p = 15" in msg
+    rc, msg = _calc_acf_rc(_acf_params(correlatorp=32.0))
+    assert rc == -2 and "chanum = 73" in msg and "145" in msg
     rc, msg = _calc_acf_rc(_acf_params(cfXDistance=2))   # partner column 3 + 2 >= w_temp 5
     assert rc == -2 and "exceeds" in msg
     lag, samp, _, _ = imfcs.correlator_tables(16, 8, 4096, 1e-3)

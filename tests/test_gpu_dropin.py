@@ -33,9 +33,9 @@ def _java_params(env, a):
     return env.acf_parameters(**fields)
 
 
-def _params(W, H, T, binning, cf, poly_order, bg=0, frametime=1e-3):
+def _params(W, H, T, binning, cf, poly_order, bg=0, frametime=1e-3, p=16, q=8):
     return plugin_glue.initialize_gpu_params(W, H, W - cf[0], H - cf[1], binning, cf, frametime, 1, T, bg,
-                                             poly_order)
+                                             poly_order, p=p, q=q)
 
 
 def test_calc_binning_is_exact_through_jni():
@@ -151,18 +151,21 @@ def _oracle_series(stack, a, px, py, bg, coef=None, frametime=1e-3):
     return d
 
 
-@pytest.mark.parametrize("poly_order", [0, 4])
-def test_calc_acf_through_jni_matches_oracle(poly_order):
+@pytest.mark.parametrize("poly_order,p,q", [(0, 16, 8), (4, 16, 8), (4, 32, 6)])
+def test_calc_acf_through_jni_matches_oracle(poly_order, p, q):
+    """calcACF with the panel's default correlator and with P = 32 (the
+    table-driven correlator), against the oracle pixel by pixel."""
     import oracle
     W, H, T, cf = 7, 6, 8192, (1, 0)
     stack = imfcs.simulate_2d(W, H, T, 60, tau_bleach_s=15.0 if poly_order else 0.0, seed=41 + poly_order)
     bg = int(stack.min())
-    a = _params(W, H, T, 1, cf, poly_order, bg=bg)
+    a = _params(W, H, T, 1, cf, poly_order, bg=bg, p=p, q=q)
     pixels = plugin_glue.get_pixels(stack, a)
     bc = None
     if poly_order:
         bc, _ = plugin_glue.bleach_correction(pixels, a)
-    lag, samp, _, _ = imfcs.correlator_tables(16, 8, T, 1e-3)
+    lag, samp, _, _ = imfcs.correlator_tables(p, q, T, 1e-3)
+    assert a.chanum == len(lag)
     n = a.width * a.height * a.chanum
     env = FakeEnv()
     outs = [np.zeros(n) for _ in range(3)]
@@ -185,9 +188,9 @@ def test_calc_acf_through_jni_matches_oracle(poly_order):
                 cb = bc[(y * a.w_temp + x + cf[0]) * nc:(y * a.w_temp + x + cf[0] + 1) * nc]
             da = _oracle_series(stack, a, x, y, bg, ca)
             db = _oracle_series(stack, a, x + cf[0], y, bg, cb)
-            oind, _ = oracle.block_transform(da, db)
+            oind, _ = oracle.block_transform(da, db, p=p, q=q)
             assert ind[y, x] == oind
-            ocor, ovar, _, _ = oracle.calculate_cf(da, db, oind)
+            ocor, ovar, _, _ = oracle.calculate_cf(da, db, oind, p=p, q=q)
             amp = np.max(np.abs(ocor[1:]))
             e = rel_err(acf[:, y, x], ocor, 1e-4 * amp)
             worst = max(worst, e.max())

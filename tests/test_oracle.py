@@ -29,6 +29,25 @@ def test_tables_match_manual_groups():
     np.testing.assert_allclose(lagtime, lag * FT)
 
 
+def test_tables_of_other_panel_p():
+    """p = 32: delays 0..32 at sample time 1, then 16 channels per group
+    with lags 2^(g-1) (16 + y); p = 18 (2 mod 4): Java's integer base / 4
+    gives the first group y + 8, so lag 9 appears twice and lag 18 not at all
+    (IFCS:7398-7408)."""
+    lag, samp, mtab, _ = oracle.tables(32, 6, 50000, FT)
+    assert len(lag) == 32 + 5 * 16 + 1
+    assert list(lag[:33]) == list(range(33)) and list(samp[:33]) == [1] * 33
+    for g in range(2, 7):
+        sl = slice(16 * g + 1, 16 * g + 17)
+        assert list(lag[sl]) == [2 ** (g - 1) * (16 + y) for y in range(1, 17)]
+        assert set(samp[sl]) == {2 ** (g - 1)}
+    lag, samp, _, _ = oracle.tables(18, 4, 50000, FT)
+    assert len(lag) == 18 + 3 * 9 + 1
+    assert list(lag[:19]) == list(range(10)) + list(range(9, 18))
+    assert list(samp[:19]) == [1] * 19
+    assert list(lag[19:28]) == [2 * (y + 8) for y in range(1, 10)]
+
+
 # ---------------------------------------------------------------- intensity
 def _rand_stack(T=60, H=6, W=7, seed=0, lo=90, hi=140):
     rng = np.random.default_rng(seed)
@@ -128,6 +147,19 @@ def test_calculate_cf_matches_bruteforce(seed):
     for ind in (7, 8):
         corav, bvar, _, _ = oracle.calculate_cf(np.r_[0, a], np.r_[0, b], ind)
         G, V = _multitau_bruteforce(a, b, ind)
+        np.testing.assert_allclose(corav, G, rtol=1e-9, atol=1e-13)
+        np.testing.assert_allclose(bvar[1:], V[1:], rtol=1e-9)
+
+
+@pytest.mark.parametrize("p,q", [(32, 6), (18, 5), (8, 8)])
+def test_calculate_cf_other_panel_p_matches_bruteforce(p, q):
+    rng = np.random.default_rng(p)
+    num = 8192 + 11
+    a = rng.poisson(150, num).astype(float) + np.convolve(rng.normal(0, 25, num), np.ones(12) / 12, "same")
+    b = np.roll(a, 2) + rng.normal(0, 4, num)
+    for ind in (q - 1, q + 1):
+        corav, bvar, _, _ = oracle.calculate_cf(np.r_[0, a], np.r_[0, b], ind, p=p, q=q)
+        G, V = _multitau_bruteforce(a, b, ind, p=p, q=q)
         np.testing.assert_allclose(corav, G, rtol=1e-9, atol=1e-13)
         np.testing.assert_allclose(bvar[1:], V[1:], rtol=1e-9)
 
