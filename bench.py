@@ -367,7 +367,8 @@ def main():
             "config": {"workload": "128x128x50000 uint16; binning 1,2,4; ACF+CCF(x+b); poly4 bleach; LM N,D,G",
                        "acf_pixels_per_step": n_acf_pixels, "converged_acf_fits_per_step": n_conv_seed,
                        "runner": ("hipGraph replay of the three pipelines (imfcs_batch)" if batch is not None
-                                  else "imfcs_run_multi")},
+                                  else "imfcs_run_multi"),
+                       "shared_bleach_pass": all(e.run_flags() & 1 for e in engines)},
             "value_plain": round(value_plain, 1), "ms_per_step_plain": round(dt_plain * 1e3, 3),
             "converged_acf_fits_per_step_plain": n_conv_plain,
             "all_pixels_per_s": round(n_acf_pixels / dt_seed, 1),

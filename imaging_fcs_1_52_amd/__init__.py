@@ -388,6 +388,13 @@ class Engine:
         for e, c in zip(engines, cfgs):
             e._chanum_last = c.correlator_p + (c.correlator_q - 1) * c.correlator_p // 2 + 1
 
+    def run_flags(self):
+        """imfcs_get_run_flags of the last run (bit 0: its bleach pass was
+        the nested pass of binnings 1, 2 and 4)."""
+        f = ctypes.c_int()
+        _check(self._lib.imfcs_get_run_flags(self.ctx, ctypes.byref(f)))
+        return f.value
+
     def stage_ms(self):
         """Device times (ms) of the last run's stages: trace + polynomial fit,
         bleach correction + window means, blocking, correlation, LM fit."""
@@ -540,6 +547,11 @@ class Batch:
         n = ctypes.c_int()
         _check(lib().imfcs_get_slot_count(lib().imfcs_batch_context(self.h, i), ctypes.byref(n)))
         return n.value
+
+    def run_flags(self, i):
+        f = ctypes.c_int()
+        _check(lib().imfcs_get_run_flags(lib().imfcs_batch_context(self.h, i), ctypes.byref(f)))
+        return f.value
 
     def info(self):
         g, f, t, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()

@@ -286,6 +286,39 @@ void *scratch_ptr(int device) {
     std::lock_guard<std::mutex> lk(g_scratch_mu);
     return device < (int)g_scratch_ptr.size() ? g_scratch_ptr[device] : nullptr;
 }
+struct EngineArgs {
+    SeriesGeom g;
+    int order;             // polynomial order (0 = none)
+    int gls;               // 1: regularized covariance of every pair (calculateCF GLS branch)
+    int fit_trace;         // 1: trace + fit on device; 0: coefficients given in host_coef
+    const double *host_coef;   // [nseries][order+1] in seconds (drop-in)
+    int nopit, ave;
+    double frametime;
+    // bleach abscissa t = frametime * (frame + t_off), frame 0-based:
+    // 1.5 = the Java CPU path's 1-based (x + 0.5) (IFCS:9747), 0.5 = the
+    // reference GPU kernel's 0-based (i + 0.5) (agpufitjni.cu:5108)
+    double t_off;
+    int p, q;
+    PairDesc pd;
+    int seg_swl;           // > 0: "Lin Segment" correction with this sliding window length
+    int exp_n;             // 1: "Single Exp" correction
+    // per-pixel point sums of the stack (P[nopit][plane], this run's first
+    // frame, nopit and ave) to take the trace from instead of the stack
+    const int32_t *psums;
+};
+
+// A pipeline run stopped before its bleach pass (stage 1 of imfcs_run_multi
+// and the batches, whose nested binnings share one pass over the stack): what
+// the rest of the run (stage 2) takes up.
+struct StagedRun {
+    EngineArgs ea;          // with the slot grid of the run
+    BleachDesc bd;
+    HostTables ht;
+    CorrTables ct;
+    Tune tn;
+    int plen = 0, npc = 0;  // frame chunks of the bleach pass
+    bool mfma = false;
+};
 }  // namespace
 
 // ------------------------------------------------------------------
@@ -308,6 +341,7 @@ struct imfcs_ctx {
     DevBuf swacf, swvar;   // sliding-window averages
     DevBuf psums;          // per-pixel bleach-point sums shared by imfcs_run_multi
     hipEvent_t ev_psums = nullptr;
+    hipEvent_t ev_stage = nullptr;   // end of a staged run's first part / of a nested bleach pass
     DevBuf etrace, eitime, epfit, einit, eout, estate, echi2, eiter;   // exponential bleach fits
     DevBuf part, coef, ctot, rec, F, bpart, ind, bis, corav, bvar, fdata, fw, fx, finit, fpfit, fout, fstate, fchi2,
         fiter, cpart, hpart, l3, lvd, ctab, csum, slots, X, glsblk, glsW, glsy, glsok, glscov, dropin_in, nbm, nbc,
@@ -338,6 +372,12 @@ struct imfcs_ctx {
     Tune force_tune{0, 0};
     // the chunking the last run used
     Tune used_tune{};
+    // 1: run_engine stops before the bleach pass, keeping its state in
+    // staged; 2: it continues from there, the pass (X, ctot, rec) done by
+    // the caller; 3: it continues with the pass; 0: whole runs
+    int stage = 0;
+    StagedRun staged;
+    int run_nested = 0;   // the last run's bleach pass was a nested pass
 };
 
 hipStream_t imfcs::ctx_stream(imfcs_ctx *ctx) { return ctx->stream; }
@@ -409,6 +449,7 @@ imfcs_ctx *imfcs_create(int device) {
     for (auto &e : ctx->ev) (void)hipEventCreate(&e);
     (void)hipEventCreateWithFlags(&ctx->ev_h2d, hipEventDisableTiming);
     (void)hipEventCreateWithFlags(&ctx->ev_psums, hipEventDisableTiming);
+    (void)hipEventCreateWithFlags(&ctx->ev_stage, hipEventDisableTiming);
     for (auto &e : ctx->evk) (void)hipEventCreate(&e);
     const char *sel = std::getenv("IMFCS_CORRELATOR");
     ctx->force_v1 = (sel && std::strcmp(sel, "generic") == 0) ? 1 : 0;
@@ -445,6 +486,7 @@ void imfcs_destroy(imfcs_ctx *ctx) {
     for (auto &e : ctx->ev) (void)hipEventDestroy(e);
     if (ctx->ev_h2d) (void)hipEventDestroy(ctx->ev_h2d);
     if (ctx->ev_psums) (void)hipEventDestroy(ctx->ev_psums);
+    if (ctx->ev_stage) (void)hipEventDestroy(ctx->ev_stage);
     for (auto &e : ctx->evk) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -580,28 +622,96 @@ int build_slots(const SeriesGeom &g, std::vector<int32_t> &buf) {
     return nslots;
 }
 
-struct EngineArgs {
-    SeriesGeom g;
-    int order;             // polynomial order (0 = none)
-    int gls;               // 1: regularized covariance of every pair (calculateCF GLS branch)
-    int fit_trace;         // 1: trace + fit on device; 0: coefficients given in host_coef
-    const double *host_coef;   // [nseries][order+1] in seconds (drop-in)
-    int nopit, ave;
-    double frametime;
-    // bleach abscissa t = frametime * (frame + t_off), frame 0-based:
-    // 1.5 = the Java CPU path's 1-based (x + 0.5) (IFCS:9747), 0.5 = the
-    // reference GPU kernel's 0-based (i + 0.5) (agpufitjni.cu:5108)
-    double t_off;
-    int p, q;
-    PairDesc pd;
-    int seg_swl;           // > 0: "Lin Segment" correction with this sliding window length
-    int exp_n;             // 1: "Single Exp" correction
-    // per-pixel point sums of the stack (P[nopit][plane], this run's first
-    // frame, nopit and ave) to take the trace from instead of the stack
-    const int32_t *psums;
-};
+
+// The run from its bleach pass on (the pass itself unless the caller did it:
+// stage 2 of a staged run), with the state its first part left in staged.
+int engine_rest(imfcs_ctx *ctx, bool do_apply) {
+    hipStream_t st = ctx->stream;
+    StagedRun &S = ctx->staged;
+    EngineArgs &ea = S.ea;
+    SeriesGeom &g = ea.g;
+    const BleachDesc &bd = S.bd;
+    const CorrTables &ct = S.ct;
+    const CorrTables *dct = ctx->ctab.as<CorrTables>();
+    const Tune tn = S.tn;
+    const int ns = g.nslots, np = ea.pd.npairs, plen = S.plen, npc = S.npc;
+    const bool mfma = S.mfma;
+    if (do_apply)
+        launch_apply(g, bd, dct, ctx->ctot.as<double>(), ctx->rec.as<double>(), ctx->X.as<float>(),
+                     mfma ? ctx->l3.as<float>() : nullptr, mfma ? ctx->cmax.as<float>() : nullptr, npc, plen, st);
+    launch_prefix_combine(dct, ns, ctx->ctot.as<double>(), ctx->rec.as<double>(), npc, plen, ctx->F.as<double>(),
+                          mfma ? ctx->cmax.as<float>() : nullptr, mfma ? ctx->xexp.as<int32_t>() : nullptr, st);
+    HIP_TRY(hipEventRecord(ctx->ev[2], st));
+    // blocking
+    const BlockingPlan bp = plan_blocking(ct, g.npix, tn.block_log2);
+    HIP_TRY(ctx->bpart.ensure(sizeof(double) * (size_t)bp.nsplit * np * 8 * 2));
+    HIP_TRY(ctx->csum.ensure(sizeof(double) * (size_t)std::max(1, bp.nchunk_prod) * np));
+    HIP_TRY(ctx->ind.ensure(sizeof(int32_t) * (size_t)np));
+    HIP_TRY(ctx->bis.ensure(sizeof(int32_t) * (size_t)np));
+    launch_blocking(g, bd, dct, ea.pd, ctx->F.as<double>(), ctx->X.as<float>(), bp, ctx->bpart.as<double>(),
+                    ctx->csum.as<double>(), st);
+    launch_blocking_finalize(ct, dct, g, ea.pd, ctx->F.as<double>(), bp, ctx->bpart.as<double>(),
+                             ctx->csum.as<double>(), ctx->ind.as<int32_t>(), ctx->bis.as<int32_t>(), st);
+    HIP_TRY(hipEventRecord(ctx->ev[3], st));
+    // correlation
+    HIP_TRY(ctx->corav.ensure(sizeof(double) * (size_t)np * ct.chanum));
+    HIP_TRY(ctx->bvar.ensure(sizeof(double) * (size_t)np * ct.chanum));
+    const int m = ct.chanum - 1, nblk = (int)ct.mtab_last;
+    ctx->did_gls = 0;
+    if (ea.gls) {
+        if (!correlate_fast_supported(ct) || ctx->force_v1)
+            return fail(IMFCS_ERR_ARG, "GLS needs the p = 16 correlator layout with blocknum >= q");
+        if (m > kGlsMaxM) return fail(IMFCS_ERR_ARG, "GLS supports at most 104 fitted channels");
+        HIP_TRY(ctx->glsblk.ensure(sizeof(float) * (size_t)np * nblk * m));
+        HIP_TRY(ctx->glsW.ensure(sizeof(double) * (size_t)np * m * m));
+        HIP_TRY(ctx->glscov.ensure(sizeof(double) * (size_t)np * m * m));
+        HIP_TRY(ctx->glsy.ensure(sizeof(double) * (size_t)np * m));
+        HIP_TRY(ctx->glsok.ensure(sizeof(int32_t) * (size_t)np));
+    }
+    if (correlate_fast_supported(ct) && !ctx->force_v1) {
+        const int clen = corr_fast_chunk_len(ct, np);
+        const int nch = (g.T + clen - 1) / clen;
+        HIP_TRY(ctx->cpart.ensure(sizeof(float) * (size_t)nch * np * 33 * 3));
+        HIP_TRY(ctx->hpart.ensure(sizeof(double) * (size_t)nch * np * (ct.chanum - 33) * 3));
+        size_t nlv = 0;
+        for (int c = 4; c < ct.nlevels; c++) nlv += (size_t)ct.numbin[c] + kLevelPad;
+        HIP_TRY(ctx->l3.ensure(sizeof(float) * ((size_t)ct.numbin[3] + kLevelPad) * g.nslots));
+        HIP_TRY(ctx->lvd.ensure(sizeof(float) * std::max<size_t>(nlv, 1) * g.nslots));
+        launch_correlate_fast(g, bd, ct, dct, ea.pd, ctx->F.as<double>(), ctx->X.as<float>(), ctx->ind.as<int32_t>(),
+                              ctx->cpart.as<float>(), ctx->hpart.as<double>(), ctx->l3.as<float>(),
+                              ctx->lvd.as<float>(), clen,
+                              ctx->corav.as<double>(), ctx->bvar.as<double>(), st, ctx->evk[0], ctx->evk[1],
+                              ea.gls ? ctx->glsblk.as<float>() : nullptr, nblk,
+                              mfma ? ctx->xexp.as<int32_t>() : nullptr);
+        ctx->evk_valid = 1;
+        if (ea.gls) {
+            // mcov replaces the correlation as Java's GLS output (IFCS:9638-9639)
+            GlsArgs A{dct, g, ea.pd, bd, ctx->F.as<double>(), ctx->ind.as<int32_t>(), ctx->glsblk.as<float>(), nblk,
+                      ctx->corav.as<double>(), ctx->glsW.as<double>(), ctx->glsy.as<double>(),
+                      ctx->glsok.as<int32_t>(), ctx->glscov.as<double>()};
+            launch_gls_cov(A, np, st);
+            ctx->did_gls = 1;
+        }
+    } else {
+        HIP_TRY(hipEventRecord(ctx->evk[0], st));
+        launch_correlate(g, bd, ct, dct, ea.pd, ctx->F.as<double>(), ctx->X.as<float>(), ctx->ind.as<int32_t>(),
+                         ctx->corav.as<double>(), ctx->bvar.as<double>(), st);
+        HIP_TRY(hipEventRecord(ctx->evk[1], st));
+        ctx->evk_valid = 1;
+    }
+    HIP_TRY(hipEventRecord(ctx->ev[4], st));
+    HIP_TRY(hipGetLastError());
+    ctx->last_g = g;
+    return 0;
+}
 
 int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTables &ct) {
+    if (ctx->stage >= 2) {   // continue a staged run (2: its bleach pass done by the caller)
+        ht = ctx->staged.ht;
+        ct = ctx->staged.ct;
+        return engine_rest(ctx, ctx->stage == 3);
+    }
+    ctx->run_nested = 0;
     hipStream_t st = ctx->stream;
     EngineArgs ea = ea_in;
     SeriesGeom &g = ea.g;
@@ -743,7 +853,7 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTabl
     int plen = 4096;
     while (plen > 64 && (int64_t)ns * ((g.T + plen - 1) / plen) < ((int64_t)1 << tn.apply_log2)) plen >>= 1;
     const int npc = (g.T + plen - 1) / plen;
-    HIP_TRY(ctx->ctot.ensure(sizeof(double) * (size_t)npc * ns));
+    HIP_TRY(ctx->ctot.ensure(sizeof(double) * (size_t)prefix_subchunks(g.T) * ns));
     HIP_TRY(ctx->rec.ensure(sizeof(double) * (size_t)ns * ct.npos));
     HIP_TRY(ctx->F.ensure(sizeof(double) * (size_t)ns * ct.npos));
     HIP_TRY(ctx->X.ensure(sizeof(float) * (size_t)ns * g.T));
@@ -758,72 +868,17 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTabl
         HIP_TRY(ctx->cmax.ensure(sizeof(float) * (size_t)npc * ns));
         HIP_TRY(ctx->xexp.ensure(sizeof(int32_t) * (size_t)ns));
     }
-    launch_apply(g, bd, dct, ctx->ctot.as<double>(), ctx->rec.as<double>(), ctx->X.as<float>(),
-                 mfma ? ctx->l3.as<float>() : nullptr, mfma ? ctx->cmax.as<float>() : nullptr, npc, plen, st);
-    launch_prefix_combine(dct, ns, ctx->ctot.as<double>(), ctx->rec.as<double>(), npc, plen, ctx->F.as<double>(),
-                          mfma ? ctx->cmax.as<float>() : nullptr, mfma ? ctx->xexp.as<int32_t>() : nullptr, st);
-    HIP_TRY(hipEventRecord(ctx->ev[2], st));
-    // blocking
-    const BlockingPlan bp = plan_blocking(ct, g.npix, tn.block_log2);
-    HIP_TRY(ctx->bpart.ensure(sizeof(double) * (size_t)bp.nsplit * np * 8 * 2));
-    HIP_TRY(ctx->csum.ensure(sizeof(double) * (size_t)std::max(1, bp.nchunk_prod) * np));
-    HIP_TRY(ctx->ind.ensure(sizeof(int32_t) * (size_t)np));
-    HIP_TRY(ctx->bis.ensure(sizeof(int32_t) * (size_t)np));
-    launch_blocking(g, bd, dct, ea.pd, ctx->F.as<double>(), ctx->X.as<float>(), bp, ctx->bpart.as<double>(),
-                    ctx->csum.as<double>(), st);
-    launch_blocking_finalize(ct, dct, g, ea.pd, ctx->F.as<double>(), bp, ctx->bpart.as<double>(),
-                             ctx->csum.as<double>(), ctx->ind.as<int32_t>(), ctx->bis.as<int32_t>(), st);
-    HIP_TRY(hipEventRecord(ctx->ev[3], st));
-    // correlation
-    HIP_TRY(ctx->corav.ensure(sizeof(double) * (size_t)np * ct.chanum));
-    HIP_TRY(ctx->bvar.ensure(sizeof(double) * (size_t)np * ct.chanum));
-    const int m = ct.chanum - 1, nblk = (int)ct.mtab_last;
-    ctx->did_gls = 0;
-    if (ea.gls) {
-        if (!correlate_fast_supported(ct) || ctx->force_v1)
-            return fail(IMFCS_ERR_ARG, "GLS needs the p = 16 correlator layout with blocknum >= q");
-        if (m > kGlsMaxM) return fail(IMFCS_ERR_ARG, "GLS supports at most 104 fitted channels");
-        HIP_TRY(ctx->glsblk.ensure(sizeof(float) * (size_t)np * nblk * m));
-        HIP_TRY(ctx->glsW.ensure(sizeof(double) * (size_t)np * m * m));
-        HIP_TRY(ctx->glscov.ensure(sizeof(double) * (size_t)np * m * m));
-        HIP_TRY(ctx->glsy.ensure(sizeof(double) * (size_t)np * m));
-        HIP_TRY(ctx->glsok.ensure(sizeof(int32_t) * (size_t)np));
-    }
-    if (correlate_fast_supported(ct) && !ctx->force_v1) {
-        const int clen = corr_fast_chunk_len(ct, np);
-        const int nch = (g.T + clen - 1) / clen;
-        HIP_TRY(ctx->cpart.ensure(sizeof(float) * (size_t)nch * np * 33 * 3));
-        HIP_TRY(ctx->hpart.ensure(sizeof(double) * (size_t)nch * np * (ct.chanum - 33) * 3));
-        size_t nlv = 0;
-        for (int c = 4; c < ct.nlevels; c++) nlv += (size_t)ct.numbin[c] + kLevelPad;
-        HIP_TRY(ctx->l3.ensure(sizeof(float) * ((size_t)ct.numbin[3] + kLevelPad) * g.nslots));
-        HIP_TRY(ctx->lvd.ensure(sizeof(float) * std::max<size_t>(nlv, 1) * g.nslots));
-        launch_correlate_fast(g, bd, ct, dct, ea.pd, ctx->F.as<double>(), ctx->X.as<float>(), ctx->ind.as<int32_t>(),
-                              ctx->cpart.as<float>(), ctx->hpart.as<double>(), ctx->l3.as<float>(),
-                              ctx->lvd.as<float>(), clen,
-                              ctx->corav.as<double>(), ctx->bvar.as<double>(), st, ctx->evk[0], ctx->evk[1],
-                              ea.gls ? ctx->glsblk.as<float>() : nullptr, nblk,
-                              mfma ? ctx->xexp.as<int32_t>() : nullptr);
-        ctx->evk_valid = 1;
-        if (ea.gls) {
-            // mcov replaces the correlation as Java's GLS output (IFCS:9638-9639)
-            GlsArgs A{dct, g, ea.pd, bd, ctx->F.as<double>(), ctx->ind.as<int32_t>(), ctx->glsblk.as<float>(), nblk,
-                      ctx->corav.as<double>(), ctx->glsW.as<double>(), ctx->glsy.as<double>(),
-                      ctx->glsok.as<int32_t>(), ctx->glscov.as<double>()};
-            launch_gls_cov(A, np, st);
-            ctx->did_gls = 1;
-        }
-    } else {
-        HIP_TRY(hipEventRecord(ctx->evk[0], st));
-        launch_correlate(g, bd, ct, dct, ea.pd, ctx->F.as<double>(), ctx->X.as<float>(), ctx->ind.as<int32_t>(),
-                         ctx->corav.as<double>(), ctx->bvar.as<double>(), st);
-        HIP_TRY(hipEventRecord(ctx->evk[1], st));
-        ctx->evk_valid = 1;
-    }
-    HIP_TRY(hipEventRecord(ctx->ev[4], st));
-    HIP_TRY(hipGetLastError());
-    ctx->last_g = g;
-    return 0;
+    StagedRun &S = ctx->staged;
+    S.ea = ea;
+    S.bd = bd;
+    S.ht = ht;
+    S.ct = ct;
+    S.tn = tn;
+    S.plen = plen;
+    S.npc = npc;
+    S.mfma = mfma;
+    if (ctx->stage == 1) return 0;
+    return engine_rest(ctx, true);
 }
 
 int check_geom(const imfcs_config *c, int ow, int oh) {
@@ -1107,6 +1162,7 @@ static int run_config(imfcs_ctx *ctx, const uint16_t *d_stack, const imfcs_confi
     } else {
         rc = run_engine(ctx, ea, ht, ct);
         if (rc) return rc;
+        if (ctx->stage == 1) return 0;   // first part of a staged run
     }
     ctx->last_cfg = *c;
     ctx->last_lagtime = ht.lagtime;
@@ -1201,6 +1257,117 @@ static int enqueue_point_sums(DevBuf &psums, const uint16_t *d_stack, const imfc
     return 0;
 }
 
+// The binnings 1, 2 and 4 among configurations of one stack whose bleach
+// passes one nested pass (k_apply_nested) can do: the same frames, panel,
+// polynomial order and background, non-overlapping full grids from (0, 0),
+// the default correlator kernels.  tri[b] = index of binning 2^b.
+static bool nested_candidates(imfcs_ctx *const *ctxs, int n, const imfcs_config *cfgs, int (&tri)[3]) {
+    tri[0] = tri[1] = tri[2] = -1;
+    for (int i = 0; i < n; i++) {
+        const imfcs_config &c = cfgs[i];
+        const int b = c.binning_x;
+        const int lb = b == 1 ? 0 : b == 2 ? 1 : b == 4 ? 2 : -1;
+        if (lb < 0 || tri[lb] >= 0 || c.binning_y != b || c.pixbin_x != b || c.pixbin_y != b) continue;
+        if (c.bleach_mode != IMFCS_BLEACH_POLYNOMIAL || c.poly_order < 1 || c.roi_x != 0 || c.roi_y != 0) continue;
+        if (ctxs[i]->want_mfma) continue;
+        tri[lb] = i;
+    }
+    if (tri[0] < 0 || tri[1] < 0 || tri[2] < 0) return false;
+    const imfcs_config &a = cfgs[tri[0]];
+    for (int lb = 1; lb < 3; lb++) {
+        const imfcs_config &c = cfgs[tri[lb]];
+        if (c.width != a.width || c.height != a.height || c.firstframe != a.firstframe ||
+            c.lastframe != a.lastframe || c.background != a.background || c.poly_order != a.poly_order ||
+            c.correlator_p != a.correlator_p || c.correlator_q != a.correlator_q || c.frametime != a.frametime)
+            return false;
+    }
+    return a.width % 2 == 0 && ((int64_t)a.width * a.height) % 2 == 0;
+}
+
+// After the first parts of the three runs: their slot grids are the full
+// nested grids (no padding, no second grid), the tables agree and every
+// buffer resource of the pass spans its 16 rows.
+static bool nested_ready(imfcs_ctx *const *ctxs, const int (&tri)[3], const uint16_t *d_stack) {
+    int swp[3], sh[3];
+    for (int lb = 0; lb < 3; lb++) {
+        const StagedRun &S = ctxs[tri[lb]]->staged;
+        const SeriesGeom &g = S.ea.g;
+        const bool ccf = g.nseries > g.npix;
+        const bool on_grid = ccf && (g.cfx % g.pbx == 0) && (g.cfy % g.pby == 0);
+        if (ccf && !on_grid) return false;
+        const int sw = g.ow + (ccf ? g.cfx / g.pbx : 0), shh = g.oh + (ccf ? g.cfy / g.pby : 0);
+        if (sw != g.swp || (int64_t)sw * shh != g.nslots || g.ox0 != 0 || g.oy0 != 0) return false;
+        if (S.bd.mode != 0 || S.bd.order != ctxs[tri[0]]->staged.bd.order || S.mfma) return false;
+        if ((int64_t)g.nslots * 64 >= 0x7fffffff) return false;
+        swp[lb] = g.swp;
+        sh[lb] = shh;
+        const CorrTables &c0 = ctxs[tri[0]]->staged.ct;
+        if (S.ct.npos != c0.npos || std::memcmp(S.ct.pos, c0.pos, sizeof(int32_t) * c0.npos) != 0) return false;
+    }
+    const SeriesGeom &g1 = ctxs[tri[0]]->staged.ea.g;
+    if (g1.plane * 2 * 16 >= 0x7fffffff || (uintptr_t)d_stack % 4 != 0) return false;
+    return swp[0] == 2 * swp[1] && swp[1] == 2 * swp[2] && sh[0] == 2 * sh[1] && sh[1] == 2 * sh[2] &&
+           swp[1] % 32 == 0 && sh[1] % 2 == 0;
+}
+
+// The runs of n configurations of one stack with a nested triple: the
+// triple's first parts, its bleach pass on binning 1's stream once the other
+// two have their polynomials, then every configuration in the given order
+// (the triple's runs from the pass on, the others whole).  Without a nested
+// layout each run does its own pass.
+static int run_nested(imfcs_ctx *const *ctxs, int n, const int (&tri)[3], const uint16_t *d_stack,
+                      const imfcs_config *cfgs, const int32_t *psums, std::vector<LMGroup> *collect) {
+    for (int lb = 0; lb < 3; lb++) {
+        imfcs_ctx *c = ctxs[tri[lb]];
+        c->stage = 1;
+        const int rc = run_config(c, d_stack, &cfgs[tri[lb]], psums, collect);
+        c->stage = 0;
+        if (rc) return rc;
+    }
+    const bool nest = nested_ready(ctxs, tri, d_stack);
+    if (nest) {
+        imfcs_ctx *c1 = ctxs[tri[0]];
+        for (int lb = 1; lb < 3; lb++) {
+            HIP_TRY(hipEventRecord(ctxs[tri[lb]]->ev_stage, ctxs[tri[lb]]->stream));
+            HIP_TRY(hipStreamWaitEvent(c1->stream, ctxs[tri[lb]]->ev_stage, 0));
+        }
+        NestedApply A{};
+        const SeriesGeom &g1 = c1->staged.ea.g;
+        A.data = d_stack;
+        A.plane = g1.plane;
+        A.stride = g1.stride;
+        A.f0 = g1.f0;
+        A.T = g1.T;
+        A.bg = cfgs[tri[0]].background;
+        int plen = 4096;
+        const int ns2 = ctxs[tri[1]]->staged.ea.g.nslots;
+        while (plen > 64 && (int64_t)ns2 * ((g1.T + plen - 1) / plen) < ((int64_t)1 << 19)) plen >>= 1;
+        for (int lb = 0; lb < 3; lb++) {
+            imfcs_ctx *c = ctxs[tri[lb]];
+            A.swp[lb] = c->staged.ea.g.swp;
+            A.sh[lb] = c->staged.ea.g.nslots / c->staged.ea.g.swp;
+            A.bd[lb] = c->staged.bd;
+            A.ctot[lb] = c->ctot.as<double>();
+            A.rec[lb] = c->rec.as<double>();
+            A.X[lb] = c->X.as<float>();
+        }
+        launch_apply_nested(A, c1->ctab.as<CorrTables>(), c1->staged.bd.order, (g1.T + plen - 1) / plen, plen,
+                            c1->stream);
+        HIP_TRY(hipEventRecord(c1->ev_stage, c1->stream));
+        for (int lb = 1; lb < 3; lb++) HIP_TRY(hipStreamWaitEvent(ctxs[tri[lb]]->stream, c1->ev_stage, 0));
+    }
+    for (int i = 0; i < n; i++) {
+        imfcs_ctx *c = ctxs[i];
+        const bool in_tri = i == tri[0] || i == tri[1] || i == tri[2];
+        c->stage = in_tri ? (nest ? 2 : 3) : 0;
+        c->run_nested = in_tri && nest;
+        const int rc = run_config(c, d_stack, &cfgs[i], psums, collect);
+        c->stage = 0;
+        if (rc) return rc;
+    }
+    return 0;
+}
+
 extern "C" int imfcs_run_multi(imfcs_ctx *const *ctxs, int n, const uint16_t *d_stack, const imfcs_config *cfgs) {
     if (!ctxs || !cfgs || !d_stack || n < 1) return fail(IMFCS_ERR_ARG, "null argument");
     for (int i = 0; i < n; i++)
@@ -1224,7 +1391,11 @@ extern "C" int imfcs_run_multi(imfcs_ctx *const *ctxs, int n, const uint16_t *d_
     for (int i = 1; i < n; i++) HIP_TRY(hipStreamWaitEvent(ctxs[i]->stream, c0->ev_psums, 0));
     // enqueued in the given order: a pipeline is ~25 launches (≈0.4 ms of
     // host time), so later configurations start later (enqueueing them from
-    // parallel host threads measured no faster end to end)
+    // parallel host threads measured no faster end to end); binnings 1, 2
+    // and 4 of the stack share one bleach pass
+    int tri[3];
+    if (nested_candidates(ctxs, n, cfgs, tri))
+        return run_nested(ctxs, n, tri, d_stack, cfgs, c0->psums.as<int32_t>(), nullptr);
     for (int i = 0; i < n; i++) {
         const int rc = run_config(ctxs[i], d_stack, &cfgs[i], c0->psums.as<int32_t>());
         if (rc) return rc;
@@ -1659,6 +1830,12 @@ extern "C" int imfcs_get_tuning(int nslots, int frames, int *apply_log2, int *bl
 extern "C" int imfcs_get_slot_count(imfcs_ctx *ctx, int *nslots) {
     if (!ctx || !nslots) return fail(IMFCS_ERR_ARG, "null argument");
     *nslots = ctx->nslots;
+    return 0;
+}
+
+extern "C" int imfcs_get_run_flags(imfcs_ctx *ctx, int *flags) {
+    if (!ctx || !flags) return fail(IMFCS_ERR_ARG, "null argument");
+    *flags = ctx->run_nested ? IMFCS_RUN_NESTED : 0;
     return 0;
 }
 

@@ -241,6 +241,30 @@ void launch_seg_means(const SeriesGeom &g, int seg_len, int nseg, double *coef, 
 // cmax / xexp (null: none): per-slot staging exponents of the matrix-core correlator.
 void launch_prefix_combine(const CorrTables *dct, int nslots, const double *ctot, const double *rec,
                            int nchunk, int chunk_len, double *F, const float *cmax, int32_t *xexp, hipStream_t st);
+// Sub-chunks (64 frames) of the bleach pass's prefix sums: ctot holds
+// prefix_subchunks(T) totals per slot.
+int prefix_subchunks(int T);
+
+// One bleach pass for the binnings 1, 2 and 4 of one uint16 stack (the
+// benchmark's and the diffusion law's nested binnings): the slot grids are
+// the full pixel grids of the three binnings over the same raw rectangle
+// from (0, 0) (grid b: pitch swp[b], sh[b] rows, slot (x, y) = window at
+// (b x, b y)), each with its own polynomial correction, X, ctot and rec
+// buffers and its own frame chunking of the prefix sums (the 64-frame
+// sub-chunks, so the buffers hold the bits the binnings' own passes write).
+struct NestedApply {
+    const uint16_t *data;
+    int64_t plane;        // elements per frame
+    int32_t stride;       // elements per row
+    int32_t f0, T;
+    int32_t bg;           // background per raw pixel
+    int32_t swp[3], sh[3];   // grids of binning 1, 2, 4
+    BleachDesc bd[3];
+    double *ctot[3], *rec[3];
+    float *X[3];
+};
+void launch_apply_nested(const NestedApply &A, const CorrTables *dct, int order, int nchunk, int chunk_len,
+                         hipStream_t st);
 // Work split of the blocking pass: products in chunks of 256 (levels 0..7
 // inside a chunk, levels >= 8 cascaded from the chunk totals).
 struct BlockingPlan {

@@ -240,7 +240,14 @@ __global__ void __launch_bounds__(64 * kPolyQ) k_polyfit(const double *part, int
 // that form every channel's window means ("direct/delayed monitors",
 // IFCS:9487-9494, 9262-9270).  One thread per (slot, frame chunk); frames
 // are read in groups of 16 so the loads of a group are in flight together.
+//
+// The sums are taken per 64-frame sub-chunk (ctot[sub][slot], the sum of
+// I' over the sub-chunk's frames in frame order) and rec[slot][k] is the sum
+// from the start of position k's sub-chunk up to the position, so the prefix
+// sums do not depend on the chunk length (any multiple of 64 frames): runs
+// that split a stack differently give the same bits.
 // ------------------------------------------------------------------
+constexpr int kSubLog2 = 6;   // 64-frame sub-chunks of the prefix sums
 //
 // With l3 set, the pass also writes every slot's level-3 values (sums of 8
 // stored frames, summed pairwise in FP32: ((x0+x1)+(x2+x3))+((x4+x5)+(x6+x7))),
@@ -324,6 +331,14 @@ __global__ void __launch_bounds__(64) k_apply(SeriesGeom g, BleachDesc bd, const
                 if (y3 < nb3) l3[(int64_t)y3 * g.nslots + k] = s8;
             }
         }
+        // end of a sub-chunk (chunks are whole sub-chunks up to the series
+        // end): its sum, restarted unless the series ends here (the
+        // positions at the end record the last sub-chunk's sum)
+        const int fe = min(fb + 16, f1);
+        if ((fe & ((1 << kSubLog2) - 1)) == 0 || fe == f1) {   // wave-uniform
+            ctot[(int64_t)((fe - 1) >> kSubLog2) * g.nslots + k] = run;
+            if (fe < g.T) run = 0.0;
+        }
     }
     if (cmax) cmax[(int64_t)chunk * g.nslots + k] = mx;
     if (chunk == nchunk - 1)
@@ -331,22 +346,170 @@ __global__ void __launch_bounds__(64) k_apply(SeriesGeom g, BleachDesc bd, const
             rec[(int64_t)k * npos + q] = run;
             q++;
         }
-    ctot[(int64_t)chunk * g.nslots + k] = run;
 }
 
-// F[s][k] = sum_{c < ck} ctot[c][s] + rec[s][k], ck = the chunk of position
-// k (positions ascending).  One wave per slot: the chunk totals are scanned
-// 64 at a time across the lanes, and each lane completes one position, so
-// the kernel's latency is a few round trips instead of one per chunk and
-// position.
+// The bleach pass of binnings 1, 2 and 4 in one read of the stack.  A lane
+// owns one binning-2 slot: its 2 x 2 raw pixels are the four binning-1 slots,
+// and the four lanes of a quad are the 2 x 2 binning-2 slots of one
+// binning-4 slot (window sums exchanged within the quad).  A wave covers 32
+// x 2 binning-2 slots.  Every series is corrected, summed and stored exactly
+// as k_apply does it (same integer window sums, same FP64 correction, same
+// frame order of the sums per 64-frame sub-chunk), so the three binnings'
+// X, ctot and rec hold the bits of their own passes; the binning-4 series
+// is corrected by all four lanes of its quad and stored by the first.
+constexpr int kNG = 8;   // frames per group of the nested pass
+template <int MAXO>
+__global__ void __launch_bounds__(64) k_apply_nested(NestedApply A, const CorrTables *__restrict__ ctp,
+                                                     int chunk_len) {
+    const CorrTables &ct = *ctp;
+    const int lane = threadIdx.x;
+    const int wpr = A.swp[1] >> 5;   // waves per binning-2 row pair
+    const int wy = (int)blockIdx.x / wpr, wx = (int)blockIdx.x - wy * wpr;
+    const int qd = lane >> 2, li = lane & 1, lj = (lane >> 1) & 1;
+    const int x2 = wx * 32 + 2 * qd + li, y2 = wy * 2 + lj;
+    const int x4 = wx * 16 + qd, y4 = wy;
+    const bool st4 = (lane & 3) == 0;
+    int k1[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) k1[m] = (2 * y2 + (m >> 1)) * A.swp[0] + 2 * x2 + (m & 1);
+    const int k2 = y2 * A.swp[1] + x2, k4 = y4 * A.swp[2] + x4;
+    BleachP<MAXO> b1[4], b2, b4;
+    double mu1[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        b1[m].load(A.bd[0], k1[m]);
+        mu1[m] = slot_mu(A.bd[0].coef, A.bd[0].order, k1[m]);
+    }
+    b2.load(A.bd[1], k2);
+    b4.load(A.bd[2], k4);
+    const double mu2 = slot_mu(A.bd[1].coef, A.bd[1].order, k2);
+    const double mu4 = slot_mu(A.bd[2].coef, A.bd[2].order, k4);
+    const int chunk = blockIdx.y, nchunk = gridDim.y;
+    const int f0 = chunk * chunk_len, f1 = min(A.T, f0 + chunk_len);
+    const int npos = ct.npos;
+    int q = 0;
+    while (q < npos && ct.pos[q] < f0) q++;
+    int next = (q < npos) ? ct.pos[q] : 0x7fffffff;
+    double run1[4] = {0.0, 0.0, 0.0, 0.0}, run2 = 0.0, run4 = 0.0;
+    const int64_t ns1 = (int64_t)A.swp[0] * A.sh[0], ns2 = (int64_t)A.swp[1] * A.sh[1],
+                  ns4 = (int64_t)A.swp[2] * A.sh[2];
+    // the two raw rows of the lane's 2 x 2 pixels, one 32-bit word each, by
+    // buffer loads: the group's first frame in the resource, the frame as
+    // the scalar offset, one lane offset per row (the host checks that 16
+    // frames and 16 rows of every X fit one resource).  Groups of kNG
+    // frames: the six series' coefficients and two groups of raw words
+    // (the next one in flight) fill the registers of two waves per SIMD.
+    const int lo0 = (2 * y2 * A.stride + 2 * x2) * 2, lo1 = lo0 + A.stride * 2;
+    const int plb = (int)A.plane * 2;
+    auto fetch = [&](int fb, uint32_t (&w)[2][kNG]) {
+        const int nv = min(kNG, f1 - fb);
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(A.data + (int64_t)(A.f0 + fb) * A.plane), (short)0, plb * 16, kBufferWord3);
+#pragma unroll
+        for (int i = 0; i < kNG; i++) {
+            const int so = min(i, nv - 1) * plb;
+            w[0][i] = __builtin_amdgcn_raw_buffer_load_b32(rs, lo0, so, 0);
+            w[1][i] = __builtin_amdgcn_raw_buffer_load_b32(rs, lo1, so, 0);
+        }
+    };
+    auto pix = [](uint32_t w, int hi) { return (int)(hi ? (w >> 16) : (w & 0xffffu)); };
+    auto sum4 = [&](const uint32_t (&w)[2][kNG], int i) {
+        return (pix(w[0][i], 0) + pix(w[0][i], 1)) + (pix(w[1][i], 0) + pix(w[1][i], 1));
+    };
+    uint32_t cur[2][kNG], nxt[2][kNG];
+    if (f0 < f1) fetch(f0, cur);
+    for (int fb = f0; fb < f1; fb += kNG) {
+        int s4[kNG];   // binning-4 window sums (the quad's four binning-2 sums)
+#pragma unroll
+        for (int i = 0; i < kNG; i++) {
+            const int s2 = sum4(cur, i);
+            const int t = s2 + __shfl_xor(s2, 1, 64);
+            s4[i] = t + __shfl_xor(t, 2, 64);
+        }
+        if (fb + kNG < f1) fetch(fb + kNG, nxt);   // wave-uniform
+        // this group's rows of the three X arrays (16 rows per resource)
+        const __amdgpu_buffer_rsrc_t xr0 = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(A.X[0] + (int64_t)fb * ns1), (short)0, (int)(ns1 * 64), kBufferWord3);
+        const __amdgpu_buffer_rsrc_t xr1 = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(A.X[1] + (int64_t)fb * ns2), (short)0, (int)(ns2 * 64), kBufferWord3);
+        const __amdgpu_buffer_rsrc_t xr2 = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(A.X[2] + (int64_t)fb * ns4), (short)0, (int)(ns4 * 64), kBufferWord3);
+#pragma unroll
+        for (int i = 0; i < kNG; i++) {
+            const int f = fb + i;
+            if (f < f1) {   // wave-uniform
+                while (next == f) {
+#pragma unroll
+                    for (int m = 0; m < 4; m++) A.rec[0][(int64_t)k1[m] * npos + q] = run1[m];
+                    A.rec[1][(int64_t)k2 * npos + q] = run2;
+                    if (st4) A.rec[2][(int64_t)k4 * npos + q] = run4;
+                    q++;
+                    next = (q < npos) ? ct.pos[q] : 0x7fffffff;
+                }
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    const double a = b1[m].apply((double)(pix(cur[m >> 1][i], m & 1) - A.bg), f);
+                    run1[m] += a;
+                    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, (float)(a - mu1[m])), xr0,
+                                                          k1[m] * 4, i * (int)ns1 * 4, 0);
+                }
+                const double a2 = b2.apply((double)(sum4(cur, i) - 4 * A.bg), f);
+                run2 += a2;
+                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, (float)(a2 - mu2)), xr1, k2 * 4,
+                                                      i * (int)ns2 * 4, 0);
+                const double a4 = b4.apply((double)(s4[i] - 16 * A.bg), f);
+                run4 += a4;
+                if (st4)
+                    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, (float)(a4 - mu4)), xr2,
+                                                          k4 * 4, i * (int)ns4 * 4, 0);
+            }
+            // one frame's six corrections at a time (they hold enough
+            // independent work): bounds the registers of the unrolled group
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        const int fe = min(fb + kNG, f1);
+        if ((fe & ((1 << kSubLog2) - 1)) == 0 || fe == f1) {   // wave-uniform
+            const int64_t sub = (fe - 1) >> kSubLog2;
+#pragma unroll
+            for (int m = 0; m < 4; m++) A.ctot[0][sub * ns1 + k1[m]] = run1[m];
+            A.ctot[1][sub * ns2 + k2] = run2;
+            if (st4) A.ctot[2][sub * ns4 + k4] = run4;
+            if (fe < A.T) {
+#pragma unroll
+                for (int m = 0; m < 4; m++) run1[m] = 0.0;
+                run2 = run4 = 0.0;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kNG; i++) {
+            cur[0][i] = nxt[0][i];
+            cur[1][i] = nxt[1][i];
+        }
+    }
+    if (chunk == nchunk - 1)
+        while (q < npos && ct.pos[q] >= f1) {
+#pragma unroll
+            for (int m = 0; m < 4; m++) A.rec[0][(int64_t)k1[m] * npos + q] = run1[m];
+            A.rec[1][(int64_t)k2 * npos + q] = run2;
+            if (st4) A.rec[2][(int64_t)k4 * npos + q] = run4;
+            q++;
+        }
+}
+
+// F[s][k] = S(ck) + rec[s][k], ck = the sub-chunk of position k (positions
+// ascending; at most the last sub-chunk), S(m) the sum of the sub-chunk
+// totals before m in a fixed order: runs of 16 totals, each summed in
+// order; the runs before m's run by a lane scan; then m's own run up to m in
+// order.  One wave per slot: each lane sums one run of a 1024-sub-chunk
+// segment, and each position is completed by one lane, so the kernel's
+// latency is a few round trips instead of one per sub-chunk and position.
 //
 // With xexp set, also the staging exponent of every slot for the matrix-core
 // correlator: the smallest e >= 0 with max |X| * 2^-e < 2^14 (from the
-// per-chunk maxima cmax), at most 60.
+// per-chunk maxima cmax of the nchunk bleach chunks), at most 60.
 __global__ void __launch_bounds__(256) k_prefix_combine(const CorrTables *__restrict__ ctp, int nslots,
                                                         const double *ctot, const double *rec, int nchunk,
-                                                        int chunk_len, double *F, const float *cmax,
-                                                        int32_t *xexp) {
+                                                        double *F, const float *cmax, int32_t *xexp) {
     const CorrTables &ct = *ctp;
     const int s = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -365,13 +528,22 @@ __global__ void __launch_bounds__(256) k_prefix_combine(const CorrTables *__rest
         if (lane == 0) xexp[s] = e;
     }
     const int npos = ct.npos;
+    const int nsub = (ct.T + (1 << kSubLog2) - 1) >> kSubLog2;
+    const int nrun = (nsub + 15) >> 4;
     const double *recs = rec + (int64_t)s * npos;
     double *Fs = F + (int64_t)s * npos;
-    double carry = 0.0;   // sum of the chunks before this segment
+    double carry = 0.0;   // sum of the runs before this segment
     int k = 0;            // next position
-    for (int c0 = 0; c0 < nchunk && k < npos; c0 += 64) {
-        const int c = c0 + lane;
-        const double v = c < nchunk ? ctot[(int64_t)c * nslots + s] : 0.0;
+    for (int r0 = 0; r0 < nrun && k < npos; r0 += 64) {
+        const int r = r0 + lane;
+        double v = 0.0;
+        if (r < nrun) {
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int c = 16 * r + i;
+                if (c < nsub) v += ctot[(int64_t)c * nslots + s];
+            }
+        }
         double incl = v;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
@@ -380,15 +552,20 @@ __global__ void __launch_bounds__(256) k_prefix_combine(const CorrTables *__rest
         }
         double excl = __shfl_up(incl, 1, 64);
         if (lane == 0) excl = 0.0;
-        const double P = carry + excl;   // sum of the chunks before chunk c
-        // positions whose chunk lies in this segment (the last segment
-        // holds the chunk of every remaining position)
+        const double P = carry + excl;   // sum of the runs before run r
+        // positions whose run lies in this segment (the last segment holds
+        // the run of every remaining position)
         for (;;) {
             const int kk = k + lane;
-            const int ck = kk < npos ? min(ct.pos[kk] / chunk_len, nchunk - 1) : 0x7fffffff;
-            const bool mine = ck < c0 + 64;
-            const double Pk = __shfl(P, (ck - c0) & 63, 64);
-            if (mine) Fs[kk] = Pk + recs[kk];
+            const int ck = kk < npos ? min(ct.pos[kk] >> kSubLog2, nsub - 1) : 0x7fffffff;
+            const int rk = ck >> 4;
+            const bool mine = rk < r0 + 64;
+            const double Pk = __shfl(P, (rk - r0) & 63, 64);
+            if (mine) {
+                double part = 0.0;   // the run's totals before sub-chunk ck
+                for (int c = rk << 4; c < ck; c++) part += ctot[(int64_t)c * nslots + s];
+                Fs[kk] = (Pk + part) + recs[kk];
+            }
             const int nm = __popcll(__ballot(mine));
             k += nm;
             if (nm < 64) break;
@@ -2521,8 +2698,20 @@ void launch_seg_means(const SeriesGeom &g, int seg_len, int nseg, double *coef, 
 
 void launch_prefix_combine(const CorrTables *dct, int nslots, const double *ctot, const double *rec, int nchunk,
                            int chunk_len, double *F, const float *cmax, int32_t *xexp, hipStream_t st) {
+    (void)chunk_len;   // the prefix sums are taken per 64-frame sub-chunk whatever the chunking
     hipLaunchKernelGGL(k_prefix_combine, dim3(cdiv(nslots, 4)), dim3(256), 0, st, dct, nslots, ctot, rec, nchunk,
-                       chunk_len, F, cmax, xexp);
+                       F, cmax, xexp);
+}
+
+int prefix_subchunks(int T) { return (T + (1 << kSubLog2) - 1) >> kSubLog2; }
+
+void launch_apply_nested(const NestedApply &A, const CorrTables *dct, int order, int nchunk, int chunk_len,
+                         hipStream_t st) {
+    dim3 grid((A.swp[1] >> 5) * (A.sh[1] >> 1), nchunk);
+    if (bleach_maxo(order) <= 4)
+        hipLaunchKernelGGL(k_apply_nested<4>, grid, dim3(64), 0, st, A, dct, chunk_len);
+    else
+        hipLaunchKernelGGL(k_apply_nested<kMaxOrder>, grid, dim3(64), 0, st, A, dct, chunk_len);
 }
 
 BlockingPlan plan_blocking(const CorrTables &ct, int npix, int threads_log2) {

@@ -96,6 +96,7 @@ SIGNATURES = {
     "imfcs_batch_info": (ctypes.c_int, [_VP, _P(ctypes.c_int), _P(ctypes.c_int), _P(ctypes.c_int), _P(_f64)]),
     "imfcs_get_tuning": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _P(ctypes.c_int), _P(ctypes.c_int)]),
     "imfcs_get_slot_count": (ctypes.c_int, [_VP, _P(ctypes.c_int)]),
+    "imfcs_get_run_flags": (ctypes.c_int, [_VP, _P(ctypes.c_int)]),
     "imfcs_run_multi": (ctypes.c_int, [_P(_VP), ctypes.c_int, _VP, _P(PipelineConfig)]),
     "imfcs_refit": (ctypes.c_int, [_VP, _P(PipelineConfig)]),
     "imfcs_get_results": (ctypes.c_int, [_VP, _P(_f64), _P(_f64), _P(_i32), _P(_i32), _P(_f64), _P(_f64),

@@ -86,3 +86,46 @@ def test_run_multi_frame_range_and_roi(engine):
     finally:
         for e in engines:
             e.close()
+
+
+@pytest.mark.parametrize("T", [5000, 4321])
+def test_nested_binnings_share_one_bleach_pass(engine, T):
+    """Binnings 1, 2 and 4 of one stack (full grids from (0, 0), each with
+    its CCF partner one output pixel to the right) share one bleach pass in
+    imfcs_run_multi; a binning-3 configuration of the same stack runs its
+    own.  Every result equals the separate runs' bit for bit (the nested pass
+    corrects, sums and stores each series as the binning's own pass does,
+    and the window-mean sums are taken per 64-frame sub-chunk whatever the
+    chunking); T = 4321 ends inside a sub-chunk."""
+    W, H = 64, 32
+    print(f"nested T={T}", flush=True)
+    stack = imfcs.simulate_2d(W, H, T, 200, D_um2s=1.0, tau_bleach_s=10.0, seed=11 + T)
+    d = engine.upload(stack)
+    bg = int(stack.min())
+    cfgs = [_cfg(W, H, T, 2, (2, 0), bg, 4), _cfg(W, H, T, 4, (4, 0), bg, 4), _cfg(W, H, T, 1, (1, 0), bg, 4),
+            _cfg(W, H, T, 3, (0, 0), bg, 4)]
+    sep = []
+    for c in cfgs:
+        engine.run(c, d)
+        engine.synchronize()
+        assert engine.run_flags() == 0
+        sep.append(engine.results(c))
+    engines = [imfcs.Engine(0) for _ in cfgs]
+    try:
+        imfcs.Engine.run_multi(engines, cfgs, d)
+        assert [e.run_flags() for e in engines] == [1, 1, 1, 0]
+        for e, c, r in zip(engines, cfgs, sep):
+            e.synchronize()
+            m = e.results(c)
+            for k in KEYS:
+                np.testing.assert_array_equal(np.asarray(m[k]), np.asarray(r[k]), err_msg=k)
+    finally:
+        for e in engines:
+            e.close()
+
+
+@pytest.mark.skip(reason="the nested bleach pass is switched off in batches: a batch run of this "
+                         "triple ended in a host segmentation fault whose cause is not found yet")
+def test_nested_binnings_in_a_batch_graph(engine):
+    """A batch's captured graph of the nested triple, replayed, against the
+    separate runs bit for bit."""
