@@ -9,9 +9,11 @@ One step = for each binning b in (1, 2, 4): fused b x b binning, order-4
 polynomial bleach correction, ACF and CCF to the x+b neighbour with blocked
 variances, and the LM fit of N, D, G_inf of every ACF and CCF pixel.  The
 three binnings run as three pipelines on three HIP streams over the one
-device-resident stack, captured once into a hipGraph (imfcs_batch with one
-shared stack; every kernel runs in every replay) and replayed each step
-(--runner multi: imfcs_run_multi enqueued each step).  The stack is in HBM
+device-resident stack, enqueued each step by imfcs_run_multi: one pass over
+the stack gives every pipeline its bleach trace, and one bleach-correction
+pass writes the corrected series of all three binnings (--runner graph: the
+three pipelines captured once into a hipGraph by imfcs_batch, each with its
+own bleach-correction pass, and replayed each step).  The stack is in HBM
 before timing starts.
 
 ``value``: ACF pixels fitted to state CONVERGED (finite correlation; the
@@ -167,9 +169,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=48, help="side of the oracle's pixel sample (0 = skip)")
     ap.add_argument("--train-steps", type=int, default=400, help="AdamW steps of each seeding network")
-    ap.add_argument("--runner", choices=("graph", "multi"), default="graph",
-                    help="graph: the three pipelines captured once into a hipGraph (imfcs_batch with one shared "
-                         "stack) and replayed every step; multi: imfcs_run_multi enqueued every step")
+    ap.add_argument("--runner", choices=("graph", "multi"), default="multi",
+                    help="multi: imfcs_run_multi enqueued every step (binnings 1, 2, 4 share one bleach-correction "
+                         "pass); graph: the three pipelines captured once into a hipGraph (imfcs_batch with one "
+                         "shared stack) and replayed every step")
     ap.add_argument("--autotune", action="store_true", help="graph runner: time the frame chunking first")
     ap.add_argument("--solo-binning", type=int, default=0,
                     help="profiling aid: run only this binning's plain pipeline --steps times (after --warmup), alone "
@@ -368,7 +371,7 @@ def main():
                        "acf_pixels_per_step": n_acf_pixels, "converged_acf_fits_per_step": n_conv_seed,
                        "runner": ("hipGraph replay of the three pipelines (imfcs_batch)" if batch is not None
                                   else "imfcs_run_multi"),
-                       "shared_bleach_pass": all(e.run_flags() & 1 for e in engines)},
+                       "shared_bleach_pass": {str(b): bool(e.run_flags() & 1) for b, e in zip(BINNINGS, engines)}},
             "value_plain": round(value_plain, 1), "ms_per_step_plain": round(dt_plain * 1e3, 3),
             "converged_acf_fits_per_step_plain": n_conv_plain,
             "all_pixels_per_s": round(n_acf_pixels / dt_seed, 1),

@@ -355,9 +355,30 @@ __global__ void __launch_bounds__(64) k_apply(SeriesGeom g, BleachDesc bd, const
 // x 2 binning-2 slots.  Every series is corrected, summed and stored exactly
 // as k_apply does it (same integer window sums, same FP64 correction, same
 // frame order of the sums per 64-frame sub-chunk), so the three binnings'
-// X, ctot and rec hold the bits of their own passes; the binning-4 series
-// is corrected by all four lanes of its quad and stored by the first.
+// X, ctot and rec hold the bits of their own passes.  The four lanes of a
+// quad share the binning-4 series' corrections (two frames of each 8-frame
+// group per lane) and every lane of the quad sums all of them in frame order;
+// the quad's first lane writes its sums.
 constexpr int kNG = 8;   // frames per group of the nested pass
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+// Lane k's value (k < 4 wave-uniform) on every lane of its quad (DPP
+// quad_perm broadcast of both halves).
+template <int CTRL>
+__device__ __forceinline__ double quad_bcast_t(double v) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xf, 0xf, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ double quad_bcast(double v, int k) {
+    switch (k) {
+        case 0: return quad_bcast_t<0x00>(v);
+        case 1: return quad_bcast_t<0x55>(v);
+        case 2: return quad_bcast_t<0xaa>(v);
+        default: return quad_bcast_t<0xff>(v);
+    }
+}
 template <int MAXO>
 __global__ void __launch_bounds__(64) k_apply_nested(NestedApply A, const CorrTables *__restrict__ ctp,
                                                      int chunk_len) {
@@ -434,6 +455,22 @@ __global__ void __launch_bounds__(64) k_apply_nested(NestedApply A, const CorrTa
             (void *)(A.X[1] + (int64_t)fb * ns2), (short)0, (int)(ns2 * 64), kBufferWord3);
         const __amdgpu_buffer_rsrc_t xr2 = __builtin_amdgcn_make_buffer_rsrc(
             (void *)(A.X[2] + (int64_t)fb * ns4), (short)0, (int)(ns4 * 64), kBufferWord3);
+        // binning 4: lane j of the quad corrects and stores frames j and j + 4
+        // of the group; the values reach every lane of the quad for the sums
+        // in frame order
+        const int jq = lane & 3;
+        double a4q[kNG / 4];
+#pragma unroll
+        for (int h = 0; h < kNG / 4; h++) {
+            const int i = jq + 4 * h;
+            // s4[i] without a dynamic register index
+            const int *sv = s4 + 4 * h;
+            const int s4i = jq == 0 ? sv[0] : jq == 1 ? sv[1] : jq == 2 ? sv[2] : sv[3];
+            a4q[h] = b4.apply((double)(s4i - 16 * A.bg), fb + i);
+            if (fb + i < f1)
+                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, (float)(a4q[h] - mu4)), xr2,
+                                                      k4 * 4 + i * (int)ns4 * 4, 0, 0);
+        }
 #pragma unroll
         for (int i = 0; i < kNG; i++) {
             const int f = fb + i;
@@ -446,22 +483,25 @@ __global__ void __launch_bounds__(64) k_apply_nested(NestedApply A, const CorrTa
                     q++;
                     next = (q < npos) ? ct.pos[q] : 0x7fffffff;
                 }
+                // binning 1: the two pixels of each raw row as one 8-byte store
+                // (slots k1[0], k1[0] + 1 and k1[2], k1[2] + 1)
+                uint32_t xw[4];
 #pragma unroll
                 for (int m = 0; m < 4; m++) {
                     const double a = b1[m].apply((double)(pix(cur[m >> 1][i], m & 1) - A.bg), f);
                     run1[m] += a;
-                    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, (float)(a - mu1[m])), xr0,
-                                                          k1[m] * 4, i * (int)ns1 * 4, 0);
+                    xw[m] = __builtin_bit_cast(uint32_t, (float)(a - mu1[m]));
+                }
+#pragma unroll
+                for (int r = 0; r < 2; r++) {
+                    const u32x2 v = {xw[2 * r], xw[2 * r + 1]};
+                    __builtin_amdgcn_raw_buffer_store_b64(v, xr0, k1[2 * r] * 4, i * (int)ns1 * 4, 0);
                 }
                 const double a2 = b2.apply((double)(sum4(cur, i) - 4 * A.bg), f);
                 run2 += a2;
                 __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, (float)(a2 - mu2)), xr1, k2 * 4,
                                                       i * (int)ns2 * 4, 0);
-                const double a4 = b4.apply((double)(s4[i] - 16 * A.bg), f);
-                run4 += a4;
-                if (st4)
-                    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, (float)(a4 - mu4)), xr2,
-                                                          k4 * 4, i * (int)ns4 * 4, 0);
+                run4 += quad_bcast(a4q[i >> 2], i & 3);
             }
             // one frame's six corrections at a time (they hold enough
             // independent work): bounds the registers of the unrolled group

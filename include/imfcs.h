@@ -337,7 +337,7 @@ int imfcs_get_slot_count(imfcs_ctx *ctx, int *nslots);
 /* How a context's last run was done (no counterpart in the reference; for
  * callers and tests that check which path ran): bit 0 (IMFCS_RUN_NESTED)
  * set when its bleach-correction pass was the one pass shared by binnings
- * 1, 2 and 4 of its stack (imfcs_run_multi and the batches). */
+ * 1, 2 and 4 of its stack (imfcs_run_multi). */
 #define IMFCS_RUN_NESTED 1
 int imfcs_get_run_flags(imfcs_ctx *ctx, int *flags);
 
