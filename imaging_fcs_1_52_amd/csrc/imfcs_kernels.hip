@@ -539,78 +539,85 @@ __global__ void __launch_bounds__(64) k_apply_nested(NestedApply A, const CorrTa
 // F[s][k] = S(ck) + rec[s][k], ck = the sub-chunk of position k (positions
 // ascending; at most the last sub-chunk), S(m) the sum of the sub-chunk
 // totals before m in a fixed order: runs of 16 totals, each summed in
-// order; the runs before m's run by a lane scan; then m's own run up to m in
-// order.  One wave per slot: each lane sums one run of a 1024-sub-chunk
-// segment, and each position is completed by one lane, so the kernel's
-// latency is a few round trips instead of one per sub-chunk and position.
+// order; the runs before m's run in order; then m's own run up to m in
+// order.  A workgroup takes 64 consecutive slots (lane = slot, so every load
+// of ctot[sub][*] is one contiguous 512-byte row): its four waves sum the
+// runs of kPcRuns-run segments into LDS, then each lane accumulates its
+// slot's runs and the waves complete the positions of the segment (every
+// fourth position each; the positions are the same for every slot).
 //
 // With xexp set, also the staging exponent of every slot for the matrix-core
 // correlator: the smallest e >= 0 with max |X| * 2^-e < 2^14 (from the
 // per-chunk maxima cmax of the nchunk bleach chunks), at most 60.
+constexpr int kPcRuns = 64;   // runs of 16 sub-chunks per LDS segment
 __global__ void __launch_bounds__(256) k_prefix_combine(const CorrTables *__restrict__ ctp, int nslots,
                                                         const double *ctot, const double *rec, int nchunk,
                                                         double *F, const float *cmax, int32_t *xexp) {
+    __shared__ double runsum[kPcRuns][64];
+    __shared__ double runpre[kPcRuns][64];   // sum of the runs before run r (segment-local index)
     const CorrTables &ct = *ctp;
-    const int s = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (s >= nslots) return;   // wave-uniform
-    if (xexp) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int s = blockIdx.x * 64 + lane;
+    const bool live = s < nslots;
+    const int sc = live ? s : nslots - 1;   // (dead lanes load a copy, write nothing)
+    if (xexp && w == 0 && live) {
         float mx = 0.f;
-        for (int c = lane; c < nchunk; c += 64) mx = fmaxf(mx, cmax[(int64_t)c * nslots + s]);
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+        for (int c = 0; c < nchunk; c++) mx = fmaxf(mx, cmax[(int64_t)c * nslots + s]);
         int e = 0;
         if (mx > 0.f && mx <= 3.0e38f) {
             int ex;
             (void)frexpf(mx, &ex);   // mx < 2^ex
             e = min(max(ex - 14, 0), 60);
         }
-        if (lane == 0) xexp[s] = e;
+        xexp[s] = e;
     }
     const int npos = ct.npos;
     const int nsub = (ct.T + (1 << kSubLog2) - 1) >> kSubLog2;
     const int nrun = (nsub + 15) >> 4;
-    const double *recs = rec + (int64_t)s * npos;
-    double *Fs = F + (int64_t)s * npos;
-    double carry = 0.0;   // sum of the runs before this segment
-    int k = 0;            // next position
-    for (int r0 = 0; r0 < nrun && k < npos; r0 += 64) {
-        const int r = r0 + lane;
-        double v = 0.0;
-        if (r < nrun) {
+    const double *recs = rec + (int64_t)sc * npos;
+    double *Fs = F + (int64_t)sc * npos;
+    double carry = 0.0;   // (wave 0) sum of the runs before this segment
+    int k0 = 0;           // first position of this segment
+    for (int r0 = 0; r0 < nrun; r0 += kPcRuns) {
+        const int nr = min(kPcRuns, nrun - r0);
+        // run sums of the segment, every fourth run per wave
+        for (int rr = w; rr < nr; rr += 4) {
+            const int c0 = 16 * (r0 + rr);
+            double v[16];
 #pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const int c = 16 * r + i;
-                if (c < nsub) v += ctot[(int64_t)c * nslots + s];
+            for (int i = 0; i < 16; i++) v[i] = c0 + i < nsub ? ctot[(int64_t)(c0 + i) * nslots + sc] : 0.0;
+            double t = 0.0;
+#pragma unroll
+            for (int i = 0; i < 16; i++) t += v[i];
+            runsum[rr][lane] = t;
+        }
+        __syncthreads();
+        if (w == 0) {
+            for (int rr = 0; rr < nr; rr++) {
+                runpre[rr][lane] = carry;
+                carry += runsum[rr][lane];
             }
         }
-        double incl = v;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const double t = __shfl_up(incl, off, 64);
-            if (lane >= off) incl += t;
-        }
-        double excl = __shfl_up(incl, 1, 64);
-        if (lane == 0) excl = 0.0;
-        const double P = carry + excl;   // sum of the runs before run r
+        __syncthreads();
         // positions whose run lies in this segment (the last segment holds
         // the run of every remaining position)
-        for (;;) {
-            const int kk = k + lane;
-            const int ck = kk < npos ? min(ct.pos[kk] >> kSubLog2, nsub - 1) : 0x7fffffff;
+        int k1 = k0;
+        while (k1 < npos && (r0 + kPcRuns >= nrun || (min(ct.pos[k1] >> kSubLog2, nsub - 1) >> 4) < r0 + kPcRuns)) k1++;
+        for (int k = k0 + w; k < k1; k += 4) {
+            const int ck = min(ct.pos[k] >> kSubLog2, nsub - 1);
             const int rk = ck >> 4;
-            const bool mine = rk < r0 + 64;
-            const double Pk = __shfl(P, (rk - r0) & 63, 64);
-            if (mine) {
-                double part = 0.0;   // the run's totals before sub-chunk ck
-                for (int c = rk << 4; c < ck; c++) part += ctot[(int64_t)c * nslots + s];
-                Fs[kk] = (Pk + part) + recs[kk];
-            }
-            const int nm = __popcll(__ballot(mine));
-            k += nm;
-            if (nm < 64) break;
+            const int c0 = rk << 4;
+            double v[15];
+#pragma unroll
+            for (int i = 0; i < 15; i++) v[i] = c0 + i < ck ? ctot[(int64_t)(c0 + i) * nslots + sc] : 0.0;
+            double part = 0.0;   // the run's totals before sub-chunk ck, in order
+#pragma unroll
+            for (int i = 0; i < 15; i++)
+                if (c0 + i < ck) part += v[i];
+            if (live) Fs[k] = (runpre[rk - r0][lane] + part) + recs[k];
         }
-        carry += __shfl(incl, 63, 64);
+        k0 = k1;
+        __syncthreads();   // the next segment overwrites runsum and runpre
     }
 }
 
@@ -2739,7 +2746,7 @@ void launch_seg_means(const SeriesGeom &g, int seg_len, int nseg, double *coef, 
 void launch_prefix_combine(const CorrTables *dct, int nslots, const double *ctot, const double *rec, int nchunk,
                            int chunk_len, double *F, const float *cmax, int32_t *xexp, hipStream_t st) {
     (void)chunk_len;   // the prefix sums are taken per 64-frame sub-chunk whatever the chunking
-    hipLaunchKernelGGL(k_prefix_combine, dim3(cdiv(nslots, 4)), dim3(256), 0, st, dct, nslots, ctot, rec, nchunk,
+    hipLaunchKernelGGL(k_prefix_combine, dim3(cdiv(nslots, 64)), dim3(256), 0, st, dct, nslots, ctot, rec, nchunk,
                        F, cmax, xexp);
 }
 
