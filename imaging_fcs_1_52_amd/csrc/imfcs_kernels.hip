@@ -544,13 +544,16 @@ __global__ void __launch_bounds__(64) k_apply_nested(NestedApply A, const CorrTa
 // of ctot[sub][*] is one contiguous 512-byte row): its four waves sum the
 // runs of kPcRuns-run segments into LDS, then each lane accumulates its
 // slot's runs and the waves complete the positions of the segment (every
-// fourth position each; the positions are the same for every slot).
+// kPcWaves-th position each, four at a time; the positions are the same for
+// every slot).  Eight waves per workgroup: each wave's part is a few
+// dependent round trips to memory.
 //
 // With xexp set, also the staging exponent of every slot for the matrix-core
 // correlator: the smallest e >= 0 with max |X| * 2^-e < 2^14 (from the
 // per-chunk maxima cmax of the nchunk bleach chunks), at most 60.
-constexpr int kPcRuns = 64;   // runs of 16 sub-chunks per LDS segment
-__global__ void __launch_bounds__(256) k_prefix_combine(const CorrTables *__restrict__ ctp, int nslots,
+constexpr int kPcRuns = 64;    // runs of 16 sub-chunks per LDS segment
+constexpr int kPcWaves = 8;    // waves per workgroup (64 slots)
+__global__ void __launch_bounds__(64 * kPcWaves) k_prefix_combine(const CorrTables *__restrict__ ctp, int nslots,
                                                         const double *ctot, const double *rec, int nchunk,
                                                         double *F, const float *cmax, int32_t *xexp) {
     __shared__ double runsum[kPcRuns][64];
@@ -580,8 +583,8 @@ __global__ void __launch_bounds__(256) k_prefix_combine(const CorrTables *__rest
     int k0 = 0;           // first position of this segment
     for (int r0 = 0; r0 < nrun; r0 += kPcRuns) {
         const int nr = min(kPcRuns, nrun - r0);
-        // run sums of the segment, every fourth run per wave
-        for (int rr = w; rr < nr; rr += 4) {
+        // run sums of the segment, every kPcWaves-th run per wave
+        for (int rr = w; rr < nr; rr += kPcWaves) {
             const int c0 = 16 * (r0 + rr);
             double v[16];
 #pragma unroll
@@ -603,18 +606,30 @@ __global__ void __launch_bounds__(256) k_prefix_combine(const CorrTables *__rest
         // the run of every remaining position)
         int k1 = k0;
         while (k1 < npos && (r0 + kPcRuns >= nrun || (min(ct.pos[k1] >> kSubLog2, nsub - 1) >> 4) < r0 + kPcRuns)) k1++;
-        for (int k = k0 + w; k < k1; k += 4) {
-            const int ck = min(ct.pos[k] >> kSubLog2, nsub - 1);
-            const int rk = ck >> 4;
-            const int c0 = rk << 4;
-            double v[15];
+        // four positions per wave at a time, their loads in flight together
+        for (int kb = k0 + w; kb < k1; kb += 4 * kPcWaves) {
+            int ck[4];
+            double v[4][15], rv[4];
 #pragma unroll
-            for (int i = 0; i < 15; i++) v[i] = c0 + i < ck ? ctot[(int64_t)(c0 + i) * nslots + sc] : 0.0;
-            double part = 0.0;   // the run's totals before sub-chunk ck, in order
+            for (int u = 0; u < 4; u++) {
+                const int k = kb + kPcWaves * u;
+                ck[u] = k < k1 ? min(ct.pos[k] >> kSubLog2, nsub - 1) : 0;
+                const int c0 = ck[u] & ~15;
 #pragma unroll
-            for (int i = 0; i < 15; i++)
-                if (c0 + i < ck) part += v[i];
-            if (live) Fs[k] = (runpre[rk - r0][lane] + part) + recs[k];
+                for (int i = 0; i < 15; i++)
+                    v[u][i] = c0 + i < ck[u] ? ctot[(int64_t)(c0 + i) * nslots + sc] : 0.0;
+                rv[u] = k < k1 ? recs[k] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int k = kb + kPcWaves * u;
+                const int c0 = ck[u] & ~15;
+                double part = 0.0;   // the run's totals before sub-chunk ck, in order
+#pragma unroll
+                for (int i = 0; i < 15; i++)
+                    if (c0 + i < ck[u]) part += v[u][i];
+                if (live && k < k1) Fs[k] = (runpre[(ck[u] >> 4) - r0][lane] + part) + rv[u];
+            }
         }
         k0 = k1;
         __syncthreads();   // the next segment overwrites runsum and runpre
@@ -2746,8 +2761,8 @@ void launch_seg_means(const SeriesGeom &g, int seg_len, int nseg, double *coef, 
 void launch_prefix_combine(const CorrTables *dct, int nslots, const double *ctot, const double *rec, int nchunk,
                            int chunk_len, double *F, const float *cmax, int32_t *xexp, hipStream_t st) {
     (void)chunk_len;   // the prefix sums are taken per 64-frame sub-chunk whatever the chunking
-    hipLaunchKernelGGL(k_prefix_combine, dim3(cdiv(nslots, 64)), dim3(256), 0, st, dct, nslots, ctot, rec, nchunk,
-                       F, cmax, xexp);
+    hipLaunchKernelGGL(k_prefix_combine, dim3(cdiv(nslots, 64)), dim3(64 * kPcWaves), 0, st, dct, nslots, ctot, rec,
+                       nchunk, F, cmax, xexp);
 }
 
 int prefix_subchunks(int T) { return (T + (1 << kSubLog2) - 1) >> kSubLog2; }

