@@ -129,3 +129,34 @@ def test_nested_binnings_share_one_bleach_pass(engine, T):
 def test_nested_binnings_in_a_batch_graph(engine):
     """A batch's captured graph of the nested triple, replayed, against the
     separate runs bit for bit."""
+
+
+def test_nested_pass_with_vertical_partners_and_its_fallback(engine):
+    """Binnings 1, 2 and 4 with partners one output pixel below (grids
+    extended by a row) take the nested pass and match separate runs bit for
+    bit; with a different background for binning 4 the set is not nested
+    (every run does its own pass) and again matches."""
+    W, H, T = 64, 32, 4500
+    stack = imfcs.simulate_2d(W, H, T, 200, D_um2s=1.0, tau_bleach_s=10.0, seed=77)
+    d = engine.upload(stack)
+    bg = int(stack.min())
+    for bg4, nested in ((bg, True), (bg - 2, False)):
+        cfgs = [_cfg(W, H, T, 2, (0, 2), bg, 4), _cfg(W, H, T, 4, (0, 4), bg4, 4), _cfg(W, H, T, 1, (0, 1), bg, 4)]
+        print(f"nested={nested}", flush=True)
+        sep = []
+        for c in cfgs:
+            engine.run(c, d)
+            engine.synchronize()
+            sep.append(engine.results(c))
+        engines = [imfcs.Engine(0) for _ in cfgs]
+        try:
+            imfcs.Engine.run_multi(engines, cfgs, d)
+            assert [e.run_flags() for e in engines] == [int(nested)] * 3
+            for e, c, r in zip(engines, cfgs, sep):
+                e.synchronize()
+                m = e.results(c)
+                for k in KEYS:
+                    np.testing.assert_array_equal(np.asarray(m[k]), np.asarray(r[k]), err_msg=k)
+        finally:
+            for e in engines:
+                e.close()
