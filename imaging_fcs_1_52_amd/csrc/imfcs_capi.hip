@@ -726,7 +726,6 @@ int run_engine(imfcs_ctx *ctx, const EngineArgs &ea_in, HostTables &ht, CorrTabl
     HIP_TRY(hipStreamIsCapturing(st, &cap));
     if (cap == hipStreamCaptureStatusNone) HIP_TRY(hipEventSynchronize(ctx->ev_h2d));
     HIP_TRY(ctx->up_ctab.put(ctx->ctab, &ct, sizeof(CorrTables), st));
-    const CorrTables *dct = ctx->ctab.as<CorrTables>();
     g.nslots = build_slots(g, ctx->host_slots);
     g.swp = slot_pitch(g);
     // vectorized window rows: every slot origin, the row stride and the
