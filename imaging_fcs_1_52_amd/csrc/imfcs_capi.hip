@@ -345,7 +345,7 @@ struct imfcs_ctx {
     DevBuf etrace, eitime, epfit, einit, eout, estate, echi2, eiter;   // exponential bleach fits
     DevBuf part, coef, ctot, rec, F, bpart, ind, bis, corav, bvar, fdata, fw, fx, finit, fpfit, fout, fstate, fchi2,
         fiter, cpart, hpart, l3, lvd, ctab, csum, slots, X, glsblk, glsW, glsy, glsok, glscov, dropin_in, nbm, nbc,
-        cmax, xexp;
+        cmax, xexp, lvg;
     SeriesGeom last_g{};   // geometry (with device slot tables) of the last engine run
     std::vector<int32_t> host_slots;   // slot_of then slot origins
     std::vector<double> host_coef;     // drop-in bleach coefficients per slot (source of the async copy)   // source of the async copy into ctab
@@ -465,7 +465,7 @@ static std::vector<DevBuf *> all_bufs(imfcs_ctx *ctx) {
                       &ctx->X, &ctx->glsblk, &ctx->glsW, &ctx->glsy, &ctx->glsok, &ctx->glscov,
                       &ctx->dropin_in, &ctx->nbm, &ctx->nbc, &ctx->swacf, &ctx->swvar, &ctx->etrace,
                       &ctx->eitime, &ctx->epfit, &ctx->einit, &ctx->eout, &ctx->estate, &ctx->echi2,
-                      &ctx->eiter, &ctx->psums, &ctx->fseed, &ctx->cmax, &ctx->xexp};
+                      &ctx->eiter, &ctx->psums, &ctx->fseed, &ctx->cmax, &ctx->xexp, &ctx->lvg};
 }
 
 static void invalidate_uploads(imfcs_ctx *ctx) {
@@ -693,9 +693,12 @@ int engine_rest(imfcs_ctx *ctx, bool do_apply) {
             ctx->did_gls = 1;
         }
     } else {
+        LevelOffsets lo;
+        const size_t nlv = generic_level_values(ct, g.nslots, lo);
+        HIP_TRY(ctx->lvg.ensure(sizeof(double) * std::max<size_t>(nlv, 1)));
         HIP_TRY(hipEventRecord(ctx->evk[0], st));
         launch_correlate(g, bd, ct, dct, ea.pd, ctx->F.as<double>(), ctx->X.as<float>(), ctx->ind.as<int32_t>(),
-                         ctx->corav.as<double>(), ctx->bvar.as<double>(), st);
+                         ctx->corav.as<double>(), ctx->bvar.as<double>(), ctx->lvg.as<double>(), lo, st);
         HIP_TRY(hipEventRecord(ctx->evk[1], st));
         ctx->evk_valid = 1;
     }

@@ -281,9 +281,15 @@ void launch_blocking(const SeriesGeom &g, const BleachDesc &bd, const CorrTables
 void launch_blocking_finalize(const CorrTables &ct, const CorrTables *dct, const SeriesGeom &g, const PairDesc &pd,
                               const double *F, const BlockingPlan &bp, const double *bpart, const double *csum,
                               int32_t *ind, int32_t *bis, hipStream_t st);
+// Level c >= 1 of every slot (FP64) for the table-driven correlator, at
+// lvg + off[c]: generic_level_values returns the values to allocate.
+struct LevelOffsets {
+    int64_t off[kMaxLevel];
+};
+size_t generic_level_values(const CorrTables &ct, int nslots, LevelOffsets &lo);
 void launch_correlate(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const CorrTables *dct,
                       const PairDesc &pd, const double *F, const float *X, const int32_t *ind, double *corav,
-                      double *bvar, hipStream_t st);
+                      double *bvar, double *lvg, const LevelOffsets &lo, hipStream_t st);
 bool correlate_fast_supported(const CorrTables &ct);
 int corr_fast_chunk_len(const CorrTables &ct, int npairs);
 // xexp non-null: levels 0..2 on the matrix cores (k_corr_mfma; the level-3

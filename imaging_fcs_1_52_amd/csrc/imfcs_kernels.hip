@@ -972,15 +972,13 @@ __global__ void __launch_bounds__(64) k_blocking_finalize(const CorrTables *__re
 // ------------------------------------------------------------------
 struct LevelCtx {
     const float *X;
+    const double *lv;   // level c >= 1: [numbin][ld] FP64 pairwise sums (k_level_sums)
     int64_t ld;
     int ka, kb, c, numbin, same;
     // level-c value: sum of 2^c stored deviations, i.e. I'-sum minus 2^c mu
     __device__ __forceinline__ double val(int k, int y) const {
         if (y >= numbin) return 0.0;
-        const int64_t f0 = (int64_t)y << c;
-        double acc = 0.0;
-        for (int i = 0; i < (1 << c); i++) acc += (double)X[(f0 + i) * ld + k];
-        return acc;
+        return c == 0 ? (double)X[(int64_t)y * ld + k] : lv[(int64_t)y * ld + k];
     }
     __device__ __forceinline__ double val_a(int y) const { return val(ka, y); }
     __device__ __forceinline__ double val_b(int y) const { return val(kb, y); }
@@ -1088,9 +1086,21 @@ __device__ void correlate_run(const LevelCtx &L, int G, int d0, const double *al
 // the level's channels are taken in runs of at most 17 consecutive delays
 // (p = 16: delays 0..16 on level 0, 9..16 on the others; a repeated delay
 // starts a new run), each run one pass over the level's values.
+// Level c >= 1 of every slot for the table-driven correlator, FP64 sums of
+// the two values of level c - 1 (level 0: the stored FP32 deviations X):
+// dst[y][s] = src[2y][s] + src[2y+1][s], one thread per (value, slot).
+__global__ void __launch_bounds__(256) k_level_sums(const float *X, const double *src, double *dst, int numbin,
+                                                    int nslots) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (int64_t)numbin * nslots) return;
+    const int64_t y = i / nslots, s = i - y * nslots;
+    const int64_t a = (2 * y) * nslots + s, b = a + nslots;
+    dst[i] = X ? (double)X[a] + (double)X[b] : src[a] + src[b];
+}
+
 __global__ void __launch_bounds__(64) k_correlate(SeriesGeom g, BleachDesc bd, const CorrTables *__restrict__ ctp,
                                                   PairDesc pd, const double *F, const float *X, const int32_t *ind,
-                                                  double *corav, double *bvar) {
+                                                  double *corav, double *bvar, const double *lvg, LevelOffsets lo) {
     const CorrTables &ct = *ctp;
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     int c = blockIdx.y;
@@ -1099,6 +1109,7 @@ __global__ void __launch_bounds__(64) k_correlate(SeriesGeom g, BleachDesc bd, c
     pair_series(pd, p, sa, sb);
     LevelCtx L;
     L.X = X;
+    L.lv = c > 0 ? lvg + lo.off[c] : nullptr;
     L.ld = g.nslots;
     L.ka = g.slot_of[sa];
     L.kb = g.slot_of[sb];
@@ -2811,11 +2822,27 @@ void launch_blocking_finalize(const CorrTables &ct, const CorrTables *dct, const
                        bpart, bp.nsplit, csum, ind, bis);
 }
 
+size_t generic_level_values(const CorrTables &ct, int nslots, LevelOffsets &lo) {
+    size_t tot = 0;
+    for (int c = 0; c < kMaxLevel; c++) lo.off[c] = 0;
+    for (int c = 1; c < ct.nlevels; c++) {
+        lo.off[c] = (int64_t)tot;
+        tot += (size_t)ct.numbin[c] * nslots;
+    }
+    return tot;
+}
+
 void launch_correlate(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const CorrTables *dct,
                       const PairDesc &pd, const double *F, const float *X, const int32_t *ind, double *corav,
-                      double *bvar, hipStream_t st) {
+                      double *bvar, double *lvg, const LevelOffsets &lo, hipStream_t st) {
+    const int ns = g.nslots;
+    for (int c = 1; c < ct.nlevels; c++) {
+        const int64_t n = (int64_t)ct.numbin[c] * ns;
+        hipLaunchKernelGGL(k_level_sums, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, st, c == 1 ? X : nullptr,
+                           c == 1 ? nullptr : lvg + lo.off[c - 1], lvg + lo.off[c], ct.numbin[c], ns);
+    }
     dim3 grid(cdiv(pd.npairs, 64), ct.nlevels);
-    hipLaunchKernelGGL(k_correlate, grid, dim3(64), 0, st, g, bd, dct, pd, F, X, ind, corav, bvar);
+    hipLaunchKernelGGL(k_correlate, grid, dim3(64), 0, st, g, bd, dct, pd, F, X, ind, corav, bvar, lvg, lo);
 }
 
 static void launch_lm(const LMArgs &a, unsigned free_mask, hipStream_t st) {
