@@ -287,9 +287,12 @@ struct LevelOffsets {
     int64_t off[kMaxLevel];
 };
 size_t generic_level_values(const CorrTables &ct, int nslots, LevelOffsets &lo);
+// Frames per time chunk of the table-driven correlator (part holds
+// ceil(T / chunk) x npairs x chanum x 3 sums).
+int generic_chunk_len(const CorrTables &ct);
 void launch_correlate(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const CorrTables *dct,
                       const PairDesc &pd, const double *F, const float *X, const int32_t *ind, double *corav,
-                      double *bvar, double *lvg, const LevelOffsets &lo, hipStream_t st);
+                      double *bvar, double *lvg, const LevelOffsets &lo, double *part, hipStream_t st);
 bool correlate_fast_supported(const CorrTables &ct);
 int corr_fast_chunk_len(const CorrTables &ct, int npairs);
 // xexp non-null: levels 0..2 on the matrix cores (k_corr_mfma; the level-3

@@ -992,7 +992,7 @@ struct LevelCtx {
 // B products.
 template <int G, int NL>
 __device__ void correlate_level(const LevelCtx &L, int d0, const double *alpha, const double *beta, int B, int pnum,
-                                double *cross, double *a1, double *a2) {
+                                int ybeg, int yend, double *cross, double *a1, double *a2) {
     constexpr int WN = G + NL - 1;
     double w[WN];   // b~ values [y0 + d0, y0 + d0 + WN)
     double ga[G];   // a~ values [y0, y0 + G)
@@ -1003,10 +1003,10 @@ __device__ void correlate_level(const LevelCtx &L, int d0, const double *alpha, 
         cross[i] = a1[i] = a2[i] = 0.0;
     }
 #pragma unroll
-    for (int k = 0; k < WN; k++) w[k] = L.val_b(d0 + k);
+    for (int k = 0; k < WN; k++) w[k] = L.val_b(ybeg + d0 + k);
     const int numbin = L.numbin;
     const bool reuse = L.same && d0 == 0;   // the a window is the b window
-    for (int y0 = 0; y0 < numbin; y0 += G) {
+    for (int y0 = ybeg; y0 < yend; y0 += G) {
 #pragma unroll
         for (int k = 0; k < G; k++) ga[k] = reuse ? w[k] : L.val_a(y0 + k);
         if (y0 + d0 + WN <= numbin) {
@@ -1043,9 +1043,9 @@ __device__ void correlate_level(const LevelCtx &L, int d0, const double *alpha, 
                 sb[i] += t;
             }
         }
-        int yend = y0 + G;
-        if ((yend & (B - 1)) == 0) {
-            int j = yend / B - 1;
+        const int yb = y0 + G;
+        if ((yb & (B - 1)) == 0) {
+            int j = yb / B - 1;
             bool inblk = j < pnum;
 #pragma unroll
             for (int i = 0; i < NL; i++) {
@@ -1072,13 +1072,13 @@ __device__ void correlate_level(const LevelCtx &L, int d0, const double *alpha, 
 
 template <int NL>
 __device__ void correlate_run(const LevelCtx &L, int G, int d0, const double *alpha, const double *beta, int B,
-                              int pnum, double *cross, double *a1, double *a2) {
+                              int pnum, int ybeg, int yend, double *cross, double *a1, double *a2) {
     switch (G) {
-        case 16: correlate_level<16, NL>(L, d0, alpha, beta, B, pnum, cross, a1, a2); break;
-        case 8: correlate_level<8, NL>(L, d0, alpha, beta, B, pnum, cross, a1, a2); break;
-        case 4: correlate_level<4, NL>(L, d0, alpha, beta, B, pnum, cross, a1, a2); break;
-        case 2: correlate_level<2, NL>(L, d0, alpha, beta, B, pnum, cross, a1, a2); break;
-        default: correlate_level<1, NL>(L, d0, alpha, beta, B, pnum, cross, a1, a2); break;
+        case 16: correlate_level<16, NL>(L, d0, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2); break;
+        case 8: correlate_level<8, NL>(L, d0, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2); break;
+        case 4: correlate_level<4, NL>(L, d0, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2); break;
+        case 2: correlate_level<2, NL>(L, d0, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2); break;
+        default: correlate_level<1, NL>(L, d0, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2); break;
     }
 }
 
@@ -1098,13 +1098,25 @@ __global__ void __launch_bounds__(256) k_level_sums(const float *X, const double
     dst[i] = X ? (double)X[a] + (double)X[b] : src[a] + src[b];
 }
 
+// One thread per (pair, level, time chunk) for every channel layout of
+// setParameters: the level's channels are taken in runs of at most 17
+// consecutive delays (p = 16: delays 0..16 on level 0, 9..16 on the others;
+// a repeated delay starts a new run), each run one pass over the chunk's
+// values (and the delays past its end).  Chunks of clen frames hold whole
+// blocks of the blocked variance at every level; each thread writes its
+// channels' sums of the chunk, part[chunk][pair][channel] = {cross, a1, a2},
+// and k_correlate_finalize adds the chunks.
 __global__ void __launch_bounds__(64) k_correlate(SeriesGeom g, BleachDesc bd, const CorrTables *__restrict__ ctp,
                                                   PairDesc pd, const double *F, const float *X, const int32_t *ind,
-                                                  double *corav, double *bvar, const double *lvg, LevelOffsets lo) {
+                                                  const double *lvg, LevelOffsets lo, int clen, double *part) {
     const CorrTables &ct = *ctp;
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    int c = blockIdx.y;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y, chunk = blockIdx.z;
     if (p >= pd.npairs || c >= ct.nlevels) return;
+    const int len = clen >> c;   // values of level c per chunk
+    const int ybeg = chunk * len;
+    if (ybeg >= ct.numbin[c]) return;
+    const int yend = min(ct.numbin[c], ybeg + len);
     int sa, sb;
     pair_series(pd, p, sa, sb);
     LevelCtx L;
@@ -1128,41 +1140,73 @@ __global__ void __launch_bounds__(64) k_correlate(SeriesGeom g, BleachDesc bd, c
     int G = 16;
     const int bmin = 1 << max(ct.q - 1 - c, 0);
     if (bmin < G) G = bmin;
+    double *out = part + ((int64_t)chunk * pd.npairs + p) * ct.chanum * 3;
     for (int r0 = 0; r0 < nch;) {
         int nl = 1;
         while (r0 + nl < nch && nl < 17 && ct.del[chan0 + r0 + nl] == ct.del[chan0 + r0] + nl) nl++;
         const int d0 = ct.del[chan0 + r0];
-        double am[17], bmn[17], alpha[17], beta[17];
+        double alpha[17], beta[17];
 #pragma unroll
-        for (int i = 0; i < 17; i++) {
-            am[i] = bmn[i] = 1.0;
-            alpha[i] = beta[i] = 0.0;
-        }
+        for (int i = 0; i < 17; i++) alpha[i] = beta[i] = 0.0;
         for (int i = 0; i < nl; i++) {
             const int x = chan0 + r0 + i;
             const double n = (double)(ct.numbin[c] - ct.del[x]);
-            am[i] = Fa[ct.pos_a_end[x]] / n;
-            bmn[i] = (Fb[ct.pos_b_end[x]] - Fb[ct.pos_b_beg[x]]) / n;
-            alpha[i] = am[i] - shift_a;
-            beta[i] = bmn[i] - shift_b;
+            alpha[i] = Fa[ct.pos_a_end[x]] / n - shift_a;
+            beta[i] = (Fb[ct.pos_b_end[x]] - Fb[ct.pos_b_beg[x]]) / n - shift_b;
         }
         double cross[17], a1[17], a2[17];
         if (nl <= 8)
-            correlate_run<8>(L, G, d0, alpha, beta, B, pnum, cross, a1, a2);
+            correlate_run<8>(L, G, d0, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2);
         else
-            correlate_run<17>(L, G, d0, alpha, beta, B, pnum, cross, a1, a2);
+            correlate_run<17>(L, G, d0, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2);
         for (int i = 0; i < nl; i++) {
-            const int x = chan0 + r0 + i;
-            const double n = (double)(ct.numbin[c] - ct.del[x]);
-            const double tot = cross[i] - n * alpha[i] * beta[i];
-            const double ab = am[i] * bmn[i];
-            corav[(int64_t)p * ct.chanum + x] = tot / (n * am[i] * bmn[i]);
-            const double sp = a1[i] / B, sp2 = a2[i] / ((double)B * B);
-            const double pn = (double)pnum;
-            bvar[(int64_t)p * ct.chanum + x] = (sp2 / pn - (sp / pn) * (sp / pn)) / ((pn - 1) * (ab * ab));
+            double *o = out + (chan0 + r0 + i) * 3;
+            o[0] = cross[i];
+            o[1] = a1[i];
+            o[2] = a2[i];
         }
         r0 += nl;
     }
+}
+
+// G and the blocked variance of every (pair, channel) from the chunks' sums,
+// added in chunk order.
+__global__ void __launch_bounds__(64) k_correlate_finalize(SeriesGeom g, BleachDesc bd,
+                                                           const CorrTables *__restrict__ ctp, PairDesc pd,
+                                                           const double *F, const int32_t *ind, const double *part,
+                                                           int clen, double *corav, double *bvar) {
+    const CorrTables &ct = *ctp;
+    const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (i >= (int64_t)pd.npairs * ct.chanum) return;
+    const int p = (int)(i / ct.chanum), x = (int)(i - (int64_t)p * ct.chanum);
+    int sa, sb;
+    pair_series(pd, p, sa, sb);
+    const int ka = g.slot_of[sa], kb = g.slot_of[sb];
+    const int c = ct.level[x];
+    const double *Fa = F + (int64_t)ka * ct.npos, *Fb = F + (int64_t)kb * ct.npos;
+    const double n = (double)(ct.numbin[c] - ct.del[x]);
+    const double am = Fa[ct.pos_a_end[x]] / n;
+    const double bmn = (Fb[ct.pos_b_end[x]] - Fb[ct.pos_b_beg[x]]) / n;
+    const double alpha = am - slot_mu(bd.coef, bd.order, ka) * (double)(1 << c);
+    const double beta = bmn - slot_mu(bd.coef, bd.order, kb) * (double)(1 << c);
+    double cross = 0.0, a1 = 0.0, a2 = 0.0;
+    // chunks that hold level-c values
+    const int nchunk = (ct.numbin[c] + (clen >> c) - 1) / (clen >> c);
+    for (int k = 0; k < nchunk; k++) {
+        const double *o = part + (((int64_t)k * pd.npairs + p) * ct.chanum + x) * 3;
+        cross += o[0];
+        a1 += o[1];
+        a2 += o[2];
+    }
+    const int indp = ind[p];
+    const int B = 1 << (indp - c);
+    const int pnum = (int)floor(ct.mtab_last / pow(2.0, fmax(indp - ct.log2_samp_last, 0.0)));
+    const double tot = cross - n * alpha * beta;
+    const double ab = am * bmn;
+    corav[i] = tot / (n * am * bmn);
+    const double sp = a1 / B, sp2 = a2 / ((double)B * B);
+    const double pn = (double)pnum;
+    bvar[i] = (sp2 / pn - (sp / pn) * (sp / pn)) / ((pn - 1) * (ab * ab));
 }
 
 // ------------------------------------------------------------------
@@ -2832,17 +2876,29 @@ size_t generic_level_values(const CorrTables &ct, int nslots, LevelOffsets &lo) 
     return tot;
 }
 
+int generic_chunk_len(const CorrTables &ct);
+
 void launch_correlate(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const CorrTables *dct,
                       const PairDesc &pd, const double *F, const float *X, const int32_t *ind, double *corav,
-                      double *bvar, double *lvg, const LevelOffsets &lo, hipStream_t st) {
+                      double *bvar, double *lvg, const LevelOffsets &lo, double *part, hipStream_t st) {
     const int ns = g.nslots;
     for (int c = 1; c < ct.nlevels; c++) {
         const int64_t n = (int64_t)ct.numbin[c] * ns;
         hipLaunchKernelGGL(k_level_sums, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, st, c == 1 ? X : nullptr,
                            c == 1 ? nullptr : lvg + lo.off[c - 1], lvg + lo.off[c], ct.numbin[c], ns);
     }
-    dim3 grid(cdiv(pd.npairs, 64), ct.nlevels);
-    hipLaunchKernelGGL(k_correlate, grid, dim3(64), 0, st, g, bd, dct, pd, F, X, ind, corav, bvar, lvg, lo);
+    const int clen = generic_chunk_len(ct);
+    const int nchunk = (ct.T + clen - 1) / clen;
+    dim3 grid(cdiv(pd.npairs, 64), ct.nlevels, nchunk);
+    hipLaunchKernelGGL(k_correlate, grid, dim3(64), 0, st, g, bd, dct, pd, F, X, ind, lvg, lo, clen, part);
+    hipLaunchKernelGGL(k_correlate_finalize, dim3(cdiv((int64_t)pd.npairs * ct.chanum, 64)), dim3(64), 0, st, g, bd,
+                       dct, pd, F, ind, part, clen, corav, bvar);
+}
+
+int generic_chunk_len(const CorrTables &ct) {
+    // a multiple of every block of the blocked variance on every level:
+    // blocks of 2^ind frames, ind <= max(blocknum - 1, q - 1)
+    return 1 << std::max(ct.blocknum - 1, ct.q - 1);
 }
 
 static void launch_lm(const LMArgs &a, unsigned free_mask, hipStream_t st) {
