@@ -698,7 +698,7 @@ int engine_rest(imfcs_ctx *ctx, bool do_apply) {
         HIP_TRY(ctx->lvg.ensure(sizeof(double) * std::max<size_t>(nlv, 1)));
         const int gclen = generic_chunk_len(ct);
         const size_t gnch = (size_t)((g.T + gclen - 1) / gclen);
-        HIP_TRY(ctx->hpart.ensure(sizeof(double) * gnch * np * ct.chanum * 3));
+        HIP_TRY(ctx->hpart.ensure(sizeof(double) * (gnch * 3 + 2) * np * ct.chanum));
         HIP_TRY(hipEventRecord(ctx->evk[0], st));
         launch_correlate(g, bd, ct, dct, ea.pd, ctx->F.as<double>(), ctx->X.as<float>(), ctx->ind.as<int32_t>(),
                          ctx->corav.as<double>(), ctx->bvar.as<double>(), ctx->lvg.as<double>(), lo,

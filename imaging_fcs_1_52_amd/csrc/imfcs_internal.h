@@ -288,7 +288,8 @@ struct LevelOffsets {
 };
 size_t generic_level_values(const CorrTables &ct, int nslots, LevelOffsets &lo);
 // Frames per time chunk of the table-driven correlator (part holds
-// ceil(T / chunk) x npairs x chanum x 3 sums).
+// ceil(T / chunk) x npairs x chanum x 3 sums, then npairs x chanum x 2
+// window means).
 int generic_chunk_len(const CorrTables &ct);
 void launch_correlate(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct, const CorrTables *dct,
                       const PairDesc &pd, const double *F, const float *X, const int32_t *ind, double *corav,

@@ -974,118 +974,129 @@ struct LevelCtx {
     const float *X;
     const double *lv;   // level c >= 1: [numbin][ld] FP64 pairwise sums (k_level_sums)
     int64_t ld;
-    int ka, kb, c, numbin, same;
-    // level-c value: sum of 2^c stored deviations, i.e. I'-sum minus 2^c mu
-    __device__ __forceinline__ double val(int k, int y) const {
-        if (y >= numbin) return 0.0;
-        return c == 0 ? (double)X[(int64_t)y * ld + k] : lv[(int64_t)y * ld + k];
+    int ka, kb, c, numbin;
+    // level-c value: sum of 2^c stored deviations, i.e. I'-sum minus 2^c mu.
+    // y is wave-uniform: the row's base goes into a scalar buffer resource
+    // and the lane offset is the slot's; values past the series end read 0.
+    // CHECK false: the caller has shown y < numbin; F32: level 0 (X)
+    template <bool CHECK, bool F32>
+    __device__ __forceinline__ double get(int k, int y) const {
+        if (CHECK && y >= numbin) return 0.0;
+        if (F32) {
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                (void *)(X + (int64_t)y * ld), (short)0, (int)(ld * 4), kBufferWord3);
+            return (double)__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, k * 4, 0, 0));
+        }
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(lv + (int64_t)y * ld), (short)0, (int)(ld * 8), kBufferWord3);
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, k * 8, 0, 0));
     }
-    __device__ __forceinline__ double val_a(int y) const { return val(ka, y); }
-    __device__ __forceinline__ double val_b(int y) const { return val(kb, y); }
 };
 
 // One run of nl <= NL channels of a level with the consecutive delays
-// d0 .. d0 + nl - 1: a register window of b~ values [y0 + d0, y0 + d0 + WN)
-// slides by G.  Channels i >= nl are computed and dropped.  Products past
-// the series end (y + d >= numbin) do not exist; the block sums Sa, Sb run
-// over the existing products, which for every block j < pnum are all of its
-// B products.
-template <int G, int NL>
-__device__ void correlate_level(const LevelCtx &L, int d0, const double *alpha, const double *beta, int B, int pnum,
-                                int ybeg, int yend, double *cross, double *a1, double *a2) {
+// d0 .. d0 + nl - 1 over the values [ybeg, yend): a register window of b~
+// values [y0 + d0, y0 + d0 + WN) slides by G.  Values past the series end
+// read as 0, so products with y + d >= numbin add nothing.  Per block only
+// Sab (per channel), Sa and the channel-d0 sum Sb0 are accumulated; the
+// other channels' Sb follow from Sb0 at the block's end,
+//   Sb_i = Sb0 + sum_{k<i} (b~[ye + d0 + k] - b~[ys + d0 + k]),
+// with the first values past the block in the window and those at its start
+// read again.  Every block j < pnum holds all of its B products.  The run's
+// sums {cross, a1, a2} of channel i live at op[(3 i + m) ps] (m = 0, 1, 2)
+// and are added to at block ends; alpha_i, beta_i are at ab[(2 i + m) ps].
+template <int G, int NL, bool F32>
+__device__ __forceinline__ void correlate_level(const LevelCtx &L, int d0, int nl, int B, int pnum, int ybeg, int yend,
+                                const double *ab, double *op, int64_t ps) {
     constexpr int WN = G + NL - 1;
     double w[WN];   // b~ values [y0 + d0, y0 + d0 + WN)
     double ga[G];   // a~ values [y0, y0 + G)
-    double sab[NL], sb[NL], sa = 0.0;
+    double sab[NL], sa = 0.0, sb0 = 0.0;
 #pragma unroll
     for (int i = 0; i < NL; i++) {
-        sab[i] = sb[i] = 0.0;
-        cross[i] = a1[i] = a2[i] = 0.0;
+        sab[i] = 0.0;
+        if (i < nl) op[(3 * i) * ps] = op[(3 * i + 1) * ps] = op[(3 * i + 2) * ps] = 0.0;
     }
 #pragma unroll
-    for (int k = 0; k < WN; k++) w[k] = L.val_b(ybeg + d0 + k);
-    const int numbin = L.numbin;
-    const bool reuse = L.same && d0 == 0;   // the a window is the b window
-    for (int y0 = ybeg; y0 < yend; y0 += G) {
+    for (int k = 0; k < WN; k++) w[k] = L.template get<true, F32>(L.kb, ybeg + d0 + k);
+    // one group of G values; CHECK: the group reaches past the series end
+    auto step = [&](int y0, auto check) {
+        constexpr bool CHECK = decltype(check)::value;
 #pragma unroll
-        for (int k = 0; k < G; k++) ga[k] = reuse ? w[k] : L.val_a(y0 + k);
-        if (y0 + d0 + WN <= numbin) {
-            double sg = 0.0;
+        for (int k = 0; k < G; k++) ga[k] = L.template get<CHECK, F32>(L.ka, y0 + k);
+        double sg = 0.0, tg = 0.0;
 #pragma unroll
-            for (int k = 0; k < G; k++) sg += ga[k];
-            sa += sg;
+        for (int k = 0; k < G; k++) {
+            sg += ga[k];
+            tg += w[k];
+        }
+        sa += sg;
+        sb0 += tg;
 #pragma unroll
-            for (int i = 0; i < NL; i++) {
-                double s = 0.0, t = 0.0;
+        for (int i = 0; i < NL; i++) {
+            double s = 0.0;
 #pragma unroll
-                for (int k = 0; k < G; k++) {
-                    s = fma(ga[k], w[k + i], s);
-                    t += w[k + i];
-                }
-                sab[i] += s;
-                sb[i] += t;
-            }
-        } else {
-            // series tail: only products with y + d < numbin exist
-#pragma unroll
-            for (int k = 0; k < G; k++)
-                if (y0 + k < numbin) sa += ga[k];
-#pragma unroll
-            for (int i = 0; i < NL; i++) {
-                double s = 0.0, t = 0.0;
-#pragma unroll
-                for (int k = 0; k < G; k++)
-                    if (y0 + k + d0 + i < numbin) {
-                        s = fma(ga[k], w[k + i], s);
-                        t += w[k + i];
-                    }
-                sab[i] += s;
-                sb[i] += t;
-            }
+            for (int k = 0; k < G; k++) s = fma(ga[k], w[k + i], s);
+            sab[i] += s;
         }
         const int yb = y0 + G;
         if ((yb & (B - 1)) == 0) {
-            int j = yb / B - 1;
-            bool inblk = j < pnum;
+            const bool inblk = yb / B - 1 < pnum;
+            double hd = 0.0;   // Sb_i - Sb0
 #pragma unroll
             for (int i = 0; i < NL; i++) {
-                if (inblk) {
-                    double S = sab[i] - beta[i] * sa - alpha[i] * sb[i] + (double)B * alpha[i] * beta[i];
-                    a1[i] += S;
-                    a2[i] += S * S;
+                if (i < nl) {
+                    double *o = op + (3 * i) * ps;
+                    o[0] += sab[i];
+                    if (inblk) {
+                        const double al = ab[(2 * i) * ps], be = ab[(2 * i + 1) * ps];
+                        const double S = sab[i] - be * sa - al * (sb0 + hd) + (double)B * al * be;
+                        o[ps] += S;
+                        o[2 * ps] += S * S;
+                    }
+                    if (inblk && i + 1 < nl) hd += w[G + i] - L.template get<true, F32>(L.kb, yb - B + d0 + i);
                 }
-                cross[i] += sab[i];
                 sab[i] = 0.0;
-                sb[i] = 0.0;
             }
-            sa = 0.0;
+            sa = sb0 = 0.0;
         }
         // slide the window by G
 #pragma unroll
         for (int k = 0; k < NL - 1; k++) w[k] = w[k + G];
 #pragma unroll
-        for (int k = 0; k < G; k++) w[NL - 1 + k] = L.val_b(y0 + d0 + WN + k);
+        for (int k = 0; k < G; k++) w[NL - 1 + k] = L.template get<CHECK, F32>(L.kb, y0 + d0 + WN + k);
+    };
+    for (int y0 = ybeg; y0 < yend; y0 += G) {
+        if (y0 + d0 + WN + G <= L.numbin)
+            step(y0, std::false_type{});
+        else
+            step(y0, std::true_type{});
     }
 #pragma unroll
-    for (int i = 0; i < NL; i++) cross[i] += sab[i];
+    for (int i = 0; i < NL; i++)
+        if (i < nl) op[(3 * i) * ps] += sab[i];
+}
+
+template <int NL, bool F32>
+__device__ __forceinline__ void correlate_run_t(const LevelCtx &L, int G, int d0, int nl, int B, int pnum, int ybeg,
+                                                int yend, const double *ab, double *op, int64_t ps) {
+    switch (G) {
+        case 16: correlate_level<16, NL, F32>(L, d0, nl, B, pnum, ybeg, yend, ab, op, ps); break;
+        case 8: correlate_level<8, NL, F32>(L, d0, nl, B, pnum, ybeg, yend, ab, op, ps); break;
+        case 4: correlate_level<4, NL, F32>(L, d0, nl, B, pnum, ybeg, yend, ab, op, ps); break;
+        case 2: correlate_level<2, NL, F32>(L, d0, nl, B, pnum, ybeg, yend, ab, op, ps); break;
+        default: correlate_level<1, NL, F32>(L, d0, nl, B, pnum, ybeg, yend, ab, op, ps); break;
+    }
 }
 
 template <int NL>
-__device__ void correlate_run(const LevelCtx &L, int G, int d0, const double *alpha, const double *beta, int B,
-                              int pnum, int ybeg, int yend, double *cross, double *a1, double *a2) {
-    switch (G) {
-        case 16: correlate_level<16, NL>(L, d0, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2); break;
-        case 8: correlate_level<8, NL>(L, d0, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2); break;
-        case 4: correlate_level<4, NL>(L, d0, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2); break;
-        case 2: correlate_level<2, NL>(L, d0, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2); break;
-        default: correlate_level<1, NL>(L, d0, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2); break;
-    }
+__device__ __forceinline__ void correlate_run(const LevelCtx &L, int G, int d0, int nl, int B, int pnum, int ybeg,
+                                              int yend, const double *ab, double *op, int64_t ps) {
+    if (L.c == 0)
+        correlate_run_t<NL, true>(L, G, d0, nl, B, pnum, ybeg, yend, ab, op, ps);
+    else
+        correlate_run_t<NL, false>(L, G, d0, nl, B, pnum, ybeg, yend, ab, op, ps);
 }
 
-// One thread per (pair, level) for every channel layout of setParameters:
-// the level's channels are taken in runs of at most 17 consecutive delays
-// (p = 16: delays 0..16 on level 0, 9..16 on the others; a repeated delay
-// starts a new run), each run one pass over the level's values.
 // Level c >= 1 of every slot for the table-driven correlator, FP64 sums of
 // the two values of level c - 1 (level 0: the stored FP32 deviations X):
 // dst[y][s] = src[2y][s] + src[2y+1][s], one thread per (value, slot).
@@ -1098,17 +1109,46 @@ __global__ void __launch_bounds__(256) k_level_sums(const float *X, const double
     dst[i] = X ? (double)X[a] + (double)X[b] : src[a] + src[b];
 }
 
+// Window means of every (pair, channel): am, bmn of the channel's products
+// and alpha, beta relative to the level's shift; ab[(2 x + m) npairs + p].
+__device__ __forceinline__ void channel_means(const SeriesGeom &g, const BleachDesc &bd, const CorrTables &ct,
+                                              const PairDesc &pd, const double *F, int p, int x, double &am,
+                                              double &bmn, double &alpha, double &beta) {
+    int sa, sb;
+    pair_series(pd, p, sa, sb);
+    const int ka = g.slot_of[sa], kb = g.slot_of[sb];
+    const int c = ct.level[x];
+    const double *Fa = F + (int64_t)ka * ct.npos, *Fb = F + (int64_t)kb * ct.npos;
+    const double n = (double)(ct.numbin[c] - ct.del[x]);
+    am = Fa[ct.pos_a_end[x]] / n;
+    bmn = (Fb[ct.pos_b_end[x]] - Fb[ct.pos_b_beg[x]]) / n;
+    alpha = am - slot_mu(bd.coef, bd.order, ka) * (double)(1 << c);
+    beta = bmn - slot_mu(bd.coef, bd.order, kb) * (double)(1 << c);
+}
+
+__global__ void __launch_bounds__(64) k_correlate_means(SeriesGeom g, BleachDesc bd,
+                                                        const CorrTables *__restrict__ ctp, PairDesc pd,
+                                                        const double *F, double *ab) {
+    const CorrTables &ct = *ctp;
+    const int p = blockIdx.x * 64 + threadIdx.x, x = blockIdx.y;
+    if (p >= pd.npairs) return;
+    double am, bmn, alpha, beta;
+    channel_means(g, bd, ct, pd, F, p, x, am, bmn, alpha, beta);
+    ab[(2 * (int64_t)x) * pd.npairs + p] = alpha;
+    ab[(2 * (int64_t)x + 1) * pd.npairs + p] = beta;
+}
+
 // One thread per (pair, level, time chunk) for every channel layout of
 // setParameters: the level's channels are taken in runs of at most 17
 // consecutive delays (p = 16: delays 0..16 on level 0, 9..16 on the others;
 // a repeated delay starts a new run), each run one pass over the chunk's
 // values (and the delays past its end).  Chunks of clen frames hold whole
-// blocks of the blocked variance at every level; each thread writes its
-// channels' sums of the chunk, part[chunk][pair][channel] = {cross, a1, a2},
-// and k_correlate_finalize adds the chunks.
-__global__ void __launch_bounds__(64) k_correlate(SeriesGeom g, BleachDesc bd, const CorrTables *__restrict__ ctp,
-                                                  PairDesc pd, const double *F, const float *X, const int32_t *ind,
-                                                  const double *lvg, LevelOffsets lo, int clen, double *part) {
+// blocks of the blocked variance at every level; the chunk's sums go to
+// part[((chunk chanum + x) 3 + m) npairs + p] (m: cross, a1, a2) and
+// k_correlate_finalize adds the chunks.
+__global__ void __launch_bounds__(64) k_correlate(SeriesGeom g, const CorrTables *__restrict__ ctp, PairDesc pd,
+                                                  const float *X, const int32_t *ind, const double *lvg,
+                                                  LevelOffsets lo, const double *ab, int clen, double *part) {
     const CorrTables &ct = *ctp;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const int c = blockIdx.y, chunk = blockIdx.z;
@@ -1127,10 +1167,6 @@ __global__ void __launch_bounds__(64) k_correlate(SeriesGeom g, BleachDesc bd, c
     L.kb = g.slot_of[sb];
     L.c = c;
     L.numbin = ct.numbin[c];
-    L.same = (L.ka == L.kb);
-    const double *Fa = F + (int64_t)L.ka * ct.npos, *Fb = F + (int64_t)L.kb * ct.npos;
-    const double shift_a = slot_mu(bd.coef, bd.order, L.ka) * (double)(1 << c);
-    const double shift_b = slot_mu(bd.coef, bd.order, L.kb) * (double)(1 << c);
     const int chan0 = ct.first_chan[c], nch = ct.nchan[c];
     const int indp = ind[p];
     const int B = 1 << (indp - c);
@@ -1140,31 +1176,14 @@ __global__ void __launch_bounds__(64) k_correlate(SeriesGeom g, BleachDesc bd, c
     int G = 16;
     const int bmin = 1 << max(ct.q - 1 - c, 0);
     if (bmin < G) G = bmin;
-    double *out = part + ((int64_t)chunk * pd.npairs + p) * ct.chanum * 3;
+    const int64_t ps = pd.npairs;
     for (int r0 = 0; r0 < nch;) {
         int nl = 1;
-        while (r0 + nl < nch && nl < 17 && ct.del[chan0 + r0 + nl] == ct.del[chan0 + r0] + nl) nl++;
-        const int d0 = ct.del[chan0 + r0];
-        double alpha[17], beta[17];
-#pragma unroll
-        for (int i = 0; i < 17; i++) alpha[i] = beta[i] = 0.0;
-        for (int i = 0; i < nl; i++) {
-            const int x = chan0 + r0 + i;
-            const double n = (double)(ct.numbin[c] - ct.del[x]);
-            alpha[i] = Fa[ct.pos_a_end[x]] / n - shift_a;
-            beta[i] = (Fb[ct.pos_b_end[x]] - Fb[ct.pos_b_beg[x]]) / n - shift_b;
-        }
-        double cross[17], a1[17], a2[17];
-        if (nl <= 8)
-            correlate_run<8>(L, G, d0, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2);
-        else
-            correlate_run<17>(L, G, d0, alpha, beta, B, pnum, ybeg, yend, cross, a1, a2);
-        for (int i = 0; i < nl; i++) {
-            double *o = out + (chan0 + r0 + i) * 3;
-            o[0] = cross[i];
-            o[1] = a1[i];
-            o[2] = a2[i];
-        }
+        while (r0 + nl < nch && nl < 8 && ct.del[chan0 + r0 + nl] == ct.del[chan0 + r0] + nl) nl++;
+        const int x0 = chan0 + r0;
+        const double *abr = ab + 2 * x0 * ps + p;
+        double *op = part + ((int64_t)chunk * ct.chanum + x0) * 3 * ps + p;
+        correlate_run<8>(L, G, ct.del[x0], nl, B, pnum, ybeg, yend, abr, op, ps);
         r0 += nl;
     }
 }
@@ -1176,37 +1195,32 @@ __global__ void __launch_bounds__(64) k_correlate_finalize(SeriesGeom g, BleachD
                                                            const double *F, const int32_t *ind, const double *part,
                                                            int clen, double *corav, double *bvar) {
     const CorrTables &ct = *ctp;
-    const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
-    if (i >= (int64_t)pd.npairs * ct.chanum) return;
-    const int p = (int)(i / ct.chanum), x = (int)(i - (int64_t)p * ct.chanum);
-    int sa, sb;
-    pair_series(pd, p, sa, sb);
-    const int ka = g.slot_of[sa], kb = g.slot_of[sb];
+    const int p = blockIdx.x * 64 + threadIdx.x, x = blockIdx.y;
+    if (p >= pd.npairs) return;
     const int c = ct.level[x];
-    const double *Fa = F + (int64_t)ka * ct.npos, *Fb = F + (int64_t)kb * ct.npos;
+    double am, bmn, alpha, beta;
+    channel_means(g, bd, ct, pd, F, p, x, am, bmn, alpha, beta);
     const double n = (double)(ct.numbin[c] - ct.del[x]);
-    const double am = Fa[ct.pos_a_end[x]] / n;
-    const double bmn = (Fb[ct.pos_b_end[x]] - Fb[ct.pos_b_beg[x]]) / n;
-    const double alpha = am - slot_mu(bd.coef, bd.order, ka) * (double)(1 << c);
-    const double beta = bmn - slot_mu(bd.coef, bd.order, kb) * (double)(1 << c);
+    const int64_t ps = pd.npairs;
     double cross = 0.0, a1 = 0.0, a2 = 0.0;
     // chunks that hold level-c values
     const int nchunk = (ct.numbin[c] + (clen >> c) - 1) / (clen >> c);
     for (int k = 0; k < nchunk; k++) {
-        const double *o = part + (((int64_t)k * pd.npairs + p) * ct.chanum + x) * 3;
+        const double *o = part + ((int64_t)k * ct.chanum + x) * 3 * ps + p;
         cross += o[0];
-        a1 += o[1];
-        a2 += o[2];
+        a1 += o[ps];
+        a2 += o[2 * ps];
     }
     const int indp = ind[p];
     const int B = 1 << (indp - c);
     const int pnum = (int)floor(ct.mtab_last / pow(2.0, fmax(indp - ct.log2_samp_last, 0.0)));
     const double tot = cross - n * alpha * beta;
-    const double ab = am * bmn;
+    const double abm = am * bmn;
+    const int64_t i = (int64_t)p * ct.chanum + x;
     corav[i] = tot / (n * am * bmn);
     const double sp = a1 / B, sp2 = a2 / ((double)B * B);
     const double pn = (double)pnum;
-    bvar[i] = (sp2 / pn - (sp / pn) * (sp / pn)) / ((pn - 1) * (ab * ab));
+    bvar[i] = (sp2 / pn - (sp / pn) * (sp / pn)) / ((pn - 1) * (abm * abm));
 }
 
 // ------------------------------------------------------------------
@@ -2889,10 +2903,14 @@ void launch_correlate(const SeriesGeom &g, const BleachDesc &bd, const CorrTable
     }
     const int clen = generic_chunk_len(ct);
     const int nchunk = (ct.T + clen - 1) / clen;
+    double *ab = part + (size_t)nchunk * pd.npairs * ct.chanum * 3;
+    hipLaunchKernelGGL(k_correlate_means, dim3(cdiv(pd.npairs, 64), ct.chanum), dim3(64), 0, st, g, bd, dct, pd, F,
+                       ab);
     dim3 grid(cdiv(pd.npairs, 64), ct.nlevels, nchunk);
-    hipLaunchKernelGGL(k_correlate, grid, dim3(64), 0, st, g, bd, dct, pd, F, X, ind, lvg, lo, clen, part);
-    hipLaunchKernelGGL(k_correlate_finalize, dim3(cdiv((int64_t)pd.npairs * ct.chanum, 64)), dim3(64), 0, st, g, bd,
-                       dct, pd, F, ind, part, clen, corav, bvar);
+    hipLaunchKernelGGL(k_correlate, grid, dim3(64), 0, st, g, dct, pd, X, ind, lvg, lo, (const double *)ab, clen,
+                       part);
+    hipLaunchKernelGGL(k_correlate_finalize, dim3(cdiv(pd.npairs, 64), ct.chanum), dim3(64), 0, st, g, bd, dct, pd, F,
+                       ind, (const double *)part, clen, corav, bvar);
 }
 
 int generic_chunk_len(const CorrTables &ct) {
