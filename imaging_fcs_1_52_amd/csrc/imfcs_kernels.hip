@@ -1006,7 +1006,7 @@ struct LevelCtx {
 // and are added to at block ends; alpha_i, beta_i are at ab[(2 i + m) ps].
 template <int G, int NL, bool F32>
 __device__ __forceinline__ void correlate_level(const LevelCtx &L, int d0, int nl, int B, int pnum, int ybeg, int yend,
-                                const double *ab, double *op, int64_t ps) {
+                                                const double *ab, double *op, int64_t ps, double *sl) {
     constexpr int WN = G + NL - 1;
     double w[WN];   // b~ values [y0 + d0, y0 + d0 + WN)
     double ga[G];   // a~ values [y0, y0 + G)
@@ -1040,22 +1040,29 @@ __device__ __forceinline__ void correlate_level(const LevelCtx &L, int d0, int n
         }
         const int yb = y0 + G;
         if ((yb & (B - 1)) == 0) {
-            const bool inblk = yb / B - 1 < pnum;
-            double hd = 0.0;   // Sb_i - Sb0
+            // block end: the channels one at a time (not unrolled) from
+            // the lane's LDS copy of Sab, so the loads and updates of the
+            // block sums do not all claim registers at once
 #pragma unroll
             for (int i = 0; i < NL; i++) {
-                if (i < nl) {
-                    double *o = op + (3 * i) * ps;
-                    o[0] += sab[i];
-                    if (inblk) {
-                        const double al = ab[(2 * i) * ps], be = ab[(2 * i + 1) * ps];
-                        const double S = sab[i] - be * sa - al * (sb0 + hd) + (double)B * al * be;
-                        o[ps] += S;
-                        o[2 * ps] += S * S;
-                    }
-                    if (inblk && i + 1 < nl) hd += w[G + i] - L.template get<true, F32>(L.kb, yb - B + d0 + i);
-                }
+                sl[i * 64 + threadIdx.x] = sab[i];
                 sab[i] = 0.0;
+            }
+            const bool inblk = yb / B - 1 < pnum;
+            double hd = 0.0;   // Sb_i - Sb0
+#pragma unroll 1
+            for (int i = 0; i < nl; i++) {
+                const double sv = sl[i * 64 + threadIdx.x];
+                double *o = op + (3 * i) * ps;
+                o[0] += sv;
+                if (inblk) {
+                    const double al = ab[(2 * i) * ps], be = ab[(2 * i + 1) * ps];
+                    const double S = sv - be * sa - al * (sb0 + hd) + (double)B * al * be;
+                    o[ps] += S;
+                    o[2 * ps] += S * S;
+                    hd += L.template get<true, F32>(L.kb, yb + d0 + i) -
+                          L.template get<true, F32>(L.kb, yb - B + d0 + i);
+                }
             }
             sa = sb0 = 0.0;
         }
@@ -1078,23 +1085,23 @@ __device__ __forceinline__ void correlate_level(const LevelCtx &L, int d0, int n
 
 template <int NL, bool F32>
 __device__ __forceinline__ void correlate_run_t(const LevelCtx &L, int G, int d0, int nl, int B, int pnum, int ybeg,
-                                                int yend, const double *ab, double *op, int64_t ps) {
+                                                int yend, const double *ab, double *op, int64_t ps, double *sl) {
     switch (G) {
-        case 16: correlate_level<16, NL, F32>(L, d0, nl, B, pnum, ybeg, yend, ab, op, ps); break;
-        case 8: correlate_level<8, NL, F32>(L, d0, nl, B, pnum, ybeg, yend, ab, op, ps); break;
-        case 4: correlate_level<4, NL, F32>(L, d0, nl, B, pnum, ybeg, yend, ab, op, ps); break;
-        case 2: correlate_level<2, NL, F32>(L, d0, nl, B, pnum, ybeg, yend, ab, op, ps); break;
-        default: correlate_level<1, NL, F32>(L, d0, nl, B, pnum, ybeg, yend, ab, op, ps); break;
+        case 16:
+        case 8: correlate_level<8, NL, F32>(L, d0, nl, B, pnum, ybeg, yend, ab, op, ps, sl); break;
+        case 4: correlate_level<4, NL, F32>(L, d0, nl, B, pnum, ybeg, yend, ab, op, ps, sl); break;
+        case 2: correlate_level<2, NL, F32>(L, d0, nl, B, pnum, ybeg, yend, ab, op, ps, sl); break;
+        default: correlate_level<1, NL, F32>(L, d0, nl, B, pnum, ybeg, yend, ab, op, ps, sl); break;
     }
 }
 
 template <int NL>
 __device__ __forceinline__ void correlate_run(const LevelCtx &L, int G, int d0, int nl, int B, int pnum, int ybeg,
-                                              int yend, const double *ab, double *op, int64_t ps) {
+                                              int yend, const double *ab, double *op, int64_t ps, double *sl) {
     if (L.c == 0)
-        correlate_run_t<NL, true>(L, G, d0, nl, B, pnum, ybeg, yend, ab, op, ps);
+        correlate_run_t<NL, true>(L, G, d0, nl, B, pnum, ybeg, yend, ab, op, ps, sl);
     else
-        correlate_run_t<NL, false>(L, G, d0, nl, B, pnum, ybeg, yend, ab, op, ps);
+        correlate_run_t<NL, false>(L, G, d0, nl, B, pnum, ybeg, yend, ab, op, ps, sl);
 }
 
 // Level c >= 1 of every slot for the table-driven correlator, FP64 sums of
@@ -1177,13 +1184,14 @@ __global__ void __launch_bounds__(64) k_correlate(SeriesGeom g, const CorrTables
     const int bmin = 1 << max(ct.q - 1 - c, 0);
     if (bmin < G) G = bmin;
     const int64_t ps = pd.npairs;
+    __shared__ double sabl[8][64];   // each lane's Sab at block ends
     for (int r0 = 0; r0 < nch;) {
         int nl = 1;
         while (r0 + nl < nch && nl < 8 && ct.del[chan0 + r0 + nl] == ct.del[chan0 + r0] + nl) nl++;
         const int x0 = chan0 + r0;
         const double *abr = ab + 2 * x0 * ps + p;
         double *op = part + ((int64_t)chunk * ct.chanum + x0) * 3 * ps + p;
-        correlate_run<8>(L, G, ct.del[x0], nl, B, pnum, ybeg, yend, abr, op, ps);
+        correlate_run<8>(L, G, ct.del[x0], nl, B, pnum, ybeg, yend, abr, op, ps, &sabl[0][0]);
         r0 += nl;
     }
 }
