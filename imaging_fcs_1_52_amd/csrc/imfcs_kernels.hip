@@ -1087,7 +1087,6 @@ template <int NL, bool F32>
 __device__ __forceinline__ void correlate_run_t(const LevelCtx &L, int G, int d0, int nl, int B, int pnum, int ybeg,
                                                 int yend, const double *ab, double *op, int64_t ps, double *sl) {
     switch (G) {
-        case 16:
         case 8: correlate_level<8, NL, F32>(L, d0, nl, B, pnum, ybeg, yend, ab, op, ps, sl); break;
         case 4: correlate_level<4, NL, F32>(L, d0, nl, B, pnum, ybeg, yend, ab, op, ps, sl); break;
         case 2: correlate_level<2, NL, F32>(L, d0, nl, B, pnum, ybeg, yend, ab, op, ps, sl); break;
@@ -1155,15 +1154,29 @@ __global__ void __launch_bounds__(64) k_correlate_means(SeriesGeom g, BleachDesc
 // k_correlate_finalize adds the chunks.
 __global__ void __launch_bounds__(64) k_correlate(SeriesGeom g, const CorrTables *__restrict__ ctp, PairDesc pd,
                                                   const float *X, const int32_t *ind, const double *lvg,
-                                                  LevelOffsets lo, const double *ab, int clen, double *part) {
+                                                  LevelOffsets lo, const double *ab, int clen, int nrun, double *part) {
     const CorrTables &ct = *ctp;
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    // blockIdx.x = ((pair block / 8) nrun + run) 8 + pair block % 8: the
+    // runs of one pair block read the same values and sit on one XCD
+    // (workgroups go to the XCDs round-robin), back to back
+    const int xcd = blockIdx.x & 7, t = blockIdx.x >> 3;
+    const int run = t % nrun, pblk = (t / nrun) * 8 + xcd;
+    const int p = pblk * 64 + threadIdx.x;
     const int c = blockIdx.y, chunk = blockIdx.z;
     if (p >= pd.npairs || c >= ct.nlevels) return;
     const int len = clen >> c;   // values of level c per chunk
     const int ybeg = chunk * len;
     if (ybeg >= ct.numbin[c]) return;
     const int yend = min(ct.numbin[c], ybeg + len);
+    // the run's channels: runs of at most 8 consecutive delays
+    const int chan0 = ct.first_chan[c], nch = ct.nchan[c];
+    int r0 = 0, nl = 0;
+    for (int r = 0; r <= run; r++) {
+        r0 += nl;
+        if (r0 >= nch) return;
+        nl = 1;
+        while (r0 + nl < nch && nl < 8 && ct.del[chan0 + r0 + nl] == ct.del[chan0 + r0] + nl) nl++;
+    }
     int sa, sb;
     pair_series(pd, p, sa, sb);
     LevelCtx L;
@@ -1174,26 +1187,20 @@ __global__ void __launch_bounds__(64) k_correlate(SeriesGeom g, const CorrTables
     L.kb = g.slot_of[sb];
     L.c = c;
     L.numbin = ct.numbin[c];
-    const int chan0 = ct.first_chan[c], nch = ct.nchan[c];
     const int indp = ind[p];
     const int B = 1 << (indp - c);
     const int pnum = (int)floor(ct.mtab_last / pow(2.0, fmax(indp - ct.log2_samp_last, 0.0)));
-    // products per group: 16, or fewer on levels whose blocks are shorter
+    // products per group: 8, or fewer on levels whose blocks are shorter
     // (B >= 2^(q-1-c) since ind >= q - 1)
-    int G = 16;
+    int G = 8;
     const int bmin = 1 << max(ct.q - 1 - c, 0);
     if (bmin < G) G = bmin;
     const int64_t ps = pd.npairs;
     __shared__ double sabl[8][64];   // each lane's Sab at block ends
-    for (int r0 = 0; r0 < nch;) {
-        int nl = 1;
-        while (r0 + nl < nch && nl < 8 && ct.del[chan0 + r0 + nl] == ct.del[chan0 + r0] + nl) nl++;
-        const int x0 = chan0 + r0;
-        const double *abr = ab + 2 * x0 * ps + p;
-        double *op = part + ((int64_t)chunk * ct.chanum + x0) * 3 * ps + p;
-        correlate_run<8>(L, G, ct.del[x0], nl, B, pnum, ybeg, yend, abr, op, ps, &sabl[0][0]);
-        r0 += nl;
-    }
+    const int x0 = chan0 + r0;
+    const double *abr = ab + 2 * x0 * ps + p;
+    double *op = part + ((int64_t)chunk * ct.chanum + x0) * 3 * ps + p;
+    correlate_run<8>(L, G, ct.del[x0], nl, B, pnum, ybeg, yend, abr, op, ps, &sabl[0][0]);
 }
 
 // G and the blocked variance of every (pair, channel) from the chunks' sums,
@@ -2914,9 +2921,20 @@ void launch_correlate(const SeriesGeom &g, const BleachDesc &bd, const CorrTable
     double *ab = part + (size_t)nchunk * pd.npairs * ct.chanum * 3;
     hipLaunchKernelGGL(k_correlate_means, dim3(cdiv(pd.npairs, 64), ct.chanum), dim3(64), 0, st, g, bd, dct, pd, F,
                        ab);
-    dim3 grid(cdiv(pd.npairs, 64), ct.nlevels, nchunk);
+    // most runs of at most 8 consecutive delays on one level
+    int nrun = 1;
+    for (int c = 0; c < ct.nlevels; c++) {
+        int n = 0;
+        for (int r0 = 0, nl; r0 < ct.nchan[c]; r0 += nl, n++) {
+            const int x = ct.first_chan[c] + r0;
+            nl = 1;
+            while (r0 + nl < ct.nchan[c] && nl < 8 && ct.del[x + nl] == ct.del[x] + nl) nl++;
+        }
+        nrun = std::max(nrun, n);
+    }
+    dim3 grid((unsigned)(cdiv(cdiv(pd.npairs, 64), 8) * 8 * nrun), ct.nlevels, nchunk);
     hipLaunchKernelGGL(k_correlate, grid, dim3(64), 0, st, g, dct, pd, X, ind, lvg, lo, (const double *)ab, clen,
-                       part);
+                       nrun, part);
     hipLaunchKernelGGL(k_correlate_finalize, dim3(cdiv(pd.npairs, 64), ct.chanum), dim3(64), 0, st, g, bd, dct, pd, F,
                        ind, (const double *)part, clen, corav, bvar);
 }
