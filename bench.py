@@ -105,13 +105,13 @@ def models(b):
     return imfcs.make_fit_model(ax, ax, PSF), imfcs.make_fit_model(ax, ax, PSF, rx=PIX * b)
 
 
-def configs(stack_min):
+def configs(stack_min, p=16):
     import imaging_fcs_1_52_amd as imfcs
     out = []
     for b in BINNINGS:
         acf, ccf = models(b)
         out.append(imfcs.make_pipeline_config(W, H, T, binning=b, cf=(b, 0), background=stack_min, poly_order=4,
-                                              do_acf=True, do_ccf=True, do_fit=True, acf_model=acf,
+                                              p=p, do_acf=True, do_ccf=True, do_fit=True, acf_model=acf,
                                               ccf_model=ccf, fit_tolerance=1e-16, fit_max_iterations=2000))
     return out
 
@@ -137,7 +137,7 @@ def solo(imfcs, stack, smin, args):
     b = args.solo_binning
     if b not in BINNINGS:
         raise SystemExit("--solo-binning must be one of %s" % (BINNINGS,))
-    cfg = configs(smin)[BINNINGS.index(b)]
+    cfg = configs(smin, args.correlator_p)[BINNINGS.index(b)]
     e = imfcs.Engine(0)
     d = e.upload(stack)
     rows = []
@@ -147,7 +147,7 @@ def solo(imfcs, stack, smin, args):
         if i >= args.warmup:
             rows.append(e.stage_ms())
     med = np.median(np.asarray(rows), 0)
-    print(json.dumps({"solo_binning": b, "runs": args.steps,
+    print(json.dumps({"solo_binning": b, "correlator_p": args.correlator_p, "runs": args.steps,
                       "stage_ms_median": {k: round(float(v), 3) for k, v in
                                           zip(("trace+fit", "correction+means", "blocking", "correlation", "lm"),
                                               med)}}), flush=True)
@@ -177,6 +177,9 @@ def main():
     ap.add_argument("--solo-binning", type=int, default=0,
                     help="profiling aid: run only this binning's plain pipeline --steps times (after --warmup), alone "
                          "on the GPU, and print its per-run stage times instead of the benchmark line")
+    ap.add_argument("--correlator-p", type=int, default=16,
+                    help="with --solo-binning: correlator panel P (16 runs the fast kernels, any other even P the "
+                         "table-driven correlator)")
     ap.add_argument("--enqueue-order", default=",".join(str(b) for b in ENQUEUE_ORDER),
                     help="order in which the binnings' pipelines are enqueued (captured), e.g. 1,2,4")
     ap.add_argument("--copy-calibration", action="store_true",
