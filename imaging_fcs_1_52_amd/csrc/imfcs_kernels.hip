@@ -1103,16 +1103,37 @@ __device__ __forceinline__ void correlate_run(const LevelCtx &L, int G, int d0, 
         correlate_run_t<NL, false>(L, G, d0, nl, B, pnum, ybeg, yend, ab, op, ps, sl);
 }
 
-// Level c >= 1 of every slot for the table-driven correlator, FP64 sums of
-// the two values of level c - 1 (level 0: the stored FP32 deviations X):
-// dst[y][s] = src[2y][s] + src[2y+1][s], one thread per (value, slot).
-__global__ void __launch_bounds__(256) k_level_sums(const float *X, const double *src, double *dst, int numbin,
-                                                    int nslots) {
+// Levels s+1 .. s+3 of every slot for the table-driven correlator in one
+// pass over level s (s = 0: the stored FP32 deviations X; else the FP64
+// level array src): each value is the FP64 sum of the two values below it,
+// dj[y][k] = d(j-1)[2y][k] + d(j-1)[2y+1][k].  One thread per (group of 4
+// level-(s+1) values, slot); a null dj (past the last level) is skipped and
+// n(j) = floor(n(j-1) / 2) bounds each level.
+__global__ void __launch_bounds__(256) k_level_sums(const float *X, const double *src, double *d1, double *d2,
+                                                    double *d3, int n1, int n2, int n3, int nslots) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= (int64_t)numbin * nslots) return;
-    const int64_t y = i / nslots, s = i - y * nslots;
-    const int64_t a = (2 * y) * nslots + s, b = a + nslots;
-    dst[i] = X ? (double)X[a] + (double)X[b] : src[a] + src[b];
+    const int64_t ngroup = (n1 + 3) / 4;
+    if (i >= ngroup * nslots) return;
+    const int64_t gi = i / nslots, k = i - gi * nslots;
+    double v1[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int64_t y = 4 * gi + j;
+        if (y < n1) {
+            const int64_t a = (2 * y) * nslots + k, b = a + nslots;
+            v1[j] = X ? (double)X[a] + (double)X[b] : src[a] + src[b];
+            d1[y * nslots + k] = v1[j];
+        }
+    }
+    if (!d2) return;
+    double v2[2];
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int64_t y = 2 * gi + j;
+        v2[j] = v1[2 * j] + v1[2 * j + 1];
+        if (y < n2) d2[y * nslots + k] = v2[j];
+    }
+    if (d3 && gi < n3) d3[gi * nslots + k] = v2[0] + v2[1];
 }
 
 // Window means of every (pair, channel): am, bmn of the channel's products
@@ -2911,10 +2932,13 @@ void launch_correlate(const SeriesGeom &g, const BleachDesc &bd, const CorrTable
                       const PairDesc &pd, const double *F, const float *X, const int32_t *ind, double *corav,
                       double *bvar, double *lvg, const LevelOffsets &lo, double *part, hipStream_t st) {
     const int ns = g.nslots;
-    for (int c = 1; c < ct.nlevels; c++) {
-        const int64_t n = (int64_t)ct.numbin[c] * ns;
+    for (int c = 1; c < ct.nlevels; c += 3) {
+        const int64_t n = (int64_t)((ct.numbin[c] + 3) / 4) * ns;
+        double *d2 = c + 1 < ct.nlevels ? lvg + lo.off[c + 1] : nullptr;
+        double *d3 = c + 2 < ct.nlevels ? lvg + lo.off[c + 2] : nullptr;
         hipLaunchKernelGGL(k_level_sums, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, st, c == 1 ? X : nullptr,
-                           c == 1 ? nullptr : lvg + lo.off[c - 1], lvg + lo.off[c], ct.numbin[c], ns);
+                           c == 1 ? nullptr : lvg + lo.off[c - 1], lvg + lo.off[c], d2, d3, ct.numbin[c],
+                           d2 ? ct.numbin[c + 1] : 0, d3 ? ct.numbin[c + 2] : 0, ns);
     }
     const int clen = generic_chunk_len(ct);
     const int nchunk = (ct.T + clen - 1) / clen;
