@@ -1018,11 +1018,19 @@ __device__ __forceinline__ void correlate_level(const LevelCtx &L, int d0, int n
     }
 #pragma unroll
     for (int k = 0; k < WN; k++) w[k] = L.template get<true, F32>(L.kb, ybeg + d0 + k);
-    // one group of G values; CHECK: the group reaches past the series end
+#pragma unroll
+    for (int k = 0; k < G; k++) ga[k] = L.template get<true, F32>(L.ka, ybeg + k);
+    // one group of G values; the next group's a~ values and window values
+    // are loaded first, in flight while this group's products run.  CHECK:
+    // those loads reach past the series end.
     auto step = [&](int y0, auto check) {
         constexpr bool CHECK = decltype(check)::value;
+        double na[G], nb[G];
 #pragma unroll
-        for (int k = 0; k < G; k++) ga[k] = L.template get<CHECK, F32>(L.ka, y0 + k);
+        for (int k = 0; k < G; k++) {
+            na[k] = L.template get<CHECK, F32>(L.ka, y0 + G + k);
+            nb[k] = L.template get<CHECK, F32>(L.kb, y0 + d0 + WN + k);
+        }
         double sg = 0.0, tg = 0.0;
 #pragma unroll
         for (int k = 0; k < G; k++) {
@@ -1070,10 +1078,13 @@ __device__ __forceinline__ void correlate_level(const LevelCtx &L, int d0, int n
 #pragma unroll
         for (int k = 0; k < NL - 1; k++) w[k] = w[k + G];
 #pragma unroll
-        for (int k = 0; k < G; k++) w[NL - 1 + k] = L.template get<CHECK, F32>(L.kb, y0 + d0 + WN + k);
+        for (int k = 0; k < G; k++) {
+            w[NL - 1 + k] = nb[k];
+            ga[k] = na[k];
+        }
     };
     for (int y0 = ybeg; y0 < yend; y0 += G) {
-        if (y0 + d0 + WN + G <= L.numbin)
+        if (y0 + 2 * G + d0 + NL - 1 <= L.numbin && y0 + 2 * G <= L.numbin)
             step(y0, std::false_type{});
         else
             step(y0, std::true_type{});
